@@ -1,0 +1,158 @@
+/*
+ * plask_fem.h -- C ABI of the MI355X (gfx950) FEM thermal / electrical solve path.
+ *
+ * Drop-in boundary for the steady-state FEM path of PLaSK's `thermal.static` and
+ * `electrical.shockley` solvers.  In the reference the seam is the FemMatrix
+ * family (plask/common/fem/matrix.hpp:21-123) driven by each solver's
+ * setMatrix()+solve() loop; a matrix-free GPU operator cannot sit behind
+ * FemMatrix::operator()(r,c) += v, so this ABI replaces the pair
+ * "setMatrix + FemMatrix::solve" of one compute() loop:
+ *
+ *   pf_grid_set + pf_coeffs_set + pf_source_set + pf_dirichlet_set
+ *       replace ThermalFem3DSolver::setMatrix (solvers/thermal/static/therm3d.cpp:170-279),
+ *       ThermalFem2DSolver<..>::setMatrix (therm2d.cpp:176-298 Cartesian, :301-440 cylindrical),
+ *       ElectricalFem2DSolver<..>::setMatrix (solvers/electrical/shockley/electr2d.cpp:234-329),
+ *       ElectricalFem3DSolver::setMatrix (electr3d.cpp:239-354)
+ *       and FemMatrix::applyBC (plask/common/fem/matrix.hpp:111-118)
+ *   pf_solve replaces FemMatrix::solve -> DpbMatrix::factorize/solverhs
+ *       (plask/common/fem/cholesky_matrix.hpp:90-111) and
+ *       SparseMatrix::solverhs (plask/common/fem/iterative_matrix.hpp:141-339)
+ *   pf_apply replaces FemMatrix::mult (matrix.hpp:89; cholesky_matrix.hpp:113-115;
+ *       iterative_matrix.hpp:341-344)
+ *
+ * Conventions
+ *   - Status codes, never exceptions: 0 = OK, < 0 = error (pf_last_error has the text).
+ *   - Node numbering is natural order, axis 0 fastest: node = i0 + n0*(i1 + n1*i2).
+ *     Element numbering likewise over (n0-1, n1-1, n2-1).  (The reference numbers
+ *     nodes by the mesh iteration order, plask/mesh/rectilinear3d.cpp:20-23; the
+ *     host solver classes translate.)
+ *   - Units as in the reference: coordinates in um, conductivities in W/(m K)
+ *     (thermal) or S/m (electrical), heat density in W/m^3.  The reference's unit
+ *     scalings (x1e-6 in 3D, therm3d.cpp:215 / electr3d.cpp:312-313; x r_mid in
+ *     cylindrical 2D, therm2d.cpp:333) are applied inside.
+ *   - Host buffers are owned by the caller and copied; device buffers are owned by
+ *     the context.  A context is not thread-safe; distinct contexts are independent.
+ *     All entry points return after their work is complete on the device.
+ */
+#ifndef PLASK_FEM_H
+#define PLASK_FEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pf_ctx pf_ctx;
+
+/* status codes */
+#define PF_OK 0
+#define PF_EINVAL -1   /* bad argument */
+#define PF_ESTATE -2   /* call order (e.g. solve before grid) */
+#define PF_EDEVICE -3  /* HIP runtime error */
+#define PF_ESHAPE -4   /* grid shape outside what the kernels support */
+#define PF_ENOCONV -5  /* did not reach the tolerance within maxit (result still written) */
+#define PF_EBREAK -6   /* CG breakdown: (p,Ap) <= 0 or (r,z) <= 0 -- matrix not positive definite */
+#define PF_ENONFINITE -8
+
+/* geometry */
+#define PF_GEOM_CARTESIAN 0
+#define PF_GEOM_CYLINDRICAL 1 /* 2D only: axis 0 is the radius */
+
+/* equation */
+#define PF_THERMAL 0
+#define PF_ELECTRICAL 1
+
+/* precision of a solve */
+#define PF_FP64 0         /* FP64 vectors and operator */
+#define PF_FP32_REFINE 1  /* FP32 operator + inner PCG, FP64 outer residual / update */
+
+/* start vector */
+#define PF_X0_GIVEN 0 /* x_inout holds the start vector (its Dirichlet entries are overwritten) */
+#define PF_X0_ZERO 1
+#define PF_X0_DEVICE 2 /* start from the solution kept on the device from the previous pf_solve */
+
+typedef struct {
+    int precision;    /* PF_FP64 | PF_FP32_REFINE */
+    int precond;      /* 1 = Jacobi (NSPCG 'jac'); others reserved */
+    int x0;           /* PF_X0_* */
+    int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
+    double rtol;      /* stop when ||B - A X||_2 <= rtol ||B||_2, B the eliminated load vector */
+    double inner_rtol;/* PF_FP32_REFINE: inner relative reduction per refinement step */
+    int max_outer;    /* PF_FP32_REFINE: max refinement steps */
+    int check_every;  /* iterations between host convergence checks (0 = default 32) */
+} pf_solve_opts;
+
+typedef struct {
+    int iters;          /* CG iterations (inner iterations summed for PF_FP32_REFINE) */
+    int outer;          /* refinement steps */
+    double rel_resid;   /* true ||B - A X|| / ||B|| at exit, computed in FP64 */
+    double bnorm;       /* ||B|| */
+    double seconds;     /* wall time of the solve on the device */
+    int status;         /* PF_OK or an error code */
+} pf_solve_stats;
+
+/* context ------------------------------------------------------------------ */
+int pf_ctx_create(pf_ctx** out, int device, int flags);
+void pf_ctx_destroy(pf_ctx* ctx);
+const char* pf_last_error(const pf_ctx* ctx);
+
+/* Rectilinear grid.  dim 2: (ax0,n0) x (ax1,n1); dim 3 adds (ax2,n2).  Replaces the
+ * mesh part of FemSolverWithMaskedMesh::getMatrix (plask/common/fem/fem_solver.hpp:196-241)
+ * for the full (non-masked) mesh. */
+int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, const double* ax1, int64_t n1,
+                const double* ax2, int64_t n2);
+
+/* Per-element conductivities: in-plane (c00 of Tensor2) and vertical (c11).  3D uses
+ * kx = ky = in-plane (therm3d.cpp:213-216, electr3d.cpp:310-314).  equation = PF_THERMAL
+ * or PF_ELECTRICAL selects nothing numerically (both use the same element matrix) but
+ * is recorded for the load vector (electrical has none, electr2d.cpp:268-317). */
+int pf_coeffs_set(pf_ctx* ctx, int equation, const double* k_inplane, const double* k_vert, int64_t nelem);
+
+/* Per-element heat density (W/m^3) for the load vector (therm2d.cpp:220, :346;
+ * therm3d.cpp:223).  NULL = zero. */
+int pf_source_set(pf_ctx* ctx, const double* q_elem, int64_t nelem);
+
+/* Dirichlet nodes and values (BandMatrix::setBC, matrix.hpp:132-145; first kind
+ * boundary conditions).  A node listed twice takes the last value. */
+int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n);
+
+/* Solve the eliminated system A X = B.  x_inout: host array of N node values
+ * (start vector in, solution out); may be NULL with PF_X0_ZERO / PF_X0_DEVICE to
+ * leave the solution on the device (pf_get_solution copies it out). */
+int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_stats* stats);
+
+/* Copy the device solution of the last pf_solve to the host (N values). */
+int pf_get_solution(pf_ctx* ctx, double* x_out);
+
+/* y = A x on device buffers of N values (FP64 when precision = PF_FP64, FP32 when
+ * PF_FP32_REFINE), A the Dirichlet-eliminated operator. */
+int pf_apply(pf_ctx* ctx, int precision, const void* x_dev, void* y_dev);
+
+/* Host-buffer convenience: y = A x (FP64). */
+int pf_apply_host(pf_ctx* ctx, const double* x, double* y);
+
+/* The eliminated load vector B (host, N values). */
+int pf_rhs_host(pf_ctx* ctx, double* b_out);
+
+/* Device pointer of the solution vector (FP64, N values) kept by the context. */
+void* pf_solution_device(pf_ctx* ctx);
+
+/* Number of nodes / elements of the current grid. */
+int64_t pf_num_nodes(const pf_ctx* ctx);
+int64_t pf_num_elements(const pf_ctx* ctx);
+
+/* Average duration (s) of each kernel class over the last pf_solve, measured with
+ * HIP events on the context stream: what[0] = operator apply + (p,Ap),
+ * what[1] = vector update + (r,z); counts[] receives launch counts.  For benchmarks. */
+int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
+
+/* Time `reps` back-to-back FP64 operator applies y = A x on the context stream with HIP
+ * events; returns the average seconds per apply.  For benchmarks. */
+int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""CPU oracle for the FEM thermal / electrical steady-state path.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package
+(``plask_amd``) never imports it.
+
+ctypes wrapper over ``liborc.so`` (built from ``fem_oracle.c`` by the Makefile
+beside this file) plus the reference's solver loops restated in Python:
+
+* ``thermal_solve``  -- one ``setMatrix`` + ``A.solve`` pass of
+  ``ThermalFem{2D,3D}Solver::compute`` (solvers/thermal/static/therm2d.cpp:473-496,
+  therm3d.cpp:311-334) with band Cholesky (cholesky_matrix.hpp:90-111) or
+  PCG on the symmetric diagonal storage (iterative_matrix.hpp:141-339).
+* ``electrical_compute`` -- the Picard loop of ``ElectricalFem{2D,3D}Solver::compute``
+  (solvers/electrical/shockley/electr2d.cpp:363-440, electr3d.cpp:356-442) with the
+  junction law of ``BetaSolver::activeCond`` (beta.hpp:43-46).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ST_BAND = 0
+ST_DIA = 1
+
+
+class _Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("cyl", C.c_int), ("n", C.c_int64 * 3),
+                ("ax", C.POINTER(C.c_double) * 3), ("s", C.c_int64 * 3)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        i64 = C.c_int64
+        L.orc_assemble.argtypes = [P, C.c_int, P, P, P, C.c_int, i64, i64, C.c_int, P, P, P, P, P, i64]
+        L.orc_assemble.restype = C.c_int
+        L.orc_dia_mult.argtypes = [i64, C.c_int, P, P, P, P]
+        L.orc_band_mult.argtypes = [i64, i64, i64, P, P, P]
+        L.orc_band_cholesky.argtypes = [i64, i64, i64, P]
+        L.orc_band_cholesky.restype = i64
+        L.orc_band_solve.argtypes = [i64, i64, i64, P, P]
+        L.orc_dia_pcg.argtypes = [i64, C.c_int, P, P, P, P, C.c_int, C.c_double, i64, P, P]
+        L.orc_dia_pcg.restype = i64
+        L.orc_ic0_factor.argtypes = [i64, C.c_int, P, P, P, P]
+        L.orc_ic0_factor.restype = i64
+        L.orc_apply.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.orc_junction_update.argtypes = [i64, C.c_int, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int, P]
+        L.orc_currents.argtypes = [P, P, P, P, P, P]
+        L.orc_wtime.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Grid:
+    """Rectilinear grid; ``order`` lists axes from major to minor
+    (reference iteration orders, plask/mesh/rectilinear3d.cpp:20-23)."""
+
+    def __init__(self, axes, cyl=False, order=None):
+        self.axes = [np.ascontiguousarray(a, dtype=np.float64) for a in axes]
+        self.dim = len(axes)
+        self.cyl = bool(cyl)
+        self.n = [len(a) for a in self.axes]
+        if order is None:
+            order = list(range(self.dim))[::-1]  # major = last axis, minor = axis 0
+        self.order = list(order)
+        s = [0, 0, 0]
+        st = 1
+        for a in reversed(self.order):
+            s[a] = st
+            st *= self.n[a]
+        self.s = s
+        self._c = _Grid()
+        self._c.dim = self.dim
+        self._c.cyl = int(self.cyl)
+        for a in range(3):
+            self._c.n[a] = self.n[a] if a < self.dim else 1
+            self._c.s[a] = s[a]
+        for a in range(self.dim):
+            self._c.ax[a] = self.axes[a].ctypes.data_as(C.POINTER(C.c_double))
+
+    @property
+    def nnodes(self):
+        return int(np.prod(self.n))
+
+    @property
+    def nelems(self):
+        return int(np.prod([m - 1 for m in self.n]))
+
+    def node_index(self, *ii):
+        return sum(int(i) * s for i, s in zip(ii, self.s))
+
+    def natural_perm(self):
+        """perm[k] = oracle node number of the k-th node in natural (axis-0-fastest) order."""
+        idx = np.indices(self.n[::-1]).reshape(self.dim, -1)[::-1]  # idx[a] = coordinate along axis a
+        return sum(idx[a].astype(np.int64) * self.s[a] for a in range(self.dim))
+
+    def band(self):
+        """Half bandwidth used by the reference getMatrix (fem_solver.hpp:118,126)
+        and the LAPACK leading dimension of DpbMatrix (cholesky_matrix.hpp:64-65)."""
+        kd = sum(self.s[:self.dim])
+        ld = ((kd + 1 + 1) & ~1) - 1
+        return kd, ld
+
+    def dia_offsets(self):
+        """Stored upper-diagonal offsets of SparseBandMatrix (iterative_matrix.hpp:355-390)."""
+        minor = self.order[-1]
+        if self.dim == 2:
+            m = self.n[minor]
+            return np.array([0, 1, m - 1, m, m + 1], dtype=np.int64)
+        m = self.n[minor]
+        M = m * self.n[self.order[1]]
+        return np.array([0, 1, m - 1, m, m + 1, M - m - 1, M - m, M - m + 1, M - 1, M, M + 1,
+                         M + m - 1, M + m, M + m + 1], dtype=np.int64)
+
+
+class Assembled:
+    def __init__(self, grid, kind, data, B, kd=0, ld=0, offsets=None):
+        self.grid, self.kind, self.data, self.B = grid, kind, data, B
+        self.kd, self.ld, self.offsets = kd, ld, offsets
+        self.rank = grid.nnodes
+
+    def mult(self, x):
+        y = np.zeros(self.rank)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if self.kind == ST_BAND:
+            lib().orc_band_mult(self.rank, self.ld, self.kd, _p(self.data), _p(x), _p(y))
+        else:
+            lib().orc_dia_mult(self.rank, len(self.offsets), _p(self.offsets), _p(self.data), _p(x), _p(y))
+        return y
+
+    def dense(self):
+        n = self.rank
+        return np.column_stack([self.mult(np.eye(n)[:, j]) for j in range(n)])
+
+
+def assemble(grid, mode, kh, kv, heat, kind, dnodes=(), dvals=()):
+    """Assemble the eliminated system (orc_assemble). ``mode`` 0 thermal, 1 electrical.
+    Element arrays in natural element order; dnodes are oracle node numbers."""
+    kh = np.ascontiguousarray(kh, dtype=np.float64)
+    kv = np.ascontiguousarray(kv, dtype=np.float64)
+    heat = None if heat is None else np.ascontiguousarray(heat, dtype=np.float64)
+    dn = np.ascontiguousarray(dnodes, dtype=np.int64)
+    dv = np.ascontiguousarray(dvals, dtype=np.float64)
+    N = grid.nnodes
+    B = np.zeros(N)
+    if kind == ST_BAND:
+        kd, ld = grid.band()
+        data = np.zeros(N * (ld + 1))
+        rc = lib().orc_assemble(C.byref(grid._c), mode, _p(kh), _p(kv), _p(heat), kind,
+                                ld, kd, 0, None, _p(data), _p(B), _p(dn), _p(dv), len(dn))
+        A = Assembled(grid, kind, data, B, kd=kd, ld=ld)
+    else:
+        off = grid.dia_offsets()
+        data = np.zeros(N * len(off))
+        rc = lib().orc_assemble(C.byref(grid._c), mode, _p(kh), _p(kv), _p(heat), kind, 0, 0, len(off), _p(off),
+                                _p(data), _p(B), _p(dn), _p(dv), len(dn))
+        A = Assembled(grid, kind, data, B, offsets=off)
+    if rc != 0:
+        raise RuntimeError("oracle assembly failed (%d)" % rc)
+    return A
+
+
+def band_cholesky_solve(A, B=None):
+    """dpbtrf + dpbtrs restated (cholesky_matrix.hpp:90-111). Factorizes a copy."""
+    ab = A.data.copy()
+    info = lib().orc_band_cholesky(A.rank, A.ld, A.kd, _p(ab))
+    if info != 0:
+        raise ArithmeticError("leading minor of order %d of the stiffness matrix is not positive-definite" % info)
+    x = np.array(A.B if B is None else B, dtype=np.float64)
+    lib().orc_band_solve(A.rank, A.ld, A.kd, _p(ab), _p(x))
+    return x
+
+
+def dia_pcg(A, x0, precond=1, rtol=1e-8, maxit=100000):
+    """PCG on the symmetric diagonal storage; precond 0 none, 1 Jacobi, 2 IC(0)."""
+    u = np.array(x0, dtype=np.float64)
+    rel = C.c_double(0.)
+    it = C.c_int64(0)
+    st = lib().orc_dia_pcg(A.rank, len(A.offsets), _p(A.offsets), _p(A.data), _p(A.B), _p(u), precond, rtol, maxit,
+                           C.byref(rel), C.byref(it))
+    return u, int(st), int(it.value), float(rel.value)
+
+
+def apply(grid, mode, kh, kv, dmask, x):
+    """Matrix-free y = M A0 (M x) + (I - M) x (orc_apply)."""
+    kh = np.ascontiguousarray(kh, dtype=np.float64)
+    kv = np.ascontiguousarray(kv, dtype=np.float64)
+    dm = np.ascontiguousarray(dmask, dtype=np.uint8)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    lib().orc_apply(C.byref(grid._c), mode, _p(kh), _p(kv), _p(dm), _p(x), _p(y))
+    return y
+
+
+def currents(grid, sh, sv, V):
+    """Element current densities (kA/cm2) and Joule heat densities (W/m3)."""
+    E = grid.nelems
+    cur = np.zeros(E * grid.dim)
+    heat = np.zeros(E)
+    lib().orc_currents(C.byref(grid._c), _p(np.ascontiguousarray(sh, np.float64)),
+                       _p(np.ascontiguousarray(sv, np.float64)), _p(np.ascontiguousarray(V, np.float64)),
+                       _p(cur), _p(heat))
+    return cur.reshape(E, grid.dim), heat
+
+
+def junction_update(elems, corners, ncorner, V, height, beta, js, stable, sig_v):
+    e = np.ascontiguousarray(elems, dtype=np.int64)
+    c = np.ascontiguousarray(corners, dtype=np.int64)
+    lib().orc_junction_update(len(e), ncorner, _p(e), _p(c), _p(np.ascontiguousarray(V, np.float64)), height, beta,
+                              js, int(stable), _p(sig_v))
+
+
+def wtime():
+    return lib().orc_wtime()

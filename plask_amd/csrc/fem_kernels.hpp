@@ -1,0 +1,667 @@
+// gfx950 kernels for the matrix-free Q1 FEM operator and the fused PCG steps.
+//
+// Operator.  For an axis-aligned box element with diagonal conductivity the Q1
+// stiffness matrix is  K = kx' S(x)M(y)M(z) + ky' M(x)S(y)M(z) + kz' M(x)M(y)S(z)
+// with S = [1 -1; -1 1], M = [1/3 1/6; 1/6 1/3] and k' the conductivity scaled by
+// the element sizes (the closed forms of solvers/thermal/static/therm3d.cpp:218-237
+// and therm2d.cpp:216-228).  In the per-dimension basis (sum, difference)
+//   sigma = x0 + x1,  delta = x1 - x0
+// both S and M are diagonal (S: sigma->0, delta->2 delta; M: sigma->sigma/2,
+// delta->delta/6), so K x = G^T diag(L) G x with G the 2^d-point sum/difference
+// transform.  One element costs ~55 flops for all 8 outputs instead of 64 FMAs,
+// and the two in-plane transform stages are shared by the two element layers that
+// touch a node plane.
+//
+// Data layout in HBM: nodes and elements in natural order, axis 0 fastest
+// (node = i + n0*(j + n1*k)).  Element data: two conductivities per element
+// (in-plane, vertical) and nothing else; element sizes come from 1D axis arrays.
+// Dirichlet nodes are a bitmask (1 bit per node).
+//
+// Kernel shape (3D): a block holds the whole x extent (BX = n0 threads per row)
+// and BY rows in y; thread (tx,ty) owns element column (tx, jb-1+ty) and node
+// column (tx, jb-1+ty).  Row ty = 0 is a halo row (its elements feed row 1; its
+// nodes are not written).  The block marches along z over a chunk of planes:
+// per plane one coalesced load of x into LDS, one LDS exchange of the element
+// corner outputs, two barriers.  Nothing is recomputed across z except one halo
+// element layer per chunk.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+enum { MODE_OP = 0, MODE_RES = 1 };
+
+// Scalars of one PCG solve, resident on the device.  Every kernel of the
+// iteration reads them; the last block of a reduction kernel writes them.
+struct PcgState {
+    double rz;       // (r, z) of the current iterate
+    double rz_prev;  // (r, z) of the previous iterate
+    double pap;      // (p, A p)
+    double alpha;
+    double rr;       // (r, r)
+    double stop2;    // stop when rr <= stop2
+    double bb;       // (b, b)
+    double spare;
+    int iters;
+    int maxit;
+    int done;    // 1: converged or stopped
+    int status;  // 0 ok, 1 maxit reached, -6 (p,Ap) <= 0, -7 (r,z) <= 0, -8 non-finite
+    unsigned int counter[4];  // block tickets for the last-block reductions
+};
+
+template <typename T> struct Axes3 {
+    int n0, n1, n2;
+    const T* hx;  // element sizes along each axis (um)
+    const T* hy;
+    const T* hz;
+};
+
+template <typename T> struct Axes2 {
+    int n0, n1;
+    const T* hx;    // element sizes (um)
+    const T* hy;
+    const T* rmid;  // cylindrical: element mid radius, else nullptr
+};
+
+__device__ __forceinline__ bool dbit(const uint64_t* __restrict__ m, long long node) {
+    return (m[node >> 6] >> (node & 63)) & 1ull;
+}
+
+// ---------------------------------------------------------------------------
+// reductions: wave64 shuffle tree, LDS across waves, per-block partial, and the
+// last block (ticket counter) sums the partials in fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-reduce NV values; returns true in thread 0 of the last block, which then
+// holds the grid totals in tot[].
+template <int NV>
+__device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsigned int* counter, double (&tot)[NV]) {
+    __shared__ double red[NV][16];
+    __shared__ bool last;
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthr = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = (nthr + 63) >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        double s = wave_sum(v[a]);
+        if (lane == 0) red[a][wid] = s;
+    }
+    __syncthreads();
+    const unsigned int nblk = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (tid == 0) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            double s = 0.;
+            for (int w = 0; w < nw; ++w) s += red[a][w];
+            partials[(size_t)a * nblk + bid] = s;
+        }
+        __threadfence();
+        unsigned int t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (t == nblk - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    // last block: sum partials in fixed order (deterministic)
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        double s = 0.;
+        for (unsigned int b = tid; b < nblk; b += nthr)
+            s += __hip_atomic_load(partials + (size_t)a * nblk + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = wave_sum(s);
+        __syncthreads();
+        if (lane == 0) red[a][wid] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.;
+            for (int w = 0; w < nw; ++w) t += red[a][w];
+            tot[a] = t;
+        }
+    }
+    if (tid == 0) *counter = 0u;
+    return tid == 0;
+}
+
+// ---------------------------------------------------------------------------
+// 3D operator apply, z-marching.
+//   MODE_OP : y = M A0 (M x) + (I-M) x        (the Dirichlet-eliminated operator)
+//   MODE_RES: y = M (F - A0 x)                (residual of the full field x)
+//   FUSEP   : x = z + beta p is formed on the fly and stored back into p
+//   DOT     : (x, y) over owned free nodes -> PcgState.pap, alpha = rz / pap
+// T is the storage and arithmetic type of vectors and element data.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE, bool FUSEP, bool DOT>
+__global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, const T* __restrict__ xin,
+                                                  const T* __restrict__ zin, T* __restrict__ pio, T* __restrict__ y,
+                                                  const T* __restrict__ F, const uint64_t* __restrict__ dmask,
+                                                  const T* __restrict__ kh, const T* __restrict__ kv,
+                                                  PcgState* __restrict__ st, double* __restrict__ partials) {
+    extern __shared__ unsigned char smem_raw[];
+    const int BX = g.n0;
+    const int PX = BX + 1;  // padded row: column BX holds zeros
+    T* xs = reinterpret_cast<T*>(smem_raw);       // [2][BY+1][PX]
+    T* oc = xs + 2 * (BY + 1) * PX;               // [4][BY][PX], corner outputs at column tx+1
+    if (FUSEP || DOT) {
+        if (st->done) return;
+    }
+    const int tid = threadIdx.x;
+    const int tx = tid % BX, ty = tid / BX;
+    const bool active = ty < BY;
+    const int jb = blockIdx.x * (BY - 1);
+    const int j = jb - 1 + ty;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int kb = blockIdx.y * KC;
+    const int ke = min(kb + KC, n2);
+    const bool nvalid = active && j >= 0 && j < n1;
+    const bool owned = nvalid && ty >= 1;
+    const bool evalid = active && tx < n0 - 1 && j >= 0 && j < n1 - 1;
+    const bool xrow = active && ty == BY - 1 && (j + 1) < n1;  // also loads node row j+1
+    const long long plane = (long long)n0 * n1;
+
+    double beta = 0.;
+    if (FUSEP) beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+
+    // per-column geometric factors, folded with 1e-6 (W/m -> W/um) and 1/48
+    T gxf = 0, gyf = 0, gzf = 0;
+    if (evalid) {
+        T hx = g.hx[tx], hy = g.hy[j];
+        const T c = T(1e-6 / 48.);
+        gxf = c * hy / hx;
+        gyf = c * hx / hy;
+        gzf = c * hx * hy;
+    }
+    // zero the padding column of both plane buffers and of the corner outputs
+    for (int r = tid; r < 2 * (BY + 1); r += blockDim.x) xs[r * PX + BX] = T(0);
+    for (int r = tid; r < 4 * BY; r += blockDim.x) oc[r * PX] = T(0);
+
+    // load own node (and the extra row) of plane k into xs[buf]; returns the
+    // unmasked own value; masked values (MODE_OP) go to LDS
+    auto load_plane = [&](int k, int buf, T& own, bool& own_dir, bool store_p) {
+        T v = T(0);
+        bool d = false;
+        if (nvalid) {
+            long long node = (long long)k * plane + (long long)j * n0 + tx;
+            if (FUSEP) {
+                v = zin[node] + T(beta) * pio[node];
+                if (store_p) pio[node] = v;
+            } else {
+                v = xin[node];
+            }
+            d = dbit(dmask, node);
+        }
+        own = v;
+        own_dir = d;
+        if (active) xs[(buf * (BY + 1) + ty) * PX + tx] = (MODE == MODE_OP && d) ? T(0) : v;
+        if (xrow) {
+            long long node = (long long)k * plane + (long long)(j + 1) * n0 + tx;
+            T w;
+            if (FUSEP) w = zin[node] + T(beta) * pio[node];
+            else w = xin[node];
+            bool dd = dbit(dmask, node);
+            xs[(buf * (BY + 1) + BY) * PX + tx] = (MODE == MODE_OP && dd) ? T(0) : w;
+        }
+    };
+    // in-plane sum/difference transform of the element column's 4 corners
+    auto plane_tf = [&](int buf, T (&t)[4]) {
+        const T* r0 = xs + (buf * (BY + 1) + ty) * PX;
+        const T* r1 = r0 + PX;
+        T v00 = r0[tx], v10 = r0[tx + 1], v01 = r1[tx], v11 = r1[tx + 1];
+        T s0 = v00 + v10, d0 = v10 - v00, s1 = v01 + v11, d1 = v11 - v01;
+        t[0] = s0 + s1;  // sigma sigma
+        t[1] = d0 + d1;  // delta sigma   (x delta)
+        t[2] = s1 - s0;  // sigma delta   (y delta)
+        t[3] = d1 - d0;  // delta delta
+    };
+
+    const int P0 = (kb > 0) ? kb - 1 : 0;
+    int buf = 0;
+    T own_cur, own_next = T(0);
+    bool dir_cur, dir_next = false;
+    __syncthreads();
+    load_plane(P0, 0, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
+    __syncthreads();
+    T xt_cur[4], xt_next[4] = {T(0), T(0), T(0), T(0)};
+    if (active) plane_tf(0, xt_cur);
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    double dot = 0.;
+
+    for (int P = P0; P < ke; ++P) {
+        const bool has_next = (P + 1) < n2;
+        T bot[4] = {T(0), T(0), T(0), T(0)}, top[4] = {T(0), T(0), T(0), T(0)};
+        if (has_next) {
+            load_plane(P + 1, buf ^ 1, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
+            T a = T(0), b = T(0), c = T(0);
+            if (evalid) {
+                long long e = (long long)P * (n0 - 1) * (n1 - 1) + (long long)j * (n0 - 1) + tx;
+                T hz = g.hz[P];
+                T kxy = kh[e], kz = kv[e];
+                a = kxy * (gxf * hz);
+                b = kxy * (gyf * hz);
+                c = kz * (gzf / hz);
+            }
+            __syncthreads();
+            if (active) {
+                plane_tf(buf ^ 1, xt_next);
+                // z stage: w[pq sigma] = T + B, w[pq delta] = T - B, scaled by the eigenvalues
+                const T ab = a + b, ac = a + c, bc = b + c, abc3 = (ab + c) * T(1. / 3.);
+                // components: 0 = ss, 1 = ds, 2 = sd, 3 = dd (x,y); z sigma / delta
+                T ws1 = (xt_next[1] + xt_cur[1]) * (T(3) * a);   // d s s
+                T ws2 = (xt_next[2] + xt_cur[2]) * (T(3) * b);   // s d s
+                T ws3 = (xt_next[3] + xt_cur[3]) * ab;           // d d s
+                T wd0 = (xt_next[0] - xt_cur[0]) * (T(3) * c);   // s s d
+                T wd1 = (xt_next[1] - xt_cur[1]) * ac;           // d s d
+                T wd2 = (xt_next[2] - xt_cur[2]) * bc;           // s d d
+                T wd3 = (xt_next[3] - xt_cur[3]) * abc3;         // d d d
+                bot[0] = -wd0;
+                top[0] = wd0;
+                bot[1] = ws1 - wd1;
+                top[1] = ws1 + wd1;
+                bot[2] = ws2 - wd2;
+                top[2] = ws2 + wd2;
+                bot[3] = ws3 - wd3;
+                top[3] = ws3 + wd3;
+            }
+        }
+        if (P >= kb) {
+            // plane P is complete: inverse in-plane transform -> 4 corner outputs
+            if (active) {
+                T t0 = acc[0] + bot[0], t1 = acc[1] + bot[1], t2 = acc[2] + bot[2], t3 = acc[3] + bot[3];
+                T as = t0 - t2, ad = t1 - t3;  // row j   (y sigma - y delta)
+                T bs = t0 + t2, bd = t1 + t3;  // row j+1
+                oc[(0 * BY + ty) * PX + tx + 1] = as - ad;  // corner (0,0)
+                oc[(1 * BY + ty) * PX + tx + 1] = as + ad;  // corner (1,0)
+                oc[(2 * BY + ty) * PX + tx + 1] = bs - bd;  // corner (0,1)
+                oc[(3 * BY + ty) * PX + tx + 1] = bs + bd;  // corner (1,1)
+            }
+            __syncthreads();
+            if (owned) {
+                T s = oc[(0 * BY + ty) * PX + tx + 1] + oc[(1 * BY + ty) * PX + tx] +
+                      oc[(2 * BY + ty - 1) * PX + tx + 1] + oc[(3 * BY + ty - 1) * PX + tx];
+                long long node = (long long)P * plane + (long long)j * n0 + tx;
+                T out;
+                if (MODE == MODE_OP) {
+                    out = dir_cur ? own_cur : s;
+                } else {
+                    out = dir_cur ? T(0) : (F[node] - s);
+                }
+                y[node] = out;
+                if (DOT && !dir_cur) dot += (double)own_cur * (double)out;
+            }
+        }
+        for (int a = 0; a < 4; ++a) {
+            acc[a] = top[a];
+            xt_cur[a] = xt_next[a];
+        }
+        own_cur = own_next;
+        dir_cur = dir_next;
+        buf ^= 1;
+    }
+    if (DOT) {
+        double v[1] = {dot}, tot[1];
+        if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
+            st->pap = tot[0];
+            if (!(tot[0] > 0.) || !isfinite(tot[0])) {
+                st->status = isfinite(tot[0]) ? -6 : -8;
+                st->done = 1;
+                st->alpha = 0.;
+            } else {
+                st->alpha = st->rz / tot[0];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D operator apply (Cartesian or cylindrical), marching along axis 1.
+// One block = the whole row (BX = n0 threads); thread tx owns element column tx
+// and node column tx.  Element matrix therm2d.cpp:216-228, x r_mid for
+// cylindrical geometry (therm2d.cpp:333,410-420; electr2d.cpp:206-231).
+// ---------------------------------------------------------------------------
+template <typename T, int MODE, bool FUSEP, bool DOT>
+__global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* __restrict__ xin,
+                                                  const T* __restrict__ zin, T* __restrict__ pio, T* __restrict__ y,
+                                                  const T* __restrict__ F, const uint64_t* __restrict__ dmask,
+                                                  const T* __restrict__ kh, const T* __restrict__ kv,
+                                                  PcgState* __restrict__ st, double* __restrict__ partials) {
+    extern __shared__ unsigned char smem_raw[];
+    const int n0 = g.n0, n1 = g.n1;
+    const int PX = n0 + 1;
+    T* xs = reinterpret_cast<T*>(smem_raw);  // [2][PX]
+    T* oc = xs + 2 * PX;                     // [2][PX], corner outputs at column tx+1
+    if (FUSEP || DOT) {
+        if (st->done) return;
+    }
+    const int tx = threadIdx.x;
+    const bool nvalid = tx < n0;
+    const bool evalid = tx < n0 - 1;
+    const int kb = blockIdx.x * KC;
+    const int ke = min(kb + KC, n1);
+    double beta = 0.;
+    if (FUSEP) beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+
+    T gxf = 0, gyf = 0;
+    if (evalid) {
+        T w = g.hx[tx];
+        T r = g.rmid ? g.rmid[tx] : T(1);
+        gxf = r / (T(12) * w);  // kx' / 12 = kx h r / (12 w)
+        gyf = r * w / T(12);    // ky' / 12 = ky w r / (12 h)
+    }
+    if (threadIdx.x < 2) {
+        xs[threadIdx.x * PX + n0] = T(0);
+        oc[threadIdx.x * PX] = T(0);
+    }
+    auto load_row = [&](int k, int buf, T& own, bool& d, bool store_p) {
+        T v = T(0);
+        d = false;
+        if (nvalid) {
+            long long node = (long long)k * n0 + tx;
+            if (FUSEP) {
+                v = zin[node] + T(beta) * pio[node];
+                if (store_p) pio[node] = v;
+            } else {
+                v = xin[node];
+            }
+            d = dbit(dmask, node);
+            xs[buf * PX + tx] = (MODE == MODE_OP && d) ? T(0) : v;
+        }
+        own = v;
+    };
+    const int P0 = (kb > 0) ? kb - 1 : 0;
+    int buf = 0;
+    T own_cur, own_next = T(0);
+    bool dir_cur, dir_next = false;
+    __syncthreads();
+    load_row(P0, 0, own_cur, dir_cur, FUSEP && P0 >= kb);
+    __syncthreads();
+    T s_cur = T(0), d_cur = T(0);
+    if (nvalid) {
+        T v0 = xs[tx], v1 = xs[tx + 1];
+        s_cur = v0 + v1;
+        d_cur = v1 - v0;
+    }
+    T acc0 = T(0), acc1 = T(0);
+    double dot = 0.;
+    for (int P = P0; P < ke; ++P) {
+        const bool has_next = (P + 1) < n1;
+        T bot0 = T(0), bot1 = T(0), top0 = T(0), top1 = T(0);
+        T s_next = T(0), d_next = T(0);
+        if (has_next) {
+            load_row(P + 1, buf ^ 1, own_next, dir_next, FUSEP && (P + 1) < ke);
+            T a = T(0), b = T(0);
+            if (evalid) {
+                long long e = (long long)P * (n0 - 1) + tx;
+                T h = g.hy[P];
+                a = kh[e] * (gxf * h);
+                b = kv[e] * (gyf / h);
+            }
+            __syncthreads();
+            if (nvalid) {
+                T v0 = xs[(buf ^ 1) * PX + tx], v1 = xs[(buf ^ 1) * PX + tx + 1];
+                s_next = v0 + v1;
+                d_next = v1 - v0;
+                // w[x p][z q]: (s,s) eigenvalue 0, (d,s) 3a, (s,d) 3b, (d,d) a+b
+                T wds = (d_next + d_cur) * (T(3) * a);
+                T wsd = (s_next - s_cur) * (T(3) * b);
+                T wdd = (d_next - d_cur) * (a + b);
+                bot0 = -wsd;
+                top0 = wsd;
+                bot1 = wds - wdd;
+                top1 = wds + wdd;
+            }
+        }
+        if (P >= kb) {
+            if (nvalid) {
+                T t0 = acc0 + bot0, t1 = acc1 + bot1;
+                oc[0 * PX + tx + 1] = t0 - t1;  // corner at node tx
+                oc[1 * PX + tx + 1] = t0 + t1;  // corner at node tx+1
+            }
+            __syncthreads();
+            if (nvalid) {
+                T s = oc[0 * PX + tx + 1] + oc[1 * PX + tx];
+                long long node = (long long)P * n0 + tx;
+                T out;
+                if (MODE == MODE_OP) out = dir_cur ? own_cur : s;
+                else out = dir_cur ? T(0) : (F[node] - s);
+                y[node] = out;
+                if (DOT && !dir_cur) dot += (double)own_cur * (double)out;
+            }
+        }
+        acc0 = top0;
+        acc1 = top1;
+        s_cur = s_next;
+        d_cur = d_next;
+        own_cur = own_next;
+        dir_cur = dir_next;
+        buf ^= 1;
+    }
+    if (DOT) {
+        double v[1] = {dot}, tot[1];
+        if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
+            st->pap = tot[0];
+            if (!(tot[0] > 0.) || !isfinite(tot[0])) {
+                st->status = isfinite(tot[0]) ? -6 : -8;
+                st->done = 1;
+                st->alpha = 0.;
+            } else {
+                st->alpha = st->rz / tot[0];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector step: x += alpha p; r -= alpha q; z = Dinv r; (r,z), (r,r).
+// Grid-stride over all nodes; vectors of type T, reductions in FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ x, const T* __restrict__ p,
+                                                   T* __restrict__ r, const T* __restrict__ q,
+                                                   const T* __restrict__ dinv, T* __restrict__ z,
+                                                   PcgState* __restrict__ st, double* __restrict__ partials) {
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    double rz = 0., rr = 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        T pi = p[i], qi = q[i];
+        T xi = x[i] + alpha * pi;
+        T ri = r[i] - alpha * qi;
+        T zi = dinv[i] * ri;
+        x[i] = xi;
+        r[i] = ri;
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+        st->rz_prev = st->rz;
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->iters += 1;
+        if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+            st->status = -8;
+            st->done = 1;
+        } else if (tot[1] <= st->stop2) {
+            st->done = 1;
+        } else if (!(tot[0] > 0.)) {
+            st->status = -7;
+            st->done = 1;
+        } else if (st->iters >= st->maxit) {
+            st->status = 1;
+            st->done = 1;
+        }
+    }
+}
+
+// r = given residual; z = Dinv r; (r,z), (r,r) -> state (start of a CG run)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_start(long long n, const T* __restrict__ r, const T* __restrict__ dinv,
+                                                  T* __restrict__ z, PcgState* __restrict__ st,
+                                                  double* __restrict__ partials) {
+    double rz = 0., rr = 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        T ri = r[i];
+        T zi = dinv[i] * ri;
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+        st->rz = tot[0];
+        st->rz_prev = tot[0];
+        st->rr = tot[1];
+        st->iters = 0;
+        st->status = 0;
+        st->done = (tot[1] <= st->stop2) ? 1 : 0;
+        if (!isfinite(tot[1])) {
+            st->status = -8;
+            st->done = 1;
+        }
+    }
+}
+
+// sum of squares of a vector -> st->bb
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm2(long long n, const T* __restrict__ v_, PcgState* __restrict__ st,
+                                               double* __restrict__ partials, int slot) {
+    double s = 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double v = (double)v_[i];
+        s += v * v;
+    }
+    double v[1] = {s}, tot[1];
+    if (grid_reduce<1>(v, partials, &st->counter[2], tot)) {
+        if (slot == 0) st->bb = tot[0];
+        else st->rr = tot[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// setup kernels (node-centric gathers over the elements touching a node)
+// ---------------------------------------------------------------------------
+
+// Dinv = 1 / diag(A) (1 at Dirichlet nodes); 3D diag = sum_e (kx'+ky'+kz')/9.
+template <typename T>
+__global__ void k_diag3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                         const uint64_t* __restrict__ dmask, T* __restrict__ dinv) {
+    const long long N = (long long)g.n0 * g.n1 * g.n2;
+    for (long long node = blockIdx.x * (long long)blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        int i = node % g.n0;
+        int j = (node / g.n0) % g.n1;
+        int k = node / ((long long)g.n0 * g.n1);
+        double d = 0.;
+        for (int c = 0; c < 8; ++c) {
+            int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1), ek = k - 1 + ((c >> 2) & 1);
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= g.n0 - 1 || ej >= g.n1 - 1 || ek >= g.n2 - 1) continue;
+            long long e = ((long long)ek * (g.n1 - 1) + ej) * (g.n0 - 1) + ei;
+            double dx = g.hx[ei], dy = g.hy[ej], dz = g.hz[ek];
+            double kxy = kh[e] * 1e-6, kz = kv[e] * 1e-6;
+            d += (kxy * dy * dz / dx + kxy * dx * dz / dy + kz * dx * dy / dz) / 9.;
+        }
+        dinv[node] = dbit(dmask, node) ? T(1) : T(1. / d);
+    }
+}
+
+template <typename T>
+__global__ void k_diag2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                         const uint64_t* __restrict__ dmask, T* __restrict__ dinv) {
+    const long long N = (long long)g.n0 * g.n1;
+    for (long long node = blockIdx.x * (long long)blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        int i = node % g.n0;
+        int j = node / g.n0;
+        double d = 0.;
+        for (int c = 0; c < 4; ++c) {
+            int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1);
+            if (ei < 0 || ej < 0 || ei >= g.n0 - 1 || ej >= g.n1 - 1) continue;
+            long long e = (long long)ej * (g.n0 - 1) + ei;
+            double w = g.hx[ei], h = g.hy[ej];
+            double r = g.rmid ? g.rmid[ei] : 1.;
+            d += r * (kh[e] * h / w + kv[e] * w / h) / 3.;
+        }
+        dinv[node] = dbit(dmask, node) ? T(1) : T(1. / d);
+    }
+}
+
+// Node load vector F: 3D sum_e 0.125e-18 dx dy dz q_e (therm3d.cpp:223)
+__global__ void k_load3d(Axes3<double> g, const double* __restrict__ q, double* __restrict__ F) {
+    const long long N = (long long)g.n0 * g.n1 * g.n2;
+    for (long long node = blockIdx.x * (long long)blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        int i = node % g.n0;
+        int j = (node / g.n0) % g.n1;
+        int k = node / ((long long)g.n0 * g.n1);
+        double f = 0.;
+        for (int c = 0; c < 8; ++c) {
+            int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1), ek = k - 1 + ((c >> 2) & 1);
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= g.n0 - 1 || ej >= g.n1 - 1 || ek >= g.n2 - 1) continue;
+            long long e = ((long long)ek * (g.n1 - 1) + ej) * (g.n0 - 1) + ei;
+            f += 0.125e-18 * g.hx[ei] * g.hy[ej] * g.hz[ek] * q[e];
+        }
+        F[node] = f;
+    }
+}
+
+// 2D: sum_e 0.25e-12 (r) w h q_e (therm2d.cpp:220, :346)
+__global__ void k_load2d(Axes2<double> g, const double* __restrict__ q, double* __restrict__ F) {
+    const long long N = (long long)g.n0 * g.n1;
+    for (long long node = blockIdx.x * (long long)blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        int i = node % g.n0;
+        int j = node / g.n0;
+        double f = 0.;
+        for (int c = 0; c < 4; ++c) {
+            int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1);
+            if (ei < 0 || ej < 0 || ei >= g.n0 - 1 || ej >= g.n1 - 1) continue;
+            long long e = (long long)ej * (g.n0 - 1) + ei;
+            double r = g.rmid ? g.rmid[ei] : 1.;
+            f += 0.25e-12 * r * g.hx[ei] * g.hy[ej] * q[e];
+        }
+        F[node] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small vector kernels
+// ---------------------------------------------------------------------------
+template <typename TO, typename TI>
+__global__ void k_convert(long long n, const TI* __restrict__ a, TO* __restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = TO(a[i]);
+}
+
+// x[node] = val for the Dirichlet list; optional second target
+__global__ void k_scatter(long long m, const long long* __restrict__ nodes, const double* __restrict__ vals,
+                          double* __restrict__ x) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+        x[nodes[i]] = vals[i];
+}
+
+// x += s * (double) d
+template <typename T>
+__global__ void k_add_correction(long long n, double* __restrict__ x, const T* __restrict__ d, double s) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] += s * (double)d[i];
+}
+
+// b32 = (float)(r * s), zero initial guess handled by caller
+__global__ void k_scale_to_f32(long long n, const double* __restrict__ r, double s, float* __restrict__ r32) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        r32[i] = (float)(r[i] * s);
+}
+
+template <typename T>
+__global__ void k_fill(long long n, T* __restrict__ a, T v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+}  // namespace pf

@@ -1,0 +1,693 @@
+// C ABI of the gfx950 FEM solve path (declarations and reference citations in
+// include/plask_fem.h).  Owns device buffers, builds the eliminated system
+// matrix-free and drives the Jacobi-preconditioned CG: one fused
+// apply + (p,Ap) kernel and one vector-update + (r,z) kernel per iteration,
+// scalars kept on the device, chunks of iterations replayed from a hipGraph,
+// and the host reading back a 64-byte state once per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plask_fem.h"
+#include "fem_kernels.hpp"
+
+using namespace pf;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int dim = 0, geom = 0, equation = PF_THERMAL;
+    int64_t n[3] = {1, 1, 1};
+    int64_t N = 0, E = 0;
+    std::vector<double> ax[3];
+
+    // geometry (FP64 and FP32 copies)
+    double *hx = nullptr, *hy = nullptr, *hz = nullptr, *rmid = nullptr;
+    float *hx32 = nullptr, *hy32 = nullptr, *hz32 = nullptr, *rmid32 = nullptr;
+    // element data
+    double *kh = nullptr, *kv = nullptr, *q = nullptr;
+    float *kh32 = nullptr, *kv32 = nullptr;
+    bool have_coeffs = false, have_source = false;
+    // Dirichlet
+    uint64_t* dmask = nullptr;
+    long long* dnodes = nullptr;
+    double* dvals = nullptr;
+    int64_t nd = 0;
+    double dvals_sq = 0.;
+    // vectors
+    double *x = nullptr, *r = nullptr, *p = nullptr, *qv = nullptr, *z = nullptr, *dinv = nullptr, *F = nullptr,
+           *tmp = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *dinv32 = nullptr;
+    PcgState* st = nullptr;
+    PcgState* hst = nullptr;  // pinned host mirror
+    double* partials = nullptr;
+    size_t npartials = 0;
+    bool dirty = true;  // F, Dinv, ||B|| need recomputing
+    double bnorm = 0.;
+    bool have_solution = false;
+
+    // launch configuration
+    int BY = 8, KC = 8, KC2 = 16;
+    int upd_blocks = 1024;
+    // cached graphs of one chunk of iterations
+    hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+    int graph_chunk = 0;
+
+    // profiling
+    bool profile = false;
+    double kt[2] = {0., 0.};
+    int64_t kc[2] = {0, 0};
+
+    std::vector<void*> allocs;
+};
+
+#define HIPCK(call)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                            \
+            return PF_EDEVICE;                                                                       \
+        }                                                                                            \
+    } while (0)
+
+static int fail(pf_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+template <typename T> static int dalloc(pf_ctx* ctx, T** ptr, size_t count) {
+    if (*ptr) {
+        hipFree(*ptr);
+        auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), (void*)*ptr);
+        if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+        *ptr = nullptr;
+    }
+    if (count == 0) return PF_OK;
+    HIPCK(hipMalloc((void**)ptr, sizeof(T) * count));
+    ctx->allocs.push_back((void*)*ptr);
+    return PF_OK;
+}
+
+static void drop_graphs(pf_ctx* ctx) {
+    if (ctx->graph64) hipGraphExecDestroy(ctx->graph64);
+    if (ctx->graph32) hipGraphExecDestroy(ctx->graph32);
+    ctx->graph64 = ctx->graph32 = nullptr;
+}
+
+static inline int grid1(long long n, int bs = 256, int maxb = 2048) {
+    long long b = (n + bs - 1) / bs;
+    return (int)std::max(1LL, std::min<long long>(b, maxb));
+}
+
+extern "C" {
+
+int pf_ctx_create(pf_ctx** out, int device, int flags) {
+    if (!out) return PF_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PF_EDEVICE;
+    if (device < 0 || device >= ndev) return PF_EINVAL;
+    pf_ctx* ctx = new pf_ctx();
+    ctx->device = device;
+    ctx->profile = (flags & 1) != 0;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PF_EDEVICE;
+    }
+    if (hipMalloc((void**)&ctx->st, sizeof(PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->hst, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return PF_EDEVICE;
+    }
+    hipMemset(ctx->st, 0, sizeof(PcgState));
+    *out = ctx;
+    return PF_OK;
+}
+
+void pf_ctx_destroy(pf_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
+    for (void* p : ctx->allocs) hipFree(p);
+    if (ctx->st) hipFree(ctx->st);
+    if (ctx->hst) hipHostFree(ctx->hst);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* pf_last_error(const pf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t pf_num_nodes(const pf_ctx* ctx) { return ctx ? ctx->N : 0; }
+int64_t pf_num_elements(const pf_ctx* ctx) { return ctx ? ctx->E : 0; }
+void* pf_solution_device(pf_ctx* ctx) { return ctx ? (void*)ctx->x : nullptr; }
+
+int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, const double* ax1, int64_t n1,
+                const double* ax2, int64_t n2) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    if (dim != 2 && dim != 3) return fail(ctx, PF_EINVAL, "dim must be 2 or 3");
+    if (geom == PF_GEOM_CYLINDRICAL && dim != 2) return fail(ctx, PF_EINVAL, "cylindrical geometry is 2D only");
+    if (!ax0 || !ax1 || n0 < 2 || n1 < 2 || (dim == 3 && (!ax2 || n2 < 2)))
+        return fail(ctx, PF_EINVAL, "each axis needs at least 2 nodes");
+    if (n0 > 1024) return fail(ctx, PF_ESHAPE, "axis 0 longer than 1024 nodes is not supported by the row kernels");
+    if (dim == 3 && n0 > 512) return fail(ctx, PF_ESHAPE, "3D: axis 0 longer than 512 nodes is not supported");
+    const double* axs[3] = {ax0, ax1, ax2};
+    int64_t ns[3] = {n0, n1, dim == 3 ? n2 : 1};
+    for (int a = 0; a < dim; ++a) {
+        for (int64_t i = 0; i + 1 < ns[a]; ++i)
+            if (!(axs[a][i + 1] > axs[a][i])) return fail(ctx, PF_EINVAL, "axis coordinates must increase strictly");
+    }
+    if (geom == PF_GEOM_CYLINDRICAL && ax0[0] < 0.) return fail(ctx, PF_EINVAL, "radius must be non-negative");
+    drop_graphs(ctx);
+    ctx->dim = dim;
+    ctx->geom = geom;
+    for (int a = 0; a < 3; ++a) {
+        ctx->n[a] = ns[a];
+        ctx->ax[a].assign(axs[a] ? axs[a] : nullptr, axs[a] ? axs[a] + ns[a] : nullptr);
+    }
+    ctx->N = ns[0] * ns[1] * ns[2];
+    ctx->E = (ns[0] - 1) * (ns[1] - 1) * (dim == 3 ? ns[2] - 1 : 1);
+    // element sizes
+    double** hd[3] = {&ctx->hx, &ctx->hy, &ctx->hz};
+    float** hf[3] = {&ctx->hx32, &ctx->hy32, &ctx->hz32};
+    for (int a = 0; a < dim; ++a) {
+        std::vector<double> h(ns[a] - 1);
+        std::vector<float> hf32(ns[a] - 1);
+        for (int64_t i = 0; i + 1 < ns[a]; ++i) {
+            h[i] = axs[a][i + 1] - axs[a][i];
+            hf32[i] = (float)h[i];
+        }
+        int rc;
+        if ((rc = dalloc(ctx, hd[a], h.size()))) return rc;
+        if ((rc = dalloc(ctx, hf[a], h.size()))) return rc;
+        HIPCK(hipMemcpy(*hd[a], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(*hf[a], hf32.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    }
+    if (geom == PF_GEOM_CYLINDRICAL) {
+        std::vector<double> rm(n0 - 1);
+        std::vector<float> rm32(n0 - 1);
+        for (int64_t i = 0; i + 1 < n0; ++i) {
+            rm[i] = 0.5 * (ax0[i] + ax0[i + 1]);  // midpoint.rad_r() (therm2d.cpp:333)
+            rm32[i] = (float)rm[i];
+        }
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->rmid, rm.size()))) return rc;
+        if ((rc = dalloc(ctx, &ctx->rmid32, rm.size()))) return rc;
+        HIPCK(hipMemcpy(ctx->rmid, rm.data(), sizeof(double) * rm.size(), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->rmid32, rm32.data(), sizeof(float) * rm.size(), hipMemcpyHostToDevice));
+    } else {
+        dalloc(ctx, &ctx->rmid, 0);
+        dalloc(ctx, &ctx->rmid32, 0);
+    }
+    // vectors
+    const size_t N = ctx->N;
+    int rc;
+    double** v64[] = {&ctx->x, &ctx->r, &ctx->p, &ctx->qv, &ctx->z, &ctx->dinv, &ctx->F, &ctx->tmp};
+    for (auto pp : v64)
+        if ((rc = dalloc(ctx, pp, N))) return rc;
+    float** v32[] = {&ctx->x32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->z32, &ctx->dinv32};
+    for (auto pp : v32)
+        if ((rc = dalloc(ctx, pp, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->kh, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->kv, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->q, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->kh32, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->kv32, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->dmask, (N + 63) / 64))) return rc;
+    HIPCK(hipMemset(ctx->dmask, 0, sizeof(uint64_t) * ((N + 63) / 64)));
+    HIPCK(hipMemset(ctx->q, 0, sizeof(double) * ctx->E));
+    HIPCK(hipMemset(ctx->x, 0, sizeof(double) * N));
+    ctx->nd = 0;
+    ctx->dvals_sq = 0.;
+    ctx->have_coeffs = false;
+    ctx->have_source = false;
+    ctx->have_solution = false;
+    // launch shapes
+    if (dim == 3) {
+        ctx->BY = (int)std::max<int64_t>(2, std::min<int64_t>(16, 1024 / n0));
+        int ytiles = (int)((n1 + ctx->BY - 2) / (ctx->BY - 1));
+        int kc = (int)std::max<int64_t>(4, std::min<int64_t>(32, (n2 * ytiles) / 1024));
+        ctx->KC = kc;
+    } else {
+        ctx->KC2 = (int)std::max<int64_t>(4, std::min<int64_t>(64, n1 / 64));
+    }
+    ctx->upd_blocks = grid1(N, 256, 1024);
+    size_t napply = 0;
+    if (dim == 3) {
+        int ytiles = (int)((n1 + ctx->BY - 2) / (ctx->BY - 1));
+        napply = (size_t)ytiles * ((n2 + ctx->KC - 1) / ctx->KC);
+    } else {
+        napply = (size_t)((n1 + ctx->KC2 - 1) / ctx->KC2);
+    }
+    ctx->npartials = 2 * std::max<size_t>(napply, 2048);
+    if ((rc = dalloc(ctx, &ctx->partials, ctx->npartials))) return rc;
+    ctx->dirty = true;
+    return PF_OK;
+}
+
+int pf_coeffs_set(pf_ctx* ctx, int equation, const double* k_inplane, const double* k_vert, int64_t nelem) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (nelem != ctx->E || !k_inplane || !k_vert) return fail(ctx, PF_EINVAL, "coefficient arrays must have one value per element");
+    HIPCK(hipSetDevice(ctx->device));
+    for (int64_t e = 0; e < nelem; ++e)
+        if (!std::isfinite(k_inplane[e]) || !std::isfinite(k_vert[e]) || k_vert[e] <= 0. || k_inplane[e] < 0.)
+            return fail(ctx, PF_EINVAL, "conductivities must be finite, in-plane >= 0, vertical > 0 (element " +
+                                            std::to_string(e) + ")");
+    ctx->equation = equation;
+    HIPCK(hipMemcpyAsync(ctx->kh, k_inplane, sizeof(double) * nelem, hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->kv, k_vert, sizeof(double) * nelem, hipMemcpyHostToDevice, ctx->stream));
+    int b = grid1(nelem);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(b), dim3(256), 0, ctx->stream, (long long)nelem, ctx->kh, ctx->kh32);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(b), dim3(256), 0, ctx->stream, (long long)nelem, ctx->kv, ctx->kv32);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->have_coeffs = true;
+    ctx->dirty = true;
+    return PF_OK;
+}
+
+int pf_source_set(pf_ctx* ctx, const double* q_elem, int64_t nelem) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    if (!q_elem) {
+        HIPCK(hipMemsetAsync(ctx->q, 0, sizeof(double) * ctx->E, ctx->stream));
+    } else {
+        if (nelem != ctx->E) return fail(ctx, PF_EINVAL, "heat array must have one value per element");
+        HIPCK(hipMemcpyAsync(ctx->q, q_elem, sizeof(double) * nelem, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->have_source = q_elem != nullptr;
+    ctx->dirty = true;
+    return PF_OK;
+}
+
+int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (n < 0 || (n > 0 && (!nodes || !vals))) return fail(ctx, PF_EINVAL, "bad Dirichlet list");
+    HIPCK(hipSetDevice(ctx->device));
+    // last value wins for repeated nodes (setBC applied in list order, matrix.hpp:111-118)
+    std::vector<uint64_t> mask((ctx->N + 63) / 64, 0ull);
+    std::vector<int64_t> pos(0);
+    std::vector<long long> un;
+    std::vector<double> uv;
+    un.reserve(n);
+    uv.reserve(n);
+    std::vector<int64_t> where;  // node -> index in un (only for repeated detection)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t nd = nodes[i];
+        if (nd < 0 || nd >= ctx->N) return fail(ctx, PF_EINVAL, "Dirichlet node out of range");
+        if (!std::isfinite(vals[i])) return fail(ctx, PF_EINVAL, "Dirichlet value not finite");
+        uint64_t bit = 1ull << (nd & 63);
+        if (mask[nd >> 6] & bit) {
+            for (size_t k = 0; k < un.size(); ++k)
+                if (un[k] == nd) uv[k] = vals[i];
+            continue;
+        }
+        mask[nd >> 6] |= bit;
+        un.push_back(nd);
+        uv.push_back(vals[i]);
+    }
+    ctx->nd = (int64_t)un.size();
+    double sq = 0.;
+    for (double v : uv) sq += v * v;
+    ctx->dvals_sq = sq;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->dnodes, std::max<size_t>(1, un.size())))) return rc;
+    if ((rc = dalloc(ctx, &ctx->dvals, std::max<size_t>(1, un.size())))) return rc;
+    if (!un.empty()) {
+        HIPCK(hipMemcpy(ctx->dnodes, un.data(), sizeof(long long) * un.size(), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->dvals, uv.data(), sizeof(double) * uv.size(), hipMemcpyHostToDevice));
+    }
+    HIPCK(hipMemcpy(ctx->dmask, mask.data(), sizeof(uint64_t) * mask.size(), hipMemcpyHostToDevice));
+    ctx->dirty = true;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// kernel launch helpers
+// ---------------------------------------------------------------------------
+template <typename T> static Axes3<T> axes3(pf_ctx* ctx);
+template <> Axes3<double> axes3<double>(pf_ctx* ctx) {
+    return Axes3<double>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx, ctx->hy, ctx->hz};
+}
+template <> Axes3<float> axes3<float>(pf_ctx* ctx) {
+    return Axes3<float>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx32, ctx->hy32, ctx->hz32};
+}
+template <typename T> static Axes2<T> axes2(pf_ctx* ctx);
+template <> Axes2<double> axes2<double>(pf_ctx* ctx) {
+    return Axes2<double>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx, ctx->hy, ctx->rmid};
+}
+template <> Axes2<float> axes2<float>(pf_ctx* ctx) {
+    return Axes2<float>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx32, ctx->hy32, ctx->rmid32};
+}
+
+template <typename T, int MODE, bool FUSEP, bool DOT>
+static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, T* pio, T* y, const T* F, const T* kh, const T* kv) {
+    if (ctx->dim == 3) {
+        const int BX = (int)ctx->n[0], BY = ctx->BY;
+        int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
+        int zt = (int)((ctx->n[2] + ctx->KC - 1) / ctx->KC);
+        size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
+        hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
+                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pio, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+    } else {
+        const int BX = (int)ctx->n[0];
+        int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
+        size_t shm = sizeof(T) * (size_t)(4 * (BX + 1));
+        hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, axes2<T>(ctx),
+                           ctx->KC2, xin, zin, pio, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+    }
+}
+
+// F, Dinv (FP64/FP32) and ||B||: B = M (F - A0 xD) + xD, xD = Dirichlet values
+static int prepare(pf_ctx* ctx) {
+    if (!ctx->dirty) return PF_OK;
+    if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
+    const long long N = ctx->N;
+    const int gb = grid1(N, 256, 4096);
+    if (ctx->dim == 3) {
+        hipLaunchKernelGGL(k_load3d, dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->q, ctx->F);
+        hipLaunchKernelGGL((k_diag3d<double>), dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
+                           ctx->dmask, ctx->dinv);
+        hipLaunchKernelGGL((k_diag3d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
+                           ctx->dmask, ctx->dinv32);
+    } else {
+        hipLaunchKernelGGL(k_load2d, dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->q, ctx->F);
+        hipLaunchKernelGGL((k_diag2d<double>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
+                           ctx->dmask, ctx->dinv);
+        hipLaunchKernelGGL((k_diag2d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
+                           ctx->dmask, ctx->dinv32);
+    }
+    if (ctx->equation == PF_ELECTRICAL || !ctx->have_source) {
+        // electrical: no load vector (electr2d.cpp:268-317); thermal without source: zero
+        hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->F, 0.);
+    }
+    // xD into tmp, then r = M (F - A0 xD)
+    hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->tmp, 0.);
+    if (ctx->nd)
+        hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
+                           ctx->dvals, ctx->tmp);
+    launch_apply<double, MODE_RES, false, false>(ctx, ctx->tmp, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
+    hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, ctx->r, ctx->st,
+                       ctx->partials, 0);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->bnorm = std::sqrt(ctx->hst->bb + ctx->dvals_sq);
+    ctx->dirty = false;
+    return PF_OK;
+}
+
+// One CG run on (x, r already holding b - A x).  T selects FP64 or FP32 vectors.
+template <typename T>
+static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* q, T* z, const T* dinv, const T* kh, const T* kv, double stop2,
+                  int maxit, int check_every, hipGraphExec_t* gcache, int* iters_out, int* status_out) {
+    const long long N = ctx->N;
+    PcgState init{};
+    init.stop2 = stop2;
+    init.maxit = maxit;
+    *ctx->hst = init;  // pinned staging: the host does not touch it again before the next sync
+    HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
+                       ctx->partials);
+    HIPCK(hipGetLastError());
+    auto one_iter = [&]() {
+        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, q, nullptr, kh, kv);
+        hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, p, r, q, dinv, z,
+                           ctx->st, ctx->partials);
+    };
+    const int chunk = check_every > 0 ? check_every : 32;
+    if (ctx->profile) {
+        // event-timed launches, no graph
+        hipEvent_t ev[3];
+        for (auto& e : ev) HIPCK(hipEventCreate(&e));
+        for (;;) {
+            for (int c = 0; c < chunk; ++c) {
+                hipEventRecord(ev[0], ctx->stream);
+                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, q, nullptr, kh, kv);
+                hipEventRecord(ev[1], ctx->stream);
+                hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, p, r, q,
+                                   dinv, z, ctx->st, ctx->partials);
+                hipEventRecord(ev[2], ctx->stream);
+                HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+                HIPCK(hipStreamSynchronize(ctx->stream));
+                float t1 = 0.f, t2 = 0.f;
+                hipEventElapsedTime(&t1, ev[0], ev[1]);
+                hipEventElapsedTime(&t2, ev[1], ev[2]);
+                ctx->kt[0] += 1e-3 * t1;
+                ctx->kt[1] += 1e-3 * t2;
+                ctx->kc[0] += 1;
+                ctx->kc[1] += 1;
+                if (ctx->hst->done) break;
+            }
+            if (ctx->hst->done) break;
+        }
+        for (auto& e : ev) hipEventDestroy(e);
+    } else {
+        if (!*gcache || ctx->graph_chunk != chunk) {
+            if (*gcache) hipGraphExecDestroy(*gcache);
+            *gcache = nullptr;
+            hipGraph_t graph;
+            HIPCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            for (int c = 0; c < chunk; ++c) one_iter();
+            HIPCK(hipStreamEndCapture(ctx->stream, &graph));
+            HIPCK(hipGraphInstantiate(gcache, graph, nullptr, nullptr, 0));
+            hipGraphDestroy(graph);
+            ctx->graph_chunk = chunk;
+        }
+        for (;;) {
+            HIPCK(hipGraphLaunch(*gcache, ctx->stream));
+            HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCK(hipStreamSynchronize(ctx->stream));
+            if (ctx->hst->done) break;
+        }
+    }
+    *iters_out = ctx->hst->iters;
+    *status_out = ctx->hst->status;
+    return PF_OK;
+}
+
+// ||M(F - A0 x)||^2 + 0 (Dirichlet rows of x hold the values exactly) into r; returns sqrt
+static int true_residual(pf_ctx* ctx, double* rn) {
+    launch_apply<double, MODE_RES, false, false>(ctx, ctx->x, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
+    hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, (long long)ctx->N, ctx->r,
+                       ctx->st, ctx->partials, 1);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    *rn = std::sqrt(ctx->hst->rr);
+    return PF_OK;
+}
+
+extern "C" {
+
+int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_stats* stats) {
+    if (!ctx || !opts) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = prepare(ctx))) return rc;
+    const long long N = ctx->N;
+    const int gb = grid1(N, 256, 4096);
+    auto t0 = std::chrono::steady_clock::now();
+    ctx->kt[0] = ctx->kt[1] = 0.;
+    ctx->kc[0] = ctx->kc[1] = 0;
+    // start vector with the Dirichlet values
+    if (opts->x0 == PF_X0_GIVEN) {
+        if (!x_inout) return fail(ctx, PF_EINVAL, "x_inout required with PF_X0_GIVEN");
+        HIPCK(hipMemcpyAsync(ctx->x, x_inout, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    } else if (opts->x0 == PF_X0_ZERO || !ctx->have_solution) {
+        hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, 0.);
+    }
+    if (ctx->nd)
+        hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
+                           ctx->dvals, ctx->x);
+    const double bnorm = ctx->bnorm > 0. ? ctx->bnorm : 1.;
+    const double rtol = opts->rtol > 0. ? opts->rtol : 1e-8;
+    const int maxit = opts->maxit > 0 ? opts->maxit : 100000;
+    int total_iters = 0, outer = 0, status = 0;
+    double rn = 0.;
+    if ((rc = true_residual(ctx, &rn))) return rc;
+    if (opts->precision == PF_FP64) {
+        // restarts only if the recursive residual drifted from the true one
+        for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
+            int it = 0, stt = 0;
+            double stop2 = (rtol * bnorm) * (rtol * bnorm);
+            if ((rc = run_cg<double>(ctx, ctx->x, ctx->r, ctx->p, ctx->qv, ctx->z, ctx->dinv, ctx->kh, ctx->kv, stop2,
+                                     maxit - total_iters, opts->check_every, &ctx->graph64, &it, &stt)))
+                return rc;
+            total_iters += it;
+            status = stt;
+            if ((rc = true_residual(ctx, &rn))) return rc;
+            if (stt != 0) break;
+            outer++;
+        }
+    } else if (opts->precision == PF_FP32_REFINE) {
+        const double inner = opts->inner_rtol > 0. ? opts->inner_rtol : 1e-4;
+        const int max_outer = opts->max_outer > 0 ? opts->max_outer : 50;
+        while (rn > rtol * bnorm && outer < max_outer && total_iters < maxit) {
+            // r holds M(F - A0 x) in FP64; scale to unit norm and solve A d = r in FP32
+            double s = 1. / rn;
+            hipLaunchKernelGGL(k_scale_to_f32, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->r, s, ctx->r32);
+            hipLaunchKernelGGL((k_fill<float>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x32, 0.f);
+            int it = 0, stt = 0;
+            // the target for this step: reduce by `inner`, but not beyond what the outer tolerance needs
+            double goal = std::max(inner, 0.5 * rtol * bnorm / rn);
+            if ((rc = run_cg<float>(ctx, ctx->x32, ctx->r32, ctx->p32, ctx->q32, ctx->z32, ctx->dinv32, ctx->kh32,
+                                    ctx->kv32, goal * goal, maxit - total_iters, opts->check_every, &ctx->graph32, &it,
+                                    &stt)))
+                return rc;
+            total_iters += it;
+            hipLaunchKernelGGL((k_add_correction<float>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, ctx->x32, rn);
+            HIPCK(hipGetLastError());
+            outer++;
+            if ((rc = true_residual(ctx, &rn))) return rc;
+            if (stt < 0) {
+                status = stt;
+                break;
+            }
+        }
+    } else {
+        return fail(ctx, PF_EINVAL, "unknown precision");
+    }
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    ctx->have_solution = true;
+    if (x_inout && opts->x0 != PF_X0_DEVICE) {
+        HIPCK(hipMemcpy(x_inout, ctx->x, sizeof(double) * N, hipMemcpyDeviceToHost));
+    } else if (x_inout) {
+        HIPCK(hipMemcpy(x_inout, ctx->x, sizeof(double) * N, hipMemcpyDeviceToHost));
+    }
+    int ret = PF_OK;
+    if (status == -6 || status == -7) ret = PF_EBREAK;
+    else if (status == -8) ret = PF_ENONFINITE;
+    else if (rn > rtol * bnorm) ret = PF_ENOCONV;
+    if (stats) {
+        stats->iters = total_iters;
+        stats->outer = outer;
+        stats->rel_resid = rn / bnorm;
+        stats->bnorm = bnorm;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->status = ret;
+    }
+    if (ret == PF_EBREAK) ctx->err = "CG breakdown: stiffness matrix is not positive definite";
+    if (ret == PF_ENONFINITE) ctx->err = "non-finite value in CG iteration";
+    if (ret == PF_ENOCONV) ctx->err = "did not converge to the requested tolerance";
+    return ret;
+}
+
+int pf_get_solution(pf_ctx* ctx, double* x_out) {
+    if (!ctx || !x_out) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no solution on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipMemcpy(x_out, ctx->x, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+int pf_apply(pf_ctx* ctx, int precision, const void* x_dev, void* y_dev) {
+    if (!ctx || !x_dev || !y_dev) return PF_EINVAL;
+    if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    if (precision == PF_FP64)
+        launch_apply<double, MODE_OP, false, false>(ctx, (const double*)x_dev, nullptr, nullptr, (double*)y_dev, nullptr,
+                                                    ctx->kh, ctx->kv);
+    else
+        launch_apply<float, MODE_OP, false, false>(ctx, (const float*)x_dev, nullptr, nullptr, (float*)y_dev, nullptr,
+                                                   ctx->kh32, ctx->kv32);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+}
+
+int pf_apply_host(pf_ctx* ctx, const double* x, double* y) {
+    if (!ctx || !x || !y) return PF_EINVAL;
+    if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipMemcpy(ctx->tmp, x, sizeof(double) * ctx->N, hipMemcpyHostToDevice));
+    launch_apply<double, MODE_OP, false, false>(ctx, ctx->tmp, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipMemcpy(y, ctx->qv, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+int pf_rhs_host(pf_ctx* ctx, double* b_out) {
+    if (!ctx || !b_out) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    ctx->dirty = true;
+    if ((rc = prepare(ctx))) return rc;  // leaves M(F - A0 xD) in r
+    std::vector<double> h(ctx->N);
+    HIPCK(hipMemcpy(b_out, ctx->r, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    if (ctx->nd) {
+        std::vector<long long> nodes(ctx->nd);
+        std::vector<double> vals(ctx->nd);
+        HIPCK(hipMemcpy(nodes.data(), ctx->dnodes, sizeof(long long) * ctx->nd, hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(vals.data(), ctx->dvals, sizeof(double) * ctx->nd, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < ctx->nd; ++i) b_out[nodes[i]] = vals[i];
+    }
+    return PF_OK;
+}
+
+int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
+    if (!ctx) return PF_EINVAL;
+    for (int i = 0; i < n && i < 2; ++i) {
+        if (seconds) seconds[i] = ctx->kc[i] ? ctx->kt[i] / ctx->kc[i] : 0.;
+        if (counts) counts[i] = ctx->kc[i];
+    }
+    return PF_OK;
+}
+
+int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds) {
+    if (!ctx || reps <= 0 || !seconds) return PF_EINVAL;
+    if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    hipEvent_t a, b;
+    HIPCK(hipEventCreate(&a));
+    HIPCK(hipEventCreate(&b));
+    auto go = [&]() {
+        if (precision == PF_FP64)
+            launch_apply<double, MODE_OP, false, false>(ctx, ctx->p, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
+        else
+            launch_apply<float, MODE_OP, false, false>(ctx, ctx->p32, nullptr, nullptr, ctx->q32, nullptr, ctx->kh32,
+                                                       ctx->kv32);
+    };
+    go();
+    HIPCK(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < reps; ++i) go();
+    HIPCK(hipEventRecord(b, ctx->stream));
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventSynchronize(b));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    *seconds = 1e-3 * ms / reps;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return PF_OK;
+}
+
+}  // extern "C"

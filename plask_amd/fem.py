@@ -1,0 +1,211 @@
+"""ctypes binding of the C ABI in ``include/plask_fem.h`` (``libplaskfem.so``).
+
+The library is the only compute path: if it is missing or no GPU is visible,
+``FemContext`` raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libplaskfem.so")
+
+PF_OK, PF_EINVAL, PF_ESTATE, PF_EDEVICE, PF_ESHAPE, PF_ENOCONV, PF_EBREAK, PF_ENONFINITE = 0, -1, -2, -3, -4, -5, -6, -8
+PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
+PF_THERMAL, PF_ELECTRICAL = 0, 1
+PF_FP64, PF_FP32_REFINE = 0, 1
+PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE = 0, 1, 2
+
+EXPORTED = [
+    "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
+    "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
+    "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply",
+]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("x0", C.c_int), ("maxit", C.c_int),
+                ("rtol", C.c_double), ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("rel_resid", C.c_double), ("bnorm", C.c_double),
+                ("seconds", C.c_double), ("status", C.c_int)]
+
+
+class FemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class NoConvergence(FemError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libplaskfem.so and declare the C signatures.  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libplaskfem.so not built (%s); run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    P, i64, I = C.c_void_p, C.c_int64, C.c_int
+    L.pf_ctx_create.argtypes = [C.POINTER(P), I, I]
+    L.pf_ctx_destroy.argtypes = [P]
+    L.pf_ctx_destroy.restype = None
+    L.pf_last_error.argtypes = [P]
+    L.pf_last_error.restype = C.c_char_p
+    L.pf_grid_set.argtypes = [P, I, I, P, i64, P, i64, P, i64]
+    L.pf_coeffs_set.argtypes = [P, I, P, P, i64]
+    L.pf_source_set.argtypes = [P, P, i64]
+    L.pf_dirichlet_set.argtypes = [P, P, P, i64]
+    L.pf_solve.argtypes = [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]
+    L.pf_get_solution.argtypes = [P, P]
+    L.pf_apply.argtypes = [P, I, P, P]
+    L.pf_apply_host.argtypes = [P, P, P]
+    L.pf_rhs_host.argtypes = [P, P]
+    L.pf_solution_device.argtypes = [P]
+    L.pf_solution_device.restype = P
+    L.pf_num_nodes.argtypes = [P]
+    L.pf_num_nodes.restype = i64
+    L.pf_num_elements.argtypes = [P]
+    L.pf_num_elements.restype = i64
+    L.pf_kernel_times.argtypes = [P, P, P, I]
+    L.pf_time_apply.argtypes = [P, I, I, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class FemContext:
+    """One device context: a grid, element coefficients, sources and Dirichlet nodes.
+
+    Node order is natural (axis 0 fastest), element order likewise.
+    """
+
+    def __init__(self, device=0, profile=False):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.pf_ctx_create(C.byref(h), int(device), 1 if profile else 0)
+        if rc != PF_OK:
+            raise FemError(rc, "pf_ctx_create failed: no usable GPU device %d" % device)
+        self.h = h
+        self.dim = 0
+        self.shape = ()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != PF_OK:
+            msg = self.L.pf_last_error(self.h).decode(errors="replace")
+            cls = NoConvergence if rc == PF_ENOCONV else FemError
+            raise cls(rc, "%s: %s" % (what, msg))
+
+    # -- setup ---------------------------------------------------------------
+    def set_grid(self, axes, cylindrical=False):
+        axes = [_f64(a) for a in axes]
+        self._axes = axes
+        self.dim = len(axes)
+        self.shape = tuple(len(a) for a in axes)
+        a2 = axes[2] if self.dim == 3 else None
+        n2 = len(a2) if a2 is not None else 1
+        rc = self.L.pf_grid_set(self.h, self.dim, PF_GEOM_CYLINDRICAL if cylindrical else PF_GEOM_CARTESIAN,
+                                _p(axes[0]), len(axes[0]), _p(axes[1]), len(axes[1]), _p(a2), n2)
+        self._ck(rc, "pf_grid_set")
+
+    @property
+    def nnodes(self):
+        return int(self.L.pf_num_nodes(self.h))
+
+    @property
+    def nelems(self):
+        return int(self.L.pf_num_elements(self.h))
+
+    def set_coeffs(self, k_inplane, k_vert, equation=PF_THERMAL):
+        self._kh = _f64(np.ravel(k_inplane))
+        self._kv = _f64(np.ravel(k_vert))
+        self._ck(self.L.pf_coeffs_set(self.h, equation, _p(self._kh), _p(self._kv), len(self._kh)), "pf_coeffs_set")
+
+    def set_source(self, q):
+        if q is None:
+            self._ck(self.L.pf_source_set(self.h, None, 0), "pf_source_set")
+        else:
+            self._q = _f64(np.ravel(q))
+            self._ck(self.L.pf_source_set(self.h, _p(self._q), len(self._q)), "pf_source_set")
+
+    def set_dirichlet(self, nodes, values):
+        self._dn = np.ascontiguousarray(nodes, dtype=np.int64)
+        self._dv = _f64(np.broadcast_to(values, self._dn.shape))
+        self._ck(self.L.pf_dirichlet_set(self.h, _p(self._dn), _p(self._dv), len(self._dn)), "pf_dirichlet_set")
+
+    # -- compute -------------------------------------------------------------
+    def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, max_outer=50,
+              check_every=32, keep_on_device=False, raise_noconv=True):
+        o = SolveOpts(precision, 1, PF_X0_GIVEN if x0 is not None else PF_X0_ZERO, int(maxit), float(rtol),
+                      float(inner_rtol), int(max_outer), int(check_every))
+        st = SolveStats()
+        if x0 is not None:
+            x = _f64(np.array(x0, dtype=np.float64, copy=True))
+        else:
+            x = None if keep_on_device else np.zeros(self.nnodes)
+        rc = self.L.pf_solve(self.h, C.byref(o), _p(x), C.byref(st))
+        if rc != PF_OK and not (rc == PF_ENOCONV and not raise_noconv):
+            self._ck(rc, "pf_solve")
+        return x, st
+
+    def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32):
+        """Start from the solution already on the device (no host transfers)."""
+        o = SolveOpts(precision, 1, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every))
+        st = SolveStats()
+        rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
+        self._ck(rc, "pf_solve")
+        return st
+
+    def solution(self):
+        x = np.zeros(self.nnodes)
+        self._ck(self.L.pf_get_solution(self.h, _p(x)), "pf_get_solution")
+        return x
+
+    def apply(self, x):
+        x = _f64(x)
+        y = np.zeros_like(x)
+        self._ck(self.L.pf_apply_host(self.h, _p(x), _p(y)), "pf_apply_host")
+        return y
+
+    def rhs(self):
+        b = np.zeros(self.nnodes)
+        self._ck(self.L.pf_rhs_host(self.h, _p(b)), "pf_rhs_host")
+        return b
+
+    def kernel_times(self):
+        s = np.zeros(2)
+        c = np.zeros(2, dtype=np.int64)
+        self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), 2), "pf_kernel_times")
+        return s, c
+
+    def time_apply(self, precision=PF_FP64, reps=50):
+        t = C.c_double(0.)
+        self._ck(self.L.pf_time_apply(self.h, precision, int(reps), C.byref(t)), "pf_time_apply")
+        return t.value
