@@ -72,23 +72,33 @@ typedef struct pf_ctx pf_ctx;
 #define PF_X0_GIVEN 0 /* x_inout holds the start vector (its Dirichlet entries are overwritten) */
 #define PF_X0_ZERO 1
 #define PF_X0_DEVICE 2 /* start from the solution kept on the device from the previous pf_solve */
+#define PF_X0_UNIFORM 3 /* start from init_value everywhere (the reference's inittemp field), on the device */
+
+/* denominator of the relative residual */
+#define PF_NORM_INIT 0   /* ||B - A X_init||, X_init = init_value at free nodes + the Dirichlet values:
+                            the residual of the reference's start field (inittemp, therm3d.cpp:79;
+                            zero potential, electr2d.cpp:176), i.e. ||b|| of the correction system */
+#define PF_NORM_B 1      /* ||B|| over the free rows */
+#define PF_NORM_B_FULL 2 /* ||B|| including the Dirichlet rows (B exactly as assembled by the reference) */
 
 typedef struct {
     int precision;    /* PF_FP64 | PF_FP32_REFINE */
     int precond;      /* 1 = Jacobi (NSPCG 'jac'); others reserved */
     int x0;           /* PF_X0_* */
     int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
-    double rtol;      /* stop when ||B - A X||_2 <= rtol ||B||_2, B the eliminated load vector */
+    double rtol;      /* stop when ||B - A X||_2 <= rtol * (denominator chosen by `norm`) */
     double inner_rtol;/* PF_FP32_REFINE: inner relative reduction per refinement step */
     int max_outer;    /* PF_FP32_REFINE: max refinement steps */
     int check_every;  /* iterations between host convergence checks (0 = default 32) */
+    int norm;         /* PF_NORM_* */
+    double init_value;/* X_init for PF_NORM_INIT */
 } pf_solve_opts;
 
 typedef struct {
     int iters;          /* CG iterations (inner iterations summed for PF_FP32_REFINE) */
     int outer;          /* refinement steps */
-    double rel_resid;   /* true ||B - A X|| / ||B|| at exit, computed in FP64 */
-    double bnorm;       /* ||B|| */
+    double rel_resid;   /* true ||B - A X|| / denominator at exit, computed in FP64 */
+    double bnorm;       /* the denominator */
     double seconds;     /* wall time of the solve on the device */
     int status;         /* PF_OK or an error code */
 } pf_solve_stats;
@@ -147,6 +157,10 @@ int64_t pf_num_elements(const pf_ctx* ctx);
  * HIP events on the context stream: what[0] = operator apply + (p,Ap),
  * what[1] = vector update + (r,z); counts[] receives launch counts.  For benchmarks. */
 int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
+
+/* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP
+ * events (no graph) and accumulates per-kernel durations for pf_kernel_times. */
+int pf_set_profile(pf_ctx* ctx, int on);
 
 /* Time `reps` back-to-back FP64 operator applies y = A x on the context stream with HIP
  * events; returns the average seconds per apply.  For benchmarks. */

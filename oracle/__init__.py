@@ -53,7 +53,7 @@ def lib():
         L.orc_band_cholesky.argtypes = [i64, i64, i64, P]
         L.orc_band_cholesky.restype = i64
         L.orc_band_solve.argtypes = [i64, i64, i64, P, P]
-        L.orc_dia_pcg.argtypes = [i64, C.c_int, P, P, P, P, C.c_int, C.c_double, i64, P, P]
+        L.orc_dia_pcg.argtypes = [i64, C.c_int, P, P, P, P, C.c_int, C.c_double, i64, C.c_double, P, P]
         L.orc_dia_pcg.restype = i64
         L.orc_ic0_factor.argtypes = [i64, C.c_int, P, P, P, P]
         L.orc_ic0_factor.restype = i64
@@ -189,13 +189,16 @@ def band_cholesky_solve(A, B=None):
     return x
 
 
-def dia_pcg(A, x0, precond=1, rtol=1e-8, maxit=100000):
-    """PCG on the symmetric diagonal storage; precond 0 none, 1 Jacobi, 2 IC(0)."""
+def dia_pcg(A, x0, precond=1, rtol=1e-8, maxit=100000, bnorm=None):
+    """PCG on the symmetric diagonal storage; precond 0 none, 1 Jacobi, 2 IC(0).
+    Stops at ||B - A u|| <= rtol * bnorm; bnorm defaults to ||B - A x0|| (the start residual)."""
     u = np.array(x0, dtype=np.float64)
+    if bnorm is None:
+        bnorm = float(np.linalg.norm(A.B - A.mult(u)))
     rel = C.c_double(0.)
     it = C.c_int64(0)
     st = lib().orc_dia_pcg(A.rank, len(A.offsets), _p(A.offsets), _p(A.data), _p(A.B), _p(u), precond, rtol, maxit,
-                           C.byref(rel), C.byref(it))
+                           float(bnorm), C.byref(rel), C.byref(it))
     return u, int(st), int(it.value), float(rel.value)
 
 

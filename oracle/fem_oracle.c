@@ -358,13 +358,13 @@ static double dot(int64_t n, const double* a, const double* b) {
  *   r = b - A u; loop { z = Q^-1 r; rz = (r,z) (<=0 -> -7); stop test;
  *   beta = rz/rz_old; p = z + beta p; Ap; alpha = rz/(p,Ap) (<=0 -> -6);
  *   u += alpha p; r -= alpha Ap }
- * The stop test is the true relative residual ||b - A u|| / ||b|| <= rtol
- * (recursive r, confirmed by an explicit residual before returning).
+ * The stop test is the true relative residual ||b - A u|| / bnorm_ref <= rtol
+ * (bnorm_ref <= 0: ||b||), recursive r confirmed by an explicit residual.
  * precond: 0 = none, 1 = Jacobi (pjac), 2 = IC(0).
  * Returns iterations >= 0, or -6/-7 on breakdown, or -1 when maxit is reached
  * (iterations still written to *iters). */
 int64_t orc_dia_pcg(int64_t rank, int ndiag, const int64_t* offsets, const double* data, const double* b, double* u,
-                    int precond, double rtol, int64_t maxit, double* rel_out, int64_t* iters) {
+                    int precond, double rtol, int64_t maxit, double bnorm_ref, double* rel_out, int64_t* iters) {
     double* r = malloc(sizeof(double) * rank);
     double* z = malloc(sizeof(double) * rank);
     double* p = malloc(sizeof(double) * rank);
@@ -376,7 +376,7 @@ int64_t orc_dia_pcg(int64_t rank, int ndiag, const int64_t* offsets, const doubl
         w = malloc(sizeof(double) * rank);
         orc_ic0_factor(rank, ndiag, offsets, data, piv, lfac);
     }
-    double bn = sqrt(dot(rank, b, b));
+    double bn = bnorm_ref > 0. ? bnorm_ref : sqrt(dot(rank, b, b));
     if (bn == 0.) bn = 1.;
     orc_dia_mult(rank, ndiag, offsets, data, u, r);
     for (int64_t i = 0; i < rank; ++i) r[i] = b[i] - r[i];

@@ -132,13 +132,15 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
 // 3D operator apply, z-marching.
 //   MODE_OP : y = M A0 (M x) + (I-M) x        (the Dirichlet-eliminated operator)
 //   MODE_RES: y = M (F - A0 x)                (residual of the full field x)
-//   FUSEP   : x = z + beta p is formed on the fly and stored back into p
+//   FUSEP   : x = z + beta p_old is formed on the fly from pin and stored into pout
+//             (two buffers: halo nodes re-form p from pin while their owners write pout)
 //   DOT     : (x, y) over owned free nodes -> PcgState.pap, alpha = rz / pap
 // T is the storage and arithmetic type of vectors and element data.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, bool FUSEP, bool DOT>
 __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, const T* __restrict__ xin,
-                                                  const T* __restrict__ zin, T* __restrict__ pio, T* __restrict__ y,
+                                                  const T* __restrict__ zin, const T* __restrict__ pin,
+                                                  T* __restrict__ pout, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
                                                   const T* __restrict__ kh, const T* __restrict__ kv,
                                                   PcgState* __restrict__ st, double* __restrict__ partials) {
@@ -188,8 +190,8 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         if (nvalid) {
             long long node = (long long)k * plane + (long long)j * n0 + tx;
             if (FUSEP) {
-                v = zin[node] + T(beta) * pio[node];
-                if (store_p) pio[node] = v;
+                v = zin[node] + T(beta) * pin[node];
+                if (store_p) pout[node] = v;
             } else {
                 v = xin[node];
             }
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         if (xrow) {
             long long node = (long long)k * plane + (long long)(j + 1) * n0 + tx;
             T w;
-            if (FUSEP) w = zin[node] + T(beta) * pio[node];
+            if (FUSEP) w = zin[node] + T(beta) * pin[node];
             else w = xin[node];
             bool dd = dbit(dmask, node);
             xs[(buf * (BY + 1) + BY) * PX + tx] = (MODE == MODE_OP && dd) ? T(0) : w;
@@ -325,7 +327,8 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, bool FUSEP, bool DOT>
 __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* __restrict__ xin,
-                                                  const T* __restrict__ zin, T* __restrict__ pio, T* __restrict__ y,
+                                                  const T* __restrict__ zin, const T* __restrict__ pin,
+                                                  T* __restrict__ pout, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
                                                   const T* __restrict__ kh, const T* __restrict__ kv,
                                                   PcgState* __restrict__ st, double* __restrict__ partials) {
@@ -362,8 +365,8 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
         if (nvalid) {
             long long node = (long long)k * n0 + tx;
             if (FUSEP) {
-                v = zin[node] + T(beta) * pio[node];
-                if (store_p) pio[node] = v;
+                v = zin[node] + T(beta) * pin[node];
+                if (store_p) pout[node] = v;
             } else {
                 v = xin[node];
             }

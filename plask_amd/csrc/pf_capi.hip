@@ -52,6 +52,8 @@ struct pf_ctx {
     int64_t nd = 0;
     double dvals_sq = 0.;
     // vectors
+    double *p2 = nullptr;
+    float *p2_32 = nullptr;
     double *x = nullptr, *r = nullptr, *p = nullptr, *qv = nullptr, *z = nullptr, *dinv = nullptr, *F = nullptr,
            *tmp = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *dinv32 = nullptr;
@@ -220,10 +222,10 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     // vectors
     const size_t N = ctx->N;
     int rc;
-    double** v64[] = {&ctx->x, &ctx->r, &ctx->p, &ctx->qv, &ctx->z, &ctx->dinv, &ctx->F, &ctx->tmp};
+    double** v64[] = {&ctx->x, &ctx->r, &ctx->p, &ctx->p2, &ctx->qv, &ctx->z, &ctx->dinv, &ctx->F, &ctx->tmp};
     for (auto pp : v64)
         if ((rc = dalloc(ctx, pp, N))) return rc;
-    float** v32[] = {&ctx->x32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->z32, &ctx->dinv32};
+    float** v32[] = {&ctx->x32, &ctx->r32, &ctx->p32, &ctx->p2_32, &ctx->q32, &ctx->z32, &ctx->dinv32};
     for (auto pp : v32)
         if ((rc = dalloc(ctx, pp, N))) return rc;
     if ((rc = dalloc(ctx, &ctx->kh, ctx->E))) return rc;
@@ -366,20 +368,22 @@ template <> Axes2<float> axes2<float>(pf_ctx* ctx) {
 }
 
 template <typename T, int MODE, bool FUSEP, bool DOT>
-static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, T* pio, T* y, const T* F, const T* kh, const T* kv) {
+static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, T* pout, T* y, const T* F, const T* kh,
+                         const T* kv) {
     if (ctx->dim == 3) {
         const int BX = (int)ctx->n[0], BY = ctx->BY;
         int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
         int zt = (int)((ctx->n[2] + ctx->KC - 1) / ctx->KC);
         size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
         hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
-                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pio, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, y, F, ctx->dmask, kh, kv, ctx->st,
+                           ctx->partials);
     } else {
         const int BX = (int)ctx->n[0];
         int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
         size_t shm = sizeof(T) * (size_t)(4 * (BX + 1));
         hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, axes2<T>(ctx),
-                           ctx->KC2, xin, zin, pio, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+                           ctx->KC2, xin, zin, pin, pout, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
     }
 }
 
@@ -411,21 +415,21 @@ static int prepare(pf_ctx* ctx) {
     if (ctx->nd)
         hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
                            ctx->dvals, ctx->tmp);
-    launch_apply<double, MODE_RES, false, false>(ctx, ctx->tmp, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
+    launch_apply<double, MODE_RES, false, false>(ctx, ctx->tmp, nullptr, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
     hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, ctx->r, ctx->st,
                        ctx->partials, 0);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
     HIPCK(hipStreamSynchronize(ctx->stream));
-    ctx->bnorm = std::sqrt(ctx->hst->bb + ctx->dvals_sq);
+    ctx->bnorm = std::sqrt(ctx->hst->bb);  // free rows
     ctx->dirty = false;
     return PF_OK;
 }
 
 // One CG run on (x, r already holding b - A x).  T selects FP64 or FP32 vectors.
 template <typename T>
-static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* q, T* z, const T* dinv, const T* kh, const T* kv, double stop2,
-                  int maxit, int check_every, hipGraphExec_t* gcache, int* iters_out, int* status_out) {
+static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* kh, const T* kv,
+                  double stop2, int maxit, int check_every, hipGraphExec_t* gcache, int* iters_out, int* status_out) {
     const long long N = ctx->N;
     PcgState init{};
     init.stop2 = stop2;
@@ -434,23 +438,33 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* q, T* z, const T* dinv, cons
     HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
                        ctx->partials);
+    // both direction buffers start at zero (beta = 0 on the first iteration)
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p, T(0));
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p2, T(0));
     HIPCK(hipGetLastError());
-    auto one_iter = [&]() {
-        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, q, nullptr, kh, kv);
-        hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, p, r, q, dinv, z,
-                           ctx->st, ctx->partials);
+    // iteration t reads p_old from pb[t % 2] and writes p_new to pb[(t + 1) % 2]
+    T* pb[2] = {p, p2};
+    auto one_iter = [&](int t) {
+        T* pin = pb[t & 1];
+        T* pout = pb[(t + 1) & 1];
+        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
+        hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q, dinv,
+                           z, ctx->st, ctx->partials);
     };
-    const int chunk = check_every > 0 ? check_every : 32;
+    int chunk = check_every > 0 ? check_every : 32;
+    chunk += chunk & 1;  // even: each graph replay starts on the same buffer parity
     if (ctx->profile) {
         // event-timed launches, no graph
         hipEvent_t ev[3];
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
         for (;;) {
             for (int c = 0; c < chunk; ++c) {
+                T* pin = pb[c & 1];
+                T* pout = pb[(c + 1) & 1];
                 hipEventRecord(ev[0], ctx->stream);
-                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, q, nullptr, kh, kv);
+                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
                 hipEventRecord(ev[1], ctx->stream);
-                hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, p, r, q,
+                hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q,
                                    dinv, z, ctx->st, ctx->partials);
                 hipEventRecord(ev[2], ctx->stream);
                 HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
@@ -473,7 +487,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* q, T* z, const T* dinv, cons
             *gcache = nullptr;
             hipGraph_t graph;
             HIPCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) one_iter();
+            for (int c = 0; c < chunk; ++c) one_iter(c);
             HIPCK(hipStreamEndCapture(ctx->stream, &graph));
             HIPCK(hipGraphInstantiate(gcache, graph, nullptr, nullptr, 0));
             hipGraphDestroy(graph);
@@ -493,7 +507,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* q, T* z, const T* dinv, cons
 
 // ||M(F - A0 x)||^2 + 0 (Dirichlet rows of x hold the values exactly) into r; returns sqrt
 static int true_residual(pf_ctx* ctx, double* rn) {
-    launch_apply<double, MODE_RES, false, false>(ctx, ctx->x, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
+    launch_apply<double, MODE_RES, false, false>(ctx, ctx->x, nullptr, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
     hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, (long long)ctx->N, ctx->r,
                        ctx->st, ctx->partials, 1);
     HIPCK(hipGetLastError());
@@ -520,13 +534,33 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     if (opts->x0 == PF_X0_GIVEN) {
         if (!x_inout) return fail(ctx, PF_EINVAL, "x_inout required with PF_X0_GIVEN");
         HIPCK(hipMemcpyAsync(ctx->x, x_inout, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    } else if (opts->x0 == PF_X0_UNIFORM) {
+        hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, opts->init_value);
     } else if (opts->x0 == PF_X0_ZERO || !ctx->have_solution) {
         hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, 0.);
     }
     if (ctx->nd)
         hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
                            ctx->dvals, ctx->x);
-    const double bnorm = ctx->bnorm > 0. ? ctx->bnorm : 1.;
+    double bnorm = ctx->bnorm;
+    if (opts->norm == PF_NORM_B_FULL) {
+        bnorm = std::sqrt(ctx->bnorm * ctx->bnorm + ctx->dvals_sq);
+    } else if (opts->norm == PF_NORM_INIT) {
+        hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->tmp, opts->init_value);
+        if (ctx->nd)
+            hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd,
+                               ctx->dnodes, ctx->dvals, ctx->tmp);
+        launch_apply<double, MODE_RES, false, false>(ctx, ctx->tmp, nullptr, nullptr, nullptr, ctx->qv, ctx->F, ctx->kh, ctx->kv);
+        hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, ctx->qv, ctx->st,
+                           ctx->partials, 1);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipStreamSynchronize(ctx->stream));
+        bnorm = std::sqrt(ctx->hst->rr);
+    } else if (opts->norm != PF_NORM_B) {
+        return fail(ctx, PF_EINVAL, "unknown residual norm");
+    }
+    if (!(bnorm > 0.)) bnorm = 1.;
     const double rtol = opts->rtol > 0. ? opts->rtol : 1e-8;
     const int maxit = opts->maxit > 0 ? opts->maxit : 100000;
     int total_iters = 0, outer = 0, status = 0;
@@ -537,7 +571,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
         for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
             int it = 0, stt = 0;
             double stop2 = (rtol * bnorm) * (rtol * bnorm);
-            if ((rc = run_cg<double>(ctx, ctx->x, ctx->r, ctx->p, ctx->qv, ctx->z, ctx->dinv, ctx->kh, ctx->kv, stop2,
+            if ((rc = run_cg<double>(ctx, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->kh, ctx->kv, stop2,
                                      maxit - total_iters, opts->check_every, &ctx->graph64, &it, &stt)))
                 return rc;
             total_iters += it;
@@ -557,7 +591,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
             int it = 0, stt = 0;
             // the target for this step: reduce by `inner`, but not beyond what the outer tolerance needs
             double goal = std::max(inner, 0.5 * rtol * bnorm / rn);
-            if ((rc = run_cg<float>(ctx, ctx->x32, ctx->r32, ctx->p32, ctx->q32, ctx->z32, ctx->dinv32, ctx->kh32,
+            if ((rc = run_cg<float>(ctx, ctx->x32, ctx->r32, ctx->p32, ctx->p2_32, ctx->q32, ctx->z32, ctx->dinv32, ctx->kh32,
                                     ctx->kv32, goal * goal, maxit - total_iters, opts->check_every, &ctx->graph32, &it,
                                     &stt)))
                 return rc;
@@ -577,11 +611,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     HIPCK(hipStreamSynchronize(ctx->stream));
     auto t1 = std::chrono::steady_clock::now();
     ctx->have_solution = true;
-    if (x_inout && opts->x0 != PF_X0_DEVICE) {
-        HIPCK(hipMemcpy(x_inout, ctx->x, sizeof(double) * N, hipMemcpyDeviceToHost));
-    } else if (x_inout) {
-        HIPCK(hipMemcpy(x_inout, ctx->x, sizeof(double) * N, hipMemcpyDeviceToHost));
-    }
+    if (x_inout) HIPCK(hipMemcpy(x_inout, ctx->x, sizeof(double) * N, hipMemcpyDeviceToHost));
     int ret = PF_OK;
     if (status == -6 || status == -7) ret = PF_EBREAK;
     else if (status == -8) ret = PF_ENONFINITE;
@@ -613,10 +643,10 @@ int pf_apply(pf_ctx* ctx, int precision, const void* x_dev, void* y_dev) {
     if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
     HIPCK(hipSetDevice(ctx->device));
     if (precision == PF_FP64)
-        launch_apply<double, MODE_OP, false, false>(ctx, (const double*)x_dev, nullptr, nullptr, (double*)y_dev, nullptr,
+        launch_apply<double, MODE_OP, false, false>(ctx, (const double*)x_dev, nullptr, nullptr, nullptr, (double*)y_dev, nullptr,
                                                     ctx->kh, ctx->kv);
     else
-        launch_apply<float, MODE_OP, false, false>(ctx, (const float*)x_dev, nullptr, nullptr, (float*)y_dev, nullptr,
+        launch_apply<float, MODE_OP, false, false>(ctx, (const float*)x_dev, nullptr, nullptr, nullptr, (float*)y_dev, nullptr,
                                                    ctx->kh32, ctx->kv32);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -628,7 +658,7 @@ int pf_apply_host(pf_ctx* ctx, const double* x, double* y) {
     if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
     HIPCK(hipSetDevice(ctx->device));
     HIPCK(hipMemcpy(ctx->tmp, x, sizeof(double) * ctx->N, hipMemcpyHostToDevice));
-    launch_apply<double, MODE_OP, false, false>(ctx, ctx->tmp, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
+    launch_apply<double, MODE_OP, false, false>(ctx, ctx->tmp, nullptr, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(ctx->stream));
     HIPCK(hipMemcpy(y, ctx->qv, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
@@ -662,6 +692,12 @@ int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
     return PF_OK;
 }
 
+int pf_set_profile(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    ctx->profile = on != 0;
+    return PF_OK;
+}
+
 int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds) {
     if (!ctx || reps <= 0 || !seconds) return PF_EINVAL;
     if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
@@ -671,9 +707,9 @@ int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds) {
     HIPCK(hipEventCreate(&b));
     auto go = [&]() {
         if (precision == PF_FP64)
-            launch_apply<double, MODE_OP, false, false>(ctx, ctx->p, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
+            launch_apply<double, MODE_OP, false, false>(ctx, ctx->p, nullptr, nullptr, nullptr, ctx->qv, nullptr, ctx->kh, ctx->kv);
         else
-            launch_apply<float, MODE_OP, false, false>(ctx, ctx->p32, nullptr, nullptr, ctx->q32, nullptr, ctx->kh32,
+            launch_apply<float, MODE_OP, false, false>(ctx, ctx->p32, nullptr, nullptr, nullptr, ctx->q32, nullptr, ctx->kh32,
                                                        ctx->kv32);
     };
     go();

@@ -15,18 +15,20 @@ PF_OK, PF_EINVAL, PF_ESTATE, PF_EDEVICE, PF_ESHAPE, PF_ENOCONV, PF_EBREAK, PF_EN
 PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
 PF_THERMAL, PF_ELECTRICAL = 0, 1
 PF_FP64, PF_FP32_REFINE = 0, 1
-PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE = 0, 1, 2
+PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM = 0, 1, 2, 3
+PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
-    "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply",
+    "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
 ]
 
 
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("x0", C.c_int), ("maxit", C.c_int),
-                ("rtol", C.c_double), ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int)]
+                ("rtol", C.c_double), ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("check_every", C.c_int),
+                ("norm", C.c_int), ("init_value", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -78,6 +80,7 @@ def load_library(path=LIB_PATH):
     L.pf_num_elements.restype = i64
     L.pf_kernel_times.argtypes = [P, P, P, I]
     L.pf_time_apply.argtypes = [P, I, I, C.POINTER(C.c_double)]
+    L.pf_set_profile.argtypes = [P, I]
     _lib = L
     return L
 
@@ -162,9 +165,13 @@ class FemContext:
 
     # -- compute -------------------------------------------------------------
     def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, max_outer=50,
-              check_every=32, keep_on_device=False, raise_noconv=True):
+              check_every=32, keep_on_device=False, raise_noconv=True, norm=PF_NORM_INIT, init_value=None):
+        """Solve; the relative residual is ||B - A X|| / ||B - A X_init|| (norm=PF_NORM_INIT,
+        X_init = init_value, default the first entry of x0 or 0) unless norm says otherwise."""
+        if init_value is None:
+            init_value = float(np.ravel(x0)[0]) if x0 is not None else 0.0
         o = SolveOpts(precision, 1, PF_X0_GIVEN if x0 is not None else PF_X0_ZERO, int(maxit), float(rtol),
-                      float(inner_rtol), int(max_outer), int(check_every))
+                      float(inner_rtol), int(max_outer), int(check_every), int(norm), float(init_value))
         st = SolveStats()
         if x0 is not None:
             x = _f64(np.array(x0, dtype=np.float64, copy=True))
@@ -175,9 +182,21 @@ class FemContext:
             self._ck(rc, "pf_solve")
         return x, st
 
-    def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32):
+    def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
+                   norm=PF_NORM_INIT, init_value=0.0):
         """Start from the solution already on the device (no host transfers)."""
-        o = SolveOpts(precision, 1, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every))
+        o = SolveOpts(precision, 1, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every),
+                      int(norm), float(init_value))
+        st = SolveStats()
+        rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
+        self._ck(rc, "pf_solve")
+        return st
+
+    def solve_uniform(self, init_value, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
+                      norm=PF_NORM_INIT, max_outer=50):
+        """Start from init_value everywhere (device fill); the solution stays on the device."""
+        o = SolveOpts(precision, 1, PF_X0_UNIFORM, int(maxit), float(rtol), float(inner_rtol), int(max_outer),
+                      int(check_every), int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
         self._ck(rc, "pf_solve")
@@ -204,6 +223,9 @@ class FemContext:
         c = np.zeros(2, dtype=np.int64)
         self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), 2), "pf_kernel_times")
         return s, c
+
+    def set_profile(self, on):
+        self._ck(self.L.pf_set_profile(self.h, 1 if on else 0), "pf_set_profile")
 
     def time_apply(self, precision=PF_FP64, reps=50):
         t = C.c_double(0.)
