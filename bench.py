@@ -25,14 +25,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from plask_amd import workloads  # noqa: E402
-from plask_amd.fem import PF_FP32_REFINE, PF_FP64, FemContext  # noqa: E402
+from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PRECOND_JACOBI, PRECOND_LINE, FemContext  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Full-solve iteration counts of the CPU reference path (serial IC(0)-CG and Jacobi-CG
 # on the 14-diagonal storage, same inputs, same stopping rule) for 128x128x256, used to
 # extrapolate the bounded per-iteration sample to a whole solve.  Iteration counts are a
 # property of the algorithm and the inputs, not of the host.
-CPU_FULL_ITERS = {"ic0": None, "jacobi": None}
+CPU_FULL_ITERS = {"ic0": 161, "jacobi": None}
 
 
 def make_ctx(p, profile=False):
@@ -89,8 +89,8 @@ def main():
     ctx = make_ctx(p)
     rtol = 1e-8
 
-    def step(prec):
-        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-4 if prec else 0.0)
+    def step(prec, pc=PRECOND_LINE):
+        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-4 if prec else 0.0, precond=pc)
 
     for _ in range(args.warmup):
         st = step(PF_FP64)
@@ -102,6 +102,14 @@ def main():
     print("fp64: %.2f solves/s, %.1f ms/solve, %d iterations, rel resid %.2e" %
           (fp64["solves_per_s"], fp64["ms"], st.iters, st.rel_resid), file=sys.stderr, flush=True)
     T64 = ctx.solution()
+
+    # plain Jacobi-PCG for reference (one untimed-warm solve + timed)
+    step(PF_FP64, PRECOND_JACOBI)
+    t0 = time.perf_counter()
+    stj = step(PF_FP64, PRECOND_JACOBI)
+    tj = time.perf_counter() - t0
+    jac = dict(solves_per_s=1.0 / tj, ms=1e3 * tj, iters=stj.iters, rel_resid=stj.rel_resid)
+    print("fp64 jacobi: %.2f solves/s, %d iterations" % (jac["solves_per_s"], stj.iters), file=sys.stderr, flush=True)
 
     mixed = None
     if not args.no_mixed:
@@ -124,15 +132,15 @@ def main():
     (t_apply, t_upd), (n_apply, n_upd) = ctx.kernel_times()
     mask_bytes = 8 * ((N + 63) // 64)
     bytes_apply = 32 * N + 16 * E + mask_bytes      # z, p_old read; p_new, q write; kxy, kz; mask
-    bytes_upd = 64 * N                              # x, r r/w; p, q, dinv read; z write
+    bytes_upd = 104 * N  # fwd: x,p,r,q,a,w read, x,r,z write; bwd: z,c',r read, z write
     t_plain = ctx.time_apply(PF_FP64, reps=50)
     bytes_plain = 16 * N + 16 * E + mask_bytes      # x read, y write, kxy, kz, mask
     kern = {
         "apply_fused": dict(avg_s=t_apply, launches=int(n_apply), bytes=bytes_apply, gbs=bytes_apply / t_apply / 1e9),
-        "cg_update": dict(avg_s=t_upd, launches=int(n_upd), bytes=bytes_upd, gbs=bytes_upd / t_upd / 1e9),
+        "cg_line": dict(avg_s=t_upd, launches=int(n_upd), bytes=bytes_upd, gbs=bytes_upd / t_upd / 1e9),
         "apply_plain": dict(avg_s=t_plain, bytes=bytes_plain, gbs=bytes_plain / t_plain / 1e9),
     }
-    dom = "apply_fused" if t_apply >= t_upd else "cg_update"
+    dom = "apply_fused" if t_apply >= t_upd else "cg_line"
     k = kern[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
@@ -162,10 +170,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded layered stack, SplitMix64)",
-        "config": {"workload": "thermal3d_%dx%dx%d_fp64_jacobi_pcg" % size, "dof": N, "elements": E,
+        "config": {"workload": "thermal3d_%dx%dx%d_fp64_zline_pcg" % size, "dof": N, "elements": E,
                    "rtol": rtol, "residual": "||B-AT||/||B-AT_init||, T_init = 300 K",
                    "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]},
         "mixed_precision": mixed,
+        "fp64_jacobi_pcg": jac,
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,

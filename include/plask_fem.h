@@ -83,7 +83,8 @@ typedef struct pf_ctx pf_ctx;
 
 typedef struct {
     int precision;    /* PF_FP64 | PF_FP32_REFINE */
-    int precond;      /* 1 = Jacobi (NSPCG 'jac'); others reserved */
+    int precond;      /* 1 = Jacobi (NSPCG 'jac'); 2 = line block-Jacobi along the last (layer) axis
+                         (the tridiagonal part of A per column, cf. NSPCG 'ljac'); 0 = default (2) */
     int x0;           /* PF_X0_* */
     int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
     double rtol;      /* stop when ||B - A X||_2 <= rtol * (denominator chosen by `norm`) */
@@ -165,6 +166,54 @@ int pf_set_profile(pf_ctx* ctx, int on);
 /* Time `reps` back-to-back FP64 operator applies y = A x on the context stream with HIP
  * events; returns the average seconds per apply.  For benchmarks. */
 int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds);
+
+/* Picard-loop support ------------------------------------------------------ */
+
+/* Set the device solution (N values), e.g. the initial field inittemp (therm2d.cpp:79). */
+int pf_solution_set(pf_ctx* ctx, const double* x);
+
+/* max |X - X_saved| and max X, then X_saved = X: the loop error and maxT of
+ * ThermalFem2DSolver::compute (therm2d.cpp:479-487). */
+int pf_solution_change(pf_ctx* ctx, double* max_change, double* max_value);
+
+/* Element means of the device solution (the element temperature T_avg of
+ * therm2d.cpp:210 / therm3d.cpp:209), E values to the host. */
+int pf_element_means(pf_ctx* ctx, double* out);
+
+/* Temperature-dependent conductivity law per element, k = k0 (300 / T_avg)^alpha
+ * (alpha = 0: constant); pf_coeffs_update evaluates it on the device from the current
+ * solution, replacing Material::thermk(T) calls of setMatrix (therm3d.cpp:211-213). */
+int pf_coeffs_powerlaw(pf_ctx* ctx, int equation, const double* k0_h, const double* k0_v, const double* a_h,
+                       const double* a_v, int64_t nelem);
+int pf_coeffs_update(pf_ctx* ctx);
+
+/* Current element conductivities (E values each) to the host. */
+int pf_get_coeffs(pf_ctx* ctx, double* k_inplane, double* k_vert);
+
+/* Junction elements of the Shockley solver (active regions found by
+ * ElectricalFem2DSolver::setupActiveRegions, electr2d.cpp:90-169): for element q its
+ * index, ncorner (2 in 2D, 4 in 3D) bottom/top node pairs spanning the whole junction
+ * (act.bottom/act.top rows, electr2d.cpp:250-253, electr3d.cpp:253-259), and its
+ * junction number; heights[nj] = junction thickness d (um). */
+int pf_junctions_set(pf_ctx* ctx, int64_t nje, int ncorner, const int64_t* elems, const int64_t* bottom_nodes,
+                     const int64_t* top_nodes, const int* junction, int nj, const double* heights);
+/* beta (1/V) and js (A/m^2) per junction element (BetaSolver getBeta/getJs, beta.hpp:50-77). */
+int pf_junction_params(pf_ctx* ctx, const double* beta, const double* js, int64_t nje);
+/* One junction conductivity update from the device potential (the loopno != 0 block of
+ * ElectricalFem2DSolver::setMatrix, electr2d.cpp:243-264; law beta.hpp:43-46).
+ * stable = 1: CONVERGENCE_STABLE averaging. */
+int pf_junction_update(pf_ctx* ctx, int stable);
+
+/* Per-element flags: select = elements whose |j| enters the loop error (junction
+ * elements, electr2d.cpp:410; NULL = all), noheat = elements with zero Joule heat
+ * (EMPTY material or 'noheat' role, electr2d.cpp:454). */
+int pf_element_flags_set(pf_ctx* ctx, const unsigned char* select, const unsigned char* noheat);
+
+/* Element current densities (kA/cm^2) and Joule heat (W/m^3) from the device potential;
+ * returns max |j - j_prev|^2 and max |j|^2 over the selected elements (the loop error
+ * of electr2d.cpp:399-422) and keeps j as j_prev. */
+int pf_currents(pf_ctx* ctx, double* max_dj2, double* max_j2);
+int pf_get_currents(pf_ctx* ctx, double* cur, double* heat);
 
 #ifdef __cplusplus
 }

@@ -233,3 +233,45 @@ def junction_update(elems, corners, ncorner, V, height, beta, js, stable, sig_v)
 
 def wtime():
     return lib().orc_wtime()
+
+
+def electrical_compute(grid, sh, sv, dnodes, dvals, junctions, loops, start_cond=5.0, stable=False, maxerr=0.05):
+    """Picard loop of ElectricalFem2DSolver::compute (electr2d.cpp:363-440) with band Cholesky.
+
+    junctions: list of dicts with 'elems' (element indices), 'bottom'/'top' (node lists per
+    element, ncorner each), 'height', 'beta', 'js'.  Element conductivities sh/sv are the
+    material values; junction elements start at (0, start_cond).  Returns (V, currents,
+    heats, sigma_v, loops_done, err).
+    """
+    sh = np.array(sh, dtype=np.float64)
+    sv = np.array(sv, dtype=np.float64)
+    for J in junctions:
+        sh[J["elems"]] = 0.0
+        sv[J["elems"]] = start_cond
+    V = np.zeros(grid.nnodes)
+    cur_prev = np.zeros((grid.nelems, grid.dim))
+    jmask = np.zeros(grid.nelems, bool)
+    for J in junctions:
+        jmask[J["elems"]] = True
+    noactive = not jmask.any()
+    loop = 0
+    loopno = 0
+    err = 0.0
+    while True:
+        if loopno != 0:
+            for J in junctions:
+                nc = len(J["bottom"][0])
+                corners = np.concatenate([np.r_[b, t] for b, t in zip(J["bottom"], J["top"])])
+                junction_update(J["elems"], corners, nc, V, J["height"], J["beta"], J["js"], stable, sv)
+        A = assemble(grid, 1, sh, sv, None, ST_BAND, dnodes, dvals)
+        V = band_cholesky_solve(A)
+        cur, heat = currents(grid, sh, sv, V)
+        acur = np.sum(cur ** 2, axis=1)
+        mcur = np.sqrt((acur if noactive else acur[jmask]).max())
+        err = 100.0 * np.sqrt(np.sum((cur - cur_prev) ** 2, axis=1).max()) / max(mcur, 100e-7)
+        cur_prev = cur
+        loop += 1
+        loopno += 1
+        if not (err > maxerr and (loops == 0 or loop < loops)):
+            break
+    return V, cur, heat, sv, loop, err

@@ -9,3 +9,11 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
     -Wno-unused-result -Wno-unused-value \
     -o "$OUT.tmp" "$HERE/pf_capi.hip"
 mv "$OUT.tmp" "$OUT"
+# host solver cores + pybind11 module (links the C ABI library by rpath)
+PYINC=$(python3 -c "import sysconfig; print(sysconfig.get_paths()['include'])")
+PBINC=$(python3 -c "import pybind11; print(pybind11.get_include())")
+EXT=$(python3 -c "import sysconfig; print(sysconfig.get_config_var('EXT_SUFFIX'))")
+g++ -O2 -std=c++17 -fPIC -shared -fvisibility=hidden -I"$PYINC" -I"$PBINC" \
+    -o "$HERE/../_core$EXT.tmp" "$HERE/host/fem_solvers.cpp" "$HERE/host/pymodule.cpp" \
+    -L"$HERE/.." -lplaskfem -Wl,-rpath,'$ORIGIN'
+mv "$HERE/../_core$EXT.tmp" "$HERE/../_core$EXT"

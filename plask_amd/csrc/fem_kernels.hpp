@@ -458,6 +458,40 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
     }
 }
 
+// end of an iteration: new (r,z), (r,r) -> scalars, stopping decisions
+__device__ inline void finish_update(PcgState* st, const double (&tot)[2]) {
+    st->rz_prev = st->rz;
+    st->rz = tot[0];
+    st->rr = tot[1];
+    st->iters += 1;
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+        st->status = -8;
+        st->done = 1;
+    } else if (tot[1] <= st->stop2) {
+        st->done = 1;
+    } else if (!(tot[0] > 0.)) {
+        st->status = -7;
+        st->done = 1;
+    } else if (st->iters >= st->maxit) {
+        st->status = 1;
+        st->done = 1;
+    }
+}
+
+// start of a CG run: scalars from (r,z), (r,r)
+__device__ inline void finish_start(PcgState* st, const double (&tot)[2]) {
+    st->rz = tot[0];
+    st->rz_prev = tot[0];
+    st->rr = tot[1];
+    st->iters = 0;
+    st->status = 0;
+    st->done = (tot[1] <= st->stop2) ? 1 : 0;
+    if (!isfinite(tot[1])) {
+        st->status = -8;
+        st->done = 1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PCG vector step: x += alpha p; r -= alpha q; z = Dinv r; (r,z), (r,r).
 // Grid-stride over all nodes; vectors of type T, reductions in FP64.
@@ -482,24 +516,7 @@ __global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ 
         rr += (double)ri * (double)ri;
     }
     double v[2] = {rz, rr}, tot[2];
-    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
-        st->rz_prev = st->rz;
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->iters += 1;
-        if (!isfinite(tot[0]) || !isfinite(tot[1])) {
-            st->status = -8;
-            st->done = 1;
-        } else if (tot[1] <= st->stop2) {
-            st->done = 1;
-        } else if (!(tot[0] > 0.)) {
-            st->status = -7;
-            st->done = 1;
-        } else if (st->iters >= st->maxit) {
-            st->status = 1;
-            st->done = 1;
-        }
-    }
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) finish_update(st, tot);
 }
 
 // r = given residual; z = Dinv r; (r,z), (r,r) -> state (start of a CG run)
@@ -516,18 +533,7 @@ __global__ __launch_bounds__(256) void k_cg_start(long long n, const T* __restri
         rr += (double)ri * (double)ri;
     }
     double v[2] = {rz, rr}, tot[2];
-    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
-        st->rz = tot[0];
-        st->rz_prev = tot[0];
-        st->rr = tot[1];
-        st->iters = 0;
-        st->status = 0;
-        st->done = (tot[1] <= st->stop2) ? 1 : 0;
-        if (!isfinite(tot[1])) {
-            st->status = -8;
-            st->done = 1;
-        }
-    }
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) finish_start(st, tot);
 }
 
 // sum of squares of a vector -> st->bb
@@ -543,6 +549,173 @@ __global__ __launch_bounds__(256) void k_norm2(long long n, const T* __restrict_
     if (grid_reduce<1>(v, partials, &st->counter[2], tot)) {
         if (slot == 0) st->bb = tot[0];
         else st->rr = tot[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Line (block-Jacobi) preconditioner along the last axis (the layer-stacking
+// axis): M = the tridiagonal part of A coupling each node to its two neighbours
+// in the same column.  Thin layers make those couplings dominate the operator,
+// so M captures most of it.  Factors of the Thomas algorithm are precomputed:
+//   w_k = 1 / (b_k - a_k c'_{k-1}),  c'_k = c_k w_k,  a_k = c_{k-1}
+// and z = M^-1 r is  d_k = (r_k - a_k d_{k-1}) w_k,  z_k = d_k - c'_k z_{k+1}.
+// Each line block is a principal submatrix of the SPD matrix, so all pivots
+// are positive.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                               const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
+                               T* __restrict__ lcp) {
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n0 * n1) return;
+    const int i = col % n0, j = col / n0;
+    const long long plane = (long long)n0 * n1;
+    double cprev = 0., cpprev = 0.;
+    for (int k = 0; k < n2; ++k) {
+        const long long node = (long long)k * plane + col;
+        double b = 0., c = 0.;
+        for (int cc = 0; cc < 8; ++cc) {
+            int ei = i - 1 + (cc & 1), ej = j - 1 + ((cc >> 1) & 1), ek = k - 1 + ((cc >> 2) & 1);
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= n0 - 1 || ej >= n1 - 1 || ek >= n2 - 1) continue;
+            long long e = ((long long)ek * (n1 - 1) + ej) * (n0 - 1) + ei;
+            double dx = g.hx[ei], dy = g.hy[ej], dz = g.hz[ek];
+            double kxy = kh[e] * 1e-6, kz = kv[e] * 1e-6;
+            double kx1 = kxy * dy * dz / dx, ky1 = kxy * dx * dz / dy, kz1 = kz * dx * dy / dz;
+            b += (kx1 + ky1 + kz1) / 9.;
+            if (ek == k) c += (kx1 + ky1 - 2. * kz1) / 18.;  // z edge (k, k+1) of this element
+        }
+        const bool dk = dbit(dmask, node);
+        if (dk) b = 1.;
+        if (dk || k == n2 - 1 || dbit(dmask, node + plane)) c = 0.;
+        const double a = cprev;
+        const double w = 1. / (b - a * cpprev);
+        const double cp = c * w;
+        la[node] = T(a);
+        lw[node] = T(w);
+        lcp[node] = T(cp);
+        cprev = c;
+        cpprev = cp;
+    }
+}
+
+// 2D: lines along axis 1; couplings (kx' - 2 ky')/6 (x r_mid), therm2d.cpp:225-228
+template <typename T>
+__global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                               const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
+                               T* __restrict__ lcp) {
+    const int n0 = g.n0, n1 = g.n1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n0) return;
+    double cprev = 0., cpprev = 0.;
+    for (int k = 0; k < n1; ++k) {
+        const long long node = (long long)k * n0 + i;
+        double b = 0., c = 0.;
+        for (int cc = 0; cc < 4; ++cc) {
+            int ei = i - 1 + (cc & 1), ek = k - 1 + ((cc >> 1) & 1);
+            if (ei < 0 || ek < 0 || ei >= n0 - 1 || ek >= n1 - 1) continue;
+            long long e = (long long)ek * (n0 - 1) + ei;
+            double w = g.hx[ei], h = g.hy[ek];
+            double r = g.rmid ? g.rmid[ei] : 1.;
+            double kx1 = r * kh[e] * h / w, ky1 = r * kv[e] * w / h;
+            b += (kx1 + ky1) / 3.;
+            if (ek == k) c += (kx1 - 2. * ky1) / 6.;
+        }
+        const bool dk = dbit(dmask, node);
+        if (dk) b = 1.;
+        if (dk || k == n1 - 1 || dbit(dmask, node + n0)) c = 0.;
+        const double a = cprev;
+        const double w = 1. / (b - a * cpprev);
+        const double cp = c * w;
+        la[node] = T(a);
+        lw[node] = T(w);
+        lcp[node] = T(cp);
+        cprev = c;
+        cpprev = cp;
+    }
+}
+
+// Column kernel: one thread per line.  START: z = M^-1 r only (beginning of a CG
+// run); otherwise x += alpha p; r -= alpha q; z = M^-1 r.  The forward sweep
+// keeps d in z, the backward sweep overwrites it with the solution.
+template <typename T, bool START>
+__global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long plane, T* __restrict__ x,
+                                                const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q,
+                                                const T* __restrict__ la, const T* __restrict__ lw,
+                                                const T* __restrict__ lcp, T* __restrict__ z,
+                                                PcgState* __restrict__ st, double* __restrict__ partials) {
+    if (!START && st->done) return;
+    constexpr int U = 8;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = col < ncol;
+    const T alpha = START ? T(0) : T(st->alpha);
+    double rr = 0., rz = 0.;
+    T dprev = T(0);
+    for (int k0 = 0; k0 < nplanes; k0 += U) {
+        T xv[U], pv[U], rv[U], qv[U], av[U], wv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u;
+            if (ok && k < nplanes) {
+                const long long node = (long long)k * plane + col;
+                rv[u] = r[node];
+                av[u] = la[node];
+                wv[u] = lw[node];
+                if (!START) {
+                    xv[u] = x[node];
+                    pv[u] = p[node];
+                    qv[u] = q[node];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u;
+            if (ok && k < nplanes) {
+                const long long node = (long long)k * plane + col;
+                T ri = rv[u];
+                if (!START) {
+                    x[node] = xv[u] + alpha * pv[u];
+                    ri = ri - alpha * qv[u];
+                    r[node] = ri;
+                }
+                rr += (double)ri * (double)ri;
+                const T d = (ri - av[u] * dprev) * wv[u];
+                z[node] = d;
+                dprev = d;
+            }
+        }
+    }
+    T znext = T(0);
+    const int kl = nplanes - 1;
+    for (int k0 = kl; k0 >= 0; k0 -= U) {
+        T dv[U], cv[U], rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 - u;
+            if (ok && k >= 0) {
+                const long long node = (long long)k * plane + col;
+                dv[u] = z[node];
+                cv[u] = lcp[node];
+                rv[u] = r[node];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 - u;
+            if (ok && k >= 0) {
+                const long long node = (long long)k * plane + col;
+                const T zk = dv[u] - cv[u] * znext;
+                z[node] = zk;
+                rz += (double)rv[u] * (double)zk;
+                znext = zk;
+            }
+        }
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+        if (START) finish_start(st, tot);
+        else finish_update(st, tot);
     }
 }
 
@@ -665,6 +838,217 @@ template <typename T>
 __global__ void k_fill(long long n, T* __restrict__ a, T v) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Picard-loop support: element means, k(T) laws, change statistics, junctions,
+// currents and Joule heat.  Element loops are one thread per element.
+// ---------------------------------------------------------------------------
+struct ElemGrid {
+    int dim, n0, n1, n2;  // nodes per axis (n2 = 1 in 2D)
+};
+
+__device__ __forceinline__ void elem_coords(const ElemGrid& g, long long e, int& i, int& j, int& k) {
+    const int m0 = g.n0 - 1, m1 = g.n1 - 1;
+    i = e % m0;
+    if (g.dim == 2) {
+        j = e / m0;
+        k = 0;
+    } else {
+        j = (e / m0) % m1;
+        k = e / ((long long)m0 * m1);
+    }
+}
+
+__device__ __forceinline__ long long node_of(const ElemGrid& g, int i, int j, int k) {
+    return i + (long long)g.n0 * (j + (long long)g.n1 * k);
+}
+
+// mean of the element's corner values (the element temperature of therm3d.cpp:209,
+// therm2d.cpp:210)
+__global__ void k_elem_mean(ElemGrid g, long long E, const double* __restrict__ x, double* __restrict__ out) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        elem_coords(g, e, i, j, k);
+        double s = 0.;
+        const int nc = g.dim == 3 ? 8 : 4;
+        for (int c = 0; c < nc; ++c) s += x[node_of(g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1))];
+        out[e] = s / nc;
+    }
+}
+
+// k = k0 (300 / T_elem)^alpha for both tensor components; writes FP64 and FP32 copies
+__global__ void k_powerlaw(ElemGrid g, long long E, const double* __restrict__ x, const double* __restrict__ k0h,
+                           const double* __restrict__ k0v, const double* __restrict__ ah, const double* __restrict__ av,
+                           double* __restrict__ kh, double* __restrict__ kv, float* __restrict__ kh32,
+                           float* __restrict__ kv32) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        elem_coords(g, e, i, j, k);
+        double s = 0.;
+        const int nc = g.dim == 3 ? 8 : 4;
+        for (int c = 0; c < nc; ++c) s += x[node_of(g, i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1))];
+        const double t = 300. / (s / nc);
+        const double h = ah[e] == 0. ? k0h[e] : k0h[e] * pow(t, ah[e]);
+        const double v = av[e] == 0. ? k0v[e] : k0v[e] * pow(t, av[e]);
+        kh[e] = h;
+        kv[e] = v;
+        kh32[e] = (float)h;
+        kv32[e] = (float)v;
+    }
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// max over the grid of NV values; last block returns totals
+template <int NV>
+__device__ bool grid_max(double (&v)[NV], double* __restrict__ partials, unsigned int* counter, double (&tot)[NV]) {
+    __shared__ double red[NV][16];
+    __shared__ bool last;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = (nthr + 63) >> 6;
+    for (int a = 0; a < NV; ++a) {
+        double s = wave_max(v[a]);
+        if (lane == 0) red[a][wid] = s;
+    }
+    __syncthreads();
+    const unsigned int nblk = gridDim.x, bid = blockIdx.x;
+    const double ninf = -__builtin_huge_val();
+    if (tid == 0) {
+        for (int a = 0; a < NV; ++a) {
+            double s = ninf;
+            for (int w = 0; w < nw; ++w) s = fmax(s, red[a][w]);
+            partials[(size_t)a * nblk + bid] = s;
+        }
+        __threadfence();
+        unsigned int t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (t == nblk - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    for (int a = 0; a < NV; ++a) {
+        double s = ninf;
+        for (unsigned int b = tid; b < nblk; b += nthr)
+            s = fmax(s, __hip_atomic_load(partials + (size_t)a * nblk + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        s = wave_max(s);
+        __syncthreads();
+        if (lane == 0) red[a][wid] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double t = ninf;
+            for (int w = 0; w < nw; ++w) t = fmax(t, red[a][w]);
+            tot[a] = t;
+        }
+    }
+    if (tid == 0) *counter = 0u;
+    return tid == 0;
+}
+
+// max |x - xprev| and max x (therm2d.cpp:480-487); then xprev = x.  Results in out[0..1].
+__global__ __launch_bounds__(256) void k_delta_stats(long long n, const double* __restrict__ x, double* __restrict__ xprev,
+                                                     double* __restrict__ out, double* __restrict__ partials,
+                                                     unsigned int* counter) {
+    double d = 0., m = -__builtin_huge_val();
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double xi = x[i];
+        d = fmax(d, fabs(xi - xprev[i]));
+        m = fmax(m, xi);
+        xprev[i] = xi;
+    }
+    double v[2] = {d, m}, tot[2];
+    if (grid_max<2>(v, partials, counter, tot)) {
+        out[0] = tot[0];
+        out[1] = tot[1];
+    }
+}
+
+// Junction conductivity update (electr2d.cpp:243-264, electr3d.cpp:246-273) with the
+// law of BetaSolver::activeCond (beta.hpp:43-46).  For junction element q:
+// U = mean_c (V[top_c] - V[bot_c]), jy = 0.1 sigma_yy U / d,
+// sigma = 10 |jy| d beta / ln(1e7 |jy| / js + 1); STABLE averages with the old value;
+// values below 1e-16 or NaN are clamped to 1e-16.  In-plane conductivity is 0.
+__global__ void k_junction_update(long long nje, int ncorner, const long long* __restrict__ elems,
+                                  const long long* __restrict__ bot, const long long* __restrict__ top,
+                                  const int* __restrict__ jid, const double* __restrict__ height,
+                                  const double* __restrict__ beta, const double* __restrict__ js, int stable,
+                                  const double* __restrict__ V, double* __restrict__ kh, double* __restrict__ kv,
+                                  float* __restrict__ kh32, float* __restrict__ kv32) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nje; q += (long long)gridDim.x * blockDim.x) {
+        double U = 0.;
+        for (int c = 0; c < ncorner; ++c) U += V[top[q * ncorner + c]] - V[bot[q * ncorner + c]];
+        U /= ncorner;
+        const long long e = elems[q];
+        const int n = jid[q];
+        const double d = height[n];
+        double jy = fabs(0.1 * kv[e] * U / d);
+        double cond = 10. * jy * d * beta[q] / log(1e7 * jy / js[q] + 1.);
+        if (stable) cond = 0.5 * (kv[e] + cond);
+        if (isnan(cond) || fabs(cond) < 1e-16) cond = 1e-16;
+        kv[e] = cond;
+        kh[e] = 0.;
+        kv32[e] = (float)cond;
+        kh32[e] = 0.f;
+    }
+}
+
+// Element current densities j = -sigma grad V in kA/cm^2 (electr2d.cpp:405-409,
+// electr3d.cpp:399-411), Joule heat sigma |grad V|^2 in W/m^3 (electr2d.cpp:457-461,
+// electr3d.cpp:459-475; zero where noheat[e]), and the Picard error inputs:
+// max |j - j_prev|^2 over all elements and max |j|^2 over the selected elements.
+__global__ __launch_bounds__(256) void k_currents(ElemGrid g, long long E, const double* __restrict__ hx,
+                                                  const double* __restrict__ hy, const double* __restrict__ hz,
+                                                  const double* __restrict__ V, const double* __restrict__ kh,
+                                                  const double* __restrict__ kv, const unsigned char* __restrict__ sel,
+                                                  const unsigned char* __restrict__ noheat, double* __restrict__ cur,
+                                                  double* __restrict__ heat, double* __restrict__ out,
+                                                  double* __restrict__ partials, unsigned int* counter) {
+    double dmax = 0., jmax = 0.;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        elem_coords(g, e, i, j, k);
+        double c[3] = {0., 0., 0.}, h = 0.;
+        if (g.dim == 2) {
+            double ll = V[node_of(g, i, j, 0)], lu = V[node_of(g, i + 1, j, 0)], ul = V[node_of(g, i, j + 1, 0)],
+                   uu = V[node_of(g, i + 1, j + 1, 0)];
+            double gx = (-ll + lu - ul + uu) / hx[i], gy = (-ll - lu + ul + uu) / hy[j];
+            c[0] = kh[e] * (-0.05 * gx);
+            c[1] = kv[e] * (-0.05 * gy);
+            double ex = 0.5e6 * gx, ey = 0.5e6 * gy;
+            h = kh[e] * ex * ex + kv[e] * ey * ey;
+        } else {
+            double sx = 0., sy = 0., sz = 0.;
+            for (int a = 0; a < 8; ++a) {
+                double v = V[node_of(g, i + (a & 1), j + ((a >> 1) & 1), k + ((a >> 2) & 1))];
+                sx += (a & 1) ? v : -v;
+                sy += (a & 2) ? v : -v;
+                sz += (a & 4) ? v : -v;
+            }
+            c[0] = -0.025 * kh[e] * sx / hx[i];
+            c[1] = -0.025 * kh[e] * sy / hy[j];
+            c[2] = -0.025 * kv[e] * sz / hz[k];
+            double ex = -0.25e6 * sx / hx[i], ey = -0.25e6 * sy / hy[j], ez = -0.25e6 * sz / hz[k];
+            h = kh[e] * ex * ex + kh[e] * ey * ey + kv[e] * ez * ez;
+        }
+        double dd = 0., jj = 0.;
+        for (int a = 0; a < g.dim; ++a) {
+            double o = cur[e * g.dim + a];
+            dd += (c[a] - o) * (c[a] - o);
+            jj += c[a] * c[a];
+            cur[e * g.dim + a] = c[a];
+        }
+        heat[e] = (noheat && noheat[e]) ? 0. : h;
+        dmax = fmax(dmax, dd);
+        if (!sel || sel[e]) jmax = fmax(jmax, jj);
+    }
+    double v[2] = {dmax, jmax}, tot[2];
+    if (grid_max<2>(v, partials, counter, tot)) {
+        out[0] = tot[0];
+        out[1] = tot[1];
+    }
 }
 
 }  // namespace pf

@@ -1,0 +1,261 @@
+#include "fem_solvers.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+namespace plask_amd {
+
+static std::string fmt(const char* f, ...) __attribute__((format(printf, 1, 2)));
+static std::string fmt(const char* f, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, f);
+    vsnprintf(buf, sizeof(buf), f, ap);
+    va_end(ap);
+    return buf;
+}
+
+FemCore::FemCore(int dim, bool cylindrical, int device, std::string id_) : id(std::move(id_)), dim_(dim), cyl_(cylindrical) {
+    if (dim != 2 && dim != 3) throw std::invalid_argument("dim must be 2 or 3");
+    int rc = pf_ctx_create(&ctx_, device, 0);
+    if (rc != PF_OK) throw std::runtime_error(id + ": no usable GPU device (pf_ctx_create " + std::to_string(rc) + ")");
+}
+
+FemCore::~FemCore() { pf_ctx_destroy(ctx_); }
+
+void FemCore::check(int rc, const char* what) {
+    if (rc == PF_OK) return;
+    std::string msg = id + ": " + what + ": " + pf_last_error(ctx_);
+    if (rc == PF_EBREAK || rc == PF_ENONFINITE || rc == PF_ENOCONV) throw ComputationError(msg);
+    throw std::runtime_error(msg);
+}
+
+void FemCore::set_mesh(const std::vector<std::vector<double>>& axes) {
+    if ((int)axes.size() != dim_) throw std::invalid_argument(id + ": mesh dimension does not match the solver");
+    const double* a2 = dim_ == 3 ? axes[2].data() : nullptr;
+    int64_t n2 = dim_ == 3 ? (int64_t)axes[2].size() : 1;
+    check(pf_grid_set(ctx_, dim_, cyl_ ? PF_GEOM_CYLINDRICAL : PF_GEOM_CARTESIAN, axes[0].data(), axes[0].size(),
+                      axes[1].data(), axes[1].size(), a2, n2),
+          "mesh");
+    N_ = pf_num_nodes(ctx_);
+    E_ = pf_num_elements(ctx_);
+    have_mesh_ = true;
+    initialized_ = false;
+}
+
+void FemCore::set_dirichlet(const std::vector<int64_t>& nodes, const std::vector<double>& values) {
+    if (nodes.size() != values.size()) throw std::invalid_argument("nodes and values differ in length");
+    check(pf_dirichlet_set(ctx_, nodes.data(), values.data(), (int64_t)nodes.size()), "boundary conditions");
+}
+
+std::vector<double> FemCore::solution() {
+    std::vector<double> x(N_);
+    check(pf_get_solution(ctx_, x.data()), "solution");
+    return x;
+}
+
+void FemCore::solve(double init_value) {
+    pf_solve_opts o{};
+    o.precision = iter.precision;
+    o.precond = iter.precond;
+    o.x0 = PF_X0_DEVICE;
+    o.maxit = iter.maxit;
+    o.rtol = iter.maxerr;
+    o.inner_rtol = iter.inner_rtol;
+    o.max_outer = 50;
+    o.check_every = 32;
+    o.norm = PF_NORM_INIT;
+    o.init_value = init_value;
+    pf_solve_stats st{};
+    log(LOG_DETAIL, id + ": Iterating linear system");
+    int rc = pf_solve(ctx_, &o, nullptr, &st);
+    iter.iters = st.iters;
+    iter.err = st.rel_resid;
+    iter.seconds = st.seconds;
+    if (rc == PF_ENOCONV) {
+        iter.converged = false;
+        // iterative_matrix.hpp:299-314
+        if (iter.noconv == 0)
+            throw ComputationError(fmt("%s: failed to converge in %d iterations (error %g)", id.c_str(), iter.maxit,
+                                       st.rel_resid));
+        if (iter.noconv == 1)
+            log(LOG_WARNING, fmt("%s: Failed to converge in %d iterations (error %g)", id.c_str(), iter.maxit, st.rel_resid));
+        else
+            log(LOG_DETAIL, fmt("%s: Did not converge yet in %d iterations (error %g)", id.c_str(), iter.maxit, st.rel_resid));
+        return;
+    }
+    check(rc, "solve");
+    iter.converged = true;
+    log(LOG_DETAIL, fmt("%s: Converged after %d iterations (error %g)", id.c_str(), st.iters, st.rel_resid));
+}
+
+// ---------------------------------------------------------------------------
+void ThermalCore::set_conductivity(const std::vector<double>& k0h, const std::vector<double>& k0v,
+                                   const std::vector<double>& ah, const std::vector<double>& av) {
+    if ((int64_t)k0h.size() != E_ || (int64_t)k0v.size() != E_ || (int64_t)ah.size() != E_ || (int64_t)av.size() != E_)
+        throw std::invalid_argument(id + ": conductivity arrays need one value per element");
+    k0h_ = k0h;
+    k0v_ = k0v;
+    ah_ = ah;
+    av_ = av;
+    tdep_ = std::any_of(ah.begin(), ah.end(), [](double a) { return a != 0.; }) ||
+            std::any_of(av.begin(), av.end(), [](double a) { return a != 0.; });
+    check(pf_coeffs_powerlaw(ctx_, PF_THERMAL, k0h.data(), k0v.data(), ah.data(), av.data(), E_), "conductivity");
+    kcb_ = nullptr;
+    if (!tdep_) check(pf_coeffs_set(ctx_, PF_THERMAL, k0h.data(), k0v.data(), E_), "conductivity");
+}
+
+void ThermalCore::set_conductivity_callback(std::function<void(const double*, double*, double*, int64_t)> cb) {
+    kcb_ = std::move(cb);
+}
+
+void ThermalCore::set_heat(const std::vector<double>& q) {
+    if (q.empty()) {
+        check(pf_source_set(ctx_, nullptr, 0), "heat");
+        return;
+    }
+    if ((int64_t)q.size() != E_) throw std::invalid_argument(id + ": heat array needs one value per element");
+    check(pf_source_set(ctx_, q.data(), E_), "heat");
+}
+
+// onInitialize: temperatures = inittemp, loopno = 0 (therm3d.cpp:71-80)
+void ThermalCore::initialize() {
+    if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
+    std::vector<double> t(N_, inittemp);
+    check(pf_solution_set(ctx_, t.data()), "initial temperature");
+    loopno = 0;
+    initialized_ = true;
+}
+
+double ThermalCore::compute(int loops) {
+    if (!initialized_) initialize();
+    log(LOG_INFO, id + ": Running thermal calculations");
+    int loop = 0;
+    double err = 0.;
+    toterr = 0.;
+    check(pf_solution_change(ctx_, nullptr, nullptr), "loop start");  // T_prev = T
+    do {
+        // setMatrix: element conductivities at the current element temperatures
+        if (kcb_) {
+            std::vector<double> tavg(E_), kh(E_), kv(E_);
+            check(pf_element_means(ctx_, tavg.data()), "element temperatures");
+            kcb_(tavg.data(), kh.data(), kv.data(), E_);
+            check(pf_coeffs_set(ctx_, PF_THERMAL, kh.data(), kv.data(), E_), "conductivity");
+        } else if (tdep_) {
+            check(pf_coeffs_update(ctx_), "conductivity");
+        }
+        solve(inittemp);
+        check(pf_solution_change(ctx_, &err, &maxT), "loop error");
+        if (err > toterr) toterr = err;
+        ++loopno;
+        ++loop;
+        log(LOG_RESULT, fmt("%s: Loop %d(%d): max(T) = %.3f K, error = %g K", id.c_str(), loop, loopno, maxT, err));
+    } while ((!iter.converged || err > maxerr) && (loops == 0 || loop < loops));
+    return toterr;
+}
+
+// ---------------------------------------------------------------------------
+void ElectricalCore::set_conductivity(const std::vector<double>& sh, const std::vector<double>& sv) {
+    if ((int64_t)sh.size() != E_ || (int64_t)sv.size() != E_)
+        throw std::invalid_argument(id + ": conductivity arrays need one value per element");
+    sh_ = sh;
+    sv_ = sv;
+}
+
+void ElectricalCore::set_junctions(const std::vector<int64_t>& elems, int ncorner, const std::vector<int64_t>& bottom,
+                                   const std::vector<int64_t>& top, const std::vector<int>& junction,
+                                   const std::vector<double>& heights) {
+    check(pf_junctions_set(ctx_, (int64_t)elems.size(), ncorner, elems.data(), bottom.data(), top.data(),
+                           junction.data(), (int)heights.size(), heights.data()),
+          "junctions");
+    jelems_ = elems;
+    jcond_.assign(elems.size(), start_cond_);
+}
+
+void ElectricalCore::set_junction_params(const std::vector<double>& beta, const std::vector<double>& js) {
+    check(pf_junction_params(ctx_, beta.data(), js.data(), (int64_t)beta.size()), "junction parameters");
+}
+
+void ElectricalCore::set_flags(const std::vector<unsigned char>& select, const std::vector<unsigned char>& noheat) {
+    check(pf_element_flags_set(ctx_, select.empty() ? nullptr : select.data(), noheat.empty() ? nullptr : noheat.data()),
+          "element flags");
+    have_flags_ = true;
+}
+
+// onInitialize: potentials = 0, loopno = 0, junction conductivity = start value (electr2d.cpp:171-187)
+void ElectricalCore::initialize() {
+    if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
+    std::vector<double> v(N_, 0.);
+    check(pf_solution_set(ctx_, v.data()), "initial potential");
+    std::vector<double> z(E_ * dim_, 0.);
+    loopno = 0;
+    jcond_.assign(jelems_.size(), start_cond_);
+    initialized_ = true;
+}
+
+double ElectricalCore::compute(int loops) {
+    if (!initialized_) initialize();
+    if ((int64_t)sh_.size() != E_) throw std::runtime_error(id + ": conductivities not set");
+    log(LOG_INFO, id + ": Running electrical calculations");
+    // loadConductivities (electr2d.cpp:331-353): materials, then the saved junction values
+    std::vector<double> kh = sh_, kv = sv_;
+    for (size_t q = 0; q < jelems_.size(); ++q) {
+        kh[jelems_[q]] = 0.;
+        kv[jelems_[q]] = (std::isnan(jcond_[q]) || std::abs(jcond_[q]) < 1e-16) ? 1e-16 : jcond_[q];
+    }
+    check(pf_coeffs_set(ctx_, PF_ELECTRICAL, kh.data(), kv.data(), E_), "conductivity");
+    const bool noactive = jelems_.empty();
+    const double minj = 100e-7;  // electr2d.cpp:391
+    unsigned loop = 0;
+    double err = 0.;
+    toterr = 0.;
+    do {
+        if (loopno != 0) check(pf_junction_update(ctx_, stable ? 1 : 0), "junction update");
+        solve(0.);
+        double dj2 = 0., j2 = 0.;
+        check(pf_currents(ctx_, &dj2, &j2), "currents");
+        double mcur = std::sqrt(j2);
+        err = 100. * std::sqrt(dj2) / std::max(mcur, minj);
+        if ((loop != 0 || mcur >= minj) && err > toterr) toterr = err;
+        maxcur = mcur;
+        ++loopno;
+        ++loop;
+        log(LOG_RESULT, fmt("%s: Loop %d(%d): max(j%s) = %g kA/cm2, error = %g%%", id.c_str(), loop, loopno,
+                            noactive ? "" : "@junc", mcur, err));
+    } while ((!iter.converged || err > maxerr) && (loops == 0 || loop < (unsigned)loops));
+    // saveConductivities (electr2d.cpp:355-361)
+    if (!jelems_.empty()) {
+        std::vector<double> cv(E_);
+        check(pf_get_coeffs(ctx_, nullptr, cv.data()), "conductivity");
+        for (size_t q = 0; q < jelems_.size(); ++q) jcond_[q] = cv[jelems_[q]];
+    }
+    return toterr;
+}
+
+std::vector<double> ElectricalCore::currents() {
+    std::vector<double> c(E_ * dim_);
+    check(pf_get_currents(ctx_, c.data(), nullptr), "currents");
+    return c;
+}
+
+std::vector<double> ElectricalCore::heats() {
+    std::vector<double> h(E_);
+    check(pf_get_currents(ctx_, nullptr, h.data()), "heats");
+    return h;
+}
+
+std::vector<double> ElectricalCore::conductivity_h() {
+    std::vector<double> k(E_);
+    check(pf_get_coeffs(ctx_, k.data(), nullptr), "conductivity");
+    return k;
+}
+
+std::vector<double> ElectricalCore::conductivity_v() {
+    std::vector<double> k(E_);
+    check(pf_get_coeffs(ctx_, nullptr, k.data()), "conductivity");
+    return k;
+}
+
+}  // namespace plask_amd

@@ -1,0 +1,137 @@
+// Host solver cores of the thermal.static and electrical.shockley path, written over
+// the C ABI of include/plask_fem.h.  They keep the reference's compute(loops) loop
+// semantics; geometry, materials and meshes are resolved by the caller into
+// per-element arrays (natural element order, axis 0 fastest).
+#pragma once
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/plask_fem.h"
+
+namespace plask_amd {
+
+// Log levels of plask/log/log.hpp:27-38 that the solvers use.
+enum LogLevel { LOG_CRITICAL_ERROR = 0, LOG_ERROR, LOG_ERROR_DETAIL, LOG_WARNING, LOG_IMPORTANT, LOG_INFO, LOG_RESULT,
+                LOG_DATA, LOG_DETAIL, LOG_DEBUG };
+
+// Iteration parameters and results (IterativeMatrixParams, plask/common/fem/iterative_matrix.hpp:25-94),
+// reinterpreted for the GPU PCG: maxit, maxerr (relative residual), noconv; outputs converged/iters/err.
+struct IterParams {
+    int maxit = 100000;
+    double maxerr = 1e-8;
+    int precond = 2;    // 1 Jacobi, 2 line (layer-axis) block Jacobi
+    int precision = 0;  // PF_FP64 or PF_FP32_REFINE
+    double inner_rtol = 1e-4;
+    int noconv = 1;     // 0 error, 1 warning, 2 continue (NoConvergenceBehavior)
+    bool converged = true;
+    int iters = 0;
+    double err = 0.;
+    double seconds = 0.;
+};
+
+class ComputationError : public std::runtime_error {
+  public:
+    using std::runtime_error::runtime_error;
+};
+
+class FemCore {
+  public:
+    FemCore(int dim, bool cylindrical, int device, std::string id);
+    virtual ~FemCore();
+    FemCore(const FemCore&) = delete;
+
+    void set_mesh(const std::vector<std::vector<double>>& axes);
+    void set_dirichlet(const std::vector<int64_t>& nodes, const std::vector<double>& values);
+    int64_t num_nodes() const { return N_; }
+    int64_t num_elements() const { return E_; }
+    int dim() const { return dim_; }
+    bool cylindrical() const { return cyl_; }
+    std::vector<double> solution();  // node values (natural order)
+    void invalidate() { initialized_ = false; }
+    bool initialized() const { return initialized_; }
+
+    IterParams iter;
+    std::function<void(int, const std::string&)> logger;
+    std::string id;
+
+  protected:
+    void check(int rc, const char* what);
+    void log(int level, const std::string& msg) const {
+        if (logger) logger(level, msg);
+    }
+    // one linear solve from the device solution; fills iter.*
+    void solve(double init_value);
+
+    pf_ctx* ctx_ = nullptr;
+    int dim_;
+    bool cyl_;
+    int64_t N_ = 0, E_ = 0;
+    bool initialized_ = false;
+    bool have_mesh_ = false;
+};
+
+// ThermalFem2DSolver / ThermalFem3DSolver compute loop (solvers/thermal/static/therm2d.cpp:443-502,
+// therm3d.cpp:281-340).  Element conductivity: k0 * (300 / T_avg)^alpha per tensor component
+// (device-evaluated), or a host callback k(T_avg) for general materials.
+class ThermalCore : public FemCore {
+  public:
+    ThermalCore(int dim, bool cylindrical, int device, std::string id) : FemCore(dim, cylindrical, device, id) {}
+
+    void set_conductivity(const std::vector<double>& k0h, const std::vector<double>& k0v,
+                          const std::vector<double>& ah, const std::vector<double>& av);
+    // callback(T_avg[E], kh[E], kv[E]) for materials the device law cannot express
+    void set_conductivity_callback(std::function<void(const double*, double*, double*, int64_t)> cb);
+    void set_heat(const std::vector<double>& q);  // W/m^3 per element (empty = none)
+    double compute(int loops);
+    void initialize();
+
+    double inittemp = 300.;
+    double maxerr = 0.05;  // K
+    int loopno = 0;
+    double toterr = 0., maxT = 0.;
+
+  private:
+    std::vector<double> k0h_, k0v_, ah_, av_;
+    bool tdep_ = false;
+    std::function<void(const double*, double*, double*, int64_t)> kcb_;
+};
+
+// ElectricalFem2DSolver / ElectricalFem3DSolver + BetaSolver compute loop
+// (solvers/electrical/shockley/electr2d.cpp:363-440, electr3d.cpp:356-442; beta.hpp:43-46).
+class ElectricalCore : public FemCore {
+  public:
+    ElectricalCore(int dim, bool cylindrical, int device, std::string id) : FemCore(dim, cylindrical, device, id) {}
+
+    // material conductivities (S/m) for every element; junction entries are ignored
+    void set_conductivity(const std::vector<double>& sh, const std::vector<double>& sv);
+    // junction elements: index, node pairs (ncorner bottom then ncorner top per element), junction number
+    void set_junctions(const std::vector<int64_t>& elems, int ncorner, const std::vector<int64_t>& bottom,
+                       const std::vector<int64_t>& top, const std::vector<int>& junction,
+                       const std::vector<double>& heights);
+    void set_junction_params(const std::vector<double>& beta, const std::vector<double>& js);
+    void set_flags(const std::vector<unsigned char>& select, const std::vector<unsigned char>& noheat);
+    void set_start_cond(double cond) { start_cond_ = cond; }
+    double compute(int loops);
+    void initialize();
+    std::vector<double> currents();  // E * dim, kA/cm^2
+    std::vector<double> heats();     // E, W/m^3
+    std::vector<double> conductivity_h();
+    std::vector<double> conductivity_v();
+
+    double maxerr = 0.05;  // %
+    bool stable = false;   // CONVERGENCE_STABLE
+    int loopno = 0;
+    double toterr = 0., maxcur = 0.;
+
+  private:
+    std::vector<double> sh_, sv_;
+    std::vector<int64_t> jelems_;
+    std::vector<double> jcond_;  // saved junction conductivity per junction element
+    double start_cond_ = 5.;     // default_junction_conductivity c11 (electr2d.cpp:24)
+    bool have_flags_ = false;
+};
+
+}  // namespace plask_amd

@@ -54,6 +54,9 @@ struct pf_ctx {
     // vectors
     double *p2 = nullptr;
     float *p2_32 = nullptr;
+    // line preconditioner factors (a, w, c')
+    double *la = nullptr, *lw = nullptr, *lcp = nullptr;
+    float *la32 = nullptr, *lw32 = nullptr, *lcp32 = nullptr;
     double *x = nullptr, *r = nullptr, *p = nullptr, *qv = nullptr, *z = nullptr, *dinv = nullptr, *F = nullptr,
            *tmp = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *dinv32 = nullptr;
@@ -70,12 +73,26 @@ struct pf_ctx {
     int upd_blocks = 1024;
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
-    int graph_chunk = 0;
+    int graph_chunk = 0, graph_precond = 0;
 
     // profiling
     bool profile = false;
     double kt[2] = {0., 0.};
     int64_t kc[2] = {0, 0};
+
+    // Picard-loop state
+    double *k0h = nullptr, *k0v = nullptr, *ah = nullptr, *av = nullptr;
+    bool have_law = false;
+    double* xprev = nullptr;
+    double *cur = nullptr, *heat = nullptr, *emean = nullptr;
+    unsigned char *sel = nullptr, *noheat = nullptr;
+    bool have_sel = false, have_noheat = false;
+    double* dout = nullptr;  // small device results
+    long long *je = nullptr, *jbot = nullptr, *jtop = nullptr;
+    int* jid = nullptr;
+    double *jheight = nullptr, *jbeta = nullptr, *jjs = nullptr;
+    int64_t nje = 0;
+    int jcorner = 0;
 
     std::vector<void*> allocs;
 };
@@ -222,10 +239,12 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     // vectors
     const size_t N = ctx->N;
     int rc;
-    double** v64[] = {&ctx->x, &ctx->r, &ctx->p, &ctx->p2, &ctx->qv, &ctx->z, &ctx->dinv, &ctx->F, &ctx->tmp};
+    double** v64[] = {&ctx->x, &ctx->r, &ctx->p, &ctx->p2, &ctx->qv, &ctx->z, &ctx->dinv, &ctx->F, &ctx->tmp,
+                      &ctx->la, &ctx->lw, &ctx->lcp};
     for (auto pp : v64)
         if ((rc = dalloc(ctx, pp, N))) return rc;
-    float** v32[] = {&ctx->x32, &ctx->r32, &ctx->p32, &ctx->p2_32, &ctx->q32, &ctx->z32, &ctx->dinv32};
+    float** v32[] = {&ctx->x32, &ctx->r32, &ctx->p32, &ctx->p2_32, &ctx->q32, &ctx->z32, &ctx->dinv32,
+                     &ctx->la32, &ctx->lw32, &ctx->lcp32};
     for (auto pp : v32)
         if ((rc = dalloc(ctx, pp, N))) return rc;
     if ((rc = dalloc(ctx, &ctx->kh, ctx->E))) return rc;
@@ -234,6 +253,15 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     if ((rc = dalloc(ctx, &ctx->kh32, ctx->E))) return rc;
     if ((rc = dalloc(ctx, &ctx->kv32, ctx->E))) return rc;
     if ((rc = dalloc(ctx, &ctx->dmask, (N + 63) / 64))) return rc;
+    if ((rc = dalloc(ctx, &ctx->xprev, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->cur, ctx->E * dim))) return rc;
+    if ((rc = dalloc(ctx, &ctx->heat, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->emean, ctx->E))) return rc;
+    if ((rc = dalloc(ctx, &ctx->dout, 8))) return rc;
+    HIPCK(hipMemset(ctx->cur, 0, sizeof(double) * ctx->E * dim));
+    HIPCK(hipMemset(ctx->xprev, 0, sizeof(double) * N));
+    ctx->have_law = ctx->have_sel = ctx->have_noheat = false;
+    ctx->nje = 0;
     HIPCK(hipMemset(ctx->dmask, 0, sizeof(uint64_t) * ((N + 63) / 64)));
     HIPCK(hipMemset(ctx->q, 0, sizeof(double) * ctx->E));
     HIPCK(hipMemset(ctx->x, 0, sizeof(double) * N));
@@ -259,7 +287,7 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     } else {
         napply = (size_t)((n1 + ctx->KC2 - 1) / ctx->KC2);
     }
-    ctx->npartials = 2 * std::max<size_t>(napply, 2048);
+    ctx->npartials = 2 * std::max<size_t>(napply, 4096);
     if ((rc = dalloc(ctx, &ctx->partials, ctx->npartials))) return rc;
     ctx->dirty = true;
     return PF_OK;
@@ -399,12 +427,22 @@ static int prepare(pf_ctx* ctx) {
                            ctx->dmask, ctx->dinv);
         hipLaunchKernelGGL((k_diag3d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
+        const int cb = grid1(ctx->n[0] * ctx->n[1], 64, 1 << 20);
+        hipLaunchKernelGGL((k_line_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
+                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+        hipLaunchKernelGGL((k_line_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
+                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
     } else {
         hipLaunchKernelGGL(k_load2d, dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->q, ctx->F);
         hipLaunchKernelGGL((k_diag2d<double>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv);
         hipLaunchKernelGGL((k_diag2d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
+        const int cb = grid1(ctx->n[0], 64, 1 << 20);
+        hipLaunchKernelGGL((k_line_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
+                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+        hipLaunchKernelGGL((k_line_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
+                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
     }
     if (ctx->equation == PF_ELECTRICAL || !ctx->have_source) {
         // electrical: no load vector (electr2d.cpp:268-317); thermal without source: zero
@@ -428,16 +466,34 @@ static int prepare(pf_ctx* ctx) {
 
 // One CG run on (x, r already holding b - A x).  T selects FP64 or FP32 vectors.
 template <typename T>
-static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* kh, const T* kv,
-                  double stop2, int maxit, int check_every, hipGraphExec_t* gcache, int* iters_out, int* status_out) {
+static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* la,
+                  const T* lw, const T* lcp, const T* kh, const T* kv, double stop2, int maxit, int check_every,
+                  hipGraphExec_t* gcache, int* iters_out, int* status_out) {
     const long long N = ctx->N;
     PcgState init{};
     init.stop2 = stop2;
     init.maxit = maxit;
     *ctx->hst = init;  // pinned staging: the host does not touch it again before the next sync
     HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
-                       ctx->partials);
+    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+    const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
+    const long long plane = ncol;
+    const int lblocks = (ncol + 63) / 64;
+    const bool line = precond == 2;
+    if (line)
+        hipLaunchKernelGGL((k_cg_line<T, true>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, x,
+                           (const T*)nullptr, r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
+    else
+        hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
+                           ctx->partials);
+    auto update = [&](T* pout) {
+        if (line)
+            hipLaunchKernelGGL((k_cg_line<T, false>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, x,
+                               (const T*)pout, r, (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
+        else
+            hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q,
+                               dinv, z, ctx->st, ctx->partials);
+    };
     // both direction buffers start at zero (beta = 0 on the first iteration)
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p, T(0));
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p2, T(0));
@@ -448,8 +504,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* din
         T* pin = pb[t & 1];
         T* pout = pb[(t + 1) & 1];
         launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
-        hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q, dinv,
-                           z, ctx->st, ctx->partials);
+        update(pout);
     };
     int chunk = check_every > 0 ? check_every : 32;
     chunk += chunk & 1;  // even: each graph replay starts on the same buffer parity
@@ -464,8 +519,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* din
                 hipEventRecord(ev[0], ctx->stream);
                 launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
                 hipEventRecord(ev[1], ctx->stream);
-                hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q,
-                                   dinv, z, ctx->st, ctx->partials);
+                update(pout);
                 hipEventRecord(ev[2], ctx->stream);
                 HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
                 HIPCK(hipStreamSynchronize(ctx->stream));
@@ -482,7 +536,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* din
         }
         for (auto& e : ev) hipEventDestroy(e);
     } else {
-        if (!*gcache || ctx->graph_chunk != chunk) {
+        if (!*gcache || ctx->graph_chunk != chunk || ctx->graph_precond != precond) {
             if (*gcache) hipGraphExecDestroy(*gcache);
             *gcache = nullptr;
             hipGraph_t graph;
@@ -492,6 +546,7 @@ static int run_cg(pf_ctx* ctx, T* x, T* r, T* p, T* p2, T* q, T* z, const T* din
             HIPCK(hipGraphInstantiate(gcache, graph, nullptr, nullptr, 0));
             hipGraphDestroy(graph);
             ctx->graph_chunk = chunk;
+            ctx->graph_precond = precond;
         }
         for (;;) {
             HIPCK(hipGraphLaunch(*gcache, ctx->stream));
@@ -562,6 +617,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     }
     if (!(bnorm > 0.)) bnorm = 1.;
     const double rtol = opts->rtol > 0. ? opts->rtol : 1e-8;
+    const int precond = opts->precond == 1 ? 1 : 2;  // 1 Jacobi, 2 (default) line
     const int maxit = opts->maxit > 0 ? opts->maxit : 100000;
     int total_iters = 0, outer = 0, status = 0;
     double rn = 0.;
@@ -571,7 +627,8 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
         for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
             int it = 0, stt = 0;
             double stop2 = (rtol * bnorm) * (rtol * bnorm);
-            if ((rc = run_cg<double>(ctx, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->kh, ctx->kv, stop2,
+            if ((rc = run_cg<double>(ctx, precond, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->la,
+                                     ctx->lw, ctx->lcp, ctx->kh, ctx->kv, stop2,
                                      maxit - total_iters, opts->check_every, &ctx->graph64, &it, &stt)))
                 return rc;
             total_iters += it;
@@ -591,8 +648,8 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
             int it = 0, stt = 0;
             // the target for this step: reduce by `inner`, but not beyond what the outer tolerance needs
             double goal = std::max(inner, 0.5 * rtol * bnorm / rn);
-            if ((rc = run_cg<float>(ctx, ctx->x32, ctx->r32, ctx->p32, ctx->p2_32, ctx->q32, ctx->z32, ctx->dinv32, ctx->kh32,
-                                    ctx->kv32, goal * goal, maxit - total_iters, opts->check_every, &ctx->graph32, &it,
+            if ((rc = run_cg<float>(ctx, precond, ctx->x32, ctx->r32, ctx->p32, ctx->p2_32, ctx->q32, ctx->z32, ctx->dinv32,
+                                    ctx->la32, ctx->lw32, ctx->lcp32, ctx->kh32, ctx->kv32, goal * goal, maxit - total_iters, opts->check_every, &ctx->graph32, &it,
                                     &stt)))
                 return rc;
             total_iters += it;
@@ -689,6 +746,199 @@ int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
         if (seconds) seconds[i] = ctx->kc[i] ? ctx->kt[i] / ctx->kc[i] : 0.;
         if (counts) counts[i] = ctx->kc[i];
     }
+    return PF_OK;
+}
+
+static ElemGrid egrid(pf_ctx* ctx) {
+    return ElemGrid{ctx->dim, (int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2]};
+}
+
+int pf_solution_set(pf_ctx* ctx, const double* x) {
+    if (!ctx || !x) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipMemcpy(ctx->x, x, sizeof(double) * ctx->N, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->xprev, ctx->x, sizeof(double) * ctx->N, hipMemcpyDeviceToDevice));
+    ctx->have_solution = true;
+    return PF_OK;
+}
+
+int pf_solution_change(pf_ctx* ctx, double* max_change, double* max_value) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no solution on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_delta_stats, dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, (long long)ctx->N, ctx->x,
+                       ctx->xprev, ctx->dout, ctx->partials, &ctx->st->counter[3]);
+    HIPCK(hipGetLastError());
+    double h[2];
+    HIPCK(hipMemcpyAsync(h, ctx->dout, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (max_change) *max_change = h[0];
+    if (max_value) *max_value = h[1];
+    return PF_OK;
+}
+
+int pf_element_means(pf_ctx* ctx, double* out) {
+    if (!ctx || !out) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no solution on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_elem_mean, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, egrid(ctx),
+                       (long long)ctx->E, ctx->x, ctx->emean);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, ctx->emean, sizeof(double) * ctx->E, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+}
+
+int pf_coeffs_powerlaw(pf_ctx* ctx, int equation, const double* k0_h, const double* k0_v, const double* a_h,
+                       const double* a_v, int64_t nelem) {
+    if (!ctx || !k0_h || !k0_v || !a_h || !a_v) return PF_EINVAL;
+    if (nelem != ctx->E) return fail(ctx, PF_EINVAL, "law arrays must have one value per element");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if (!ctx->k0h) {
+        if ((rc = dalloc(ctx, &ctx->k0h, ctx->E))) return rc;
+        if ((rc = dalloc(ctx, &ctx->k0v, ctx->E))) return rc;
+        if ((rc = dalloc(ctx, &ctx->ah, ctx->E))) return rc;
+        if ((rc = dalloc(ctx, &ctx->av, ctx->E))) return rc;
+    }
+    HIPCK(hipMemcpy(ctx->k0h, k0_h, sizeof(double) * nelem, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->k0v, k0_v, sizeof(double) * nelem, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->ah, a_h, sizeof(double) * nelem, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->av, a_v, sizeof(double) * nelem, hipMemcpyHostToDevice));
+    ctx->equation = equation;
+    ctx->have_law = true;
+    return PF_OK;
+}
+
+int pf_coeffs_update(pf_ctx* ctx) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->have_law) return fail(ctx, PF_ESTATE, "pf_coeffs_powerlaw first");
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no solution (temperature) on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_powerlaw, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, egrid(ctx),
+                       (long long)ctx->E, ctx->x, ctx->k0h, ctx->k0v, ctx->ah, ctx->av, ctx->kh, ctx->kv, ctx->kh32,
+                       ctx->kv32);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->have_coeffs = true;
+    ctx->dirty = true;
+    return PF_OK;
+}
+
+int pf_get_coeffs(pf_ctx* ctx, double* k_inplane, double* k_vert) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    if (k_inplane) HIPCK(hipMemcpy(k_inplane, ctx->kh, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
+    if (k_vert) HIPCK(hipMemcpy(k_vert, ctx->kv, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+int pf_junctions_set(pf_ctx* ctx, int64_t nje, int ncorner, const int64_t* elems, const int64_t* bottom_nodes,
+                     const int64_t* top_nodes, const int* junction, int nj, const double* heights) {
+    if (!ctx) return PF_EINVAL;
+    if (nje < 0 || (nje > 0 && (!elems || !bottom_nodes || !top_nodes || !junction || !heights)) || ncorner < 1 ||
+        ncorner > 4)
+        return fail(ctx, PF_EINVAL, "bad junction description");
+    HIPCK(hipSetDevice(ctx->device));
+    for (int64_t q = 0; q < nje; ++q) {
+        if (elems[q] < 0 || elems[q] >= ctx->E || junction[q] < 0 || junction[q] >= nj)
+            return fail(ctx, PF_EINVAL, "junction element out of range");
+        for (int c = 0; c < ncorner; ++c)
+            if (bottom_nodes[q * ncorner + c] < 0 || bottom_nodes[q * ncorner + c] >= ctx->N ||
+                top_nodes[q * ncorner + c] < 0 || top_nodes[q * ncorner + c] >= ctx->N)
+                return fail(ctx, PF_EINVAL, "junction node out of range");
+    }
+    for (int n = 0; n < nj; ++n)
+        if (!(heights[n] > 0.)) return fail(ctx, PF_EINVAL, "junction height must be positive");
+    int rc;
+    size_t m = std::max<int64_t>(1, nje);
+    if ((rc = dalloc(ctx, &ctx->je, m))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jbot, m * ncorner))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jtop, m * ncorner))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jid, m))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jheight, std::max(1, nj)))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jbeta, m))) return rc;
+    if ((rc = dalloc(ctx, &ctx->jjs, m))) return rc;
+    if (nje > 0) {
+        HIPCK(hipMemcpy(ctx->je, elems, sizeof(long long) * nje, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->jbot, bottom_nodes, sizeof(long long) * nje * ncorner, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->jtop, top_nodes, sizeof(long long) * nje * ncorner, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->jid, junction, sizeof(int) * nje, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->jheight, heights, sizeof(double) * nj, hipMemcpyHostToDevice));
+    }
+    ctx->nje = nje;
+    ctx->jcorner = ncorner;
+    return PF_OK;
+}
+
+int pf_junction_params(pf_ctx* ctx, const double* beta, const double* js, int64_t nje) {
+    if (!ctx || !beta || !js) return PF_EINVAL;
+    if (nje != ctx->nje) return fail(ctx, PF_EINVAL, "one beta and js per junction element");
+    for (int64_t q = 0; q < nje; ++q)
+        if (!(js[q] > 0.) || !std::isfinite(beta[q])) return fail(ctx, PF_EINVAL, "js must be > 0 and beta finite");
+    HIPCK(hipSetDevice(ctx->device));
+    if (nje > 0) {
+        HIPCK(hipMemcpy(ctx->jbeta, beta, sizeof(double) * nje, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->jjs, js, sizeof(double) * nje, hipMemcpyHostToDevice));
+    }
+    return PF_OK;
+}
+
+int pf_junction_update(pf_ctx* ctx, int stable) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no potential on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    if (ctx->nje > 0) {
+        hipLaunchKernelGGL(k_junction_update, dim3(grid1(ctx->nje, 256, 4096)), dim3(256), 0, ctx->stream,
+                           (long long)ctx->nje, ctx->jcorner, ctx->je, ctx->jbot, ctx->jtop, ctx->jid, ctx->jheight,
+                           ctx->jbeta, ctx->jjs, stable, ctx->x, ctx->kh, ctx->kv, ctx->kh32, ctx->kv32);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(ctx->stream));
+        ctx->dirty = true;
+    }
+    return PF_OK;
+}
+
+int pf_element_flags_set(pf_ctx* ctx, const unsigned char* select, const unsigned char* noheat) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if (select) {
+        if (!ctx->sel && (rc = dalloc(ctx, &ctx->sel, ctx->E))) return rc;
+        HIPCK(hipMemcpy(ctx->sel, select, ctx->E, hipMemcpyHostToDevice));
+    }
+    if (noheat) {
+        if (!ctx->noheat && (rc = dalloc(ctx, &ctx->noheat, ctx->E))) return rc;
+        HIPCK(hipMemcpy(ctx->noheat, noheat, ctx->E, hipMemcpyHostToDevice));
+    }
+    ctx->have_sel = select != nullptr;
+    ctx->have_noheat = noheat != nullptr;
+    return PF_OK;
+}
+
+int pf_currents(pf_ctx* ctx, double* max_dj2, double* max_j2) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no potential on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_currents, dim3(grid1(ctx->E, 256, 4096)), dim3(256), 0, ctx->stream, egrid(ctx),
+                       (long long)ctx->E, ctx->hx, ctx->hy, ctx->hz, ctx->x, ctx->kh, ctx->kv,
+                       ctx->have_sel ? ctx->sel : nullptr, ctx->have_noheat ? ctx->noheat : nullptr, ctx->cur,
+                       ctx->heat, ctx->dout + 2, ctx->partials, &ctx->st->counter[3]);
+    HIPCK(hipGetLastError());
+    double h[2];
+    HIPCK(hipMemcpyAsync(h, ctx->dout + 2, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (max_dj2) *max_dj2 = h[0];
+    if (max_j2) *max_j2 = h[1];
+    return PF_OK;
+}
+
+int pf_get_currents(pf_ctx* ctx, double* cur, double* heat) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    if (cur) HIPCK(hipMemcpy(cur, ctx->cur, sizeof(double) * ctx->E * ctx->dim, hipMemcpyDeviceToHost));
+    if (heat) HIPCK(hipMemcpy(heat, ctx->heat, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
     return PF_OK;
 }
 

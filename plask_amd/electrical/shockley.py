@@ -1,0 +1,317 @@
+"""``electrical.shockley``: Shockley-junction FEM solvers Shockley2D, ShockleyCyl, Shockley3D.
+
+Same classes and attributes as the reference module (solvers/electrical/shockley/
+python/electr_python.cpp:195-297): compute(loops=0), get_total_current(nact=0), err,
+inTemperature, outVoltage, outCurrentDensity, outHeat, outConductivity,
+voltage_boundary, maxerr, convergence, pcond, ncond, start_cond (pnjcond), beta/beta#,
+js/js# (float or callable of T), get_electrostatic_energy, get_capacitance,
+get_total_heat.  outPotential raises TypeError as in the reference.
+"""
+import math
+import re
+
+import numpy as np
+
+from .. import mesh as _mesh
+from .. import solverbase as _sb
+from .._core import ElectricalCore
+
+EPSILON0 = 8.854187817e-12  # F/m (plask/phys/constants.hpp)
+
+
+def _active_number(roles):
+    """Junction number + 1 from 'active'/'active#'/'junction'/'junction#' roles, else 0
+    (ElectricalFem2DSolver::isActive, electr2d.hpp:132-155)."""
+    no = 0
+    for r in roles:
+        m = re.fullmatch(r"(active|junction)(\d*)", r)
+        if not m:
+            continue
+        if no:
+            raise ValueError("multiple 'active'/'junction' roles specified")
+        no = int(m.group(2)) + 1 if m.group(2) else 1
+    return no
+
+
+class _ShockleyBase(_sb.FemSolver):
+    _core_class = ElectricalCore
+    inTemperature = _sb.ReceiverAttr(300.0)
+
+    def __init__(self, name=""):
+        object.__setattr__(self, "_beta", {})
+        object.__setattr__(self, "_js", {})
+        super().__init__(name)
+        self.voltage_boundary = _mesh.BoundaryConditions()
+        self.maxerr = 0.05
+        self.convergence = "fast"
+        self.pcond = 5.0
+        self.ncond = 50.0
+        self._start_cond = (0.0, 5.0)
+
+    # beta#, js# (BetaSolver, beta.hpp:49-77; Python callables electr_python.cpp:96-111)
+    def __getattr__(self, attr):
+        m = re.fullmatch(r"(beta|js)(\d*)", attr)
+        if m:
+            d = object.__getattribute__(self, "_" + m.group(1))
+            n = int(m.group(2)) if m.group(2) else 0
+            if n not in d:
+                raise AttributeError("%s: no %s given for junction %d" % (self.id, m.group(1), n))
+            return d[n]
+        raise AttributeError("'%s' object has no attribute '%s'" % (type(self).__name__, attr))
+
+    def __setattr__(self, attr, value):
+        m = re.fullmatch(r"(beta|js)(\d*)", attr)
+        if m:
+            if not (callable(value) or isinstance(value, (int, float))):
+                raise TypeError("%s: %s must be a float or a callable" % (self.id, attr))
+            n = int(m.group(2)) if m.group(2) else 0
+            getattr(self, "_" + m.group(1))[n] = value
+            self.invalidate()
+            return
+        object.__setattr__(self, attr, value)
+
+    @property
+    def start_cond(self):
+        return self._start_cond
+
+    @start_cond.setter
+    def start_cond(self, v):
+        self._start_cond = (0.0, float(v)) if np.isscalar(v) else (float(v[0]), float(v[1]))
+        self.invalidate()
+
+    pnjcond = start_cond
+
+    @property
+    def outPotential(self):
+        raise TypeError("%s: 'outPotential' is reserved for drift-diffusion model; use 'outVoltage' instead" % self.id)
+
+    # -- initialization -----------------------------------------------------------
+    def _on_initialize(self):
+        c = self._core
+        m = [len(a) - 1 for a in self._axes]
+        E = int(np.prod(m))
+        act = np.array([_active_number(r) for r in self._roles], dtype=np.int64)
+        self._act = act
+        shape = m[::-1]  # [k][j][i] or [j][i]
+        actg = act.reshape(shape)
+        nj = int(act.max()) if E else 0
+        z = self._axes[-1]
+        n0 = len(self._axes[0])
+        n1 = len(self._axes[1]) if self.DIM == 3 else 1
+        plane = n0 * n1
+        elems, bot, top, jid, heights, self._junctions = [], [], [], [], [], []
+        for n in range(1, nj + 1):
+            where = np.argwhere(actg == n)
+            if len(where) == 0:
+                heights.append(1.0)
+                self._junctions.append(None)
+                continue
+            kb, kt = int(where[:, 0].min()), int(where[:, 0].max()) + 1
+            h = float(z[kt] - z[kb])
+            heights.append(h)
+            self._junctions.append((kb, kt, h))
+            for w in where:
+                if self.DIM == 2:
+                    k, i = int(w[0]), int(w[1])
+                    e = i + m[0] * k
+                    cols = [i, i + 1]
+                else:
+                    k, j, i = int(w[0]), int(w[1]), int(w[2])
+                    e = i + m[0] * (j + m[1] * k)
+                    cols = [i + n0 * j, i + 1 + n0 * j, i + n0 * (j + 1), i + 1 + n0 * (j + 1)]
+                elems.append(e)
+                bot.extend([cc + kb * plane for cc in cols])
+                top.extend([cc + kt * plane for cc in cols])
+                jid.append(n - 1)
+        self._jelems = np.array(elems, dtype=np.int64)
+        self._jid = np.array(jid, dtype=np.int32)
+        c.set_start_cond(float(self._start_cond[1]))
+        c.set_junctions(self._jelems, 2 if self.DIM == 2 else 4, np.array(bot, dtype=np.int64),
+                        np.array(top, dtype=np.int64), list(self._jid), heights)
+        noheat = np.array([(mm is None or getattr(mm, "kind", "") == "empty" or "noheat" in r)
+                           for mm, r in zip(self._mat, self._roles)], dtype=np.uint8)
+        sel = (act > 0).astype(np.uint8) if len(elems) else np.zeros(0, np.uint8)
+        c.set_flags(sel, noheat)
+        c.initialize()
+
+    def _load_conductivities(self, T):
+        """Element conductivities (loadConductivities, electr2d.cpp:331-353)."""
+        E = len(self._mat)
+        sh = np.empty(E)
+        sv = np.empty(E)
+        for e, (mm, r) in enumerate(zip(self._mat, self._roles)):
+            if self._act[e]:
+                sh[e], sv[e] = 0.0, 0.0
+            elif "p-contact" in r:
+                sh[e] = sv[e] = self.pcond
+            elif "n-contact" in r:
+                sh[e] = sv[e] = self.ncond
+            elif mm is None:
+                sh[e] = sv[e] = 0.0
+            else:
+                v = mm.cond(float(T[e]))
+                sh[e], sv[e] = (v, v) if np.isscalar(v) else (float(v[0]), float(v[1]))
+        return sh, sv
+
+    def compute(self, loops=0):
+        """Run electrical calculations; returns the maximum current-density change (%)."""
+        self.initialize()
+        c = self._core
+        c.maxerr = float(self.maxerr)
+        c.stable = (str(self.convergence).lower() == "stable")
+        T = self.inTemperature(self._element_mesh())
+        sh, sv = self._load_conductivities(T)
+        c.set_conductivity(sh, sv)
+        # beta, js per junction element at the junction mid-row temperature (electr2d.cpp:255-256)
+        beta = np.empty(len(self._jelems))
+        js = np.empty(len(self._jelems))
+        m = [len(a) - 1 for a in self._axes]
+        for q, (e, n) in enumerate(zip(self._jelems, self._jid)):
+            kb, kt, _ = self._junctions[n]
+            kmid = (kb + kt) // 2
+            if self.DIM == 2:
+                ti = e % m[0] + m[0] * kmid
+            else:
+                ti = e % (m[0] * m[1]) + m[0] * m[1] * kmid
+            b = self._beta.get(int(n), None)
+            j = self._js.get(int(n), 1.0)
+            if b is None:
+                raise ValueError("%s: no beta given for junction %d" % (self.id, n))
+            beta[q] = b(float(T[ti])) if callable(b) else float(b)
+            js[q] = j(float(T[ti])) if callable(j) else float(j)
+        c.set_junction_params(beta, js)
+        nodes, vals = self.voltage_boundary.resolve(self.mesh, self.geometry)
+        c.set_dirichlet(nodes.astype(np.int64), vals)
+        err = c.compute(int(loops))
+        self._V = None
+        return err
+
+    @property
+    def err(self):
+        return self._core.toterr if self._core is not None else 0.0
+
+    def _volt(self):
+        if getattr(self, "_V", None) is None:
+            self._V = self._core.solution()
+        return self._V
+
+    # -- providers -----------------------------------------------------------------
+    def outVoltage(self, mesh, interpolation="linear"):
+        if self._core is None or self._core.loopno == 0:
+            raise ValueError("%s: voltage not computed" % self.id)
+        return _sb.interp_nodes(self._axes, self._volt(), mesh, fill=np.nan)
+
+    def outCurrentDensity(self, mesh, interpolation="linear"):
+        j = self._core.currents().reshape(-1, self.DIM)
+        return _sb.interp_elements(self._axes, j, mesh, fill=0.0)
+
+    def outHeat(self, mesh, interpolation="linear"):
+        """Joule heat density (W/m^3) (getHeatDensities, electr2d.cpp:541-562)."""
+        if self._core is None or self._core.loopno == 0:
+            raise ValueError("%s: heat density not computed" % self.id)
+        return _sb.interp_elements(self._axes, self._core.heats(), mesh, fill=0.0)
+
+    def outConductivity(self, mesh, interpolation="nearest"):
+        self.initialize()
+        if self._core.loopno == 0:
+            T = self.inTemperature(self._element_mesh())
+            sh, sv = self._load_conductivities(T)
+            sv = sv.copy()
+            sv[self._act > 0] = self._start_cond[1]
+            sh = sh.copy()
+            sh[self._act > 0] = self._start_cond[0]
+        else:
+            sh, sv = self._core.conductivity_h(), self._core.conductivity_v()
+        return _sb.nearest_elements(self._axes, np.column_stack([sh, sv]), mesh)
+
+    # -- integrals -------------------------------------------------------------------
+    def get_total_current(self, nact=0):
+        """Current through the middle row of junction nact (mA) (getTotalCurrent,
+        electr2d.cpp:466-505, electr3d.cpp:480-510)."""
+        if nact >= len(self._junctions) or self._junctions[nact] is None:
+            raise ValueError("%s: wrong active region number" % self.id)
+        kb, kt, _ = self._junctions[nact]
+        level = (kb + kt) // 2
+        j = self._core.currents().reshape(-1, self.DIM)
+        m = [len(a) - 1 for a in self._axes]
+        s = self._esize
+        if self.DIM == 2:
+            row = slice(level * m[0], (level + 1) * m[0])
+            jr = j[row, 1]
+            act = self._act[row] > 0
+            if self.CYL:
+                r = self._axes[0]
+                return float(np.sum(np.where(act, jr * (r[1:] ** 2 - r[:-1] ** 2), 0.0)) * math.pi * 0.01)
+            return float(np.sum(np.where(act, jr * s[0], 0.0)) * self.geometry.length * 0.01)
+        n = m[0] * m[1]
+        row = slice(level * n, (level + 1) * n)
+        area = (s[0][None, :] * s[1][:, None]).ravel()
+        act = self._act[row] > 0
+        return float(np.sum(np.where(act, j[row, 2] * area, 0.0)) * 0.01)
+
+    def get_total_heat(self):
+        """Total Joule heat (mW) (getTotalHeat, electr2d.cpp:627-649)."""
+        h = self._core.heats()
+        vol = self._element_volume()
+        if self.DIM == 2 and self.CYL:
+            r = 0.5 * (self._axes[0][1:] + self._axes[0][:-1])
+            rr = np.tile(r, len(self._axes[1]) - 1)
+            return float(2e-15 * math.pi * np.sum(vol * rr * h))
+        if self.DIM == 2:
+            return float(self.geometry.length * 1e-15 * np.sum(vol * h))
+        return float(1e-15 * np.sum(vol * h))
+
+    def _grad2(self):
+        """|grad V|^2 per element in (V/m)^2 with the reference's element gradients."""
+        V = self._volt().reshape([len(a) for a in self._axes][::-1])
+        s = self._esize
+        if self.DIM == 2:
+            gx = 0.5e6 * (-V[:-1, :-1] + V[:-1, 1:] - V[1:, :-1] + V[1:, 1:]) / s[0][None, :]
+            gy = 0.5e6 * (-V[:-1, :-1] - V[:-1, 1:] + V[1:, :-1] + V[1:, 1:]) / s[1][:, None]
+            return (gx ** 2 + gy ** 2).ravel()
+        tot = []
+        for ax in range(3):
+            acc = 0
+            for c in range(8):
+                sl = tuple(slice(1, None) if (c >> (2 - a)) & 1 else slice(None, -1) for a in range(3))
+                acc = acc + (1.0 if (c >> (2 - ax)) & 1 else -1.0) * V[sl]
+            tot.append(acc)
+        gz = 0.25e6 * tot[0] / s[2][:, None, None]
+        gy = 0.25e6 * tot[1] / s[1][None, :, None]
+        gx = 0.25e6 * tot[2] / s[0][None, None, :]
+        return (gx ** 2 + gy ** 2 + gz ** 2).ravel()
+
+    def get_electrostatic_energy(self):
+        """Energy of the electrostatic field (J) (getTotalEnergy, electr2d.cpp:574-615)."""
+        T = self.inTemperature(self._element_mesh())
+        eps = np.array([mm.eps(float(t)) if mm is not None else 1.0 for mm, t in zip(self._mat, T)])
+        w = eps * self._grad2() * self._element_volume()
+        if self.DIM == 2 and self.CYL:
+            r = np.tile(0.5 * (self._axes[0][1:] + self._axes[0][:-1]), len(self._axes[1]) - 1)
+            return float(2.0 * math.pi * 0.5e-18 * EPSILON0 * np.sum(w * r))
+        if self.DIM == 2:
+            return float(self.geometry.length * 0.5e-18 * EPSILON0 * np.sum(w))
+        return float(0.5e-18 * EPSILON0 * np.sum(w))
+
+    def get_capacitance(self):
+        """Capacitance (pF) = 2 W / U^2 with exactly two voltage conditions (electr2d.cpp:617-625)."""
+        if len(self.voltage_boundary) != 2:
+            raise ValueError("%s: cannot estimate applied voltage (exactly 2 voltage boundary conditions required)"
+                             % self.id)
+        U = self.voltage_boundary[0].value - self.voltage_boundary[1].value
+        return 2e12 * self.get_electrostatic_energy() / (U * U)
+
+
+class Shockley2D(_ShockleyBase):
+    """Finite element electrical solver for 2D Cartesian geometry."""
+    DIM, CYL = 2, False
+
+
+class ShockleyCyl(_ShockleyBase):
+    """Finite element electrical solver for 2D cylindrical geometry."""
+    DIM, CYL = 2, True
+
+
+class Shockley3D(_ShockleyBase):
+    """Finite element electrical solver for 3D Cartesian geometry."""
+    DIM, CYL = 3, False

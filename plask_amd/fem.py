@@ -17,6 +17,7 @@ PF_THERMAL, PF_ELECTRICAL = 0, 1
 PF_FP64, PF_FP32_REFINE = 0, 1
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM = 0, 1, 2, 3
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
+PRECOND_JACOBI, PRECOND_LINE = 1, 2
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
@@ -165,12 +166,13 @@ class FemContext:
 
     # -- compute -------------------------------------------------------------
     def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, max_outer=50,
-              check_every=32, keep_on_device=False, raise_noconv=True, norm=PF_NORM_INIT, init_value=None):
+              check_every=32, keep_on_device=False, raise_noconv=True, norm=PF_NORM_INIT, init_value=None,
+              precond=PRECOND_LINE):
         """Solve; the relative residual is ||B - A X|| / ||B - A X_init|| (norm=PF_NORM_INIT,
         X_init = init_value, default the first entry of x0 or 0) unless norm says otherwise."""
         if init_value is None:
             init_value = float(np.ravel(x0)[0]) if x0 is not None else 0.0
-        o = SolveOpts(precision, 1, PF_X0_GIVEN if x0 is not None else PF_X0_ZERO, int(maxit), float(rtol),
+        o = SolveOpts(precision, precond, PF_X0_GIVEN if x0 is not None else PF_X0_ZERO, int(maxit), float(rtol),
                       float(inner_rtol), int(max_outer), int(check_every), int(norm), float(init_value))
         st = SolveStats()
         if x0 is not None:
@@ -183,9 +185,9 @@ class FemContext:
         return x, st
 
     def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
-                   norm=PF_NORM_INIT, init_value=0.0):
+                   norm=PF_NORM_INIT, init_value=0.0, precond=PRECOND_LINE):
         """Start from the solution already on the device (no host transfers)."""
-        o = SolveOpts(precision, 1, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every),
+        o = SolveOpts(precision, precond, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every),
                       int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
@@ -193,9 +195,9 @@ class FemContext:
         return st
 
     def solve_uniform(self, init_value, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
-                      norm=PF_NORM_INIT, max_outer=50):
+                      norm=PF_NORM_INIT, max_outer=50, precond=PRECOND_LINE):
         """Start from init_value everywhere (device fill); the solution stays on the device."""
-        o = SolveOpts(precision, 1, PF_X0_UNIFORM, int(maxit), float(rtol), float(inner_rtol), int(max_outer),
+        o = SolveOpts(precision, precond, PF_X0_UNIFORM, int(maxit), float(rtol), float(inner_rtol), int(max_outer),
                       int(check_every), int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))

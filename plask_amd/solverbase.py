@@ -1,0 +1,244 @@
+"""Common parts of the FEM solver classes: geometry/mesh handling, element tables,
+receivers and providers (the subset of plask Solver / FemSolverWithMaskedMesh /
+ReceiverFor / ProviderFor the thermal and electrical solvers use).
+"""
+import logging
+
+import numpy as np
+from scipy.interpolate import RegularGridInterpolator
+
+from . import mesh as _mesh
+
+try:
+    from . import _core
+except ImportError as e:  # the compute path is the compiled core only
+    raise ImportError("plask_amd._core not built (%s); run __graft_entry__.build()" % e)
+
+log = logging.getLogger("plask_amd")
+
+# plask/log/log.hpp levels used by the solvers -> Python logging
+_LEVELS = {0: logging.CRITICAL, 1: logging.ERROR, 2: logging.ERROR, 3: logging.WARNING, 4: logging.WARNING,
+           5: logging.INFO, 6: logging.INFO, 7: logging.DEBUG, 8: logging.DEBUG, 9: logging.DEBUG}
+
+
+def _logger(level, msg):
+    log.log(_LEVELS.get(level, logging.INFO), msg)
+
+
+class Receiver:
+    """Receiver of a field: a constant, an array (values at the requested points), or a
+    provider (any callable taking a mesh) -- the Python receiver semantics of
+    python/plask/python/provider.hpp:114-305 (assignValue / connect)."""
+
+    def __init__(self, owner, default):
+        self._owner = owner
+        self._value = default
+        self.changed = True
+
+    def set(self, value):
+        self._value = value
+        self.changed = True
+
+    def __call__(self, mesh):
+        v = self._value
+        n = len(mesh)
+        if callable(v):
+            return np.asarray(v(mesh), dtype=np.float64).reshape(n, -1).squeeze(-1) if n else np.zeros(0)
+        if np.isscalar(v):
+            return np.full(n, float(v))
+        a = np.asarray(v, dtype=np.float64)
+        if a.shape[0] != n:
+            raise ValueError("receiver data has %d values, mesh has %d points" % (a.shape[0], n))
+        return a
+
+
+class ReceiverAttr:
+    def __init__(self, default):
+        self.default = default
+
+    def __set_name__(self, owner, name):
+        self.name = "_recv_" + name
+
+    def __get__(self, obj, objtype=None):
+        if obj is None:
+            return self
+        r = obj.__dict__.get(self.name)
+        if r is None:
+            r = Receiver(obj, self.default)
+            obj.__dict__[self.name] = r
+        return r
+
+    def __set__(self, obj, value):
+        self.__get__(obj).set(value)
+        obj._receiver_changed()
+
+
+def points_of(dest):
+    """(N, dim) points of a destination mesh or point list."""
+    if isinstance(dest, _mesh.Rectangular):
+        return dest.points()
+    return np.atleast_2d(np.asarray(list(dest), dtype=np.float64))
+
+
+def interp_nodes(axes, values, dest, fill):
+    """Linear interpolation of node values (natural order) on a rectilinear mesh."""
+    d = len(axes)
+    vals = np.asarray(values).reshape([len(a) for a in axes][::-1]).transpose(*range(d)[::-1])
+    f = RegularGridInterpolator(axes, vals, method="linear", bounds_error=False, fill_value=np.nan)
+    out = f(points_of(dest))
+    if fill is not None:
+        out = np.where(np.isnan(out), fill, out)
+    return out
+
+
+def interp_elements(axes, values, dest, inside=None, fill=0.0):
+    """Linear interpolation of element (midpoint) values; points inside the mesh but
+    outside the midpoint range take the edge value; points outside the mesh take fill."""
+    mids = [0.5 * (a[1:] + a[:-1]) for a in axes]
+    d = len(axes)
+    vals = np.asarray(values)
+    comp = vals.shape[1:] if vals.ndim > 1 else ()
+    grid = vals.reshape([len(m) for m in mids][::-1] + list(comp)).transpose(*(list(range(d)[::-1]) +
+                                                                                list(range(d, d + len(comp)))))
+    pts = points_of(dest)
+    clipped = np.column_stack([np.clip(pts[:, a], mids[a][0], mids[a][-1]) for a in range(d)])
+    if any(len(m) == 1 for m in mids):
+        idx = tuple(np.clip(np.searchsorted(m, clipped[:, a]), 0, len(m) - 1) for a, m in enumerate(mids))
+        out = grid[idx]
+    else:
+        f = RegularGridInterpolator(mids, grid, method="linear", bounds_error=False, fill_value=None)
+        out = f(clipped)
+    ok = np.all([(pts[:, a] >= axes[a][0]) & (pts[:, a] <= axes[a][-1]) for a in range(d)], axis=0)
+    if inside is not None:
+        ok &= inside(pts)
+    out = np.where(ok.reshape((-1,) + (1,) * len(comp)), out, fill)
+    return out
+
+
+def nearest_elements(axes, values, dest):
+    """Nearest-element lookup (INTERPOLATION_NEAREST on the element mesh)."""
+    pts = points_of(dest)
+    d = len(axes)
+    idx = [np.clip(np.searchsorted(axes[a], pts[:, a], side="right") - 1, 0, len(axes[a]) - 2) for a in range(d)]
+    m = [len(a) - 1 for a in axes]
+    flat = idx[0] + m[0] * (idx[1] + (m[1] * idx[2] if d == 3 else 0))
+    return np.asarray(values)[flat]
+
+
+class FemSolver:
+    """Shared geometry/mesh/boundary handling (FemSolverWithMaskedMesh, fem_solver.hpp:145-241)."""
+
+    DIM = 2
+    CYL = False
+    _core_class = None
+
+    def __init__(self, name=""):
+        self.id = name
+        self._geometry = None
+        self._mesh = None
+        self._msh = None
+        self._core = None
+        self.algorithm = "iterative"
+        self.empty_elements = "default"
+        self._iter = _core.IterParams()
+        self._initialized = False
+
+    # -- geometry / mesh ------------------------------------------------------
+    @property
+    def geometry(self):
+        return self._geometry
+
+    @geometry.setter
+    def geometry(self, g):
+        if g is not None and g.dim != self.DIM:
+            raise TypeError("%s: geometry dimension does not match" % self.id)
+        if g is not None and bool(getattr(g, "cylindrical", False)) != self.CYL:
+            raise TypeError("%s: wrong geometry type" % self.id)
+        self._geometry = g
+        self.invalidate()
+
+    @property
+    def mesh(self):
+        if self._msh is None and self._mesh is not None and self._geometry is not None:
+            m = self._mesh
+            self._msh = m if isinstance(m, _mesh.Rectangular) else m(self._geometry)
+        return self._msh
+
+    @mesh.setter
+    def mesh(self, m):
+        self._mesh = m
+        self._msh = None
+        self.invalidate()
+
+    @property
+    def iterative(self):
+        return self._iter
+
+    def invalidate(self):
+        self._initialized = False
+        if self._core is not None:
+            self._core.invalidate()
+
+    def initialize(self):
+        if self._initialized:
+            return
+        if self._geometry is None:
+            raise ValueError("%s: no geometry" % self.id)
+        if self.mesh is None:
+            raise ValueError("%s: no mesh" % self.id)
+        if self.empty_elements in ("exclude", "excluded"):
+            raise NotImplementedError("%s: empty_elements='exclude' (masked mesh) is not supported by the GPU "
+                                      "path; the full mesh is used with 'include'" % self.id)
+        msh = self.mesh
+        if self._core is None:
+            self._core = self._core_class(self.DIM, self.CYL, 0, self.id)
+            self._core.logger = _logger
+        self._core.iter = self._iter
+        self._core.set_mesh([np.asarray(a) for a in msh.axes])
+        self._tables()
+        self._on_initialize()
+        self._initialized = True
+
+    def _receiver_changed(self):
+        pass
+
+    # -- element tables --------------------------------------------------------
+    def _tables(self):
+        msh = self.mesh
+        g = self._geometry
+        mids = msh.element_mesh().points()
+        self._mids = mids
+        self._leaf = g.leaf_table(mids)
+        leaves = g._leaves
+        self._mat = [leaves[k][0].material if k >= 0 else None for k in self._leaf]
+        self._roles = [leaves[k][0].roles if k >= 0 else set() for k in self._leaf]
+        self._axes = [np.asarray(a) for a in msh.axes]
+        self._esize = [np.diff(a) for a in self._axes]
+
+    def _thickness(self):
+        """Per element the thickness of the same-material run in its column
+        (ThermalFem2DSolver::onInitialize thickness scan, therm2d.cpp:81-113)."""
+        m = [len(a) - 1 for a in self._axes]
+        ncol = int(np.prod(m[:-1]))
+        nz = m[-1]
+        lay = np.array([id(x) for x in self._mat]).reshape(nz, ncol)
+        z = self._axes[-1]
+        h = np.empty((nz, ncol))
+        for c in range(ncol):
+            k = 0
+            while k < nz:
+                k2 = k
+                while k2 + 1 < nz and lay[k2 + 1, c] == lay[k, c]:
+                    k2 += 1
+                h[k:k2 + 1, c] = z[k2 + 1] - z[k]
+                k = k2 + 1
+        return h.ravel()
+
+    def _element_volume(self):
+        s = self._esize
+        if self.DIM == 2:
+            return (s[0][None, :] * s[1][:, None]).ravel()
+        return (s[0][None, None, :] * s[1][None, :, None] * s[2][:, None, None]).ravel()
+
+    def _element_mesh(self):
+        return self.mesh.element_mesh()

@@ -1,0 +1,176 @@
+"""``thermal.static``: steady-state FEM thermal solvers Static2D, StaticCyl, Static3D.
+
+Same classes and attributes as the reference module (solvers/thermal/static/python/
+therm_python.cpp:97-152): compute(loops=0) -> err, inHeat, outTemperature,
+outHeatFlux, outThermalConductivity, temperature_boundary, heatflux_boundary,
+convection_boundary, radiation_boundary, inittemp, maxerr, algorithm, iterative,
+empty_elements.  The linear solve runs on the GPU (matrix-free operator, PCG).
+"""
+import numpy as np
+
+from .. import mesh as _mesh
+from .. import solverbase as _sb
+from .._core import ThermalCore
+
+
+class _StaticBase(_sb.FemSolver):
+    _core_class = ThermalCore
+    inHeat = _sb.ReceiverAttr(0.0)
+
+    def __init__(self, name=""):
+        super().__init__(name)
+        self.inittemp = 300.0
+        self.maxerr = 0.05
+        self.temperature_boundary = _mesh.BoundaryConditions()
+        self.heatflux_boundary = _mesh.BoundaryConditions()
+        self.convection_boundary = _mesh.BoundaryConditions()
+        self.radiation_boundary = _mesh.BoundaryConditions()
+        self._T = None
+
+    @property
+    def inHeatDensity(self):
+        return self.inHeat
+
+    @inHeatDensity.setter
+    def inHeatDensity(self, v):
+        self.inHeat = v
+
+    def _on_initialize(self):
+        c = self._core
+        c.inittemp = float(self.inittemp)
+        self._h = self._thickness()
+        laws = [m.thermk_law() if m is not None else None for m in self._mat]
+        if all(l is not None for l in laws):
+            L = np.array(laws)
+            c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
+        else:
+            mats = self._mat
+            h = self._h
+            groups = {}
+            for e, m in enumerate(mats):
+                groups.setdefault(id(m), (m, []))[1].append(e)
+            groups = [(m, np.array(ix)) for m, ix in groups.values()]
+
+            def kfun(tavg):
+                kh = np.empty(len(tavg))
+                kv = np.empty(len(tavg))
+                for m, ix in groups:
+                    if m is None:
+                        raise ValueError("%s: element without material" % self.id)
+                    for e in ix:
+                        k = m.thermk(float(tavg[e]), float(h[e]))
+                        k = (k, k) if np.isscalar(k) else k
+                        kh[e], kv[e] = float(k[0]), float(k[1])
+                return kh, kv
+
+            c.set_conductivity_callback(kfun)
+            # a placeholder law so that the device holds valid coefficients before the first loop
+            E = len(mats)
+            c.set_conductivity(np.ones(E), np.ones(E), np.zeros(E), np.zeros(E))
+            c.set_conductivity_callback(kfun)
+        c.initialize()
+        self._T = None
+
+    def compute(self, loops=0):
+        """Run thermal calculations; returns the maximum temperature update (K)."""
+        self.initialize()
+        for bc in (self.heatflux_boundary, self.convection_boundary, self.radiation_boundary):
+            if len(bc):
+                raise NotImplementedError("%s: heat flux / convection / radiation boundary conditions are not "
+                                          "implemented on the GPU path" % self.id)
+        c = self._core
+        c.inittemp = float(self.inittemp)
+        c.maxerr = float(self.maxerr)
+        q = self.inHeat(self._element_mesh())
+        c.set_heat(np.ascontiguousarray(q, dtype=np.float64))
+        nodes, vals = self.temperature_boundary.resolve(self.mesh, self.geometry)
+        c.set_dirichlet(nodes.astype(np.int64), vals)
+        err = c.compute(int(loops))
+        self._T = None
+        return err
+
+    @property
+    def err(self):
+        return self._core.toterr if self._core is not None else 0.0
+
+    def _temps(self):
+        if self._T is None:
+            self._T = self._core.solution()
+        return self._T
+
+    # -- providers ---------------------------------------------------------------
+    def outTemperature(self, mesh, interpolation="linear"):
+        """Temperatures (K); inittemp everywhere before the first compute
+        (ThermalFem2DSolver::getTemperatures, therm2d.cpp:543-552)."""
+        if not self._initialized or self._core is None or self._core.loopno == 0:
+            return np.full(len(_sb.points_of(mesh)), float(self.inittemp))
+        return _sb.interp_nodes(self._axes, self._temps(), mesh, fill=300.0)
+
+    def _element_T(self):
+        T = self._temps().reshape([len(a) for a in self._axes][::-1])
+        if self.DIM == 2:
+            return 0.25 * (T[:-1, :-1] + T[:-1, 1:] + T[1:, :-1] + T[1:, 1:]).ravel()
+        return 0.125 * (T[:-1, :-1, :-1] + T[:-1, :-1, 1:] + T[:-1, 1:, :-1] + T[:-1, 1:, 1:] +
+                        T[1:, :-1, :-1] + T[1:, :-1, 1:] + T[1:, 1:, :-1] + T[1:, 1:, 1:]).ravel()
+
+    def _element_k(self, leaf_height=False):
+        tav = self._element_T() if (self._core is not None and self._core.loopno > 0) else \
+            np.full(len(self._mat), float(self.inittemp))
+        h = self._h
+        if leaf_height:
+            leaves = self._geometry._leaves
+            h = np.array([leaves[k][2][-1] if k >= 0 else np.inf for k in self._leaf])
+        k = np.empty((len(self._mat), 2))
+        for e, m in enumerate(self._mat):
+            v = m.thermk(float(tav[e]), float(h[e])) if m is not None else (np.nan, np.nan)
+            k[e] = (v, v) if np.isscalar(v) else v
+        return k
+
+    def outThermalConductivity(self, mesh, interpolation="nearest"):
+        """Element conductivities (in-plane, vertical) at the element temperatures."""
+        self.initialize()
+        return _sb.nearest_elements(self._axes, self._element_k(), mesh)
+
+    def outHeatFlux(self, mesh, interpolation="linear"):
+        """Heat flux -k grad T (W/m^2); uses the leaf height for thermk as the reference
+        does (therm2d.cpp:525-531)."""
+        if self._core is None or self._core.loopno == 0:
+            return np.zeros((len(_sb.points_of(mesh)), self.DIM))
+        T = self._temps().reshape([len(a) for a in self._axes][::-1])
+        k = self._element_k(leaf_height=True)
+        s = self._esize
+        if self.DIM == 2:
+            gx = (-T[:-1, :-1] + T[:-1, 1:] - T[1:, :-1] + T[1:, 1:]) / s[0][None, :]
+            gy = (-T[:-1, :-1] - T[:-1, 1:] + T[1:, :-1] + T[1:, 1:]) / s[1][:, None]
+            fl = np.column_stack([-0.5e6 * k[:, 0] * gx.ravel(), -0.5e6 * k[:, 1] * gy.ravel()])
+        else:
+            def dif(ax):
+                sl0 = [slice(None, -1)] * 3
+                sl1 = [slice(None, -1)] * 3
+                tot = 0
+                for c in range(8):
+                    sl = tuple(slice(1, None) if (c >> (2 - a)) & 1 else slice(None, -1) for a in range(3))
+                    sign = 1.0 if (c >> (2 - ax)) & 1 else -1.0
+                    tot = tot + sign * T[sl]
+                return tot
+            gz = dif(0) / s[2][:, None, None]
+            gy = dif(1) / s[1][None, :, None]
+            gx = dif(2) / s[0][None, None, :]
+            fl = np.column_stack([-0.25e6 * k[:, 0] * gx.ravel(), -0.25e6 * k[:, 0] * gy.ravel(),
+                                  -0.25e6 * k[:, 1] * gz.ravel()])
+        return _sb.interp_elements(self._axes, fl, mesh, fill=0.0)
+
+
+class Static2D(_StaticBase):
+    """Finite element thermal solver for 2D Cartesian geometry."""
+    DIM, CYL = 2, False
+
+
+class StaticCyl(_StaticBase):
+    """Finite element thermal solver for 2D cylindrical geometry."""
+    DIM, CYL = 2, True
+
+
+class Static3D(_StaticBase):
+    """Finite element thermal solver for 3D geometry."""
+    DIM, CYL = 3, False

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PF_THERMAL, PF_ELECTRICAL, FemContext
+from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PF_THERMAL, PF_ELECTRICAL, PRECOND_JACOBI, PRECOND_LINE, FemContext
 
 pytestmark = pytest.mark.gpu
 
@@ -66,13 +66,14 @@ def test_rhs_matches_oracle(shape):
 
 @pytest.mark.parametrize("shape,cyl", [((9, 7, 11), False), ((24, 20, 30), False), ((40, 60), True), ((31, 45), False)])
 @pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE])
-def test_solve_matches_band_cholesky(shape, cyl, precision):
+@pytest.mark.parametrize("precond", [PRECOND_LINE, PRECOND_JACOBI])
+def test_solve_matches_band_cholesky(shape, cyl, precision, precond):
     g, axes, kh, kv, q, dn, dv = _rand_problem(shape, cyl=cyl, seed=3)
-    print("solve", shape, cyl, precision, flush=True)
+    print("solve", shape, cyl, precision, precond, flush=True)
     A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
     T_ref = O.band_cholesky_solve(A)
     c = _ctx(axes, kh, kv, q, dn, dv, cyl=cyl)
-    T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-12)
+    T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-12, precond=precond)
     assert st.rel_resid <= 1e-12
     rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
     assert rel <= 1e-8, rel
