@@ -182,31 +182,59 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     for (int r = tid; r < 2 * (BY + 1); r += blockDim.x) xs[r * PX + BX] = T(0);
     for (int r = tid; r < 4 * BY; r += blockDim.x) oc[r * PX] = T(0);
 
-    // load own node (and the extra row) of plane k into xs[buf]; returns the
-    // unmasked own value; masked values (MODE_OP) go to LDS
-    auto load_plane = [&](int k, int buf, T& own, bool& own_dir, bool store_p) {
+    // Global loads of one step, issued one step ahead (software prefetch): node values
+    // of plane k (own row and, for the last thread row, row j+1) and the element data
+    // of layer k-1.  With one 1024-thread block per CU nothing else hides HBM latency.
+    struct Raw {
+        T z, p, zx, px, kxy, kz;
+        bool d, dx;
+    };
+    auto gload = [&](int k, Raw& R) {
+        if (nvalid) {
+            const long long node = (long long)k * plane + (long long)j * n0 + tx;
+            if (FUSEP) {
+                R.z = zin[node];
+                R.p = pin[node];
+            } else {
+                R.z = xin[node];
+            }
+            R.d = dbit(dmask, node);
+        }
+        if (xrow) {
+            const long long node = (long long)k * plane + (long long)(j + 1) * n0 + tx;
+            if (FUSEP) {
+                R.zx = zin[node];
+                R.px = pin[node];
+            } else {
+                R.zx = xin[node];
+            }
+            R.dx = dbit(dmask, node);
+        }
+        if (evalid && k >= 1) {
+            const long long e = (long long)(k - 1) * (n0 - 1) * (n1 - 1) + (long long)j * (n0 - 1) + tx;
+            R.kxy = kh[e];
+            R.kz = kv[e];
+        }
+    };
+    // plane k values into xs[buf] (masked for MODE_OP), p_new stored for owned nodes
+    auto lstore = [&](int k, int buf, const Raw& R, T& own, bool& own_dir, bool store_p) {
         T v = T(0);
         bool d = false;
         if (nvalid) {
-            long long node = (long long)k * plane + (long long)j * n0 + tx;
-            if (FUSEP) {
-                v = zin[node] + T(beta) * pin[node];
-                if (store_p) pout[node] = v;
-            } else {
-                v = xin[node];
-            }
-            d = dbit(dmask, node);
+            v = FUSEP ? R.z + T(beta) * R.p : R.z;
+            d = R.d;
+            if (FUSEP && store_p) pout[(long long)k * plane + (long long)j * n0 + tx] = v;
         }
         own = v;
         own_dir = d;
         if (active) xs[(buf * (BY + 1) + ty) * PX + tx] = (MODE == MODE_OP && d) ? T(0) : v;
-        if (xrow) {
-            long long node = (long long)k * plane + (long long)(j + 1) * n0 + tx;
-            T w;
-            if (FUSEP) w = zin[node] + T(beta) * pin[node];
-            else w = xin[node];
-            bool dd = dbit(dmask, node);
-            xs[(buf * (BY + 1) + BY) * PX + tx] = (MODE == MODE_OP && dd) ? T(0) : w;
+        if (active && ty == BY - 1) {
+            T w = T(0);
+            if (xrow) {
+                w = FUSEP ? R.zx + T(beta) * R.px : R.zx;
+                if (MODE == MODE_OP && R.dx) w = T(0);
+            }
+            xs[(buf * (BY + 1) + BY) * PX + tx] = w;
         }
     };
     // in-plane sum/difference transform of the element column's 4 corners
@@ -225,8 +253,13 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     int buf = 0;
     T own_cur, own_next = T(0);
     bool dir_cur, dir_next = false;
+    Raw R;
+    R.z = R.p = R.zx = R.px = R.kxy = R.kz = T(0);
+    R.d = R.dx = false;
     __syncthreads();
-    load_plane(P0, 0, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
+    gload(P0, R);
+    lstore(P0, 0, R, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
+    if (P0 + 1 < n2) gload(P0 + 1, R);
     __syncthreads();
     T xt_cur[4], xt_next[4] = {T(0), T(0), T(0), T(0)};
     if (active) plane_tf(0, xt_cur);
@@ -237,16 +270,15 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         const bool has_next = (P + 1) < n2;
         T bot[4] = {T(0), T(0), T(0), T(0)}, top[4] = {T(0), T(0), T(0), T(0)};
         if (has_next) {
-            load_plane(P + 1, buf ^ 1, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
+            lstore(P + 1, buf ^ 1, R, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
             T a = T(0), b = T(0), c = T(0);
             if (evalid) {
-                long long e = (long long)P * (n0 - 1) * (n1 - 1) + (long long)j * (n0 - 1) + tx;
-                T hz = g.hz[P];
-                T kxy = kh[e], kz = kv[e];
-                a = kxy * (gxf * hz);
-                b = kxy * (gyf * hz);
-                c = kz * (gzf / hz);
+                const T hz = g.hz[P];
+                a = R.kxy * (gxf * hz);
+                b = R.kxy * (gyf * hz);
+                c = R.kz * (gzf / hz);
             }
+            if (P + 2 < n2 && P + 1 < ke) gload(P + 2, R);  // prefetch the next step
             __syncthreads();
             if (active) {
                 plane_tf(buf ^ 1, xt_next);

@@ -1,0 +1,169 @@
+"""CPU-only checks of the host side: the C ABI library loads and exports every
+declared entry point (no compute calls), the geometry / mesh / boundary logic, the
+seeded workload generators and the solver attribute surface."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "plask_fem.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from plask_amd import fem
+    lib = ctypes.CDLL(fem.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares the same set it relies on
+    assert set(fem.EXPORTED) <= set(names)
+
+
+def test_core_module_loads():
+    from plask_amd import _core
+    assert hasattr(_core, "ThermalCore") and hasattr(_core, "ElectricalCore")
+
+
+def test_splitmix64_reference_vector():
+    # first outputs of SplitMix64 seeded with 1234567 (the published reference stream)
+    from plask_amd.workloads import SplitMix64
+    r = SplitMix64(1234567)
+    assert [r.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                           4593380528125082431, 16408922859458223821]
+
+
+def test_workload_shapes_and_layers():
+    from plask_amd import workloads
+    p = workloads.thermal_3d(seed=2, n=(16, 12, 64), nlayers=10, active=5, width=20.0)
+    nx, ny, nz = 16, 12, 64
+    assert p["k_h"].shape == ((nx - 1) * (ny - 1) * (nz - 1),)
+    z = p["axes"][2]
+    assert len(z) == nz and np.all(np.diff(z) > 0)
+    # element edges on every layer interface
+    bounds = np.cumsum(np.r_[0.0, p["thick"]])
+    for b in bounds:
+        assert np.min(np.abs(z - b)) < 1e-12
+    assert abs(p["thick"][5] - 0.1) < 1e-15
+    assert p["q"].max() == 1e16 and p["q"].min() == 0.0
+    c1 = workloads.thermal_2d_cyl(seed=1, nr=200, nz=400)
+    assert len(c1["axes"][0]) == 200 and len(c1["axes"][1]) == 400
+
+
+def test_geometry_stack_shelf_and_boundaries():
+    from plask_amd import geometry, material, mesh
+    m1 = material.Const(thermk=1.0, cond=2.0, name="a")
+    m2 = material.Const(thermk=3.0, cond=4.0, name="b")
+    cont = geometry.Rectangle(400., 100., m2)
+    rect = geometry.Rectangle(1000., 300., m1)
+    junc = geometry.Rectangle(1000., 0.02, m2, role="active")
+    shelf = geometry.Shelf()
+    shelf.append(cont)
+    shelf.append_gap(200.)
+    shelf.append(cont)
+    stack = geometry.Stack2D()
+    stack.prepend(shelf)
+    stack.prepend(rect)
+    stack.prepend(junc)
+    stack.prepend(rect)
+    sp = geometry.Cylindrical(stack)
+    assert sp.get_material(10., 10.) is m1
+    assert sp.has_role("active", 10., 300.01)
+    assert sp.get_material(500., 700.) is None          # the shelf gap
+    assert sp.get_material(700., 700.) is m2
+    msh = mesh.Rectangular2D.DivideGenerator(prediv=(2, 1))(sp)
+    top = mesh.Rectangular2D.TopOf(cont).nodes(msh, sp)
+    pts = msh.points()[top]
+    assert np.allclose(pts[:, 1], 700.02)
+    assert np.all((pts[:, 0] <= 400) | (pts[:, 0] >= 600))
+    bot = mesh.Rectangular2D.Bottom().nodes(msh, sp)
+    assert np.allclose(msh.points()[bot][:, 1], 0.0) and len(bot) == len(msh.axes[0])
+
+
+def test_geometry_placement_paths_3d():
+    from plask_amd import geometry, material, mesh
+    c = material.Const(cond=1e9, name="c")
+    contact = geometry.Cuboid(700., 700., 1., c)
+    rect = geometry.Cuboid(1000., 1000., 300., c)
+    stack = geometry.Stack3D(xcenter=0, ycenter=0)
+    top = stack.prepend(contact)
+    stack.prepend(rect)
+    bottom = stack.prepend(contact)
+    sp = geometry.Cartesian3D(stack)
+    msh = mesh.Rectangular3D.DivideGenerator(prediv=(3, 3, 2))(sp)
+    t = mesh.Rectangular3D.TopOf(contact, top).nodes(msh, sp)
+    b = mesh.Rectangular3D.BottomOf(contact, bottom).nodes(msh, sp)
+    assert np.allclose(msh.points()[t][:, 2], 302.0)
+    assert np.allclose(msh.points()[b][:, 2], 0.0)
+    assert np.all(np.abs(msh.points()[t][:, :2]) <= 350.0 + 1e-9)
+
+
+def test_boundary_conditions_last_value_wins():
+    from plask_amd import mesh
+    msh = mesh.Rectangular2D(np.linspace(0, 1, 3), np.linspace(0, 1, 4))
+    bc = mesh.BoundaryConditions()
+    bc.append(msh.Bottom(), 1.0)
+    bc.append(msh.Left(), 2.0)
+    nodes, vals = bc.resolve(msh, None)
+    last = {}
+    for n, v in zip(nodes, vals):
+        last[int(n)] = v
+    assert last[0] == 2.0 and last[1] == 1.0
+
+
+def test_interpolation_helpers():
+    from plask_amd import solverbase as sb
+    from plask_amd import mesh
+    axes = [np.array([0.0, 1.0, 3.0]), np.array([0.0, 2.0])]
+    vals = np.array([0., 1., 3., 10., 11., 13.])  # f = x + 5 y
+    out = sb.interp_nodes(axes, vals, mesh.Rectangular2D([0.5, 2.0], [1.0]), fill=300.0)
+    assert np.allclose(out, [0.5 + 5, 2.0 + 5])
+    out = sb.interp_nodes(axes, vals, [[5.0, 0.0]], fill=300.0)
+    assert out[0] == 300.0
+
+
+def test_shockley_attribute_surface():
+    from plask_amd.electrical.shockley import Shockley2D, Shockley3D, ShockleyCyl
+    from plask_amd.thermal.static import Static2D, Static3D, StaticCyl
+    s = Shockley2D("e")
+    s.beta = 10.
+    s.js = 1.
+    s.beta1 = lambda T: math.log(70 * T)
+    assert s.beta == 10. and s.js == 1. and callable(s.beta1)
+    with pytest.raises(TypeError):
+        s.beta = "x"
+    with pytest.raises(TypeError):
+        s.outPotential
+    for cls in (Shockley2D, ShockleyCyl, Shockley3D):
+        o = cls("x")
+        for a in ("compute", "get_total_current", "outVoltage", "outCurrentDensity", "outHeat", "outConductivity",
+                  "voltage_boundary", "maxerr", "convergence", "pcond", "ncond", "start_cond", "pnjcond",
+                  "get_electrostatic_energy", "get_capacitance", "get_total_heat", "inTemperature", "iterative"):
+            assert hasattr(o, a), (cls, a)
+    for cls in (Static2D, StaticCyl, Static3D):
+        o = cls("x")
+        for a in ("compute", "inHeat", "outTemperature", "outHeatFlux", "outThermalConductivity",
+                  "temperature_boundary", "heatflux_boundary", "convection_boundary", "radiation_boundary",
+                  "inittemp", "maxerr", "algorithm", "iterative", "empty_elements"):
+            assert hasattr(o, a), (cls, a)
+    assert hasattr(Static3D("y"), "inHeatDensity")
+
+
+def test_plugin_module_names():
+    import sys
+    from plask_amd import plugins
+    sys.path.insert(0, plugins.PATH)
+    try:
+        from thermal.static import Static2D  # noqa: F401
+        from electrical.shockley import Shockley2D  # noqa: F401
+    finally:
+        sys.path.remove(plugins.PATH)
