@@ -159,6 +159,11 @@ int64_t pf_num_elements(const pf_ctx* ctx);
  * what[1] = vector update + (r,z); counts[] receives launch counts.  For benchmarks. */
 int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
 
+/* Launch shape of the operator kernels (for tuning): 3D rows = thread rows per block
+ * (rows * n0 <= 1024, one of them a halo row), planes = planes per block along axis 2;
+ * 2D planes = rows of axis 1 per block. */
+int pf_set_launch(pf_ctx* ctx, int rows, int planes);
+
 /* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP
  * events (no graph) and accumulates per-kernel durations for pf_kernel_times. */
 int pf_set_profile(pf_ctx* ctx, int on);
@@ -214,6 +219,49 @@ int pf_element_flags_set(pf_ctx* ctx, const unsigned char* select, const unsigne
  * of electr2d.cpp:399-422) and keeps j as j_prev. */
 int pf_currents(pf_ctx* ctx, double* max_dj2, double* max_j2);
 int pf_get_currents(pf_ctx* ctx, double* cur, double* heat);
+
+/* Batched uneven 2D Shockley problems --------------------------------------
+ * Many independent ElectricalFem2DSolver::compute runs (electr2d.cpp:363-440, junction
+ * law beta.hpp:43-46) -- e.g. voltage/temperature sweeps over device variants -- in one
+ * persistent kernel launch replayed from a hipGraph.  Each workgroup takes instances
+ * from a largest-first work queue and runs the whole Picard loop on the device: junction
+ * update, line-preconditioned CG (warm start), currents and loop error.  The reference
+ * runs such sweeps as separate processes (gui/plugins/launcher_batch.py:16). */
+typedef struct pf_batch pf_batch;
+
+typedef struct {
+    int maxloops;   /* Picard loops (compute(loops)), default 30 */
+    int maxit;      /* CG iterations per loop */
+    double maxerr;  /* loop error limit in % (ElectricalFem2DSolver::maxerr, default 0.05) */
+    double rtol;    /* CG: ||B - A V|| <= rtol ||B - A V_init|| (V_init = Dirichlet values, 0 elsewhere) */
+    int stable;     /* CONVERGENCE_STABLE */
+    int block;      /* threads per workgroup 256/512/1024; 0 = autotuned or 1024 */
+    int nwg;        /* persistent workgroups; 0 = autotuned or one per CU */
+} pf_batch_opts;
+
+typedef struct {
+    int loops, iters, status, pad; /* status 0 ok, 1 CG maxit, 2 Picard maxloops, <0 breakdown */
+    double err;      /* last loop error (%) */
+    double maxj;     /* max |j| at the junction (kA/cm^2) */
+    double rel;      /* last CG relative residual */
+    double current;  /* total current through the middle junction row (mA; 2D Cartesian per um of length) */
+} pf_batch_stats;
+
+int pf_batch_create(pf_batch** out, int device);
+void pf_batch_destroy(pf_batch* b);
+const char* pf_batch_last_error(const pf_batch* b);
+/* ninst instances; per instance node counts n0 (axis 0 = r) and n1 (axis 1 = z); node
+ * axes, element conductivities (material values, natural order; junction elements hold
+ * the start value (0, start_cond)), Dirichlet (local node, value) lists with ndir
+ * entries each, junction element rows [jlo, jhi) (jlo == jhi: no junction), beta, js. */
+int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, const int32_t* n1, const double* ax0,
+                 const double* ax1, const double* sh, const double* sv, const int64_t* ndir, const int64_t* dnodes,
+                 const double* dvals, const int32_t* jlo, const int32_t* jhi, const double* beta, const double* js);
+/* Times each (block, workgroups) schedule on the batch and keeps the fastest. */
+int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int* best_nwg, double* best_seconds);
+/* Solve every instance from V = Dirichlet values / 0; V_out (concatenated, may be NULL),
+ * stats[ninst] (may be NULL), seconds = device time of the graph launch. */
+int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_stats* stats, double* seconds);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,324 @@
+// Persistent batched solver for many uneven 2D (Cartesian or cylindrical) Shockley
+// problems: one launch; each workgroup takes an instance from a work queue (largest
+// first) and runs the whole Picard loop of ElectricalFem2DSolver::compute
+// (solvers/electrical/shockley/electr2d.cpp:363-440) for it on the device:
+//   junction update (electr2d.cpp:243-264, law beta.hpp:43-46)
+//   -> line-preconditioned CG on the eliminated system (warm start)
+//   -> element currents and the loop error (electr2d.cpp:399-423).
+// Inside a workgroup: node-centric operator apply (grid-stride, all threads),
+// Thomas line solves along axis 1 (one thread per column), LDS reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pf {
+
+struct BatchInst {
+    int n0, n1;                  // nodes along r (axis 0) and z (axis 1)
+    long long node_off, elem_off;
+    int h0_off, h1_off;          // offsets of this instance's element sizes in h0[] (and rmid[]) / h1[]
+    int jlo, jhi;                // junction element rows [jlo, jhi); jlo == jhi: none
+    double jheight, beta, js;
+    int cyl;
+    int pad;
+};
+
+struct BatchStats {
+    int loops;      // Picard loops done
+    int iters;      // CG iterations summed over loops
+    int status;     // 0 ok, 1 CG maxit, 2 Picard maxloops, <0 breakdown
+    int pad;
+    double err;     // last loop error (%)
+    double maxj;    // max |j| at the junction (kA/cm^2)
+    double rel;     // last CG relative residual
+    double current; // total current through the junction middle row (mA), cylindrical pi sum j (r2^2 - r1^2) 0.01
+};
+
+struct BatchParams {
+    int maxloops, maxit;
+    double maxerr, rtol;
+    int stable;
+    int pad;
+};
+
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double wg_max(double v, double* red) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = -__builtin_huge_val();
+    for (int w = 0; w < nw; ++w) s = fmax(s, red[w]);
+    __syncthreads();
+    return s;
+}
+
+// Node-centric 2D operator row: y = M A0 (M x) + (I-M) x (RES = false) or
+// y = M (0 - A0 x) (RES = true, electrical: no load vector).  Element matrix of
+// therm2d.cpp:225-228 / electr2d.cpp:296-304, x r_mid for cylindrical geometry.
+template <bool RES>
+__device__ inline double row2d(const BatchInst& I, const double* __restrict__ h0, const double* __restrict__ h1,
+                               const double* __restrict__ rm, const double* __restrict__ kh,
+                               const double* __restrict__ kv, const uint64_t* __restrict__ dmask,
+                               const double* __restrict__ x, int i, int k) {
+    const long long nb = I.node_off;
+    const long long node = nb + (long long)k * I.n0 + i;
+    const bool di = dbit(dmask, node);
+    const double xi = x[node - nb];
+    if (di) return RES ? 0. : xi;
+    double s = 0.;
+    for (int c = 0; c < 4; ++c) {
+        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
+        if (ei < 0 || ek < 0 || ei >= I.n0 - 1 || ek >= I.n1 - 1) continue;
+        const long long e = (long long)ek * (I.n0 - 1) + ei;
+        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
+        const double r = I.cyl ? rm[I.h0_off + ei] : 1.;
+        const double kx = r * kh[I.elem_off + e] * h / w, ky = r * kv[I.elem_off + e] * w / h;
+        // local position of node i in this element
+        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
+        for (int b = 0; b < 4; ++b) {
+            const int bx = b & 1, by = b >> 1;
+            const int ni = ei + bx, nk = ek + by;
+            const long long nn = nb + (long long)nk * I.n0 + ni;
+            if (!RES && dbit(dmask, nn)) continue;
+            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
+            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
+            s += (kx * Sx * My + ky * Mx * Sy) * x[nn - nb];
+        }
+    }
+    return RES ? -s : s;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __restrict__ insts, const int* __restrict__ order,
+                                                         int ninst, unsigned int* __restrict__ queue, BatchParams prm,
+                                                         const double* __restrict__ h0, const double* __restrict__ h1,
+                                                         const double* __restrict__ rm, double* __restrict__ kh,
+                                                         double* __restrict__ kv, const uint64_t* __restrict__ dmask,
+                                                         double* __restrict__ X, double* __restrict__ R,
+                                                         double* __restrict__ P, double* __restrict__ Q,
+                                                         double* __restrict__ Z, double* __restrict__ LA,
+                                                         double* __restrict__ LW, double* __restrict__ LC,
+                                                         double* __restrict__ CUR, BatchStats* __restrict__ stats) {
+    __shared__ double red[16];
+    __shared__ int s_q;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_q = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int qi = s_q;
+        __syncthreads();
+        if (qi >= ninst) break;
+        const int id = order[qi];
+        const BatchInst I = insts[id];
+        const int n0 = I.n0, n1 = I.n1;
+        const long long N = (long long)n0 * n1, E = (long long)(n0 - 1) * (n1 - 1);
+        double* x = X + I.node_off;
+        double* r = R + I.node_off;
+        double* p = P + I.node_off;
+        double* q = Q + I.node_off;
+        double* z = Z + I.node_off;
+        double* la = LA + I.node_off;
+        double* lw = LW + I.node_off;
+        double* lc = LC + I.node_off;
+        double* cur = CUR + 2 * I.elem_off;
+        double* sh = kh + I.elem_off;
+        double* sv = kv + I.elem_off;
+        const bool has_j = I.jhi > I.jlo;
+        BatchStats S{};
+        // denominator: residual of the start field (Dirichlet values, zero elsewhere)
+        for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
+        __syncthreads();
+        double bb = 0.;
+        for (long long n = tid; n < N; n += BS) bb += r[n] * r[n];
+        bb = wg_sum(bb, red);
+        const double stop2 = prm.rtol * prm.rtol * (bb > 0. ? bb : 1.);
+        double err = 0., mcur = 0.;
+        for (int loop = 0; loop < prm.maxloops; ++loop) {
+            // junction update from the current potential (not before the first solve)
+            if (loop > 0 && has_j) {
+                const long long nje = (long long)(I.jhi - I.jlo) * (n0 - 1);
+                for (long long t = tid; t < nje; t += BS) {
+                    const int ei = t % (n0 - 1), ek = I.jlo + t / (n0 - 1);
+                    const long long e = (long long)ek * (n0 - 1) + ei;
+                    const double U = 0.5 * (x[(long long)I.jhi * n0 + ei] - x[(long long)I.jlo * n0 + ei] +
+                                            x[(long long)I.jhi * n0 + ei + 1] - x[(long long)I.jlo * n0 + ei + 1]);
+                    const double jy = fabs(0.1 * sv[e] * U / I.jheight);
+                    double cond = 10. * jy * I.jheight * I.beta / log(1e7 * jy / I.js + 1.);
+                    if (prm.stable) cond = 0.5 * (sv[e] + cond);
+                    if (isnan(cond) || fabs(cond) < 1e-16) cond = 1e-16;
+                    sv[e] = cond;
+                    sh[e] = 0.;
+                }
+                __syncthreads();
+            }
+            // line factors along axis 1 (tridiagonal part of A per column)
+            for (int i = tid; i < n0; i += BS) {
+                double cprev = 0., cpprev = 0.;
+                for (int k = 0; k < n1; ++k) {
+                    const long long node = (long long)k * n0 + i;
+                    double b = 0., c = 0.;
+                    for (int cc = 0; cc < 4; ++cc) {
+                        const int ei = i - 1 + (cc & 1), ek = k - 1 + (cc >> 1);
+                        if (ei < 0 || ek < 0 || ei >= n0 - 1 || ek >= n1 - 1) continue;
+                        const long long e = (long long)ek * (n0 - 1) + ei;
+                        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
+                        const double rr = I.cyl ? rm[I.h0_off + ei] : 1.;
+                        const double kx1 = rr * sh[e] * h / w, ky1 = rr * sv[e] * w / h;
+                        b += (kx1 + ky1) / 3.;
+                        if (ek == k) c += (kx1 - 2. * ky1) / 6.;
+                    }
+                    const bool dk = dbit(dmask, I.node_off + node);
+                    if (dk) b = 1.;
+                    if (dk || k == n1 - 1 || dbit(dmask, I.node_off + node + n0)) c = 0.;
+                    const double a = cprev, w = 1. / (b - a * cpprev), cp = c * w;
+                    la[node] = a;
+                    lw[node] = w;
+                    lc[node] = cp;
+                    cprev = c;
+                    cpprev = cp;
+                }
+            }
+            __syncthreads();
+            // CG from the current potential
+            for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
+            __syncthreads();
+            double rz = 0., rr = 0.;
+            auto precond = [&]() {
+                double acc = 0.;
+                for (int i = tid; i < n0; i += BS) {
+                    double d = 0.;
+                    for (int k = 0; k < n1; ++k) {
+                        const long long node = (long long)k * n0 + i;
+                        d = (r[node] - la[node] * d) * lw[node];
+                        z[node] = d;
+                    }
+                    double zn = 0.;
+                    for (int k = n1 - 1; k >= 0; --k) {
+                        const long long node = (long long)k * n0 + i;
+                        zn = z[node] - lc[node] * zn;
+                        z[node] = zn;
+                        acc += r[node] * zn;
+                    }
+                }
+                __syncthreads();
+                return wg_sum(acc, red);
+            };
+            for (long long n = tid; n < N; n += BS) rr += r[n] * r[n];
+            rr = wg_sum(rr, red);
+            int it = 0;
+            if (rr > stop2) {
+                rz = precond();
+                for (long long n = tid; n < N; n += BS) p[n] = z[n];
+                __syncthreads();
+                for (;;) {
+                    double pq = 0.;
+                    for (long long n = tid; n < N; n += BS) {
+                        const double v = row2d<false>(I, h0, h1, rm, kh, kv, dmask, p, n % n0, n / n0);
+                        q[n] = v;
+                        pq += p[n] * v;
+                    }
+                    pq = wg_sum(pq, red);
+                    if (!(pq > 0.)) {
+                        S.status = -6;
+                        break;
+                    }
+                    const double alpha = rz / pq;
+                    rr = 0.;
+                    for (long long n = tid; n < N; n += BS) {
+                        x[n] += alpha * p[n];
+                        const double v = r[n] - alpha * q[n];
+                        r[n] = v;
+                        rr += v * v;
+                    }
+                    rr = wg_sum(rr, red);
+                    ++it;
+                    if (rr <= stop2) break;
+                    if (it >= prm.maxit) {
+                        S.status = 1;
+                        break;
+                    }
+                    const double rz_new = precond();
+                    if (!(rz_new > 0.)) {
+                        S.status = -7;
+                        break;
+                    }
+                    const double beta = rz_new / rz;
+                    rz = rz_new;
+                    for (long long n = tid; n < N; n += BS) p[n] = z[n] + beta * p[n];
+                    __syncthreads();
+                }
+            }
+            S.iters += it;
+            S.rel = sqrt(rr / (bb > 0. ? bb : 1.));
+            // currents and the loop error
+            double dmax = 0., jmax = 0.;
+            for (long long e = tid; e < E; e += BS) {
+                const int ei = e % (n0 - 1), ek = e / (n0 - 1);
+                const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
+                const long long ll = (long long)ek * n0 + ei;
+                const double vll = x[ll], vlu = x[ll + 1], vul = x[ll + n0], vuu = x[ll + n0 + 1];
+                const double jx = sh[e] * (-0.05 * (-vll + vlu - vul + vuu) / w);
+                const double jy = sv[e] * (-0.05 * (-vll - vlu + vul + vuu) / h);
+                const double dx = jx - cur[2 * e], dy = jy - cur[2 * e + 1];
+                cur[2 * e] = jx;
+                cur[2 * e + 1] = jy;
+                dmax = fmax(dmax, dx * dx + dy * dy);
+                if (!has_j || (ek >= I.jlo && ek < I.jhi)) jmax = fmax(jmax, jx * jx + jy * jy);
+            }
+            dmax = wg_max(dmax, red);
+            jmax = wg_max(jmax, red);
+            mcur = sqrt(jmax);
+            err = 100. * sqrt(dmax) / fmax(mcur, 100e-7);
+            S.loops = loop + 1;
+            if (S.status < 0) break;
+            if (S.status == 0 && err <= prm.maxerr) break;
+            if (loop + 1 == prm.maxloops) S.status = 2;
+        }
+        // total current through the middle junction row (integrateCurrent, electr2d.cpp:466-496)
+        double tot = 0.;
+        if (has_j) {
+            const int level = (I.jlo + I.jhi) / 2;
+            for (int ei = tid; ei < n0 - 1; ei += BS) {
+                const long long e = (long long)level * (n0 - 1) + ei;
+                const double j1 = cur[2 * e + 1];
+                if (I.cyl) {
+                    double r0 = 0., r1 = 0.;
+                    // element radii from cumulative sizes: rmid +- w/2
+                    const double w = h0[I.h0_off + ei];
+                    r0 = rm[I.h0_off + ei] - 0.5 * w;
+                    r1 = rm[I.h0_off + ei] + 0.5 * w;
+                    tot += j1 * (r1 * r1 - r0 * r0);
+                } else {
+                    tot += j1 * h0[I.h0_off + ei];
+                }
+            }
+        }
+        tot = wg_sum(tot, red);
+        if (tid == 0) {
+            S.err = err;
+            S.maxj = mcur;
+            S.current = I.cyl ? tot * 3.14159265358979323846 * 0.01 : tot * 0.01;
+            stats[id] = S;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counter(unsigned int* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+}  // namespace pf

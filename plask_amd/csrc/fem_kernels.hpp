@@ -133,14 +133,16 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
 //   MODE_OP : y = M A0 (M x) + (I-M) x        (the Dirichlet-eliminated operator)
 //   MODE_RES: y = M (F - A0 x)                (residual of the full field x)
 //   FUSEP   : x = z + beta p_old is formed on the fly from pin and stored into pout
-//             (two buffers: halo nodes re-form p from pin while their owners write pout)
+//             (two buffers: halo nodes re-form p from pin while their owners write pout),
+//             and the previous iteration's solution update xsol += alpha_prev p_old is
+//             applied to owned nodes (p_old is read here anyway)
 //   DOT     : (x, y) over owned free nodes -> PcgState.pap, alpha = rz / pap
 // T is the storage and arithmetic type of vectors and element data.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, bool FUSEP, bool DOT>
 __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, const T* __restrict__ xin,
                                                   const T* __restrict__ zin, const T* __restrict__ pin,
-                                                  T* __restrict__ pout, T* __restrict__ y,
+                                                  T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
                                                   const T* __restrict__ kh, const T* __restrict__ kv,
                                                   PcgState* __restrict__ st, double* __restrict__ partials) {
@@ -167,7 +169,11 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     const long long plane = (long long)n0 * n1;
 
     double beta = 0.;
-    if (FUSEP) beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+    T alpha_prev = T(0);
+    if (FUSEP) {
+        beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+        alpha_prev = T(st->alpha);
+    }
 
     // per-column geometric factors, folded with 1e-6 (W/m -> W/um) and 1/48
     T gxf = 0, gyf = 0, gzf = 0;
@@ -223,7 +229,11 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         if (nvalid) {
             v = FUSEP ? R.z + T(beta) * R.p : R.z;
             d = R.d;
-            if (FUSEP && store_p) pout[(long long)k * plane + (long long)j * n0 + tx] = v;
+            if (FUSEP && store_p) {
+                const long long node = (long long)k * plane + (long long)j * n0 + tx;
+                pout[node] = v;
+                xsol[node] += alpha_prev * R.p;
+            }
         }
         own = v;
         own_dir = d;
@@ -360,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
 template <typename T, int MODE, bool FUSEP, bool DOT>
 __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* __restrict__ xin,
                                                   const T* __restrict__ zin, const T* __restrict__ pin,
-                                                  T* __restrict__ pout, T* __restrict__ y,
+                                                  T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
                                                   const T* __restrict__ kh, const T* __restrict__ kv,
                                                   PcgState* __restrict__ st, double* __restrict__ partials) {
@@ -378,7 +388,11 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
     const int kb = blockIdx.x * KC;
     const int ke = min(kb + KC, n1);
     double beta = 0.;
-    if (FUSEP) beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+    T alpha_prev = T(0);
+    if (FUSEP) {
+        beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+        alpha_prev = T(st->alpha);
+    }
 
     T gxf = 0, gyf = 0;
     if (evalid) {
@@ -397,8 +411,12 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
         if (nvalid) {
             long long node = (long long)k * n0 + tx;
             if (FUSEP) {
-                v = zin[node] + T(beta) * pin[node];
-                if (store_p) pout[node] = v;
+                const T po = pin[node];
+                v = zin[node] + T(beta) * po;
+                if (store_p) {
+                    pout[node] = v;
+                    xsol[node] += alpha_prev * po;
+                }
             } else {
                 v = xin[node];
             }
@@ -525,23 +543,21 @@ __device__ inline void finish_start(PcgState* st, const double (&tot)[2]) {
 }
 
 // ---------------------------------------------------------------------------
-// PCG vector step: x += alpha p; r -= alpha q; z = Dinv r; (r,z), (r,r).
+// PCG vector step: r -= alpha q; z = Dinv r; (r,z), (r,r).  (x += alpha p is applied
+// by the next fused apply, which reads p anyway, or by k_x_fixup after the last one.)
 // Grid-stride over all nodes; vectors of type T, reductions in FP64.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ x, const T* __restrict__ p,
-                                                   T* __restrict__ r, const T* __restrict__ q,
+__global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ r, const T* __restrict__ q,
                                                    const T* __restrict__ dinv, T* __restrict__ z,
                                                    PcgState* __restrict__ st, double* __restrict__ partials) {
     if (st->done) return;
     const T alpha = T(st->alpha);
     double rz = 0., rr = 0.;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        T pi = p[i], qi = q[i];
-        T xi = x[i] + alpha * pi;
+        T qi = q[i];
         T ri = r[i] - alpha * qi;
         T zi = dinv[i] * ri;
-        x[i] = xi;
         r[i] = ri;
         z[i] = zi;
         rz += (double)ri * (double)zi;
@@ -549,6 +565,17 @@ __global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ 
     }
     double v[2] = {rz, rr}, tot[2];
     if (grid_reduce<2>(v, partials, &st->counter[1], tot)) finish_update(st, tot);
+}
+
+// last solution update after the iteration stopped: x += alpha p_last, p_last in
+// pb[iters & 1] (the buffer the last fused apply wrote)
+template <typename T>
+__global__ void k_x_fixup(long long n, T* __restrict__ x, const T* __restrict__ p0, const T* __restrict__ p1,
+                          const PcgState* __restrict__ st) {
+    const T alpha = T(st->alpha);
+    const T* p = (st->iters & 1) ? p1 : p0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] += alpha * p[i];
 }
 
 // r = given residual; z = Dinv r; (r,z), (r,r) -> state (start of a CG run)
@@ -668,23 +695,25 @@ __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, c
 }
 
 // Column kernel: one thread per line.  START: z = M^-1 r only (beginning of a CG
-// run); otherwise x += alpha p; r -= alpha q; z = M^-1 r.  The forward sweep
-// keeps d in z, the backward sweep overwrites it with the solution.
+// run); otherwise r -= alpha q first.  The forward sweep keeps d in z, the
+// backward sweep overwrites it with the solution; (r, z) = r^T M^-1 r is summed in
+// the forward sweep as sum d_k^2 / w_k (M = L D L^T, D_k = 1/w_k, L^-1 r = D d), so
+// r is not read again.  (x += alpha p is done by the next fused apply.)
 template <typename T, bool START>
-__global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long plane, T* __restrict__ x,
-                                                const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q,
-                                                const T* __restrict__ la, const T* __restrict__ lw,
-                                                const T* __restrict__ lcp, T* __restrict__ z,
-                                                PcgState* __restrict__ st, double* __restrict__ partials) {
+__global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long plane, T* __restrict__ r,
+                                                const T* __restrict__ q, const T* __restrict__ la,
+                                                const T* __restrict__ lw, const T* __restrict__ lcp,
+                                                T* __restrict__ z, PcgState* __restrict__ st,
+                                                double* __restrict__ partials) {
     if (!START && st->done) return;
-    constexpr int U = 8;
+    constexpr int U = 16;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = col < ncol;
     const T alpha = START ? T(0) : T(st->alpha);
     double rr = 0., rz = 0.;
     T dprev = T(0);
     for (int k0 = 0; k0 < nplanes; k0 += U) {
-        T xv[U], pv[U], rv[U], qv[U], av[U], wv[U];
+        T rv[U], qv[U], av[U], wv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = k0 + u;
@@ -693,11 +722,7 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
                 rv[u] = r[node];
                 av[u] = la[node];
                 wv[u] = lw[node];
-                if (!START) {
-                    xv[u] = x[node];
-                    pv[u] = p[node];
-                    qv[u] = q[node];
-                }
+                if (!START) qv[u] = q[node];
             }
         }
 #pragma unroll
@@ -707,12 +732,12 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
                 const long long node = (long long)k * plane + col;
                 T ri = rv[u];
                 if (!START) {
-                    x[node] = xv[u] + alpha * pv[u];
                     ri = ri - alpha * qv[u];
                     r[node] = ri;
                 }
                 rr += (double)ri * (double)ri;
                 const T d = (ri - av[u] * dprev) * wv[u];
+                rz += (double)d * (double)d / (double)wv[u];
                 z[node] = d;
                 dprev = d;
             }
@@ -721,7 +746,7 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
     T znext = T(0);
     const int kl = nplanes - 1;
     for (int k0 = kl; k0 >= 0; k0 -= U) {
-        T dv[U], cv[U], rv[U];
+        T dv[U], cv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = k0 - u;
@@ -729,7 +754,6 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
                 const long long node = (long long)k * plane + col;
                 dv[u] = z[node];
                 cv[u] = lcp[node];
-                rv[u] = r[node];
             }
         }
 #pragma unroll
@@ -739,7 +763,6 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
                 const long long node = (long long)k * plane + col;
                 const T zk = dv[u] - cv[u] * znext;
                 z[node] = zk;
-                rz += (double)rv[u] * (double)zk;
                 znext = zk;
             }
         }

@@ -16,6 +16,7 @@
 
 #include "../../include/plask_fem.h"
 #include "fem_kernels.hpp"
+#include "batch_kernels.hpp"
 
 using namespace pf;
 
@@ -397,21 +398,21 @@ template <> Axes2<float> axes2<float>(pf_ctx* ctx) {
 
 template <typename T, int MODE, bool FUSEP, bool DOT>
 static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, T* pout, T* y, const T* F, const T* kh,
-                         const T* kv) {
+                         const T* kv, T* xsol = nullptr) {
     if (ctx->dim == 3) {
         const int BX = (int)ctx->n[0], BY = ctx->BY;
         int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
         int zt = (int)((ctx->n[2] + ctx->KC - 1) / ctx->KC);
         size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
         hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
-                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, y, F, ctx->dmask, kh, kv, ctx->st,
+                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
                            ctx->partials);
     } else {
         const int BX = (int)ctx->n[0];
         int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
         size_t shm = sizeof(T) * (size_t)(4 * (BX + 1));
         hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, axes2<T>(ctx),
-                           ctx->KC2, xin, zin, pin, pout, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+                           ctx->KC2, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
     }
 }
 
@@ -481,18 +482,19 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const int lblocks = (ncol + 63) / 64;
     const bool line = precond == 2;
     if (line)
-        hipLaunchKernelGGL((k_cg_line<T, true>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, x,
-                           (const T*)nullptr, r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
+        hipLaunchKernelGGL((k_cg_line<T, true>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, r,
+                           (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
     else
         hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
                            ctx->partials);
     auto update = [&](T* pout) {
+        (void)pout;
         if (line)
-            hipLaunchKernelGGL((k_cg_line<T, false>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, x,
-                               (const T*)pout, r, (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
+            hipLaunchKernelGGL((k_cg_line<T, false>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, r,
+                               (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
         else
-            hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, x, pout, r, q,
-                               dinv, z, ctx->st, ctx->partials);
+            hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
+                               ctx->st, ctx->partials);
     };
     // both direction buffers start at zero (beta = 0 on the first iteration)
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p, T(0));
@@ -503,7 +505,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     auto one_iter = [&](int t) {
         T* pin = pb[t & 1];
         T* pout = pb[(t + 1) & 1];
-        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
+        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
         update(pout);
     };
     int chunk = check_every > 0 ? check_every : 32;
@@ -517,7 +519,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
                 T* pin = pb[c & 1];
                 T* pout = pb[(c + 1) & 1];
                 hipEventRecord(ev[0], ctx->stream);
-                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv);
+                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
                 hipEventRecord(ev[1], ctx->stream);
                 update(pout);
                 hipEventRecord(ev[2], ctx->stream);
@@ -555,6 +557,10 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             if (ctx->hst->done) break;
         }
     }
+    // the last solution update (the fused apply of a further iteration would have done it)
+    hipLaunchKernelGGL((k_x_fixup<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, x, (const T*)p,
+                       (const T*)p2, ctx->st);
+    HIPCK(hipGetLastError());
     *iters_out = ctx->hst->iters;
     *status_out = ctx->hst->status;
     return PF_OK;
@@ -942,6 +948,27 @@ int pf_get_currents(pf_ctx* ctx, double* cur, double* heat) {
     return PF_OK;
 }
 
+int pf_set_launch(pf_ctx* ctx, int rows, int planes) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (ctx->dim == 3) {
+        if (rows < 2 || rows * ctx->n[0] > 1024 || planes < 1)
+            return fail(ctx, PF_EINVAL, "3D: need rows >= 2, rows * n0 <= 1024, planes >= 1");
+        int ytiles = (int)((ctx->n[1] + rows - 2) / (rows - 1));
+        size_t need = 2 * (size_t)ytiles * ((ctx->n[2] + planes - 1) / planes);
+        if (need > ctx->npartials) return fail(ctx, PF_EINVAL, "too many blocks for the reduction buffer");
+        ctx->BY = rows;
+        ctx->KC = planes;
+    } else {
+        if (planes < 1) return fail(ctx, PF_EINVAL, "planes >= 1");
+        size_t need = 2 * (size_t)((ctx->n[1] + planes - 1) / planes);
+        if (need > ctx->npartials) return fail(ctx, PF_EINVAL, "too many blocks for the reduction buffer");
+        ctx->KC2 = planes;
+    }
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
 int pf_set_profile(pf_ctx* ctx, int on) {
     if (!ctx) return PF_EINVAL;
     ctx->profile = on != 0;
@@ -973,6 +1000,308 @@ int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds) {
     *seconds = 1e-3 * ms / reps;
     hipEventDestroy(a);
     hipEventDestroy(b);
+    return PF_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched 2D Shockley solver (persistent kernel over a work queue)
+// ===========================================================================
+struct pf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int ninst = 0;
+    long long N = 0, E = 0;
+    BatchInst* dinst = nullptr;
+    int* dorder = nullptr;
+    unsigned int* dqueue = nullptr;
+    double *h0 = nullptr, *h1 = nullptr, *rm = nullptr;
+    double *kh = nullptr, *kv = nullptr, *kh0 = nullptr, *kv0 = nullptr;
+    uint64_t* dmask = nullptr;
+    double *x = nullptr, *x0 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *la = nullptr,
+           *lw = nullptr, *lc = nullptr, *cur = nullptr;
+    BatchStats* dstats = nullptr;
+    std::vector<long long> node_off;
+    int tuned_block = 0, tuned_nwg = 0;
+    std::vector<void*> allocs;
+    hipGraphExec_t graph = nullptr;
+    int graph_block = 0, graph_nwg = 0;
+    BatchParams graph_prm{};
+};
+
+#define BHIPCK(call)                                                                \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            b->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            return PF_EDEVICE;                                                      \
+        }                                                                           \
+    } while (0)
+
+template <typename T> static int balloc(pf_batch* b, T** ptr, size_t count) {
+    BHIPCK(hipMalloc((void**)ptr, sizeof(T) * std::max<size_t>(count, 1)));
+    b->allocs.push_back((void*)*ptr);
+    return PF_OK;
+}
+
+__global__ void k_copy_d(long long n, const double* __restrict__ a, double* __restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+extern "C" {
+
+int pf_batch_create(pf_batch** out, int device) {
+    if (!out) return PF_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PF_EDEVICE;
+    if (device < 0 || device >= ndev) return PF_EINVAL;
+    pf_batch* b = new pf_batch();
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return PF_EDEVICE;
+    }
+    *out = b;
+    return PF_OK;
+}
+
+void pf_batch_destroy(pf_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (void* p : b->allocs) hipFree(p);
+    hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* pf_batch_last_error(const pf_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, const int32_t* n1, const double* ax0,
+                 const double* ax1, const double* sh, const double* sv, const int64_t* ndir, const int64_t* dnodes,
+                 const double* dvals, const int32_t* jlo, const int32_t* jhi, const double* beta, const double* js) {
+    if (!b || ninst <= 0 || !n0 || !n1 || !ax0 || !ax1 || !sh || !sv || !ndir || !jlo || !jhi || !beta || !js)
+        return PF_EINVAL;
+    if (b->ninst) {
+        b->err = "pf_batch_set may be called once per batch object";
+        return PF_ESTATE;
+    }
+    BHIPCK(hipSetDevice(b->device));
+    std::vector<BatchInst> inst(ninst);
+    std::vector<double> h0, h1, rm;
+    long long N = 0, E = 0, a0 = 0, a1 = 0, dpos = 0;
+    b->node_off.resize(ninst);
+    for (int i = 0; i < ninst; ++i) {
+        if (n0[i] < 2 || n1[i] < 2) {
+            b->err = "instance " + std::to_string(i) + ": each axis needs at least 2 nodes";
+            return PF_EINVAL;
+        }
+        BatchInst& I = inst[i];
+        I.n0 = n0[i];
+        I.n1 = n1[i];
+        I.node_off = N;
+        I.elem_off = E;
+        I.h0_off = (int)h0.size();
+        I.h1_off = (int)h1.size();
+        for (int k = 0; k + 1 < n0[i]; ++k) {
+            double w = ax0[a0 + k + 1] - ax0[a0 + k];
+            if (!(w > 0.)) {
+                b->err = "instance " + std::to_string(i) + ": axis 0 must increase";
+                return PF_EINVAL;
+            }
+            h0.push_back(w);
+            rm.push_back(0.5 * (ax0[a0 + k] + ax0[a0 + k + 1]));
+        }
+        for (int k = 0; k + 1 < n1[i]; ++k) {
+            double h = ax1[a1 + k + 1] - ax1[a1 + k];
+            if (!(h > 0.)) {
+                b->err = "instance " + std::to_string(i) + ": axis 1 must increase";
+                return PF_EINVAL;
+            }
+            h1.push_back(h);
+        }
+        if (jlo[i] < 0 || jhi[i] < jlo[i] || jhi[i] > n1[i] - 1 || !(js[i] > 0.)) {
+            b->err = "instance " + std::to_string(i) + ": bad junction rows or js";
+            return PF_EINVAL;
+        }
+        I.jlo = jlo[i];
+        I.jhi = jhi[i];
+        I.jheight = ax1[a1 + jhi[i]] - ax1[a1 + jlo[i]];
+        I.beta = beta[i];
+        I.js = js[i];
+        I.cyl = cylindrical;
+        b->node_off[i] = N;
+        a0 += n0[i];
+        a1 += n1[i];
+        N += (long long)n0[i] * n1[i];
+        E += (long long)(n0[i] - 1) * (n1[i] - 1);
+    }
+    std::vector<double> x0(N, 0.);
+    std::vector<uint64_t> mask((N + 63) / 64, 0ull);
+    for (int i = 0; i < ninst; ++i) {
+        long long Ni = (long long)n0[i] * n1[i];
+        for (int64_t d = 0; d < ndir[i]; ++d) {
+            int64_t nd = dnodes[dpos + d];
+            if (nd < 0 || nd >= Ni) {
+                b->err = "instance " + std::to_string(i) + ": Dirichlet node out of range";
+                return PF_EINVAL;
+            }
+            long long g = inst[i].node_off + nd;
+            mask[g >> 6] |= 1ull << (g & 63);
+            x0[g] = dvals[dpos + d];
+        }
+        dpos += ndir[i];
+    }
+    // largest instances first (longest-processing-time order for the work queue)
+    std::vector<int> order(ninst);
+    for (int i = 0; i < ninst; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return (long long)n0[a] * n1[a] > (long long)n0[c] * n1[c];
+    });
+    b->ninst = ninst;
+    b->N = N;
+    b->E = E;
+    int rc;
+    if ((rc = balloc(b, &b->dinst, ninst)) || (rc = balloc(b, &b->dorder, ninst)) || (rc = balloc(b, &b->dqueue, 1)) ||
+        (rc = balloc(b, &b->h0, h0.size())) || (rc = balloc(b, &b->h1, h1.size())) || (rc = balloc(b, &b->rm, rm.size())) ||
+        (rc = balloc(b, &b->kh, E)) || (rc = balloc(b, &b->kv, E)) || (rc = balloc(b, &b->kh0, E)) ||
+        (rc = balloc(b, &b->kv0, E)) || (rc = balloc(b, &b->dmask, mask.size())) || (rc = balloc(b, &b->x, N)) ||
+        (rc = balloc(b, &b->x0, N)) || (rc = balloc(b, &b->r, N)) || (rc = balloc(b, &b->p, N)) ||
+        (rc = balloc(b, &b->q, N)) || (rc = balloc(b, &b->z, N)) || (rc = balloc(b, &b->la, N)) ||
+        (rc = balloc(b, &b->lw, N)) || (rc = balloc(b, &b->lc, N)) || (rc = balloc(b, &b->cur, 2 * E)) ||
+        (rc = balloc(b, &b->dstats, ninst)))
+        return rc;
+    BHIPCK(hipMemcpy(b->dinst, inst.data(), sizeof(BatchInst) * ninst, hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->dorder, order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->h0, h0.data(), sizeof(double) * h0.size(), hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->h1, h1.data(), sizeof(double) * h1.size(), hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->rm, rm.data(), sizeof(double) * rm.size(), hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->kh0, sh, sizeof(double) * E, hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->kv0, sv, sizeof(double) * E, hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->dmask, mask.data(), sizeof(uint64_t) * mask.size(), hipMemcpyHostToDevice));
+    BHIPCK(hipMemcpy(b->x0, x0.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+}  // extern "C"
+
+static void batch_launch(pf_batch* b, int block, int nwg, const BatchParams& prm) {
+    const int gb = 2048;
+    hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->N, b->x0, b->x);
+    hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->E, b->kh0, b->kh);
+    hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->E, b->kv0, b->kv);
+    hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, b->stream, 2 * b->E, b->cur, 0.);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->stream, b->dqueue);
+#define PF_BATCH_LAUNCH(BS)                                                                                         \
+    hipLaunchKernelGGL((k_batch_shockley2d<BS>), dim3(nwg), dim3(BS), 0, b->stream, b->dinst, b->dorder, b->ninst, \
+                       b->dqueue, prm, b->h0, b->h1, b->rm, b->kh, b->kv, b->dmask, b->x, b->r, b->p, b->q, b->z,   \
+                       b->la, b->lw, b->lc, b->cur, b->dstats)
+    if (block == 256) PF_BATCH_LAUNCH(256);
+    else if (block == 512) PF_BATCH_LAUNCH(512);
+    else PF_BATCH_LAUNCH(1024);
+#undef PF_BATCH_LAUNCH
+}
+
+static bool same_prm(const BatchParams& a, const BatchParams& c) {
+    return a.maxloops == c.maxloops && a.maxit == c.maxit && a.maxerr == c.maxerr && a.rtol == c.rtol &&
+           a.stable == c.stable;
+}
+
+static int batch_run(pf_batch* b, int block, int nwg, const BatchParams& prm, double* seconds) {
+    if (!b->graph || b->graph_block != block || b->graph_nwg != nwg || !same_prm(b->graph_prm, prm)) {
+        if (b->graph) hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        hipGraph_t g;
+        BHIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        batch_launch(b, block, nwg, prm);
+        BHIPCK(hipStreamEndCapture(b->stream, &g));
+        BHIPCK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+        b->graph_block = block;
+        b->graph_nwg = nwg;
+        b->graph_prm = prm;
+    }
+    hipEvent_t e0, e1;
+    BHIPCK(hipEventCreate(&e0));
+    BHIPCK(hipEventCreate(&e1));
+    BHIPCK(hipEventRecord(e0, b->stream));
+    BHIPCK(hipGraphLaunch(b->graph, b->stream));
+    BHIPCK(hipEventRecord(e1, b->stream));
+    BHIPCK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (seconds) *seconds = 1e-3 * ms;
+    return PF_OK;
+}
+
+extern "C" {
+
+int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int* best_nwg, double* best_seconds) {
+    if (!b || !o) return PF_EINVAL;
+    if (!b->ninst) {
+        b->err = "pf_batch_set first";
+        return PF_ESTATE;
+    }
+    BHIPCK(hipSetDevice(b->device));
+    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, 0};
+    int props_cu = 256;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) props_cu = pr.multiProcessorCount;
+    const int blocks[3] = {256, 512, 1024};
+    const int mult[2] = {1, 2};
+    double best = 1e300;
+    int bb = 1024, bn = props_cu;
+    for (int bi = 0; bi < 3; ++bi)
+        for (int mi = 0; mi < 2; ++mi) {
+            int nwg = std::min(b->ninst, props_cu * mult[mi]);
+            double t = 0.;
+            int rc = batch_run(b, blocks[bi], nwg, prm, &t);
+            if (rc) return rc;
+            if (t < best) {
+                best = t;
+                bb = blocks[bi];
+                bn = nwg;
+            }
+        }
+    b->tuned_block = bb;
+    b->tuned_nwg = bn;
+    if (best_block) *best_block = bb;
+    if (best_nwg) *best_nwg = bn;
+    if (best_seconds) *best_seconds = best;
+    return PF_OK;
+}
+
+int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_stats* stats, double* seconds) {
+    if (!b || !o) return PF_EINVAL;
+    if (!b->ninst) {
+        b->err = "pf_batch_set first";
+        return PF_ESTATE;
+    }
+    BHIPCK(hipSetDevice(b->device));
+    int props_cu = 256;
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) props_cu = pr.multiProcessorCount;
+    int block = o->block ? o->block : (b->tuned_block ? b->tuned_block : 1024);
+    if (block != 256 && block != 512 && block != 1024) {
+        b->err = "block must be 256, 512 or 1024";
+        return PF_EINVAL;
+    }
+    int nwg = o->nwg ? o->nwg : (b->tuned_nwg ? b->tuned_nwg : std::min(b->ninst, props_cu));
+    BatchParams prm{o->maxloops > 0 ? o->maxloops : 30, o->maxit > 0 ? o->maxit : 100000,
+                    o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable, 0};
+    int rc = batch_run(b, block, nwg, prm, seconds);
+    if (rc) return rc;
+    if (V_out) BHIPCK(hipMemcpy(V_out, b->x, sizeof(double) * b->N, hipMemcpyDeviceToHost));
+    if (stats) {
+        static_assert(sizeof(BatchStats) == sizeof(pf_batch_stats), "stats layout");
+        BHIPCK(hipMemcpy(stats, b->dstats, sizeof(BatchStats) * b->ninst, hipMemcpyDeviceToHost));
+    }
     return PF_OK;
 }
 

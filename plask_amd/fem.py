@@ -23,6 +23,7 @@ EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
+    "pf_set_launch",
 ]
 
 
@@ -82,6 +83,7 @@ def load_library(path=LIB_PATH):
     L.pf_kernel_times.argtypes = [P, P, P, I]
     L.pf_time_apply.argtypes = [P, I, I, C.POINTER(C.c_double)]
     L.pf_set_profile.argtypes = [P, I]
+    L.pf_set_launch.argtypes = [P, I, I]
     _lib = L
     return L
 
@@ -225,6 +227,9 @@ class FemContext:
         c = np.zeros(2, dtype=np.int64)
         self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), 2), "pf_kernel_times")
         return s, c
+
+    def set_launch(self, rows, planes):
+        self._ck(self.L.pf_set_launch(self.h, int(rows), int(planes)), "pf_set_launch")
 
     def set_profile(self, on):
         self._ck(self.L.pf_set_profile(self.h, 1 if on else 0), "pf_set_profile")

@@ -5,7 +5,7 @@ The solvers ask a geometry two things per element midpoint: the material
 (``Rectangle``/``Cuboid``), vertical stacks (``Stack2D``/``Stack3D``; ``append`` adds
 on top, ``prepend`` at the bottom), horizontal shelves (``Shelf``) and the spaces
 ``Cartesian2D``, ``Cylindrical``, ``Cartesian3D``.  Points outside every object have
-no material (``None``, the reference's empty/air).
+the background material air (kind 'empty'), as in the reference spaces.
 """
 import numpy as np
 
@@ -165,7 +165,7 @@ class Space:
     def get_material(self, *pt):
         pt = pt[0] if len(pt) == 1 else pt
         leaf = self.leaf_at(pt)
-        return leaf.material if leaf is not None else None
+        return leaf.material if leaf is not None else _mat.Air()
 
     def get_roles(self, *pt):
         pt = pt[0] if len(pt) == 1 else pt

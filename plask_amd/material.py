@@ -120,20 +120,26 @@ class _GaAs(PowerLaw):
 
 
 class _Air(Material):
+    """The background material of every space (plask/material/air.cpp): kind EMPTY,
+    cond 0.55e-14 S/m, thermk 0.0258 (T/300)^0.84 W/m/K."""
     name = "air"
     kind = "empty"
 
     def thermk(self, T=300.0, h=math.inf):
-        return (0.026, 0.026)
+        k = 0.0258 * (T / 300.0) ** 0.84
+        return (k, k)
 
     def cond(self, T=300.0):
-        return (0.0, 0.0)
+        return (0.55e-14, 0.55e-14)
+
+    def eps(self, T=300.0):
+        return 1.0
 
     def thermk_law(self):
-        return (0.026, 0.026, 0.0, 0.0)
+        return (0.0258, 0.0258, -0.84, -0.84)
 
     def cond_law(self):
-        return (0.0, 0.0, 0.0, 0.0)
+        return (0.55e-14, 0.55e-14, 0.0, 0.0)
 
 
 _registry = {"GaAs": _GaAs(), "air": _Air(), "Air": _Air()}

@@ -7,6 +7,7 @@ import logging
 import numpy as np
 from scipy.interpolate import RegularGridInterpolator
 
+from . import material as _material
 from . import mesh as _mesh
 
 try:
@@ -210,7 +211,8 @@ class FemSolver:
         self._mids = mids
         self._leaf = g.leaf_table(mids)
         leaves = g._leaves
-        self._mat = [leaves[k][0].material if k >= 0 else None for k in self._leaf]
+        air = _material.Air()
+        self._mat = [leaves[k][0].material if k >= 0 else air for k in self._leaf]
         self._roles = [leaves[k][0].roles if k >= 0 else set() for k in self._leaf]
         self._axes = [np.asarray(a) for a in msh.axes]
         self._esize = [np.diff(a) for a in self._axes]

@@ -110,3 +110,45 @@ def thermal_3d(seed=2, n=(128, 128, 256), nlayers=40, active=20, width=100.0, pe
     return dict(axes=[x, y, z], cyl=False, k_h=k_h, k_v=k_v, q=q.ravel(), dir_nodes=dn,
                 dir_vals=np.full(nx * ny, 300.0), thick=np.array(t), kxy_layers=np.array(kxy),
                 kz_layers=np.array(kz), layer_of_row=lay, active=active)
+
+
+def shockley_cyl_batch(n_inst=256, seed0=3000, nlayers=12, junction=6, rmax=50.0, contact_r=10.0,
+                       nr_range=(50, 400), nz_range=(100, 800)):
+    """Uneven 2D cylindrical Shockley instances (voltage / device sweep).
+
+    Per instance i (SplitMix64 seed seed0 + i): node counts n_r ~ U_int[50,400],
+    n_z ~ U_int[100,800]; 12 layers t ~ U[0.05,2] um with the junction layer (role
+    active) 0.02 um; sigma ~ LU[1e-2,1e5] S/m per layer; js ~ LU[1e-2,1e2] A/m^2;
+    beta ~ U[10,25] 1/V; V ~ U[1.4,2.2] V on top nodes with r <= 10 um, 0 V at z = 0.
+    Junction elements start at (0, 5) S/m (electr2d.cpp:24).
+    """
+    out = []
+    for i in range(n_inst):
+        rng = SplitMix64(seed0 + i)
+        nr = rng.randint(*nr_range)
+        nz = rng.randint(*nz_range)
+        t = [rng.uniform(0.05, 2.0) for _ in range(nlayers)]
+        sig = [rng.loguniform(1e-2, 1e5) for _ in range(nlayers)]
+        js = rng.loguniform(1e-2, 1e2)
+        beta = rng.uniform(10.0, 25.0)
+        V = rng.uniform(1.4, 2.2)
+        t[junction] = 0.02
+        r = np.linspace(0.0, rmax, nr)
+        z, npl = layered_axis(t, nz - 1, minimum=1)
+        lay = element_layers(npl)
+        E0 = nr - 1
+        sv_row = np.array(sig)[lay]
+        sh = np.repeat(sv_row, E0)
+        sv = sh.copy()
+        jrows = np.nonzero(lay == junction)[0]
+        jlo, jhi = int(jrows.min()), int(jrows.max()) + 1
+        for k in range(jlo, jhi):
+            sh[k * E0:(k + 1) * E0] = 0.0
+            sv[k * E0:(k + 1) * E0] = 5.0
+        top = (nz - 1) * nr + np.nonzero(r <= contact_r + 1e-12)[0]
+        bot = np.arange(nr)
+        dn = np.r_[bot, top].astype(np.int64)
+        dv = np.r_[np.zeros(len(bot)), np.full(len(top), V)]
+        out.append(dict(axes=[r, z], sh=sh, sv=sv, dir_nodes=dn, dir_vals=dv, jlo=jlo, jhi=jhi, beta=beta, js=js,
+                        V=V, layers=np.array(t), sigma=np.array(sig)))
+    return out

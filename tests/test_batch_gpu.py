@@ -1,0 +1,44 @@
+"""Batched persistent 2D Shockley solver vs the oracle's Picard loop (band Cholesky)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from plask_amd import workloads
+from plask_amd.batch import ShockleyBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(p, loops, maxerr):
+    r, z = p["axes"]
+    g = O.Grid([r, z], cyl=True)
+    E0 = len(r) - 1
+    elems = np.arange(p["jlo"] * E0, p["jhi"] * E0)
+    bot = [[e % E0 + p["jlo"] * len(r), e % E0 + 1 + p["jlo"] * len(r)] for e in elems]
+    top = [[e % E0 + p["jhi"] * len(r), e % E0 + 1 + p["jhi"] * len(r)] for e in elems]
+    J = dict(elems=elems, bottom=bot, top=top, height=z[p["jhi"]] - z[p["jlo"]], beta=p["beta"], js=p["js"])
+    sh = p["sh"].copy()
+    sv = p["sv"].copy()
+    return O.electrical_compute(g, sh, sv, p["dir_nodes"], p["dir_vals"], [J], loops, maxerr=maxerr)
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+def test_batch_matches_oracle_small(block):
+    insts = workloads.shockley_cyl_batch(n_inst=6, seed0=3000, nr_range=(12, 40), nz_range=(30, 70))
+    b = ShockleyBatch(insts)
+    print("batch", [len(p["axes"][0]) * len(p["axes"][1]) for p in insts], flush=True)
+    V, stats, t = b.solve(maxloops=8, maxerr=1e-30, rtol=1e-13, block=block)
+    for p, v, s in zip(insts, V, stats):
+        Vr, cur, heat, sig, loops, err = _oracle(p, 8, 0.0)
+        assert s["loops"] == 8 and s["status"] in (0, 2), s
+        assert np.abs(v - Vr).max() <= 1e-8 * np.abs(Vr).max(), (np.abs(v - Vr).max(), s)
+
+
+def test_batch_autotune_and_convergence():
+    insts = workloads.shockley_cyl_batch(n_inst=24, seed0=3100, nr_range=(20, 120), nz_range=(40, 200))
+    b = ShockleyBatch(insts)
+    blk, nwg, t = b.autotune(maxloops=30)
+    assert blk in (256, 512, 1024) and nwg >= 1 and t > 0
+    V, stats, t2 = b.solve(maxloops=30)
+    assert all(s["status"] in (0, 2) for s in stats)
+    assert sum(s["status"] == 0 for s in stats) >= len(stats) // 2

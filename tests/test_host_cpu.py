@@ -78,7 +78,7 @@ def test_geometry_stack_shelf_and_boundaries():
     sp = geometry.Cylindrical(stack)
     assert sp.get_material(10., 10.) is m1
     assert sp.has_role("active", 10., 300.01)
-    assert sp.get_material(500., 700.) is None          # the shelf gap
+    assert sp.get_material(500., 700.).kind == "empty"  # the shelf gap: background air
     assert sp.get_material(700., 700.) is m2
     msh = mesh.Rectangular2D.DivideGenerator(prediv=(2, 1))(sp)
     top = mesh.Rectangular2D.TopOf(cont).nodes(msh, sp)
