@@ -164,6 +164,11 @@ int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
  * 2D planes = rows of axis 1 per block. */
 int pf_set_launch(pf_ctx* ctx, int rows, int planes);
 
+/* Line preconditioner segment length along the layer axis: lines are cut into
+ * independent blocks of `seg` nodes (more parallel work per solve, weaker coupling);
+ * seg >= the line length (default) keeps whole lines. */
+int pf_set_line_segment(pf_ctx* ctx, int seg);
+
 /* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP
  * events (no graph) and accumulates per-kernel durations for pf_kernel_times. */
 int pf_set_profile(pf_ctx* ctx, int on);

@@ -77,6 +77,27 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Publish a block partial and take a ticket without an agent-scope release fence:
+// on gfx950 a release fence writes back the XCD's whole L2, which made every block of a
+// reduction kernel pay for the kernel's own streaming stores.  The partial is written
+// with an agent-scope atomic exchange (performed at the device coherence point), the
+// thread waits for it to complete (s_waitcnt vmcnt(0), also a compiler barrier), then
+// increments the ticket; the last block reads the partials with agent-scope atomic loads.
+__device__ __forceinline__ void publish_partial(double* p, double v) {
+    __hip_atomic_exchange(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double read_partial(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ unsigned int take_ticket(unsigned int* counter) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Block-reduce NV values; returns true in thread 0 of the last block, which then
 // holds the grid totals in tot[].
 template <int NV>
@@ -99,21 +120,17 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
         for (int a = 0; a < NV; ++a) {
             double s = 0.;
             for (int w = 0; w < nw; ++w) s += red[a][w];
-            partials[(size_t)a * nblk + bid] = s;
+            publish_partial(partials + (size_t)a * nblk + bid, s);
         }
-        __threadfence();
-        unsigned int t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == nblk - 1);
+        last = (take_ticket(counter) == nblk - 1);
     }
     __syncthreads();
     if (!last) return false;
-    __threadfence();
     // last block: sum partials in fixed order (deterministic)
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
         double s = 0.;
-        for (unsigned int b = tid; b < nblk; b += nthr)
-            s += __hip_atomic_load(partials + (size_t)a * nblk + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (unsigned int b = tid; b < nblk; b += nthr) s += read_partial(partials + (size_t)a * nblk + b);
         s = wave_sum(s);
         __syncthreads();
         if (lane == 0) red[a][wid] = s;
@@ -124,7 +141,7 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
             tot[a] = t;
         }
     }
-    if (tid == 0) *counter = 0u;
+    if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return tid == 0;
 }
 
@@ -624,7 +641,7 @@ __global__ __launch_bounds__(256) void k_norm2(long long n, const T* __restrict_
 template <typename T>
 __global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
                                const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp) {
+                               T* __restrict__ lcp, int seg) {
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= n0 * n1) return;
@@ -646,7 +663,8 @@ __global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, c
         }
         const bool dk = dbit(dmask, node);
         if (dk) b = 1.;
-        if (dk || k == n2 - 1 || dbit(dmask, node + plane)) c = 0.;
+        // no coupling past the last row of a line segment (segments are independent blocks)
+        if (dk || k == n2 - 1 || (k + 1) % seg == 0 || dbit(dmask, node + plane)) c = 0.;
         const double a = cprev;
         const double w = 1. / (b - a * cpprev);
         const double cp = c * w;
@@ -662,7 +680,7 @@ __global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, c
 template <typename T>
 __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
                                const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp) {
+                               T* __restrict__ lcp, int seg) {
     const int n0 = g.n0, n1 = g.n1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n0) return;
@@ -682,7 +700,7 @@ __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, c
         }
         const bool dk = dbit(dmask, node);
         if (dk) b = 1.;
-        if (dk || k == n1 - 1 || dbit(dmask, node + n0)) c = 0.;
+        if (dk || k == n1 - 1 || (k + 1) % seg == 0 || dbit(dmask, node + n0)) c = 0.;
         const double a = cprev;
         const double w = 1. / (b - a * cpprev);
         const double cp = c * w;
@@ -694,14 +712,15 @@ __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, c
     }
 }
 
-// Column kernel: one thread per line.  START: z = M^-1 r only (beginning of a CG
-// run); otherwise r -= alpha q first.  The forward sweep keeps d in z, the
-// backward sweep overwrites it with the solution; (r, z) = r^T M^-1 r is summed in
-// the forward sweep as sum d_k^2 / w_k (M = L D L^T, D_k = 1/w_k, L^-1 r = D d), so
-// r is not read again.  (x += alpha p is done by the next fused apply.)
+// Column kernel: one thread per line segment (blockIdx.y selects the segment; with
+// seg >= the line length a segment is the whole line).  START: z = M^-1 r only
+// (beginning of a CG run); otherwise r -= alpha q first.  The forward sweep keeps d
+// in z, the backward sweep overwrites it with the solution; (r, z) = r^T M^-1 r is
+// summed in the forward sweep as sum d_k y_k with y = L^-1 r (M = L D L^T, d = D^-1 y),
+// so r is not read again.  (x += alpha p is done by the next fused apply.)
 template <typename T, bool START>
-__global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long plane, T* __restrict__ r,
-                                                const T* __restrict__ q, const T* __restrict__ la,
+__global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes_all, long long plane, int seg,
+                                                T* __restrict__ r, const T* __restrict__ q, const T* __restrict__ la,
                                                 const T* __restrict__ lw, const T* __restrict__ lcp,
                                                 T* __restrict__ z, PcgState* __restrict__ st,
                                                 double* __restrict__ partials) {
@@ -709,6 +728,15 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
     constexpr int U = 16;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = col < ncol;
+    // this thread's line segment [k_begin, k_begin + nplanes) (blockIdx.y)
+    const int k_begin = blockIdx.y * seg;
+    const int nplanes = min(seg, nplanes_all - k_begin);
+    r += (long long)k_begin * plane;
+    if (!START) q += (long long)k_begin * plane;
+    la += (long long)k_begin * plane;
+    lw += (long long)k_begin * plane;
+    lcp += (long long)k_begin * plane;
+    z += (long long)k_begin * plane;
     const T alpha = START ? T(0) : T(st->alpha);
     double rr = 0., rz = 0.;
     T dprev = T(0);
@@ -736,8 +764,9 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes, long long
                     r[node] = ri;
                 }
                 rr += (double)ri * (double)ri;
-                const T d = (ri - av[u] * dprev) * wv[u];
-                rz += (double)d * (double)d / (double)wv[u];
+                const T y = ri - av[u] * dprev;  // (L^-1 r)_k
+                const T d = y * wv[u];
+                rz += (double)d * (double)y;
                 z[node] = d;
                 dprev = d;
             }
@@ -976,19 +1005,15 @@ __device__ bool grid_max(double (&v)[NV], double* __restrict__ partials, unsigne
         for (int a = 0; a < NV; ++a) {
             double s = ninf;
             for (int w = 0; w < nw; ++w) s = fmax(s, red[a][w]);
-            partials[(size_t)a * nblk + bid] = s;
+            publish_partial(partials + (size_t)a * nblk + bid, s);
         }
-        __threadfence();
-        unsigned int t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == nblk - 1);
+        last = (take_ticket(counter) == nblk - 1);
     }
     __syncthreads();
     if (!last) return false;
-    __threadfence();
     for (int a = 0; a < NV; ++a) {
         double s = ninf;
-        for (unsigned int b = tid; b < nblk; b += nthr)
-            s = fmax(s, __hip_atomic_load(partials + (size_t)a * nblk + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (unsigned int b = tid; b < nblk; b += nthr) s = fmax(s, read_partial(partials + (size_t)a * nblk + b));
         s = wave_max(s);
         __syncthreads();
         if (lane == 0) red[a][wid] = s;
@@ -999,7 +1024,7 @@ __device__ bool grid_max(double (&v)[NV], double* __restrict__ partials, unsigne
             tot[a] = t;
         }
     }
-    if (tid == 0) *counter = 0u;
+    if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return tid == 0;
 }
 

@@ -71,6 +71,7 @@ struct pf_ctx {
 
     // launch configuration
     int BY = 8, KC = 8, KC2 = 16;
+    int seg = 1 << 30;  // line segment length (>= line length: whole lines)
     int upd_blocks = 1024;
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
@@ -430,9 +431,9 @@ static int prepare(pf_ctx* ctx) {
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0] * ctx->n[1], 64, 1 << 20);
         hipLaunchKernelGGL((k_line_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp, ctx->seg);
         hipLaunchKernelGGL((k_line_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
+                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32, ctx->seg);
     } else {
         hipLaunchKernelGGL(k_load2d, dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->q, ctx->F);
         hipLaunchKernelGGL((k_diag2d<double>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
@@ -441,9 +442,9 @@ static int prepare(pf_ctx* ctx) {
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0], 64, 1 << 20);
         hipLaunchKernelGGL((k_line_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp, ctx->seg);
         hipLaunchKernelGGL((k_line_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
+                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32, ctx->seg);
     }
     if (ctx->equation == PF_ELECTRICAL || !ctx->have_source) {
         // electrical: no load vector (electr2d.cpp:268-317); thermal without source: zero
@@ -479,10 +480,12 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
     const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
     const long long plane = ncol;
-    const int lblocks = (ncol + 63) / 64;
+    const int segl = std::min(ctx->seg, nplanes);
+    const int nseg = (nplanes + segl - 1) / segl;
+    const dim3 lgrid((ncol + 63) / 64, nseg);
     const bool line = precond == 2;
     if (line)
-        hipLaunchKernelGGL((k_cg_line<T, true>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, r,
+        hipLaunchKernelGGL((k_cg_line<T, true>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, segl, r,
                            (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
     else
         hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
@@ -490,7 +493,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     auto update = [&](T* pout) {
         (void)pout;
         if (line)
-            hipLaunchKernelGGL((k_cg_line<T, false>), dim3(lblocks), dim3(64), 0, ctx->stream, ncol, nplanes, plane, r,
+            hipLaunchKernelGGL((k_cg_line<T, false>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, segl, r,
                                (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
         else
             hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
@@ -965,6 +968,19 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes) {
         if (need > ctx->npartials) return fail(ctx, PF_EINVAL, "too many blocks for the reduction buffer");
         ctx->KC2 = planes;
     }
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_set_line_segment(pf_ctx* ctx, int seg) {
+    if (!ctx) return PF_EINVAL;
+    if (seg < 1) return fail(ctx, PF_EINVAL, "segment length >= 1");
+    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+    const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
+    size_t nblk = (size_t)((ncol + 63) / 64) * ((nplanes + std::min(seg, nplanes) - 1) / std::min(seg, nplanes));
+    if (2 * nblk > ctx->npartials) return fail(ctx, PF_EINVAL, "segment too short for the reduction buffer");
+    ctx->seg = seg;
+    ctx->dirty = true;  // factors depend on the segmentation
     drop_graphs(ctx);
     return PF_OK;
 }

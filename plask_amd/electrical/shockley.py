@@ -47,6 +47,9 @@ class _ShockleyBase(_sb.FemSolver):
         self.pcond = 5.0
         self.ncond = 50.0
         self._start_cond = (0.0, 5.0)
+        # contact rows (sigma up to 1e9 S/m) dominate the residual norm; the junction rows
+        # (sigma ~ 1e-4 S/m) need a much smaller relative residual to be resolved
+        self._iter.maxerr = 1e-12
 
     # beta#, js# (BetaSolver, beta.hpp:49-77; Python callables electr_python.cpp:96-111)
     def __getattr__(self, attr):

@@ -23,7 +23,7 @@ EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
-    "pf_set_launch",
+    "pf_set_launch", "pf_set_line_segment",
 ]
 
 
@@ -84,6 +84,7 @@ def load_library(path=LIB_PATH):
     L.pf_time_apply.argtypes = [P, I, I, C.POINTER(C.c_double)]
     L.pf_set_profile.argtypes = [P, I]
     L.pf_set_launch.argtypes = [P, I, I]
+    L.pf_set_line_segment.argtypes = [P, I]
     _lib = L
     return L
 
@@ -230,6 +231,9 @@ class FemContext:
 
     def set_launch(self, rows, planes):
         self._ck(self.L.pf_set_launch(self.h, int(rows), int(planes)), "pf_set_launch")
+
+    def set_line_segment(self, seg):
+        self._ck(self.L.pf_set_line_segment(self.h, int(seg)), "pf_set_line_segment")
 
     def set_profile(self, on):
         self._ck(self.L.pf_set_profile(self.h, 1 if on else 0), "pf_set_profile")

@@ -157,7 +157,11 @@ def _shockley3d():
     s.mesh = mesh.Rectangular3D.DivideGenerator(prediv=(3, 3, 2), gradual=False)
     s.beta = 10.
     s.js = 1.
-    s.maxerr = 1e-3
+    # The reference test stops the Picard loop at 1e-3 %.  The junction law's fixed-point
+    # map contracts by about 1 - 1/beta = 0.9 per loop, so the error left when the loop
+    # change drops below 1e-3 % is ~9x that (~1e-4 relative), above the test's own
+    # places=3 tolerance on ~22 mA; it is run here at 1e-5 % as the 2D tests are.
+    s.maxerr = 1e-5
     s.voltage_boundary.append(s.mesh.TopOf(contact, top), 0.)
     s.voltage_boundary.append(s.mesh.BottomOf(contact, bottom), 1.)
     return s, 1e6
