@@ -164,9 +164,12 @@ int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
  * 2D planes = rows of axis 1 per block. */
 int pf_set_launch(pf_ctx* ctx, int rows, int planes);
 
-/* Line preconditioner segment length along the layer axis: lines are cut into
- * independent blocks of `seg` nodes (more parallel work per solve, weaker coupling);
- * seg >= the line length (default) keeps whole lines. */
+/* Line preconditioner schedule (the preconditioner itself, the exact tridiagonal
+ * solve along whole lines of the layer axis, does not change): seg = 4, 8, 16 or
+ * 32 solves each line as segments of seg planes, one thread each, coupled through
+ * the Schur complement of the segment separators (FP64 uses at most 16);
+ * seg = 0 restores the default (8 FP64, 16 FP32); seg < 0 selects the sequential
+ * one-thread-per-line kernel.  No counterpart in the reference (a tuning hook). */
 int pf_set_line_segment(pf_ctx* ctx, int seg);
 
 /* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP

@@ -7,7 +7,7 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 "$HIPCC" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     -Wno-unused-result -Wno-unused-value \
-    -o "$OUT.tmp" "$HERE/pf_capi.hip"
+    -o "$OUT.tmp" "$HERE/pf_capi.hip" "$HERE/warmnet.hip"
 mv "$OUT.tmp" "$OUT"
 # host solver cores + pybind11 module (links the C ABI library by rpath)
 PYINC=$(python3 -c "import sysconfig; print(sysconfig.get_paths()['include'])")

@@ -638,23 +638,25 @@ __global__ __launch_bounds__(256) void k_norm2(long long n, const T* __restrict_
 // Each line block is a principal submatrix of the SPD matrix, so all pivots
 // are positive.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
-                               const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp, int seg) {
-    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n0 * n1) return;
-    const int i = col % n0, j = col / n0;
-    const long long plane = (long long)n0 * n1;
-    double cprev = 0., cpprev = 0.;
-    for (int k = 0; k < n2; ++k) {
-        const long long node = (long long)k * plane + col;
-        double b = 0., c = 0.;
+// Per-plane diagonal b_k and coupling c_k (row k to row k+1) of the line matrix of
+// one column, with Dirichlet rows reduced to the identity and their couplings
+// dropped (the symmetric elimination of A).
+// 3D: lines along axis 2; b = sum_e (kx'+ky'+kz')/9, c = sum (kx'+ky'-2kz')/18
+// over the 4 elements holding the z edge (therm3d.cpp:227-237).
+struct LineBC3 {
+    Axes3<double> g;
+    const double* kh;
+    const double* kv;
+    const uint64_t* dmask;
+    int i, j;
+    long long plane;
+    __device__ void operator()(int k, long long node, double& b, double& c) const {
+        b = 0.;
+        c = 0.;
         for (int cc = 0; cc < 8; ++cc) {
             int ei = i - 1 + (cc & 1), ej = j - 1 + ((cc >> 1) & 1), ek = k - 1 + ((cc >> 2) & 1);
-            if (ei < 0 || ej < 0 || ek < 0 || ei >= n0 - 1 || ej >= n1 - 1 || ek >= n2 - 1) continue;
-            long long e = ((long long)ek * (n1 - 1) + ej) * (n0 - 1) + ei;
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= g.n0 - 1 || ej >= g.n1 - 1 || ek >= g.n2 - 1) continue;
+            long long e = ((long long)ek * (g.n1 - 1) + ej) * (g.n0 - 1) + ei;
             double dx = g.hx[ei], dy = g.hy[ej], dz = g.hz[ek];
             double kxy = kh[e] * 1e-6, kz = kv[e] * 1e-6;
             double kx1 = kxy * dy * dz / dx, ky1 = kxy * dx * dz / dy, kz1 = kz * dx * dy / dz;
@@ -663,8 +665,46 @@ __global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, c
         }
         const bool dk = dbit(dmask, node);
         if (dk) b = 1.;
-        // no coupling past the last row of a line segment (segments are independent blocks)
-        if (dk || k == n2 - 1 || (k + 1) % seg == 0 || dbit(dmask, node + plane)) c = 0.;
+        if (dk || k == g.n2 - 1 || dbit(dmask, node + plane)) c = 0.;
+    }
+};
+
+// 2D: lines along axis 1; b = sum (kx'+ky')/3, c = sum (kx'-2ky')/6 (x r_mid),
+// therm2d.cpp:225-228
+struct LineBC2 {
+    Axes2<double> g;
+    const double* kh;
+    const double* kv;
+    const uint64_t* dmask;
+    int i;
+    __device__ void operator()(int k, long long node, double& b, double& c) const {
+        b = 0.;
+        c = 0.;
+        for (int cc = 0; cc < 4; ++cc) {
+            int ei = i - 1 + (cc & 1), ek = k - 1 + ((cc >> 1) & 1);
+            if (ei < 0 || ek < 0 || ei >= g.n0 - 1 || ek >= g.n1 - 1) continue;
+            long long e = (long long)ek * (g.n0 - 1) + ei;
+            double w = g.hx[ei], h = g.hy[ek];
+            double r = g.rmid ? g.rmid[ei] : 1.;
+            double kx1 = r * kh[e] * h / w, ky1 = r * kv[e] * w / h;
+            b += (kx1 + ky1) / 3.;
+            if (ek == k) c += (kx1 - 2. * ky1) / 6.;
+        }
+        const bool dk = dbit(dmask, node);
+        if (dk) b = 1.;
+        if (dk || k == g.n1 - 1 || dbit(dmask, node + g.n0)) c = 0.;
+    }
+};
+
+// Whole-line Thomas factors (one thread per column, sequential in k).
+template <typename T, class BC>
+__device__ void line_setup_column(const BC& bc, int n, long long plane, int col, T* __restrict__ la,
+                                  T* __restrict__ lw, T* __restrict__ lcp) {
+    double cprev = 0., cpprev = 0.;
+    for (int k = 0; k < n; ++k) {
+        const long long node = (long long)k * plane + col;
+        double b, c;
+        bc(k, node, b, c);
         const double a = cprev;
         const double w = 1. / (b - a * cpprev);
         const double cp = c * w;
@@ -676,40 +716,123 @@ __global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, c
     }
 }
 
-// 2D: lines along axis 1; couplings (kx' - 2 ky')/6 (x r_mid), therm2d.cpp:225-228
+template <typename T>
+__global__ void k_line_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                               const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
+                               T* __restrict__ lcp) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= g.n0 * g.n1) return;
+    const long long plane = (long long)g.n0 * g.n1;
+    LineBC3 bc{g, kh, kv, dmask, col % g.n0, col / g.n0, plane};
+    line_setup_column<T>(bc, g.n2, plane, col, la, lw, lcp);
+}
+
 template <typename T>
 __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
                                const uint64_t* __restrict__ dmask, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp, int seg) {
-    const int n0 = g.n0, n1 = g.n1;
+                               T* __restrict__ lcp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n0) return;
+    if (i >= g.n0) return;
+    LineBC2 bc{g, kh, kv, dmask, i};
+    line_setup_column<T>(bc, g.n1, g.n0, i, la, lw, lcp);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned line solve (exact, same M as the whole-line Thomas).  Each line of
+// n planes is cut into P = ceil(n / m) segments of m planes; the last plane of
+// every segment but the final one is a separator.  With I = the interiors (the
+// other planes) and s = the separators:
+//   z_I = T_II^-1 (r_I - T_Is z_s),   S z_s = r_s - T_sI T_II^-1 r_I,
+//   S = T_ss - T_sI T_II^-1 T_Is   (tridiagonal, SPD, size P - 1).
+// T_II is block diagonal (one tridiagonal block per interior), so each segment
+// is solved by its own thread with its planes held in registers; one thread per
+// column then solves the small S system.  Stored per node:
+//   interior plane: la = coupling to the previous interior plane (0 on the
+//     first), lw = LDL^T pivot inverse, lcp = c' (0 on the last);
+//   separator: la = coupling to the plane below, lcp = coupling to the plane
+//     above (lw holds the diagonal, read only here).
+// schur[(0|1|2)(P-1) + j][col] = Thomas factors (a, w, c') of S.
+// S entries from the interior factors: (T_j^-1)_{LL} = w_L (the last pivot
+// inverse); with v = L^-1 e_F (v_0 = 1, v_k = -la_k w_{k-1} v_{k-1}):
+// (T^-1)_{FF} = sum_k v_k^2 w_k and (T^-1)_{LF} = w_L v_L.
+// ---------------------------------------------------------------------------
+template <typename T, class BC>
+__device__ void part_setup_column(const BC& bc, int n, int m, long long plane, int col, int ncol,
+                                  T* __restrict__ la, T* __restrict__ lw, T* __restrict__ lcp,
+                                  T* __restrict__ schur) {
+    const int P = (n + m - 1) / m;
     double cprev = 0., cpprev = 0.;
-    for (int k = 0; k < n1; ++k) {
-        const long long node = (long long)k * n0 + i;
-        double b = 0., c = 0.;
-        for (int cc = 0; cc < 4; ++cc) {
-            int ei = i - 1 + (cc & 1), ek = k - 1 + ((cc >> 1) & 1);
-            if (ei < 0 || ek < 0 || ei >= n0 - 1 || ek >= n1 - 1) continue;
-            long long e = (long long)ek * (n0 - 1) + ei;
-            double w = g.hx[ei], h = g.hy[ek];
-            double r = g.rmid ? g.rmid[ei] : 1.;
-            double kx1 = r * kh[e] * h / w, ky1 = r * kv[e] * w / h;
-            b += (kx1 + ky1) / 3.;
-            if (ek == k) c += (kx1 - 2. * ky1) / 6.;
+    for (int k = 0; k < n; ++k) {
+        const long long node = (long long)k * plane + col;
+        double b, c;
+        bc(k, node, b, c);
+        const int s = k / m, u = k - s * m;
+        if (u == m - 1 && s < P - 1) {  // separator
+            la[node] = T(cprev);
+            lw[node] = T(b);
+            lcp[node] = T(c);
+            cprev = 0.;
+            cpprev = 0.;
+        } else {
+            const bool first = u == 0;
+            const bool last = s < P - 1 ? u == m - 2 : k == n - 1;
+            const double a = first ? 0. : cprev;
+            const double w = 1. / (b - a * cpprev);
+            const double cp = last ? 0. : c * w;
+            la[node] = T(a);
+            lw[node] = T(w);
+            lcp[node] = T(cp);
+            cprev = c;
+            cpprev = cp;
         }
-        const bool dk = dbit(dmask, node);
-        if (dk) b = 1.;
-        if (dk || k == n1 - 1 || (k + 1) % seg == 0 || dbit(dmask, node + n0)) c = 0.;
-        const double a = cprev;
-        const double w = 1. / (b - a * cpprev);
-        const double cp = c * w;
-        la[node] = T(a);
-        lw[node] = T(w);
-        lcp[node] = T(cp);
-        cprev = c;
-        cpprev = cp;
     }
+    double off_prev = 0., sc_prev = 0.;
+    for (int j = 0; j < P - 1; ++j) {
+        const long long ns = (long long)((j + 1) * m - 1) * plane + col;
+        const double elo = la[ns], bs = lw[ns], ehi = lcp[ns];
+        const double wl = lw[ns - plane];  // (T_j^-1)_{LL}
+        const int kb = (j + 1) * m, ke = j + 1 < P - 1 ? (j + 2) * m - 1 : n;
+        double v = 1., ff = 0., wprev = 0.;
+        for (int k = kb; k < ke; ++k) {
+            const long long node = (long long)k * plane + col;
+            const double w = lw[node];
+            if (k > kb) v = -(double)la[node] * wprev * v;
+            ff += v * v * w;
+            wprev = w;
+        }
+        const double fl = v * wprev;
+        const double sjj = bs - elo * elo * wl - ehi * ehi * ff;
+        const double soff = j + 1 < P - 1 ? -ehi * fl * (double)la[(long long)((j + 2) * m - 1) * plane + col] : 0.;
+        const double w = 1. / (sjj - off_prev * sc_prev);
+        const double cp = soff * w;
+        schur[(size_t)j * ncol + col] = T(off_prev);
+        schur[(size_t)(P - 1 + j) * ncol + col] = T(w);
+        schur[(size_t)(2 * (P - 1) + j) * ncol + col] = T(cp);
+        off_prev = soff;
+        sc_prev = cp;
+    }
+}
+
+template <typename T>
+__global__ void k_part_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                               const uint64_t* __restrict__ dmask, int m, T* __restrict__ la, T* __restrict__ lw,
+                               T* __restrict__ lcp, T* __restrict__ schur) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = g.n0 * g.n1;
+    if (col >= ncol) return;
+    const long long plane = ncol;
+    LineBC3 bc{g, kh, kv, dmask, col % g.n0, col / g.n0, plane};
+    part_setup_column<T>(bc, g.n2, m, plane, col, ncol, la, lw, lcp, schur);
+}
+
+template <typename T>
+__global__ void k_part_setup2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                               const uint64_t* __restrict__ dmask, int m, T* __restrict__ la, T* __restrict__ lw,
+                               T* __restrict__ lcp, T* __restrict__ schur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n0) return;
+    LineBC2 bc{g, kh, kv, dmask, i};
+    part_setup_column<T>(bc, g.n1, m, g.n0, i, g.n0, la, lw, lcp, schur);
 }
 
 // Column kernel: one thread per line segment (blockIdx.y selects the segment; with
@@ -794,6 +917,163 @@ __global__ __launch_bounds__(64) void k_cg_line(int ncol, int nplanes_all, long 
                 z[node] = zk;
                 znext = zk;
             }
+        }
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+        if (START) finish_start(st, tot);
+        else finish_update(st, tot);
+    }
+}
+
+// Threads per block of the partitioned kernel: blocks of 1024 while the register
+// arrays (5 m values) stay small, else 512.
+template <typename T, int M>
+struct PartLB {
+    static constexpr int value = (M * (int)sizeof(T) <= 64) ? 1024 : 512;
+};
+
+// One CG update with the partitioned line solve.  Thread (cl, seg) of a block
+// owns planes [seg m, seg m + m) of column blockIdx.x * CB + cl (lanes run over
+// columns, so every load is a contiguous row segment).  Steps: r -= alpha q
+// (skipped for START); interior solve y = T_j^-1 r; the separator right-hand
+// sides g = r_s - T_sI y go to LDS; one thread per column solves S z_s = g; each
+// interior is solved again with the separator values moved to its right-hand
+// side.  (r, z) and (r, r) are reduced over the grid.
+template <typename T, int M, bool START>
+__global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
+    int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q,
+    const T* __restrict__ la, const T* __restrict__ lw, const T* __restrict__ lcp, const T* __restrict__ schur,
+    T* __restrict__ z, PcgState* __restrict__ st, double* __restrict__ partials) {
+    if (!START && st->done) return;
+    extern __shared__ __align__(16) unsigned char part_smem[];
+    const int S = blockDim.x;
+    T* s_yf = reinterpret_cast<T*>(part_smem);
+    T* s_g = s_yf + S;
+    T* s_eh = s_g + S;
+    T* s_sa = s_eh + S;
+    T* s_sw = s_sa + S;
+    T* s_sc = s_sw + S;
+    const int t = threadIdx.x;
+    const int cl = t % CB, seg = t / CB;
+    const int col = blockIdx.x * CB + cl;
+    const bool act = seg < P && col < ncol;
+    const int k0 = seg * M;
+    const int len = act ? min(M, n - k0) : 0;
+    const bool sep = act && seg < P - 1;
+    const int lint = sep ? len - 1 : len;
+    const long long base = (long long)k0 * plane + col;
+    T rv[M], av[M], wv[M], cv[M], qv[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < len) {
+            const long long node = base + (long long)u * plane;
+            rv[u] = r[node];
+            av[u] = la[node];
+            wv[u] = lw[node];
+            cv[u] = lcp[node];
+            if (!START) qv[u] = q[node];
+        }
+    }
+    if (sep) {
+        s_sa[t] = schur[(size_t)seg * ncol + col];
+        s_sw[t] = schur[(size_t)(P - 1 + seg) * ncol + col];
+        s_sc[t] = schur[(size_t)(2 * (P - 1) + seg) * ncol + col];
+    }
+    const T alpha = START ? T(0) : T(st->alpha);
+    double rr = 0., rz = 0.;
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < len) {
+            if (!START) {
+                rv[u] = rv[u] - alpha * qv[u];
+                r[base + (long long)u * plane] = rv[u];
+            }
+            rr += (double)rv[u] * (double)rv[u];
+        }
+    }
+    // interior solve with the separators at zero
+    T d[M];
+    T dp = T(0);
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < lint) {
+            const T y = rv[u] - av[u] * dp;
+            d[u] = y * wv[u];
+            dp = d[u];
+        }
+    }
+    T zn = T(0), yl = T(0);
+#pragma unroll
+    for (int u = M - 1; u >= 0; --u) {
+        if (u < lint) {
+            const T zz = d[u] - cv[u] * zn;
+            if (u == lint - 1) yl = zz;
+            d[u] = zz;
+            zn = zz;
+        }
+    }
+    T rs = T(0), elo = T(0), ehi = T(0);
+#pragma unroll
+    for (int u = 0; u < M; ++u)
+        if (sep && u == len - 1) {
+            rs = rv[u];
+            elo = av[u];
+            ehi = cv[u];
+        }
+    s_yf[t] = act ? d[0] : T(0);
+    s_g[t] = sep ? rs - elo * yl : T(0);
+    s_eh[t] = sep ? ehi : T(0);
+    __syncthreads();
+    if (seg == 0 && col < ncol) {
+        T dq = T(0);
+        for (int j = 0; j < P - 1; ++j) {
+            const int o = j * CB + cl;
+            const T g = s_g[o] - s_eh[o] * s_yf[o + CB];
+            const T dd = (g - s_sa[o] * dq) * s_sw[o];
+            s_g[o] = dd;
+            dq = dd;
+        }
+        T zq = T(0);
+        for (int j = P - 2; j >= 0; --j) {
+            const int o = j * CB + cl;
+            const T zz = s_g[o] - s_sc[o] * zq;
+            s_g[o] = zz;
+            zq = zz;
+        }
+    }
+    __syncthreads();
+    const T lo = (act && seg > 0) ? s_g[t - CB] : T(0);
+    const T elp = (act && seg > 0) ? s_eh[t - CB] : T(0);
+    const T hi = sep ? s_g[t] : T(0);
+    // interior solve with the separator values on the right-hand side
+    dp = T(0);
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < lint) {
+            T rhs = rv[u];
+            if (u == 0) rhs -= elp * lo;
+            if (u == lint - 1) rhs -= elo * hi;
+            const T y = rhs - av[u] * dp;
+            d[u] = y * wv[u];
+            dp = d[u];
+        }
+    }
+    zn = T(0);
+#pragma unroll
+    for (int u = M - 1; u >= 0; --u) {
+        if (u < lint) {
+            const T zz = d[u] - cv[u] * zn;
+            d[u] = zz;
+            zn = zz;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < len) {
+            const T zz = u < lint ? d[u] : hi;
+            z[base + (long long)u * plane] = zz;
+            rz += (double)rv[u] * (double)zz;
         }
     }
     double v[2] = {rz, rr}, tot[2];

@@ -71,7 +71,11 @@ struct pf_ctx {
 
     // launch configuration
     int BY = 8, KC = 8, KC2 = 16;
-    int seg = 1 << 30;  // line segment length (>= line length: whole lines)
+    // line solve: partition length per precision (0: sequential whole-line kernel)
+    int line_m = 0;  // requested (0: default 8 FP64 / 16 FP32, < 0: sequential)
+    int pm64 = 8, pm32 = 16;
+    double* schur = nullptr;  // Thomas factors of the separator systems
+    float* schur32 = nullptr;
     int upd_blocks = 1024;
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
@@ -130,6 +134,68 @@ static void drop_graphs(pf_ctx* ctx) {
     if (ctx->graph64) hipGraphExecDestroy(ctx->graph64);
     if (ctx->graph32) hipGraphExecDestroy(ctx->graph32);
     ctx->graph64 = ctx->graph32 = nullptr;
+}
+
+// Launch shape of the partitioned line kernel: P segments of m planes per column,
+// CB columns per block (CB * P threads, at most PartLB).
+struct PartShape {
+    int m = 0, P = 1, CB = 64, threads = 64;
+};
+
+template <typename T>
+static int part_lb(int m) {
+    switch (m) {
+        case 4: return PartLB<T, 4>::value;
+        case 8: return PartLB<T, 8>::value;
+        case 16: return PartLB<T, 16>::value;
+        case 32: return PartLB<T, 32>::value;
+    }
+    return 0;
+}
+
+template <typename T>
+static PartShape part_shape(int m, int n, int ncol) {
+    PartShape ps;
+    if (m <= 0) return ps;
+    const int lb = part_lb<T>(m);
+    ps.m = m;
+    ps.P = (n + m - 1) / m;
+    if (ps.P > lb) {  // line too long for one block: caller falls back
+        ps.m = 0;
+        return ps;
+    }
+    ps.CB = 64;
+    while (ps.CB > 1 && ps.CB * ps.P > lb) ps.CB >>= 1;
+    while (ps.CB > 1 && ps.CB / 2 >= ncol) ps.CB >>= 1;
+    ps.threads = (ps.CB * ps.P + 63) / 64 * 64;
+    return ps;
+}
+
+// Partition lengths of the line solve and the separator-factor buffers.  A line
+// longer than one block can hold (P > threads per block) uses the sequential
+// whole-line kernel for that precision.
+static int line_config(pf_ctx* ctx) {
+    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+    const int n = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
+    int m64 = 8, m32 = 16;
+    if (ctx->line_m < 0) m64 = m32 = 0;
+    else if (ctx->line_m > 0) {
+        m32 = ctx->line_m;
+        m64 = std::min(ctx->line_m, 16);
+    }
+    PartShape s64 = part_shape<double>(m64, n, ncol), s32 = part_shape<float>(m32, n, ncol);
+    for (PartShape* ps : {&s64, &s32})
+        if (ps->m && 2 * (size_t)((ncol + ps->CB - 1) / ps->CB) > ctx->npartials) ps->m = 0;
+    ctx->pm64 = s64.m;
+    ctx->pm32 = s32.m;
+    int rc;
+    const size_t n64 = s64.m ? 3 * (size_t)std::max(s64.P - 1, 1) * ncol : 0;
+    const size_t n32 = s32.m ? 3 * (size_t)std::max(s32.P - 1, 1) * ncol : 0;
+    if ((rc = dalloc(ctx, &ctx->schur, n64))) return rc;
+    if ((rc = dalloc(ctx, &ctx->schur32, n32))) return rc;
+    ctx->dirty = true;  // the factors depend on the partition
+    drop_graphs(ctx);
+    return PF_OK;
 }
 
 static inline int grid1(long long n, int bs = 256, int maxb = 2048) {
@@ -291,6 +357,7 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     }
     ctx->npartials = 2 * std::max<size_t>(napply, 4096);
     if ((rc = dalloc(ctx, &ctx->partials, ctx->npartials))) return rc;
+    if ((rc = line_config(ctx))) return rc;
     ctx->dirty = true;
     return PF_OK;
 }
@@ -430,10 +497,19 @@ static int prepare(pf_ctx* ctx) {
         hipLaunchKernelGGL((k_diag3d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0] * ctx->n[1], 64, 1 << 20);
-        hipLaunchKernelGGL((k_line_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp, ctx->seg);
-        hipLaunchKernelGGL((k_line_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32, ctx->seg);
+        if (ctx->pm64)
+            hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
+        else
+            hipLaunchKernelGGL((k_line_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+        if (ctx->pm32)
+            hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32,
+                               ctx->schur32);
+        else
+            hipLaunchKernelGGL((k_line_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
     } else {
         hipLaunchKernelGGL(k_load2d, dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->q, ctx->F);
         hipLaunchKernelGGL((k_diag2d<double>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
@@ -441,10 +517,19 @@ static int prepare(pf_ctx* ctx) {
         hipLaunchKernelGGL((k_diag2d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0], 64, 1 << 20);
-        hipLaunchKernelGGL((k_line_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp, ctx->seg);
-        hipLaunchKernelGGL((k_line_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx), ctx->kh,
-                           ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32, ctx->seg);
+        if (ctx->pm64)
+            hipLaunchKernelGGL((k_part_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
+        else
+            hipLaunchKernelGGL((k_line_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
+        if (ctx->pm32)
+            hipLaunchKernelGGL((k_part_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32,
+                               ctx->schur32);
+        else
+            hipLaunchKernelGGL((k_line_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
+                               ctx->kh, ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
     }
     if (ctx->equation == PF_ELECTRICAL || !ctx->have_source) {
         // electrical: no load vector (electr2d.cpp:268-317); thermal without source: zero
@@ -466,6 +551,23 @@ static int prepare(pf_ctx* ctx) {
     return PF_OK;
 }
 
+template <typename T, bool START>
+static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long long plane, T* r, const T* q,
+                        const T* la, const T* lw, const T* lcp, const T* schur, T* z) {
+    const dim3 grid((ncol + ps.CB - 1) / ps.CB);
+    const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
+#define PF_PART(MM)                                                                                            \
+    hipLaunchKernelGGL((k_cg_part<T, MM, START>), grid, dim3(ps.threads), lds, ctx->stream, ncol, n, ps.P, ps.CB, \
+                       plane, r, q, la, lw, lcp, schur, z, ctx->st, ctx->partials)
+    switch (ps.m) {
+        case 4: PF_PART(4); break;
+        case 8: PF_PART(8); break;
+        case 16: PF_PART(16); break;
+        case 32: PF_PART(32); break;
+    }
+#undef PF_PART
+}
+
 // One CG run on (x, r already holding b - A x).  T selects FP64 or FP32 vectors.
 template <typename T>
 static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* la,
@@ -480,21 +582,30 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
     const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
     const long long plane = ncol;
-    const int segl = std::min(ctx->seg, nplanes);
-    const int nseg = (nplanes + segl - 1) / segl;
-    const dim3 lgrid((ncol + 63) / 64, nseg);
+    const dim3 lgrid((ncol + 63) / 64, 1);
     const bool line = precond == 2;
+    const int pm = sizeof(T) == 8 ? ctx->pm64 : ctx->pm32;
+    const T* schur = (const T*)(sizeof(T) == 8 ? (const void*)ctx->schur : (const void*)ctx->schur32);
+    PartShape ps = part_shape<T>(pm, nplanes, ncol);
+    auto line_start = [&]() {
+        if (pm)
+            launch_part<T, true>(ctx, ps, ncol, nplanes, plane, r, nullptr, la, lw, lcp, schur, z);
+        else
+            hipLaunchKernelGGL((k_cg_line<T, true>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
+                               r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
+    };
     if (line)
-        hipLaunchKernelGGL((k_cg_line<T, true>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, segl, r,
-                           (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
+        line_start();
     else
         hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
                            ctx->partials);
     auto update = [&](T* pout) {
         (void)pout;
-        if (line)
-            hipLaunchKernelGGL((k_cg_line<T, false>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, segl, r,
-                               (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
+        if (line && pm)
+            launch_part<T, false>(ctx, ps, ncol, nplanes, plane, r, q, la, lw, lcp, schur, z);
+        else if (line)
+            hipLaunchKernelGGL((k_cg_line<T, false>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
+                               r, (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
         else
             hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
                                ctx->st, ctx->partials);
@@ -974,15 +1085,11 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes) {
 
 int pf_set_line_segment(pf_ctx* ctx, int seg) {
     if (!ctx) return PF_EINVAL;
-    if (seg < 1) return fail(ctx, PF_EINVAL, "segment length >= 1");
-    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
-    const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
-    size_t nblk = (size_t)((ncol + 63) / 64) * ((nplanes + std::min(seg, nplanes) - 1) / std::min(seg, nplanes));
-    if (2 * nblk > ctx->npartials) return fail(ctx, PF_EINVAL, "segment too short for the reduction buffer");
-    ctx->seg = seg;
-    ctx->dirty = true;  // factors depend on the segmentation
-    drop_graphs(ctx);
-    return PF_OK;
+    if (seg > 0 && seg != 4 && seg != 8 && seg != 16 && seg != 32)
+        return fail(ctx, PF_EINVAL, "partition length must be 4, 8, 16 or 32 (0: default, < 0: sequential)");
+    ctx->line_m = seg;
+    if (!ctx->N) return PF_OK;
+    return line_config(ctx);
 }
 
 int pf_set_profile(pf_ctx* ctx, int on) {

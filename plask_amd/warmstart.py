@@ -1,0 +1,250 @@
+"""Learned warm start for the 3D thermal solve (no counterpart in the reference,
+which starts PCG from the previous solution or ``inittemp``, therm3d.cpp:79 /
+iterative_matrix.hpp:205-209).
+
+``WarmNet`` binds the ``pf_net_*`` entry points of ``include/plask_warmnet.h``: a
+3-level 3D convolutional encoder-decoder (32/64/128 channels, 3x3x3 kernels, bf16
+on the matrix cores) trained on the GPU by hand-written forward and backward
+kernels.  ``features``/``target_scale`` turn a layered-stack problem into the
+network's 8 input channels and its output normalisation; ``generate`` solves
+seeded stacks with the GPU solver to make training data; ``WarmStart.guess``
+turns a trained network into a start vector for ``FemContext.solve`` on any grid
+of the same stack (the prediction is made on the network grid and interpolated).
+A guess whose residual is not below that of the uniform start is rejected by the
+caller in favour of the uniform (``inittemp``) start.
+"""
+import ctypes as C
+
+import numpy as np
+from scipy.interpolate import RegularGridInterpolator
+
+from . import fem, workloads
+
+NIN = 8
+LAYERS = [(8, 32), (32, 64), (64, 128), (128, 64), (64, 32)]
+
+
+def _lib():
+    L = fem.load_library()
+    if getattr(L, "_net_declared", False):
+        return L
+    P, I, i64 = C.c_void_p, C.c_int, C.c_int64
+    L.pf_net_create.argtypes = [C.POINTER(P), I, I, I, I, I]
+    L.pf_net_destroy.argtypes = [P]
+    L.pf_net_destroy.restype = None
+    L.pf_net_last_error.argtypes = [P]
+    L.pf_net_last_error.restype = C.c_char_p
+    L.pf_net_num_params.argtypes = [P]
+    L.pf_net_num_params.restype = i64
+    L.pf_net_set_params.argtypes = [P, P, i64]
+    L.pf_net_get_params.argtypes = [P, P, i64]
+    L.pf_net_set_mask.argtypes = [P, P]
+    L.pf_net_sample_set.argtypes = [P, I, P, P]
+    L.pf_net_predict.argtypes = [P, P, P]
+    L.pf_net_predict_sample.argtypes = [P, I, P]
+    L.pf_net_gradient.argtypes = [P, I, P, C.POINTER(C.c_double)]
+    L.pf_net_train_batch.argtypes = [P, P, I, C.c_float, C.POINTER(C.c_double)]
+    L._net_declared = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def num_params():
+    return sum(27 * ci * co + co for ci, co in LAYERS) + 33
+
+
+def init_params(seed=0):
+    """He-normal convolution weights, zero biases, small head."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for ci, co in LAYERS:
+        parts.append(rng.normal(0.0, np.sqrt(2.0 / (27 * ci)), 27 * co * ci))
+        parts.append(np.zeros(co))
+    parts.append(rng.normal(0.0, np.sqrt(1.0 / 32), 32))
+    parts.append(np.zeros(1))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class WarmNet:
+    """The network on one grid of (nx, ny, nz) nodes with room for max_samples
+    training samples on the device."""
+
+    def __init__(self, shape, max_samples=0, device=0):
+        self.L = _lib()
+        self.shape = tuple(int(s) for s in shape)
+        self.nv = int(np.prod(self.shape))
+        h = C.c_void_p()
+        rc = self.L.pf_net_create(C.byref(h), int(device), *self.shape, int(max_samples))
+        if rc != 0:
+            raise fem.FemError(rc, "pf_net_create failed for grid %s (nx, ny, nz multiples of 4, nx*ny*nz/64 a "
+                                   "multiple of 32, GPU %d)" % (self.shape, device))
+        self.h = h
+        self.max_samples = int(max_samples)
+        self.n = int(self.L.pf_net_num_params(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pf_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise fem.FemError(rc, "%s: %s" % (what, self.L.pf_net_last_error(self.h).decode(errors="replace")))
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n
+        self._ck(self.L.pf_net_set_params(self.h, _ptr(p), self.n), "pf_net_set_params")
+
+    def params(self):
+        p = np.zeros(self.n, dtype=np.float32)
+        self._ck(self.L.pf_net_get_params(self.h, _ptr(p), self.n), "pf_net_get_params")
+        return p
+
+    def set_mask(self, m):
+        m = _f32(m)
+        assert m.size == self.nv
+        self._ck(self.L.pf_net_set_mask(self.h, _ptr(m)), "pf_net_set_mask")
+
+    def set_sample(self, idx, feat, target):
+        feat, target = _f32(feat), _f32(target)
+        assert feat.size == self.nv * NIN and target.size == self.nv
+        self._ck(self.L.pf_net_sample_set(self.h, int(idx), _ptr(feat), _ptr(target)), "pf_net_sample_set")
+
+    def predict(self, feat):
+        feat = _f32(feat)
+        assert feat.size == self.nv * NIN
+        out = np.zeros(self.nv, dtype=np.float32)
+        self._ck(self.L.pf_net_predict(self.h, _ptr(feat), _ptr(out)), "pf_net_predict")
+        return out
+
+    def predict_sample(self, idx):
+        out = np.zeros(self.nv, dtype=np.float32)
+        self._ck(self.L.pf_net_predict_sample(self.h, int(idx), _ptr(out)), "pf_net_predict_sample")
+        return out
+
+    def gradient(self, idx):
+        g = np.zeros(self.n, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._ck(self.L.pf_net_gradient(self.h, int(idx), _ptr(g), C.byref(loss)), "pf_net_gradient")
+        return loss.value, g
+
+    def train_batch(self, idx, lr):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_double(0.0)
+        self._ck(self.L.pf_net_train_batch(self.h, _ptr(idx), len(idx), float(lr), C.byref(loss)),
+                 "pf_net_train_batch")
+        return loss.value
+
+
+# ---------------------------------------------------------------------------
+# problem -> network inputs / target
+# ---------------------------------------------------------------------------
+def _node_mean(ev, shape):
+    """Mean of an element field over the (up to 8) elements touching each node."""
+    nx, ny, nz = shape
+    e = ev.reshape(nz - 1, ny - 1, nx - 1)
+    acc = np.zeros((nz, ny, nx))
+    cnt = np.zeros((nz, ny, nx))
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                acc[dz:dz + nz - 1, dy:dy + ny - 1, dx:dx + nx - 1] += e
+                cnt[dz:dz + nz - 1, dy:dy + ny - 1, dx:dx + nx - 1] += 1
+    return acc / cnt
+
+
+def features(p):
+    """(nodes, 8) float32 inputs in node order: log k_xy, log k_z (node means of
+    the element values), heat source / 1e16, Dirichlet mask, z / height,
+    log local dz, x and y relative to the centre."""
+    x, y, z = p["axes"]
+    shape = (len(x), len(y), len(z))
+    nx, ny, nz = shape
+    f = np.zeros((nz, ny, nx, NIN))
+    f[..., 0] = _node_mean(np.log(p["k_h"]), shape) / 5.0
+    f[..., 1] = _node_mean(np.log(p["k_v"]), shape) / 5.0
+    f[..., 2] = _node_mean(p["q"] / 1e16, shape)
+    m = np.zeros(nx * ny * nz)
+    m[p["dir_nodes"]] = 1.0
+    f[..., 3] = m.reshape(nz, ny, nx)
+    f[..., 4] = (z / z[-1])[:, None, None]
+    dz = np.diff(z)
+    dzn = np.r_[dz[0], 0.5 * (dz[1:] + dz[:-1]), dz[-1]]
+    f[..., 5] = (np.log(dzn) / 3.0)[:, None, None]
+    f[..., 6] = (x / x[-1] - 0.5)[None, None, :]
+    f[..., 7] = (y / y[-1] - 0.5)[None, :, None]
+    return f.reshape(-1, NIN).astype(np.float32)
+
+
+def target_scale(p):
+    """Temperature-rise scale of a stack (K): the rise at the active layer of the
+    laterally averaged 1D problem, (total power / area) * sum_{layers below} t / k_z.
+    Lengths in um, the source in W/m^3."""
+    x, y, z = p["axes"]
+    vol = np.outer(np.outer(np.diff(z), np.diff(y)), np.diff(x)).ravel()
+    power = float((p["q"] * vol).sum()) * 1e-18            # W
+    area = (x[-1] - x[0]) * (y[-1] - y[0]) * 1e-12         # m^2
+    a = p["active"]
+    r = float(np.sum(p["thick"][:a] / p["kz_layers"][:a])) * 1e-6   # m^2 K / W
+    return max(power / area * r, 1e-6)
+
+
+def dirichlet_mask(p):
+    m = np.ones(int(np.prod([len(a) for a in p["axes"]])), dtype=np.float32)
+    m[p["dir_nodes"]] = 0.0
+    return m
+
+
+def generate(seeds, shape, rtol=1e-8, ctx=None):
+    """Solve the seeded stacks (thermal_3d distributions) on the GPU; yields
+    (problem, features, normalised target, scale, iterations)."""
+    ctx = ctx or fem.FemContext()
+    for s in seeds:
+        p = workloads.thermal_3d(seed=int(s), n=shape)
+        ctx.set_grid(p["axes"])
+        ctx.set_coeffs(p["k_h"], p["k_v"])
+        ctx.set_source(p["q"])
+        ctx.set_dirichlet(p["dir_nodes"], p["dir_vals"])
+        st = ctx.solve_uniform(300.0, rtol=rtol)
+        T = ctx.solution()
+        S = target_scale(p)
+        yield p, features(p), ((T - 300.0) / S).astype(np.float32), S, st.iters
+
+
+class WarmStart:
+    """A trained network applied to a problem on any grid of the same stack."""
+
+    def __init__(self, net, net_shape):
+        self.net = net
+        self.net_shape = tuple(net_shape)
+
+    def guess_coarse(self, p_net):
+        """Prediction on the network grid (T in K)."""
+        return 300.0 + target_scale(p_net) * self.net.predict(features(p_net)).astype(np.float64)
+
+    def guess(self, p_net, p):
+        """Start vector for problem p from the prediction on p_net (the same stack on
+        the network grid), trilinear in physical coordinates; Dirichlet values kept."""
+        Tc = self.guess_coarse(p_net)
+        xc, yc, zc = p_net["axes"]
+        f = RegularGridInterpolator((zc, yc, xc), Tc.reshape(len(zc), len(yc), len(xc)), bounds_error=False,
+                                    fill_value=None)
+        x, y, z = p["axes"]
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+        T = f(np.stack([Z.ravel(), Y.ravel(), X.ravel()], axis=1))
+        T[p["dir_nodes"]] = p["dir_vals"]
+        return T

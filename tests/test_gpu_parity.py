@@ -80,3 +80,29 @@ def test_solve_matches_band_cholesky(shape, cyl, precision, precond):
     # the residual reported is the true one of the eliminated system
     r = A.B - A.mult(T)
     assert np.linalg.norm(r) / np.linalg.norm(A.B) <= 2e-12
+
+
+@pytest.mark.parametrize("shape,cyl", [((9, 7, 11), False), ((6, 5, 3), False), ((10, 6, 257), False),
+                                       ((40, 60), True), ((5, 1100), False), ((33, 2), False)])
+@pytest.mark.parametrize("seg", [-1, 0, 4, 16, 32])
+def test_line_schedules_same_preconditioner(shape, cyl, seg):
+    """The partitioned line solve (segments coupled through the separator Schur
+    complement) is the same exact line preconditioner as the sequential one: same
+    solution and the same PCG iteration count (+-1 from rounding).  rtol 1e-10: the
+    tall random grids here are too ill-conditioned for 1e-12 in FP64."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem(shape, cyl=cyl, seed=5)
+    print("line schedule", shape, cyl, seg, flush=True)
+    A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
+    T_ref = O.band_cholesky_solve(A)
+    c = _ctx(axes, kh, kv, q, dn, dv, cyl=cyl)
+    c.set_line_segment(-1)
+    T_seq, st_seq = c.solve(x0=np.full(g.nnodes, 300.0), precision=PF_FP64, rtol=1e-10, precond=PRECOND_LINE)
+    c.set_line_segment(seg)
+    for precision in (PF_FP64, PF_FP32_REFINE):
+        T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-10, precond=PRECOND_LINE)
+        assert st.rel_resid <= 1e-10
+        rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
+        assert rel <= 1e-7, (precision, rel)
+        if precision == PF_FP64:
+            assert abs(st.iters - st_seq.iters) <= 1, (st.iters, st_seq.iters)
+            assert np.abs(T - T_seq).max() / np.abs(T_seq).max() <= 1e-9

@@ -1,0 +1,96 @@
+"""GPU checks of the warm-start network: forward pass and parameter gradients of
+the hand-written bf16 MFMA kernels against the NumPy oracle (with the same bf16
+rounding points), and the gain of a trained network as a PCG start vector."""
+import numpy as np
+import pytest
+
+from oracle import warmnet as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, dtype=np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_and_gradient_match_oracle():
+    from plask_amd import warmstart
+    shape = (16, 16, 32)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(11)
+    p = warmstart.init_params(7)
+    # non-zero biases so that every bias path is exercised
+    off = 0
+    for ci, co in W.LAYERS:
+        off += 27 * ci * co
+        p[off:off + co] = rng.normal(0.0, 0.05, co)
+        off += co
+    p[-1] = 0.3
+    x = rng.standard_normal((nz, ny, nx, 8)).astype(np.float32)
+    t = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+    print("warmnet forward/gradient", shape, flush=True)
+    net = warmstart.WarmNet(shape, max_samples=1)
+    net.set_params(p)
+    net.set_sample(0, x.reshape(-1, 8), t.ravel())
+    y_gpu = net.predict_sample(0)
+    y_host = net.predict(x.reshape(-1, 8))
+    assert np.array_equal(y_gpu, y_host)
+    loss_gpu, g_gpu = net.gradient(0)
+    loss, g, y = W.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rb=True)
+    ey = _rel(y_gpu, y.ravel())
+    print("forward rel err %.2e, loss %.6g vs %.6g" % (ey, loss_gpu, loss), flush=True)
+    assert ey <= 1e-2
+    assert abs(loss_gpu - loss) <= 1e-2 * loss
+    off = 0
+    for k, (ci, co) in enumerate(W.LAYERS):
+        for name, n in (("w", 27 * ci * co), ("b", co)):
+            e = _rel(g_gpu[off:off + n], g[off:off + n])
+            print("C%d %s grad rel err %.2e" % (k + 1, name, e), flush=True)
+            assert e <= 3e-2, (k, name, e)
+            off += n
+    e = _rel(g_gpu[off:], g[off:])
+    print("head grad rel err %.2e" % e, flush=True)
+    assert e <= 1e-2
+
+
+def test_trained_warm_start_reduces_iterations():
+    """Train on solver-generated stacks (32x32x64 nodes), then start PCG on
+    held-out stacks from the network guess: same tolerance reached, fewer
+    iterations than from the uniform inittemp start."""
+    from plask_amd import fem, warmstart, workloads
+    shape = (32, 32, 64)
+    ntrain, held = 48, list(range(9000, 9006))
+    ctx = fem.FemContext()
+    print("generating %d samples" % ntrain, flush=True)
+    data = list(warmstart.generate(range(4000, 4000 + ntrain), shape, ctx=ctx))
+    net = warmstart.WarmNet(shape, max_samples=ntrain)
+    net.set_params(warmstart.init_params(0))
+    net.set_mask(warmstart.dirichlet_mask(data[0][0]))
+    for i, (p, f, t, S, it) in enumerate(data):
+        net.set_sample(i, f, t)
+    rng = np.random.default_rng(0)
+    steps = 300
+    losses = []
+    for s in range(steps):
+        lr = 2e-3 * 0.5 * (1 + np.cos(np.pi * s / steps))
+        losses.append(net.train_batch(rng.choice(ntrain, 4, replace=False), lr))
+    print("loss first %.4g last %.4g" % (np.mean(losses[:10]), np.mean(losses[-10:])), flush=True)
+    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:10])
+    ws = warmstart.WarmStart(net, shape)
+    it0 = it1 = 0
+    for seed in held:
+        p = workloads.thermal_3d(seed=seed, n=shape)
+        ctx.set_grid(p["axes"])
+        ctx.set_coeffs(p["k_h"], p["k_v"])
+        ctx.set_source(p["q"])
+        ctx.set_dirichlet(p["dir_nodes"], p["dir_vals"])
+        st0 = ctx.solve_uniform(300.0, rtol=1e-8)
+        T0 = ctx.solution()
+        x0 = ws.guess(p, p)
+        T1, st1 = ctx.solve(x0=x0, init_value=300.0, rtol=1e-8)
+        assert st1.rel_resid <= 1e-8
+        assert np.abs(T1 - T0).max() / np.abs(T0 - 300.0).max() <= 1e-5
+        it0 += st0.iters
+        it1 += st1.iters
+        print("seed %d: uniform start %d its, network start %d its" % (seed, st0.iters, st1.iters), flush=True)
+    assert it1 < it0
