@@ -8,10 +8,12 @@ A step is one complete solve from the reference start field (inittemp = 300 K, m
 the device) to ||B - A T|| <= 1e-8 ||B - A T_init||, with the element data, source
 and Dirichlet set already resident in HBM.  The solution stays on the device.
 
-Printed JSON line: value = solves/s of the plain FP64 Jacobi-PCG; extra fields give the
-FP32-operator + FP64-refinement form, the roofline of the dominant kernel (HIP events
-around each launch of an extra profiled solve) and the CPU reference path timed on
-this host (serial IC(0)-CG on the reference's 14-diagonal storage, bounded sample).
+Printed JSON line: value = solves/s of FP64 PCG with the exact line (layer-axis)
+preconditioner; extra fields give the FP32-operator + FP64-refinement form, the
+plain Jacobi-PCG of the comparison, the learned-warm-start form when trained weights
+are present, the roofline of the dominant kernel (HIP events around each launch of
+an extra profiled solve, on the context stream) and the CPU path timed on this host
+(serial IC(0)-CG on the reference's 14-diagonal storage, bounded sample).
 """
 import argparse
 import json
@@ -28,11 +30,13 @@ from plask_amd import workloads  # noqa: E402
 from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PRECOND_JACOBI, PRECOND_LINE, FemContext  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Full-solve iteration counts of the CPU reference path (serial IC(0)-CG and Jacobi-CG
-# on the 14-diagonal storage, same inputs, same stopping rule) for 128x128x256, used to
-# extrapolate the bounded per-iteration sample to a whole solve.  Iteration counts are a
-# property of the algorithm and the inputs, not of the host.
-CPU_FULL_ITERS = {"ic0": 161, "jacobi": None}
+# Full-solve iteration count of the CPU path (serial IC(0)-CG on the 14-diagonal
+# storage, same inputs, same stopping rule) for 128x128x256, measured once by a
+# complete CPU solve; used to scale the bounded per-iteration sample to a whole
+# solve.  Iteration counts depend on the algorithm and the inputs, not the host.
+CPU_FULL_ITERS = {(128, 128, 256): 161}
+WEIGHTS = os.path.join(ROOT, "plask_amd", "data", "warmstart_64x64x128.npz")
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def make_ctx(p, profile=False):
@@ -44,6 +48,22 @@ def make_ctx(p, profile=False):
     return c
 
 
+def kernel_bytes(shape, m=8):
+    """Algorithmic HBM bytes per launch (FP64) of the kernels of one PCG iteration.
+    apply_fused: read z, p_old, x; write p_new, q, x (48 B/node); k_xy, k_z
+      (16 B/element); Dirichlet bitmask (N/8 B).
+    line (partitioned, segments of m planes): read r, q, a, w, c'; write r, z
+      (56 B/node); separator factors 3 x 8 B per separator per column.
+    apply_plain: read x, write y (16 B/node), 16 B/element, bitmask."""
+    nx, ny, nz = shape
+    N, E = nx * ny * nz, (nx - 1) * (ny - 1) * (nz - 1)
+    mask = 8 * ((N + 63) // 64)
+    P = (nz + m - 1) // m
+    return {"apply_fused": 48 * N + 16 * E + mask,
+            "line": 56 * N + 24 * (P - 1) * nx * ny,
+            "apply_plain": 16 * N + 16 * E + mask}
+
+
 def cpu_baseline(p, seconds, full_iters):
     """Serial oracle: assemble the 14-diagonal system, time IC(0)-CG iterations."""
     import oracle as O
@@ -52,7 +72,6 @@ def cpu_baseline(p, seconds, full_iters):
     A = O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_DIA, p["dir_nodes"], p["dir_vals"])
     t_asm = time.perf_counter() - t0
     x0 = np.full(g.nnodes, 300.0)
-    # probe one iteration to size the sample
     t0 = time.perf_counter()
     O.dia_pcg(A, x0, precond=2, rtol=1e-30, maxit=1)
     t_one = time.perf_counter() - t0
@@ -68,6 +87,45 @@ def cpu_baseline(p, seconds, full_iters):
     return res
 
 
+def timed(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = fn()
+    return (time.perf_counter() - t0) / steps, st
+
+
+def learned(ctx, p, size, steps, warmup, T64):
+    """FP32+refinement PCG from the warm-start network guess (prediction on the
+    64x64x128 grid of the same stack, trilinear onto the solve grid); the network
+    forward pass and the interpolation are inside the timed step."""
+    from plask_amd import warmstart
+    from plask_amd.train_warmstart import load_weights
+    params, nshape = load_weights(WEIGHTS)
+    net = warmstart.WarmNet(nshape)
+    net.set_params(params)
+    ws = warmstart.WarmStart(net, nshape)
+    pc = workloads.thermal_3d(seed=2, n=nshape)
+    feat = warmstart.features(pc)          # inputs of the network (host, not timed)
+
+    def step():
+        ws.start_on_device(ctx, pc, feat)
+        return ctx.solve_warm(precision=PF_FP32_REFINE, rtol=1e-8, inner_rtol=1e-4, init_value=300.0, checked=True)
+
+    t, st = timed(step, steps, warmup)
+    T = ctx.solution()
+    t0 = time.perf_counter()
+    ws.start_on_device(ctx, pc, feat)
+    t_net = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    feat = warmstart.features(pc)
+    t_feat = time.perf_counter() - t0
+    return dict(solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters, outer=st.outer, rel_resid=st.rel_resid,
+                fell_back_to_uniform_start=bool(st.start),
+                network_and_prolong_ms=1e3 * t_net, host_features_ms_untimed=1e3 * t_feat, max_rel_diff_vs_fp64=float(np.abs(T - T64).max() / np.abs(T64).max()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +135,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="plain kernel launches instead of hipGraph replay")
+    ap.add_argument("--counters", action="store_true",
+                    help="short run for hardware-counter profiles: stream-copy calibration, one FP64 solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the solve path has no multi-GPU "
@@ -87,71 +148,83 @@ def main():
     N = int(np.prod(size))
     E = int(np.prod([s - 1 for s in size]))
     ctx = make_ctx(p)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     rtol = 1e-8
+    if args.counters:
+        cb = 1 << 30
+        t = ctx.stream_copy(cb, reps=3)
+        print("stream copy %d bytes: %.1f us, %.0f GB/s" % (cb, 1e6 * t, 2 * cb / t / 1e9), file=sys.stderr)
+        st = ctx.solve_uniform(300.0, rtol=rtol)
+        print(json.dumps({"counters_run": True, "copy_bytes": cb, "iters": st.iters}))
+        return
 
     def step(prec, pc=PRECOND_LINE):
         return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-4 if prec else 0.0, precond=pc)
 
-    for _ in range(args.warmup):
-        st = step(PF_FP64)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = step(PF_FP64)
-    t = time.perf_counter() - t0
-    fp64 = dict(solves_per_s=args.steps / t, ms=1e3 * t / args.steps, iters=st.iters, rel_resid=st.rel_resid)
+    t, st = timed(lambda: step(PF_FP64), args.steps, args.warmup)
+    fp64 = dict(solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
     print("fp64: %.2f solves/s, %.1f ms/solve, %d iterations, rel resid %.2e" %
           (fp64["solves_per_s"], fp64["ms"], st.iters, st.rel_resid), file=sys.stderr, flush=True)
     T64 = ctx.solution()
 
-    # plain Jacobi-PCG for reference (one untimed-warm solve + timed)
-    step(PF_FP64, PRECOND_JACOBI)
-    t0 = time.perf_counter()
-    stj = step(PF_FP64, PRECOND_JACOBI)
-    tj = time.perf_counter() - t0
+    tj, stj = timed(lambda: step(PF_FP64, PRECOND_JACOBI), 1, 1)
     jac = dict(solves_per_s=1.0 / tj, ms=1e3 * tj, iters=stj.iters, rel_resid=stj.rel_resid)
     print("fp64 jacobi: %.2f solves/s, %d iterations" % (jac["solves_per_s"], stj.iters), file=sys.stderr, flush=True)
 
     mixed = None
     if not args.no_mixed:
-        for _ in range(args.warmup):
-            st = step(PF_FP32_REFINE)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = step(PF_FP32_REFINE)
-        t = time.perf_counter() - t0
+        t, st = timed(lambda: step(PF_FP32_REFINE), args.steps, args.warmup)
         T32 = ctx.solution()
-        mixed = dict(solves_per_s=args.steps / t, ms=1e3 * t / args.steps, inner_iters=st.iters, outer=st.outer,
-                     rel_resid=st.rel_resid, max_rel_diff_vs_fp64=float(np.abs(T32 - T64).max() / np.abs(T64).max()))
+        mixed = dict(solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters, outer=st.outer, rel_resid=st.rel_resid,
+                     max_rel_diff_vs_fp64=float(np.abs(T32 - T64).max() / np.abs(T64).max()),
+                     speedup_vs_fp64=fp64["ms"] / (1e3 * t))
         print("fp32+refine: %.2f solves/s, %.1f ms/solve, %d inner its, %d outer, rel resid %.2e" %
               (mixed["solves_per_s"], mixed["ms"], st.iters, st.outer, st.rel_resid), file=sys.stderr, flush=True)
 
-    # roofline: one profiled solve (HIP events around each kernel launch on the context stream)
+    warm = None
+    if not args.no_mixed and os.path.exists(WEIGHTS):
+        warm = learned(ctx, p, size, args.steps, args.warmup, T64)
+        warm["speedup_vs_fp64"] = fp64["ms"] / warm["ms"]
+        print("fp32+refine+learned start: %.2f solves/s, %.1f ms/solve, %d inner its" %
+              (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
+
+    # roofline: one profiled FP64 solve (HIP events around each launch on the context stream)
     ctx.set_profile(True)
     ctx.solve_uniform(300.0, precision=PF_FP64, rtol=rtol)
     ctx.set_profile(False)
-    (t_apply, t_upd), (n_apply, n_upd) = ctx.kernel_times()
-    mask_bytes = 8 * ((N + 63) // 64)
-    bytes_apply = 32 * N + 16 * E + mask_bytes      # z, p_old read; p_new, q write; kxy, kz; mask
-    bytes_upd = 104 * N  # fwd: x,p,r,q,a,w read, x,r,z write; bwd: z,c',r read, z write
+    (t_apply, t_line), (n_apply, n_line) = ctx.kernel_times()
     t_plain = ctx.time_apply(PF_FP64, reps=50)
-    bytes_plain = 16 * N + 16 * E + mask_bytes      # x read, y write, kxy, kz, mask
+    kb = kernel_bytes(size)
     kern = {
-        "apply_fused": dict(avg_s=t_apply, launches=int(n_apply), bytes=bytes_apply, gbs=bytes_apply / t_apply / 1e9),
-        "cg_line": dict(avg_s=t_upd, launches=int(n_upd), bytes=bytes_upd, gbs=bytes_upd / t_upd / 1e9),
-        "apply_plain": dict(avg_s=t_plain, bytes=bytes_plain, gbs=bytes_plain / t_plain / 1e9),
+        "apply_fused": dict(avg_us=1e6 * t_apply, launches=int(n_apply), bytes=kb["apply_fused"]),
+        "line": dict(avg_us=1e6 * t_line, launches=int(n_line), bytes=kb["line"]),
+        "apply_plain": dict(avg_us=1e6 * t_plain, bytes=kb["apply_plain"]),
     }
-    dom = "apply_fused" if t_apply >= t_upd else "cg_line"
+    for k in kern.values():
+        k["gbs"] = round(k["bytes"] / (1e-6 * k["avg_us"]) / 1e9, 1)
+        k["avg_us"] = round(k["avg_us"], 2)
+    t_copy = ctx.stream_copy(1 << 30, reps=5)
+    copy_gbs = 2 * (1 << 30) / t_copy / 1e9
+    dom = "apply_fused" if t_apply >= t_line else "line"
     k = kern[dom]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": k["bytes"], "avg_launch_us": round(1e6 * k["avg_s"], 2)}
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as fh:
+            tr = json.load(fh)
+        if tr.get("workload") == "thermal3d_%dx%dx%d" % size:
+            traffic = tr.get("kernels", {}).get(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": k["bytes"], "avg_launch_us": k["avg_us"],
+                "stream_copy_gbs": round(copy_gbs, 1)}
 
     cpu = None
     if not args.no_cpu:
-        full = CPU_FULL_ITERS["ic0"] if size == (128, 128, 256) else None
+        full = CPU_FULL_ITERS.get(size)
         c = cpu_baseline(p, args.cpu_seconds, full)
         cpu = {"value": c.get("value"), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": "serial IC(0)-CG (reference default preconditioner, NSPCG ic2 semantics) on the "
+               "sample": "serial IC(0)-CG (the reference's default preconditioner, NSPCG ic2 semantics) on the "
                          "14-diagonal SparseBandMatrix storage of the same system: assembly timed whole, "
                          "%d iterations timed (%.3f s/iter) and scaled to the %s iterations of the full "
                          "solve to the same stopping rule" % (c["iters_sampled"], c["s_per_iter"], full),
@@ -170,15 +243,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded layered stack, SplitMix64)",
-        "config": {"workload": "thermal3d_%dx%dx%d_fp64_zline_pcg" % size, "dof": N, "elements": E,
-                   "rtol": rtol, "residual": "||B-AT||/||B-AT_init||, T_init = 300 K",
+        "config": {"workload": "thermal3d_%dx%dx%d" % size, "method": "fp64 pcg, exact line preconditioner",
+                   "dof": N, "elements": E, "rtol": rtol,
+                   "residual": "||B-AT||/||B-AT_init||, T_init = 300 K",
                    "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]},
         "mixed_precision": mixed,
+        "learned_warm_start": warm,
         "fp64_jacobi_pcg": jac,
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if cpu and cpu.get("value"):
+        line["speedup_vs_cpu"] = round(fp64["solves_per_s"] / cpu["value"], 1)
     print(json.dumps(line))
 
 

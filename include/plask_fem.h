@@ -73,6 +73,8 @@ typedef struct pf_ctx pf_ctx;
 #define PF_X0_ZERO 1
 #define PF_X0_DEVICE 2 /* start from the solution kept on the device from the previous pf_solve */
 #define PF_X0_UNIFORM 3 /* start from init_value everywhere (the reference's inittemp field), on the device */
+#define PF_X0_DEVICE_CHECKED 4 /* the device vector (e.g. a learned guess) unless its residual exceeds that of the
+                                  uniform init_value start, which is then used instead (norm must be PF_NORM_INIT) */
 
 /* denominator of the relative residual */
 #define PF_NORM_INIT 0   /* ||B - A X_init||, X_init = init_value at free nodes + the Dirichlet values:
@@ -102,6 +104,7 @@ typedef struct {
     double bnorm;       /* the denominator */
     double seconds;     /* wall time of the solve on the device */
     int status;         /* PF_OK or an error code */
+    int start;          /* PF_X0_DEVICE_CHECKED: 0 = the device vector was used, 1 = the uniform start */
 } pf_solve_stats;
 
 /* context ------------------------------------------------------------------ */
@@ -172,6 +175,26 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes);
  * one-thread-per-line kernel.  No counterpart in the reference (a tuning hook). */
 int pf_set_line_segment(pf_ctx* ctx, int seg);
 
+/* 3D operator kernel for one precision (PF_FP64, PF_FP32_REFINE, or -1 for both):
+ * kind 0 = tile blocks (shape from pf_set_launch; default for FP64), kind 1 = row
+ * blocks, one block per node row and chunk of `planes` >= 4 planes along axis 2
+ * (default for FP32, 16 planes).  A tuning hook, no reference counterpart. */
+int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes);
+
+/* on = 1 (default): CG iterations are replayed in chunks from hipGraphs; 0: plain
+ * launches (e.g. under tools that do not follow graph launches).  No reference
+ * counterpart. */
+int pf_set_graphs(pf_ctx* ctx, int on);
+
+/* Planes of node and element data the 3D operator kernel keeps in flight ahead
+ * of the plane it computes (1 or 2).  A tuning hook with no reference counterpart. */
+int pf_set_prefetch(pf_ctx* ctx, int depth);
+
+/* Device-to-device streaming copy of `bytes` bytes (read + write), timed over
+ * `reps` launches: the achievable HBM bandwidth and the calibration kernel of
+ * hardware traffic counters.  Diagnostic, no reference counterpart. */
+int pf_stream_copy(pf_ctx* ctx, int64_t bytes, int reps, double* seconds);
+
 /* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP
  * events (no graph) and accumulates per-kernel durations for pf_kernel_times. */
 int pf_set_profile(pf_ctx* ctx, int on);
@@ -184,6 +207,13 @@ int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds);
 
 /* Set the device solution (N values), e.g. the initial field inittemp (therm2d.cpp:79). */
 int pf_solution_set(pf_ctx* ctx, const double* x);
+
+/* Start vector from node values on a coarser rectilinear grid of the same domain
+ * (3D): x = offset + scale * coarse, trilinear in physical coordinates, then the
+ * Dirichlet values; solve next with x0 = PF_X0_DEVICE.  Used for the learned
+ * warm start (the network predicts on its own grid).  No reference counterpart. */
+int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int ncx, const double* cy, int ncy,
+                        const double* cz, int ncz, double offset, double scale);
 
 /* max |X - X_saved| and max X, then X_saved = X: the loop error and maxT of
  * ThermalFem2DSolver::compute (therm2d.cpp:479-487). */

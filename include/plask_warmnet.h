@@ -44,6 +44,11 @@ int pf_net_get_params(pf_net* net, float* params, int64_t n);
 /* Per-node loss weights (0 excludes a node, e.g. Dirichlet nodes). */
 int pf_net_set_mask(pf_net* net, const float* mask);
 
+/* Loss weight lambda of the neighbour differences:
+ * L = [sum m_v e_v^2 + lambda sum_edges (e_a - e_b)^2] / sum m_v, e = y - target,
+ * edges = x, y, z neighbour pairs (default 0). */
+int pf_net_set_loss(pf_net* net, float lambda);
+
 /* Training sample idx: inputs [v][8] and target [v] (host fp32). */
 int pf_net_sample_set(pf_net* net, int idx, const float* feat, const float* target);
 
@@ -51,7 +56,7 @@ int pf_net_sample_set(pf_net* net, int idx, const float* feat, const float* targ
 int pf_net_predict(pf_net* net, const float* feat, float* out);
 int pf_net_predict_sample(pf_net* net, int idx, float* out);
 
-/* Loss (weighted mean square error) and its gradient with respect to every
+/* Loss (as pf_net_set_loss) and its gradient with respect to every
  * parameter for stored sample idx (for gradient checks). */
 int pf_net_gradient(pf_net* net, int idx, float* grads, double* loss);
 

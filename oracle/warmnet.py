@@ -112,8 +112,9 @@ def forward(p, x, rb=False):
                                       W=[W1, W2, W3, W4, W5], h=h)
 
 
-def loss_grad(p, x, t, mask=None, rb=False):
-    """Weighted mean square error sum m (y - t)^2 / sum m and its gradient."""
+def loss_grad(p, x, t, mask=None, rb=False, lam=0.0):
+    """Loss [sum m (y - t)^2 + lam sum_edges (e_a - e_b)^2] / sum m (edges: x, y, z
+    neighbour pairs of e = y - t) and its gradient."""
     R = bf16 if rb else (lambda a: a)
     y, c = forward(p, x, rb)
     m = np.ones_like(t) if mask is None else mask
@@ -121,6 +122,17 @@ def loss_grad(p, x, t, mask=None, rb=False):
     e = y - t
     loss = float((m * e * e).sum() * inv)
     gy = 2.0 * m * e * inv
+    for ax in range(3):
+        d = np.diff(e, axis=ax)
+        loss += float(lam * (d * d).sum() * inv)
+        g = np.zeros_like(e)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        g[tuple(lo)] -= d
+        g[tuple(hi)] += d
+        gy += 2.0 * lam * g * inv
     W1, W2, W3, W4, W5 = c["W"]
     gh = (gy[..., None] * c["d1"]).reshape(-1, 32).sum(0)
     ghb = gy.sum()

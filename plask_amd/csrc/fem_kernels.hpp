@@ -156,7 +156,7 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
 //   DOT     : (x, y) over owned free nodes -> PcgState.pap, alpha = rz / pap
 // T is the storage and arithmetic type of vectors and element data.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE, bool FUSEP, bool DOT>
+template <typename T, int MODE, bool FUSEP, bool DOT, int PFD = 1>
 __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, const T* __restrict__ xin,
                                                   const T* __restrict__ zin, const T* __restrict__ pin,
                                                   T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     // of plane k (own row and, for the last thread row, row j+1) and the element data
     // of layer k-1.  With one 1024-thread block per CU nothing else hides HBM latency.
     struct Raw {
-        T z, p, zx, px, kxy, kz;
+        T z, p, zx, px, kxy, kz, x;
         bool d, dx;
     };
     auto gload = [&](int k, Raw& R) {
@@ -218,6 +218,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
             if (FUSEP) {
                 R.z = zin[node];
                 R.p = pin[node];
+                if (owned && k >= kb && k < ke) R.x = xsol[node];
             } else {
                 R.z = xin[node];
             }
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
             if (FUSEP && store_p) {
                 const long long node = (long long)k * plane + (long long)j * n0 + tx;
                 pout[node] = v;
-                xsol[node] += alpha_prev * R.p;
+                xsol[node] = R.x + alpha_prev * R.p;
             }
         }
         own = v;
@@ -280,13 +281,16 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     int buf = 0;
     T own_cur, own_next = T(0);
     bool dir_cur, dir_next = false;
-    Raw R;
-    R.z = R.p = R.zx = R.px = R.kxy = R.kz = T(0);
-    R.d = R.dx = false;
+    // PFD planes in flight: R1 = plane P + 1, R2 = plane P + 2 (PFD == 2)
+    Raw R1, R2;
+    R1.z = R1.p = R1.zx = R1.px = R1.kxy = R1.kz = R1.x = T(0);
+    R1.d = R1.dx = false;
+    R2 = R1;
     __syncthreads();
-    gload(P0, R);
-    lstore(P0, 0, R, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
-    if (P0 + 1 < n2) gload(P0 + 1, R);
+    gload(P0, R1);
+    lstore(P0, 0, R1, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
+    if (P0 + 1 < n2) gload(P0 + 1, R1);
+    if (PFD == 2 && P0 + 2 < n2 && P0 + 2 <= ke) gload(P0 + 2, R2);
     __syncthreads();
     T xt_cur[4], xt_next[4] = {T(0), T(0), T(0), T(0)};
     if (active) plane_tf(0, xt_cur);
@@ -297,15 +301,20 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         const bool has_next = (P + 1) < n2;
         T bot[4] = {T(0), T(0), T(0), T(0)}, top[4] = {T(0), T(0), T(0), T(0)};
         if (has_next) {
-            lstore(P + 1, buf ^ 1, R, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
+            lstore(P + 1, buf ^ 1, R1, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
             T a = T(0), b = T(0), c = T(0);
             if (evalid) {
                 const T hz = g.hz[P];
-                a = R.kxy * (gxf * hz);
-                b = R.kxy * (gyf * hz);
-                c = R.kz * (gzf / hz);
+                a = R1.kxy * (gxf * hz);
+                b = R1.kxy * (gyf * hz);
+                c = R1.kz * (gzf / hz);
             }
-            if (P + 2 < n2 && P + 1 < ke) gload(P + 2, R);  // prefetch the next step
+            if (PFD == 2) {
+                R1 = R2;
+                if (P + 3 < n2 && P + 3 <= ke) gload(P + 3, R2);  // prefetch two steps ahead
+            } else if (P + 2 < n2 && P + 2 <= ke) {
+                gload(P + 2, R1);  // prefetch the next step
+            }
             __syncthreads();
             if (active) {
                 plane_tf(buf ^ 1, xt_next);
@@ -362,6 +371,252 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         own_cur = own_next;
         dir_cur = dir_next;
         buf ^= 1;
+    }
+    if (DOT) {
+        double v[1] = {dot}, tot[1];
+        if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
+            st->pap = tot[0];
+            if (!(tot[0] > 0.) || !isfinite(tot[0])) {
+                st->status = isfinite(tot[0]) ? -6 : -8;
+                st->done = 1;
+                st->alpha = 0.;
+            } else {
+                st->alpha = st->rz / tot[0];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3D operator apply, row blocks: one block per node row (y index j) and chunk of
+// KR planes, thread = x index, marching along z.  Modes as k_apply3d.  Each
+// thread forms the element columns (i, j-1) and (i, j) (rows j-1, j, j+1 of
+// nodes; the neighbour rows are other blocks' own rows, read through L2), so a
+// block needs no y halo exchange: per plane it shares only the x+1 node values
+// and the x+1 corner contributions through LDS (two barriers of a 1-4 wave
+// block, which many co-resident blocks overlap).  XCD-aware block order:
+// hardware block b runs on XCD b % 8, and each XCD gets a contiguous range of
+// logical blocks (adjacent rows), so neighbour rows hit in that XCD's L2.
+// Blocks have round64(n0) <= 512 threads.  The grid is a multiple of 8; blocks
+// past the last row idle but still join
+// the grid reduction.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE, bool FUSEP, bool DOT>
+__global__ __launch_bounds__(512) void k_apply3r(Axes3<T> g, int KR, int nb, const T* __restrict__ xin,
+                                                  const T* __restrict__ zin, const T* __restrict__ pin,
+                                                  T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
+                                                  const T* __restrict__ F, const uint64_t* __restrict__ dmask,
+                                                  const T* __restrict__ kh, const T* __restrict__ kv,
+                                                  PcgState* __restrict__ st, double* __restrict__ partials) {
+    extern __shared__ unsigned char smem_raw[];
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int PX = n0 + 1;
+    T* xs = reinterpret_cast<T*>(smem_raw);  // [3][PX]: rows j-1, j, j+1 of the incoming plane
+    T* oc = xs + 3 * PX;                     // [PX]: corner contributions for node i + 1
+    if (FUSEP || DOT) {
+        if (st->done) return;
+    }
+    const int q8 = gridDim.x >> 3;
+    const int L = (int)(blockIdx.x & 7) * q8 + (int)(blockIdx.x >> 3);
+    const bool live = L < nb;
+    double dot = 0.;
+    if (live) {
+        const int j = L % n1, zc = L / n1;
+        const int kb = zc * KR, ke = min(kb + KR, n2);
+        const int i = threadIdx.x;
+        const bool ival = i < n0;
+        const bool ev = i < n0 - 1;
+        const bool hasA = ev && j >= 1, hasB = ev && j < n1 - 1;
+        const long long plane = (long long)n0 * n1;
+        const long long eplane = (long long)(n0 - 1) * (n1 - 1);
+
+        double beta = 0.;
+        T alpha_prev = T(0);
+        if (FUSEP) {
+            beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+            alpha_prev = T(st->alpha);
+        }
+        const T c48 = T(1e-6 / 48.);
+        T gxA = 0, gyA = 0, gzA = 0, gxB = 0, gyB = 0, gzB = 0;
+        if (hasA) {
+            const T hx = g.hx[i], hy = g.hy[j - 1];
+            gxA = c48 * hy / hx;
+            gyA = c48 * hx / hy;
+            gzA = c48 * hx * hy;
+        }
+        if (hasB) {
+            const T hx = g.hx[i], hy = g.hy[j];
+            gxB = c48 * hy / hx;
+            gyB = c48 * hx / hy;
+            gzB = c48 * hx * hy;
+        }
+        for (int r = threadIdx.x; r < 3; r += blockDim.x) xs[r * PX + n0] = T(0);
+        if (threadIdx.x == 0) oc[0] = T(0);
+
+        struct Raw {
+            T v[3], p[3], x, axy, az, bxy, bz;
+            unsigned d;
+        };
+        auto gload = [&](int k, Raw& R) {
+            R.d = 0;
+            if (ival) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const int row = j - 1 + r;
+                    if (row >= 0 && row < n1) {
+                        const long long node = (long long)k * plane + (long long)row * n0 + i;
+                        if (FUSEP) {
+                            R.v[r] = zin[node];
+                            R.p[r] = pin[node];
+                        } else {
+                            R.v[r] = xin[node];
+                        }
+                        if (dbit(dmask, node)) R.d |= 1u << r;
+                    }
+                }
+                if (FUSEP && k >= kb && k < ke) R.x = xsol[(long long)k * plane + (long long)j * n0 + i];
+            }
+            if (k >= 1) {
+                const long long e0 = (long long)(k - 1) * eplane + i;
+                if (hasA) {
+                    R.axy = kh[e0 + (long long)(j - 1) * (n0 - 1)];
+                    R.az = kv[e0 + (long long)(j - 1) * (n0 - 1)];
+                }
+                if (hasB) {
+                    R.bxy = kh[e0 + (long long)j * (n0 - 1)];
+                    R.bz = kv[e0 + (long long)j * (n0 - 1)];
+                }
+            }
+        };
+        // node values of plane k (rows j-1, j, j+1) into xs; own value and Dirichlet flag out
+        auto lstore = [&](int k, const Raw& R, T& own, bool& own_dir, bool store) {
+            T m[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int row = j - 1 + r;
+                T v = T(0);
+                if (ival && row >= 0 && row < n1) v = FUSEP ? R.v[r] + T(beta) * R.p[r] : R.v[r];
+                if (r == 1) {
+                    own = v;
+                    own_dir = (R.d >> 1) & 1u;
+                    if (FUSEP && store && ival) {
+                        const long long node = (long long)k * plane + (long long)j * n0 + i;
+                        pout[node] = v;
+                        xsol[node] = R.x + alpha_prev * R.p[1];
+                    }
+                }
+                m[r] = (MODE == MODE_OP && ((R.d >> r) & 1u)) ? T(0) : v;
+            }
+            if (ival) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) xs[r * PX + i] = m[r];
+            }
+        };
+        // in-plane sum/difference transforms of element columns A = (i, j-1), B = (i, j)
+        auto plane_tf = [&](T (&tA)[4], T (&tB)[4]) {
+            T v[3], w[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                v[r] = ival ? xs[r * PX + i] : T(0);
+                w[r] = ival ? xs[r * PX + i + 1] : T(0);
+            }
+            // A: lower row j-1 (r = 0), upper row j (r = 1); B: rows j, j+1
+            T s0 = v[0] + w[0], d0 = w[0] - v[0], s1 = v[1] + w[1], d1 = w[1] - v[1];
+            T s2 = v[2] + w[2], d2 = w[2] - v[2];
+            tA[0] = s0 + s1;
+            tA[1] = d0 + d1;
+            tA[2] = s1 - s0;
+            tA[3] = d1 - d0;
+            tB[0] = s1 + s2;
+            tB[1] = d1 + d2;
+            tB[2] = s2 - s1;
+            tB[3] = d2 - d1;
+        };
+        // z stage of one element column (eigenvalues of the Q1 box, a = kx'/48 ...)
+        auto zstage = [&](const T (&tc)[4], const T (&tn)[4], T a, T b, T c, T (&bot)[4], T (&top)[4]) {
+            const T ab = a + b, ac = a + c, bc = b + c, abc3 = (ab + c) * T(1. / 3.);
+            const T ws1 = (tn[1] + tc[1]) * (T(3) * a);
+            const T ws2 = (tn[2] + tc[2]) * (T(3) * b);
+            const T ws3 = (tn[3] + tc[3]) * ab;
+            const T wd0 = (tn[0] - tc[0]) * (T(3) * c);
+            const T wd1 = (tn[1] - tc[1]) * ac;
+            const T wd2 = (tn[2] - tc[2]) * bc;
+            const T wd3 = (tn[3] - tc[3]) * abc3;
+            bot[0] = -wd0;
+            top[0] = wd0;
+            bot[1] = ws1 - wd1;
+            top[1] = ws1 + wd1;
+            bot[2] = ws2 - wd2;
+            top[2] = ws2 + wd2;
+            bot[3] = ws3 - wd3;
+            top[3] = ws3 + wd3;
+        };
+
+        const int P0 = kb > 0 ? kb - 1 : 0;
+        T own_cur, own_next = T(0);
+        bool dir_cur, dir_next = false;
+        Raw R;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) R.v[r] = R.p[r] = T(0);
+        R.x = R.axy = R.az = R.bxy = R.bz = T(0);
+        R.d = 0;
+        gload(P0, R);
+        lstore(P0, R, own_cur, dir_cur, FUSEP && P0 >= kb);
+        if (P0 + 1 < n2) gload(P0 + 1, R);
+        __syncthreads();
+        T tAc[4], tBc[4], tAn[4] = {T(0), T(0), T(0), T(0)}, tBn[4] = {T(0), T(0), T(0), T(0)};
+        plane_tf(tAc, tBc);
+        T accA[4] = {T(0), T(0), T(0), T(0)}, accB[4] = {T(0), T(0), T(0), T(0)};
+        __syncthreads();
+        for (int P = P0; P < ke; ++P) {
+            const bool has_next = (P + 1) < n2;
+            T botA[4] = {T(0), T(0), T(0), T(0)}, topA[4] = {T(0), T(0), T(0), T(0)};
+            T botB[4] = {T(0), T(0), T(0), T(0)}, topB[4] = {T(0), T(0), T(0), T(0)};
+            if (has_next) {
+                lstore(P + 1, R, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke);
+                const T hz = g.hz[P];
+                const T aA = R.axy * (gxA * hz), bA = R.axy * (gyA * hz), cA = R.az * (gzA / hz);
+                const T aB = R.bxy * (gxB * hz), bB = R.bxy * (gyB * hz), cB = R.bz * (gzB / hz);
+                if (P + 2 < n2 && P + 2 <= ke) gload(P + 2, R);  // prefetch the next step
+                __syncthreads();
+                plane_tf(tAn, tBn);
+                zstage(tAc, tAn, aA, bA, cA, botA, topA);
+                zstage(tBc, tBn, aB, bB, cB, botB, topB);
+            } else {
+                __syncthreads();
+            }
+            T own_c = T(0);
+            if (P >= kb) {
+                const T a0 = accA[0] + botA[0], a1 = accA[1] + botA[1], a2 = accA[2] + botA[2],
+                        a3 = accA[3] + botA[3];
+                const T b0 = accB[0] + botB[0], b1 = accB[1] + botB[1], b2 = accB[2] + botB[2],
+                        b3 = accB[3] + botB[3];
+                // A: corners on its upper row (j); B: corners on its lower row (j)
+                const T As = a0 + a2, Ad = a1 + a3;
+                const T Bs = b0 - b2, Bd = b1 - b3;
+                own_c = (As - Ad) + (Bs - Bd);
+                if (ival) oc[i + 1] = (As + Ad) + (Bs + Bd);
+            }
+            __syncthreads();
+            if (P >= kb && ival) {
+                const T s = own_c + oc[i];
+                const long long node = (long long)P * plane + (long long)j * n0 + i;
+                T out;
+                if (MODE == MODE_OP) out = dir_cur ? own_cur : s;
+                else out = dir_cur ? T(0) : (F[node] - s);
+                y[node] = out;
+                if (DOT && !dir_cur) dot += (double)own_cur * (double)out;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                accA[a] = topA[a];
+                accB[a] = topB[a];
+                tAc[a] = tAn[a];
+                tBc[a] = tBn[a];
+            }
+            own_cur = own_next;
+            dir_cur = dir_next;
+        }
     }
     if (DOT) {
         double v[1] = {dot}, tot[1];

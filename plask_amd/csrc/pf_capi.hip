@@ -71,6 +71,10 @@ struct pf_ctx {
 
     // launch configuration
     int BY = 8, KC = 8, KC2 = 16;
+    int pfd = 1;  // planes prefetched by the 3D apply (1 or 2)
+    // 3D apply per precision ([0] FP64, [1] FP32): 0 tile blocks (BY x KC), 1 row blocks (KR planes)
+    int apply_kind[2] = {0, 1}, KR[2] = {16, 16};
+    bool graphs = true;  // replay CG chunks from hipGraphs
     // line solve: partition length per precision (0: sequential whole-line kernel)
     int line_m = 0;  // requested (0: default 8 FP64 / 16 FP32, < 0: sequential)
     int pm64 = 8, pm32 = 16;
@@ -355,6 +359,12 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     } else {
         napply = (size_t)((n1 + ctx->KC2 - 1) / ctx->KC2);
     }
+    if (dim == 3) {
+        // row-block apply: one block per row and chunk of KR >= 4 planes, grid rounded to 8
+        if (n0 > 1024) ctx->apply_kind[0] = ctx->apply_kind[1] = 0;
+        const size_t nrow = (size_t)n1 * ((n2 + 3) / 4) + 8;
+        napply = std::max(napply, nrow);
+    }
     ctx->npartials = 2 * std::max<size_t>(napply, 4096);
     if ((rc = dalloc(ctx, &ctx->partials, ctx->npartials))) return rc;
     if ((rc = line_config(ctx))) return rc;
@@ -467,14 +477,30 @@ template <> Axes2<float> axes2<float>(pf_ctx* ctx) {
 template <typename T, int MODE, bool FUSEP, bool DOT>
 static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, T* pout, T* y, const T* F, const T* kh,
                          const T* kv, T* xsol = nullptr) {
-    if (ctx->dim == 3) {
+    const int pk = sizeof(T) == 8 ? 0 : 1;
+    if (ctx->dim == 3 && ctx->apply_kind[pk] == 1) {
+        const int n0 = (int)ctx->n[0], n1 = (int)ctx->n[1], n2 = (int)ctx->n[2];
+        const int KR = ctx->KR[pk];
+        const int nb = n1 * ((n2 + KR - 1) / KR);
+        const int grid = (nb + 7) / 8 * 8;
+        const int threads = (n0 + 63) / 64 * 64;
+        const size_t shm = sizeof(T) * (size_t)(4 * (n0 + 1));
+        hipLaunchKernelGGL((k_apply3r<T, MODE, FUSEP, DOT>), dim3(grid), dim3(threads), shm, ctx->stream,
+                           axes3<T>(ctx), KR, nb, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                           ctx->partials);
+    } else if (ctx->dim == 3) {
         const int BX = (int)ctx->n[0], BY = ctx->BY;
         int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
         int zt = (int)((ctx->n[2] + ctx->KC - 1) / ctx->KC);
         size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
-        hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
-                           axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
-                           ctx->partials);
+        if (ctx->pfd == 2)
+            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
+                               axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                               ctx->partials);
+        else
+            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
+                               axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                               ctx->partials);
     } else {
         const int BX = (int)ctx->n[0];
         int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
@@ -651,6 +677,15 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             if (ctx->hst->done) break;
         }
         for (auto& e : ev) hipEventDestroy(e);
+    } else if (!ctx->graphs) {
+        // plain launches, one state read-back per chunk
+        for (;;) {
+            for (int c = 0; c < chunk; ++c) one_iter(c);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCK(hipStreamSynchronize(ctx->stream));
+            if (ctx->hst->done) break;
+        }
     } else {
         if (!*gcache || ctx->graph_chunk != chunk || ctx->graph_precond != precond) {
             if (*gcache) hipGraphExecDestroy(*gcache);
@@ -742,6 +777,18 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     int total_iters = 0, outer = 0, status = 0;
     double rn = 0.;
     if ((rc = true_residual(ctx, &rn))) return rc;
+    int start = 0;
+    if (opts->x0 == PF_X0_DEVICE_CHECKED) {
+        if (opts->norm != PF_NORM_INIT) return fail(ctx, PF_EINVAL, "PF_X0_DEVICE_CHECKED needs PF_NORM_INIT");
+        if (!(rn <= bnorm)) {  // the given start is no better than the uniform one (or not finite)
+            hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, opts->init_value);
+            if (ctx->nd)
+                hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd,
+                                   ctx->dnodes, ctx->dvals, ctx->x);
+            if ((rc = true_residual(ctx, &rn))) return rc;
+            start = 1;
+        }
+    }
     if (opts->precision == PF_FP64) {
         // restarts only if the recursive residual drifted from the true one
         for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
@@ -800,6 +847,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
         stats->bnorm = bnorm;
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
         stats->status = ret;
+        stats->start = start;
     }
     if (ret == PF_EBREAK) ctx->err = "CG breakdown: stiffness matrix is not positive definite";
     if (ret == PF_ENONFINITE) ctx->err = "non-finite value in CG iteration";
@@ -871,6 +919,90 @@ int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
 
 static ElemGrid egrid(pf_ctx* ctx) {
     return ElemGrid{ctx->dim, (int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2]};
+}
+
+// Trilinear interpolation of node values of a coarser rectilinear grid onto the
+// solution vector: per fine index along each axis, the coarse interval (lo) and
+// the weight of its upper end.
+__global__ void k_prolong3d(int nx, int ny, int nz, int cx, int cy, const float* __restrict__ src,
+                            const int* __restrict__ ix, const int* __restrict__ iy, const int* __restrict__ iz,
+                            const float* __restrict__ wx, const float* __restrict__ wy, const float* __restrict__ wz,
+                            double* __restrict__ x) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
+        const int a = ix[i], b = iy[j], c = iz[k];
+        const float u = wx[i], w = wy[j], t = wz[k];
+        const long long base = ((long long)c * cy + b) * cx + a;
+        const long long sy = cx, sz = (long long)cx * cy;
+        const float v00 = src[base] + u * (src[base + 1] - src[base]);
+        const float v10 = src[base + sy] + u * (src[base + sy + 1] - src[base + sy]);
+        const float v01 = src[base + sz] + u * (src[base + sz + 1] - src[base + sz]);
+        const float v11 = src[base + sz + sy] + u * (src[base + sz + sy + 1] - src[base + sz + sy]);
+        const float lo = v00 + w * (v10 - v00), hi = v01 + w * (v11 - v01);
+        x[v] = (double)(lo + t * (hi - lo));
+    }
+}
+
+int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int ncx, const double* cy, int ncy,
+                        const double* cz, int ncz, double offset, double scale) {
+    if (!ctx || !coarse || !cx || !cy || !cz) return PF_EINVAL;
+    if (ctx->dim != 3) return fail(ctx, PF_ESHAPE, "pf_solution_prolong: 3D grids only");
+    if (ncx < 2 || ncy < 2 || ncz < 2) return fail(ctx, PF_ESHAPE, "coarse grid needs >= 2 nodes per axis");
+    HIPCK(hipSetDevice(ctx->device));
+    const double* ca[3] = {cx, cy, cz};
+    const int nc[3] = {ncx, ncy, ncz};
+    std::vector<int> idx[3];
+    std::vector<float> wt[3];
+    for (int a = 0; a < 3; ++a) {
+        const std::vector<double>& f = ctx->ax[a];
+        idx[a].resize(f.size());
+        wt[a].resize(f.size());
+        for (size_t i = 0; i < f.size(); ++i) {
+            const double* c = ca[a];
+            int lo = (int)(std::upper_bound(c, c + nc[a], f[i]) - c) - 1;
+            lo = std::max(0, std::min(lo, nc[a] - 2));
+            double w = (f[i] - c[lo]) / (c[lo + 1] - c[lo]);
+            idx[a][i] = lo;
+            wt[a][i] = (float)std::max(0., std::min(1., w));
+        }
+    }
+    const size_t ncoarse = (size_t)ncx * ncy * ncz;
+    std::vector<float> src(ncoarse);
+    for (size_t i = 0; i < ncoarse; ++i) src[i] = (float)(offset + scale * (double)coarse[i]);
+    const size_t nidx = ctx->ax[0].size() + ctx->ax[1].size() + ctx->ax[2].size();
+    float* dsrc = nullptr;
+    char* dtab = nullptr;
+    HIPCK(hipMalloc((void**)&dsrc, sizeof(float) * ncoarse));
+    if (hipMalloc((void**)&dtab, (sizeof(int) + sizeof(float)) * nidx) != hipSuccess) {
+        hipFree(dsrc);
+        return fail(ctx, PF_EDEVICE, "pf_solution_prolong: out of device memory");
+    }
+    int* di = (int*)dtab;
+    float* dw = (float*)(dtab + sizeof(int) * nidx);
+    size_t o = 0;
+    int* dix[3];
+    float* dwx[3];
+    for (int a = 0; a < 3; ++a) {
+        dix[a] = di + o;
+        dwx[a] = dw + o;
+        hipMemcpyAsync(dix[a], idx[a].data(), sizeof(int) * idx[a].size(), hipMemcpyHostToDevice, ctx->stream);
+        hipMemcpyAsync(dwx[a], wt[a].data(), sizeof(float) * wt[a].size(), hipMemcpyHostToDevice, ctx->stream);
+        o += idx[a].size();
+    }
+    hipMemcpyAsync(dsrc, src.data(), sizeof(float) * ncoarse, hipMemcpyHostToDevice, ctx->stream);
+    hipLaunchKernelGGL(k_prolong3d, dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream, (int)ctx->n[0],
+                       (int)ctx->n[1], (int)ctx->n[2], ncx, ncy, dsrc, dix[0], dix[1], dix[2], dwx[0], dwx[1], dwx[2],
+                       ctx->x);
+    if (ctx->nd)
+        hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
+                           ctx->dvals, ctx->x);
+    hipError_t err = hipStreamSynchronize(ctx->stream);
+    hipFree(dsrc);
+    hipFree(dtab);
+    if (err != hipSuccess) return fail(ctx, PF_EDEVICE, std::string("pf_solution_prolong: ") + hipGetErrorString(err));
+    ctx->have_solution = true;
+    return PF_OK;
 }
 
 int pf_solution_set(pf_ctx* ctx, const double* x) {
@@ -1092,9 +1224,82 @@ int pf_set_line_segment(pf_ctx* ctx, int seg) {
     return line_config(ctx);
 }
 
+int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes) {
+    if (!ctx) return PF_EINVAL;
+    if (kind != 0 && kind != 1) return fail(ctx, PF_EINVAL, "apply kind 0 (tile blocks) or 1 (row blocks)");
+    if (precision != PF_FP64 && precision != PF_FP32_REFINE && precision != -1)
+        return fail(ctx, PF_EINVAL, "precision PF_FP64, PF_FP32_REFINE or -1 (both)");
+    if (kind == 1) {
+        if (ctx->dim == 3 && ctx->n[0] > 1024) return fail(ctx, PF_EINVAL, "row blocks need n0 <= 1024");
+        if (planes < 4) return fail(ctx, PF_EINVAL, "row blocks: planes >= 4");
+    }
+    for (int pk = 0; pk < 2; ++pk) {
+        if (precision != -1 && pk != (precision == PF_FP64 ? 0 : 1)) continue;
+        ctx->apply_kind[pk] = kind;
+        if (kind == 1) ctx->KR[pk] = planes;
+    }
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_set_graphs(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    ctx->graphs = on != 0;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_set_prefetch(pf_ctx* ctx, int depth) {
+    if (!ctx) return PF_EINVAL;
+    if (depth != 1 && depth != 2) return fail(ctx, PF_EINVAL, "prefetch depth 1 or 2");
+    ctx->pfd = depth;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
 int pf_set_profile(pf_ctx* ctx, int on) {
     if (!ctx) return PF_EINVAL;
     ctx->profile = on != 0;
+    return PF_OK;
+}
+
+// Plain streaming copy b = a of `bytes` bytes (16 bytes per thread and step):
+// the achievable HBM bandwidth of the device, and the calibration kernel for
+// hardware traffic counters.
+__global__ void k_stream_copy(long long n16, const double2* __restrict__ a, double2* __restrict__ b) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+int pf_stream_copy(pf_ctx* ctx, int64_t bytes, int reps, double* seconds) {
+    if (!ctx || bytes < 16 || reps <= 0 || !seconds) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    const long long n16 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    HIPCK(hipMalloc((void**)&a, 16 * (size_t)n16));
+    if (hipMalloc((void**)&b, 16 * (size_t)n16) != hipSuccess) {
+        hipFree(a);
+        return fail(ctx, PF_EDEVICE, "pf_stream_copy: out of device memory");
+    }
+    hipMemsetAsync(a, 0, 16 * (size_t)n16, ctx->stream);
+    const int blocks = grid1(n16, 256, 1 << 16);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ctx->stream, n16, a, b);
+    hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ctx->stream, n16, a, b);
+    hipEventRecord(e1, ctx->stream);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (err != hipSuccess) return fail(ctx, PF_EDEVICE, std::string("pf_stream_copy: ") + hipGetErrorString(err));
+    *seconds = 1e-3 * ms / reps;
     return PF_OK;
 }
 

@@ -46,6 +46,8 @@ struct pf_net {
     bf16 *x = nullptr, *e1 = nullptr, *p1 = nullptr, *e2 = nullptr, *p2 = nullptr, *e3 = nullptr, *d2 = nullptr,
          *d1 = nullptr;
     float* y = nullptr;
+    float* gy = nullptr;   // dL/dy
+    float lambda = 0.f;    // weight of the neighbour-difference term of the loss
     // gradients of activations
     bf16 *g5 = nullptr, *gu1 = nullptr, *g4 = nullptr, *gu2 = nullptr, *g3 = nullptr, *gp2 = nullptr, *gp1 = nullptr;
     float* fin = nullptr;  // fp32 staging of host inputs
@@ -160,9 +162,10 @@ static void backward(pf_net* net, const float* t) {
     Layer* Ly = net->layer;
     float* P = net->p;
     float* G = net->g;
-    hipLaunchKernelGGL(k_loss_head_bwd, dim3(gridn(V1.n(), 256, 1024)), dim3(256), 0, net->stream, V1.n(), net->y, t,
-                       net->mask, net->mask_norm, net->d1, P + net->h_off, net->g5, G + net->h_off, G + net->hb_off,
-                       net->dloss);
+    hipLaunchKernelGGL(k_loss, dim3(gridn(V1.n(), 256, 2048)), dim3(256), 0, net->stream, V1, net->y, t, net->mask,
+                       net->mask_norm, net->lambda, net->gy, net->dloss);
+    hipLaunchKernelGGL(k_head_bwd, dim3(gridn(V1.n(), 256, 1024)), dim3(256), 0, net->stream, V1.n(), net->gy,
+                       net->d1, P + net->h_off, net->g5, G + net->h_off, G + net->hb_off);
     // C5: d1 = relu(C5(up(d2)) + b5 + e1); g5 = gradient of its pre-activation
     chan_sum(net, V1.n(), 32, net->g5, G + Ly[4].b_off);
     wgrad<true>(net, V1, net->g5, net->d2, G + Ly[4].w_off, 32, 64);
@@ -259,6 +262,7 @@ int pf_net_create(pf_net** out, int device, int nx, int ny, int nz, int max_samp
     for (auto& a : acts)
         if ((rc = nalloc(net, a.p, a.n))) return bail(rc);
     if ((rc = nalloc(net, &net->y, n1))) return bail(rc);
+    if ((rc = nalloc(net, &net->gy, n1))) return bail(rc);
     if ((rc = nalloc(net, &net->fin, n1 * NIN))) return bail(rc);
     if ((rc = nalloc(net, &net->mask, n1))) return bail(rc);
     if ((rc = nalloc(net, &net->dloss, 1))) return bail(rc);
@@ -321,6 +325,12 @@ int pf_net_set_mask(pf_net* net, const float* mask) {
     if (!(s > 0.)) return nfail(net, PF_NET_EINVAL, "mask selects no node");
     NCK(hipMemcpy(net->mask, mask, sizeof(float) * n1, hipMemcpyHostToDevice));
     net->mask_norm = (float)(1. / s);
+    return PF_NET_OK;
+}
+
+int pf_net_set_loss(pf_net* net, float lambda) {
+    if (!net || !(lambda >= 0.f)) return PF_NET_EINVAL;
+    net->lambda = lambda;
     return PF_NET_OK;
 }
 

@@ -260,24 +260,68 @@ __global__ void k_head(long long nv, const bf16* __restrict__ d, const float* __
     }
 }
 
-// Loss and head backward: L = sum_v m_v (y_v - t_v)^2 / sum_v m_v (m_v = mask, 1
-// when null); gy = dL/dy; gpre[v][c] = gy_v h_c (d[v][c] > 0);
-// gh[c] += sum_v gy_v d[v][c]; ghb += sum_v gy_v; loss[0] += L.
-__global__ void k_loss_head_bwd(long long nv, const float* __restrict__ y, const float* __restrict__ t,
-                                const float* __restrict__ mask, float inv_norm, const bf16* __restrict__ d,
-                                const float* __restrict__ h, bf16* __restrict__ gpre, float* __restrict__ gh,
-                                float* __restrict__ ghb, double* __restrict__ loss) {
-    __shared__ float red[34][8];
+// Loss L = [sum_v m_v e_v^2 + lambda sum_edges (e_a - e_b)^2] / sum_v m_v with
+// e = y - t (m = mask, 1 when null; edges = the x, y, z neighbour pairs of the
+// node grid) and gy = dL/dy = [2 m e + 2 lambda sum_nbrs (e_v - e_nbr)] / sum m.
+// The gradient term weights errors that vary from node to node, which are the
+// ones that enlarge the residual of a start vector.
+__global__ void k_loss(Vol o, const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ mask,
+                       float inv_norm, float lambda, float* __restrict__ gy, double* __restrict__ loss) {
+    __shared__ float red[8];
+    const long long nv = o.n();
+    float ll = 0.f;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(v % o.nx), yy = (int)((v / o.nx) % o.ny), z = (int)(v / ((long long)o.nx * o.ny));
+        const float m = mask ? mask[v] : 1.f;
+        const float e = y[v] - t[v];
+        ll += m * e * e;
+        float lap = 0.f;
+        if (lambda != 0.f) {
+            const long long sx = 1, sy = o.nx, sz = (long long)o.nx * o.ny;
+            if (x > 0) lap += e - (y[v - sx] - t[v - sx]);
+            if (x < o.nx - 1) {
+                const float d = e - (y[v + sx] - t[v + sx]);
+                lap += d;
+                ll += lambda * d * d;
+            }
+            if (yy > 0) lap += e - (y[v - sy] - t[v - sy]);
+            if (yy < o.ny - 1) {
+                const float d = e - (y[v + sy] - t[v + sy]);
+                lap += d;
+                ll += lambda * d * d;
+            }
+            if (z > 0) lap += e - (y[v - sz] - t[v - sz]);
+            if (z < o.nz - 1) {
+                const float d = e - (y[v + sz] - t[v + sz]);
+                lap += d;
+                ll += lambda * d * d;
+            }
+        }
+        gy[v] = 2.f * (m * e + lambda * lap) * inv_norm;
+    }
+    for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ll;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s2 = 0.f;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s2 += red[k];
+        atomicAdd(loss, (double)s2 * inv_norm);
+    }
+}
+
+// Head backward: gpre[v][c] = gy_v h_c (d[v][c] > 0); gh[c] += sum_v gy_v d[v][c];
+// ghb += sum_v gy_v.
+__global__ void k_head_bwd(long long nv, const float* __restrict__ gy, const bf16* __restrict__ d,
+                           const float* __restrict__ h, bf16* __restrict__ gpre, float* __restrict__ gh,
+                           float* __restrict__ ghb) {
+    __shared__ float red[33][8];
     float lh[32];
 #pragma unroll
     for (int c = 0; c < 32; ++c) lh[c] = 0.f;
-    float lb = 0.f, ll = 0.f;
+    float lb = 0.f;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (long long)gridDim.x * blockDim.x) {
-        const float m = mask ? mask[v] : 1.f;
-        const float e = y[v] - t[v];
-        ll += m * e * e * inv_norm;
-        const float gy = 2.f * m * e * inv_norm;
-        lb += gy;
+        const float g = gy[v];
+        lb += g;
         const bf16x8* p = reinterpret_cast<const bf16x8*>(d + v * 32);
         bf16x8* q = reinterpret_cast<bf16x8*>(gpre + v * 32);
 #pragma unroll
@@ -287,13 +331,12 @@ __global__ void k_loss_head_bwd(long long nv, const float* __restrict__ y, const
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const float dv = b2f(u[j]);
-                lh[8 * k + j] += gy * dv;
-                o[j] = f2b(dv > 0.f ? gy * h[8 * k + j] : 0.f);
+                lh[8 * k + j] += g * dv;
+                o[j] = f2b(dv > 0.f ? g * h[8 * k + j] : 0.f);
             }
             q[k] = o;
         }
     }
-    // block reduction of the 34 sums (wave shuffles, then across the 4 waves)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     auto wsum = [](float v) {
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -307,16 +350,13 @@ __global__ void k_loss_head_bwd(long long nv, const float* __restrict__ y, const
     {
         const float s = wsum(lb);
         if (lane == 0) red[32][wid] = s;
-        const float s2 = wsum(ll);
-        if (lane == 0) red[33][wid] = s2;
     }
     __syncthreads();
-    if (threadIdx.x < 34) {
+    if (threadIdx.x < 33) {
         float s = 0.f;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[threadIdx.x][k];
         if (threadIdx.x < 32) atomicAdd(&gh[threadIdx.x], s);
-        else if (threadIdx.x == 32) atomicAdd(ghb, s);
-        else atomicAdd(loss, (double)s);
+        else atomicAdd(ghb, s);
     }
 }
 

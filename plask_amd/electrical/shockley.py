@@ -91,9 +91,11 @@ class _ShockleyBase(_sb.FemSolver):
     # -- initialization -----------------------------------------------------------
     def _on_initialize(self):
         c = self._core
+        self._cond_groups = None
         m = [len(a) - 1 for a in self._axes]
         E = int(np.prod(m))
-        act = np.array([_active_number(r) for r in self._roles], dtype=np.int64)
+        leaf_act = np.array([_active_number(r) for r in self._leaf_roles], dtype=np.int64)
+        act = leaf_act[self._leaf]                       # index -1: background (no junction)
         self._act = act
         shape = m[::-1]  # [k][j][i] or [j][i]
         actg = act.reshape(shape)
@@ -131,29 +133,72 @@ class _ShockleyBase(_sb.FemSolver):
         c.set_start_cond(float(self._start_cond[1]))
         c.set_junctions(self._jelems, 2 if self.DIM == 2 else 4, np.array(bot, dtype=np.int64),
                         np.array(top, dtype=np.int64), list(self._jid), heights)
-        noheat = np.array([(mm is None or getattr(mm, "kind", "") == "empty" or "noheat" in r)
-                           for mm, r in zip(self._mat, self._roles)], dtype=np.uint8)
+        leaves = self._geometry._leaves
+        air_noheat = True
+        leaf_noheat = np.array([(lf[0].material is None or getattr(lf[0].material, "kind", "") == "empty" or
+                                 "noheat" in r) for lf, r in zip(leaves, self._leaf_roles)] + [air_noheat],
+                               dtype=np.uint8)
+        noheat = leaf_noheat[self._leaf]
         sel = (act > 0).astype(np.uint8) if len(elems) else np.zeros(0, np.uint8)
         c.set_flags(sel, noheat)
         c.initialize()
 
+    @staticmethod
+    def _mat_air():
+        from .. import material
+        return material.Air()
+
     def _load_conductivities(self, T):
-        """Element conductivities (loadConductivities, electr2d.cpp:331-353)."""
+        """Element conductivities (loadConductivities, electr2d.cpp:331-353).  Elements
+        are grouped by (material, role class); materials with a closed-form law
+        (cond_law: s0 (300/T)^alpha) are evaluated for a whole group at once."""
         E = len(self._mat)
+        if getattr(self, "_cond_groups", None) is None:
+            leaves = self._geometry._leaves
+            nl = len(leaves)
+            leaf = np.where(self._leaf < 0, nl, self._leaf)   # background air -> entry nl
+            groups = {}
+            for k in range(nl + 1):
+                mm = leaves[k][0].material if k < nl else self._mat_air()
+                r = self._leaf_roles[k] if k < nl else frozenset()
+                if _active_number(r):
+                    key = ("act", None)
+                elif "p-contact" in r:
+                    key = ("p", None)
+                elif "n-contact" in r:
+                    key = ("n", None)
+                elif mm is None:
+                    key = ("none", None)
+                else:
+                    key = ("mat", id(mm))
+                groups.setdefault(key, (mm, []))[1].append(k)
+            order = np.argsort(leaf, kind="stable")
+            bounds = np.searchsorted(leaf[order], np.arange(nl + 2))
+            self._cond_groups = []
+            for key, (mm, ks) in groups.items():
+                ix = np.concatenate([order[bounds[k]:bounds[k + 1]] for k in ks])
+                if len(ix):
+                    self._cond_groups.append((key[0], mm, np.sort(ix)))
         sh = np.empty(E)
         sv = np.empty(E)
-        for e, (mm, r) in enumerate(zip(self._mat, self._roles)):
-            if self._act[e]:
-                sh[e], sv[e] = 0.0, 0.0
-            elif "p-contact" in r:
-                sh[e] = sv[e] = self.pcond
-            elif "n-contact" in r:
-                sh[e] = sv[e] = self.ncond
-            elif mm is None:
-                sh[e] = sv[e] = 0.0
+        T = np.asarray(T, dtype=np.float64)
+        for kind, mm, ix in self._cond_groups:
+            if kind in ("act", "none"):
+                sh[ix] = sv[ix] = 0.0
+            elif kind == "p":
+                sh[ix] = sv[ix] = self.pcond
+            elif kind == "n":
+                sh[ix] = sv[ix] = self.ncond
             else:
-                v = mm.cond(float(T[e]))
-                sh[e], sv[e] = (v, v) if np.isscalar(v) else (float(v[0]), float(v[1]))
+                law = mm.cond_law()
+                if law is not None:
+                    t = 300.0 / T[ix]
+                    sh[ix] = law[0] * t ** law[2]
+                    sv[ix] = law[1] * t ** law[3]
+                else:
+                    for e in ix:
+                        v = mm.cond(float(T[e]))
+                        sh[e], sv[e] = (v, v) if np.isscalar(v) else (float(v[0]), float(v[1]))
         return sh, sv
 
     def compute(self, loops=0):

@@ -15,7 +15,7 @@ PF_OK, PF_EINVAL, PF_ESTATE, PF_EDEVICE, PF_ESHAPE, PF_ENOCONV, PF_EBREAK, PF_EN
 PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
 PF_THERMAL, PF_ELECTRICAL = 0, 1
 PF_FP64, PF_FP32_REFINE = 0, 1
-PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM = 0, 1, 2, 3
+PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
 PRECOND_JACOBI, PRECOND_LINE = 1, 2
 
@@ -23,7 +23,7 @@ EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
-    "pf_set_launch", "pf_set_line_segment",
+    "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_solution_prolong",
 ]
 
 
@@ -35,7 +35,7 @@ class SolveOpts(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("rel_resid", C.c_double), ("bnorm", C.c_double),
-                ("seconds", C.c_double), ("status", C.c_int)]
+                ("seconds", C.c_double), ("status", C.c_int), ("start", C.c_int)]
 
 
 class FemError(RuntimeError):
@@ -85,6 +85,11 @@ def load_library(path=LIB_PATH):
     L.pf_set_profile.argtypes = [P, I]
     L.pf_set_launch.argtypes = [P, I, I]
     L.pf_set_line_segment.argtypes = [P, I]
+    L.pf_set_prefetch.argtypes = [P, I]
+    L.pf_set_apply.argtypes = [P, I, I, I]
+    L.pf_set_graphs.argtypes = [P, I]
+    L.pf_stream_copy.argtypes = [P, i64, I, C.POINTER(C.c_double)]
+    L.pf_solution_prolong.argtypes = [P, P, P, I, P, I, P, I, C.c_double, C.c_double]
     _lib = L
     return L
 
@@ -188,10 +193,11 @@ class FemContext:
         return x, st
 
     def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
-                   norm=PF_NORM_INIT, init_value=0.0, precond=PRECOND_LINE):
-        """Start from the solution already on the device (no host transfers)."""
-        o = SolveOpts(precision, precond, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol), 50, int(check_every),
-                      int(norm), float(init_value))
+                   norm=PF_NORM_INIT, init_value=0.0, precond=PRECOND_LINE, checked=False):
+        """Start from the solution already on the device (no host transfers); checked:
+        fall back to the uniform init_value start if that has the smaller residual."""
+        o = SolveOpts(precision, precond, PF_X0_DEVICE_CHECKED if checked else PF_X0_DEVICE, int(maxit), float(rtol),
+                      float(inner_rtol), 50, int(check_every), int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
         self._ck(rc, "pf_solve")
@@ -234,6 +240,32 @@ class FemContext:
 
     def set_line_segment(self, seg):
         self._ck(self.L.pf_set_line_segment(self.h, int(seg)), "pf_set_line_segment")
+
+    def set_apply(self, kind, planes=16, precision=-1):
+        """3D operator kernel: kind 0 tile blocks, 1 row blocks of `planes` planes;
+        precision PF_FP64, PF_FP32_REFINE or -1 (both)."""
+        self._ck(self.L.pf_set_apply(self.h, int(precision), int(kind), int(planes)), "pf_set_apply")
+
+    def set_graphs(self, on):
+        self._ck(self.L.pf_set_graphs(self.h, 1 if on else 0), "pf_set_graphs")
+
+    def set_prefetch(self, depth):
+        self._ck(self.L.pf_set_prefetch(self.h, int(depth)), "pf_set_prefetch")
+
+    def prolong(self, coarse, axes, offset=0.0, scale=1.0):
+        """Device start vector: offset + scale * coarse (node values on the 3D grid
+        `axes`), trilinear onto this grid, Dirichlet values imposed."""
+        c = np.ascontiguousarray(coarse, dtype=np.float32)
+        ax = [_f64(a) for a in axes]
+        assert c.size == np.prod([len(a) for a in ax])
+        self._ck(self.L.pf_solution_prolong(self.h, _p(c), _p(ax[0]), len(ax[0]), _p(ax[1]), len(ax[1]), _p(ax[2]),
+                                            len(ax[2]), float(offset), float(scale)), "pf_solution_prolong")
+
+    def stream_copy(self, nbytes, reps=20):
+        """Seconds per device-to-device copy of nbytes (read nbytes, write nbytes)."""
+        t = C.c_double(0.)
+        self._ck(self.L.pf_stream_copy(self.h, int(nbytes), int(reps), C.byref(t)), "pf_stream_copy")
+        return t.value
 
     def set_profile(self, on):
         self._ck(self.L.pf_set_profile(self.h, 1 if on else 0), "pf_set_profile")

@@ -94,9 +94,10 @@ class Rectangular:
         return Boundary("bottomof", (obj, path))
 
     @staticmethod
-    def Predicate(fn):
-        """Nodes whose coordinates satisfy fn(point) (a convenience place)."""
-        return Boundary("predicate", fn)
+    def Predicate(fn, vectorized=False):
+        """Nodes where fn(point) is true; vectorized=True: fn takes the (N, dim)
+        array of all points and returns N booleans."""
+        return Boundary("predicate", (fn, bool(vectorized)))
 
 
 class Rectangular2D(Rectangular):
@@ -141,8 +142,14 @@ class Boundary:
             sl[d - 1 - ax] = pos
             return idx[tuple(sl)].ravel()
         if self.kind == "predicate":
+            fn, vec = self.arg if isinstance(self.arg, tuple) else (self.arg, False)
             pts = mesh.points()
-            return np.nonzero([bool(self.arg(p)) for p in pts])[0].astype(np.int64)
+            if vec:
+                ok = np.asarray(fn(pts), dtype=bool)
+                if ok.shape != (len(pts),):
+                    raise ValueError("vectorized predicate must return one boolean per point")
+                return np.nonzero(ok)[0].astype(np.int64)
+            return np.nonzero([bool(fn(p)) for p in pts])[0].astype(np.int64)
         if self.kind in ("topof", "bottomof"):
             if geometry is None:
                 raise ValueError("TopOf/BottomOf need a geometry")

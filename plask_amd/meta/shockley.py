@@ -1,0 +1,111 @@
+"""``meta.shockley``: self-consistent thermo-electrical loop ThermoElectric2D /
+ThermoElectricCyl / ThermoElectric3D (solvers/meta/shockley/thermoelectric.py:69-220).
+
+The electrical solver takes its temperatures from the thermal solver
+(``electrical.inTemperature = thermal.outTemperature``) and the thermal solver its
+heat sources from the electrical one (``thermal.inHeat = electrical.outHeat``);
+``compute`` alternates ``electrical.compute(tfreq)`` and ``thermal.compute(1)``
+until both report errors below their ``maxerr`` (thermoelectric.py:181-219).
+When both solvers use the same mesh the coupled fields are exchanged per element
+without interpolation.  Saving to HDF5 (``save=True`` in the reference) needs
+h5py, which is not part of this build: ``save`` is accepted and logged as skipped.
+"""
+import logging
+
+from ..electrical.shockley import Shockley2D, Shockley3D, ShockleyCyl
+from ..thermal.static import Static2D, Static3D, StaticCyl
+
+log = logging.getLogger("plask_amd")
+
+
+class ThermoElectric:
+    _Thermal = None
+    _Electrical = None
+
+    def __init__(self, name=""):
+        self.id = name
+        self.thermal = self._Thermal(name)
+        self.electrical = self._Electrical(name)
+        self.tfreq = 6
+        self._reconnect()
+
+    def _reconnect(self):
+        self.electrical.inTemperature = self.thermal.outTemperature
+        self.thermal.inHeat = self.electrical.outHeat
+
+    def reconnect(self):
+        """Reconnect the internal solvers (after replacing one of them)."""
+        self._reconnect()
+
+    # convenience: one geometry / mesh for both solvers
+    @property
+    def geometry(self):
+        return self.thermal.geometry
+
+    @geometry.setter
+    def geometry(self, g):
+        self.thermal.geometry = g
+        self.electrical.geometry = g
+
+    @property
+    def mesh(self):
+        return self.thermal.mesh
+
+    @mesh.setter
+    def mesh(self, m):
+        self.thermal.mesh = m
+        self.electrical.mesh = m
+
+    def initialize(self):
+        self.thermal.initialize()
+        self.electrical.initialize()
+
+    def invalidate(self):
+        self.thermal.invalidate()
+        self.electrical.invalidate()
+
+    def compute(self, save=True, invalidate=True, group="ThermoElectric"):
+        """Alternate electrical.compute(tfreq) and thermal.compute(1) until both
+        errors are below their maxerr (thermoelectric.py:205-211)."""
+        if invalidate:
+            self.invalidate()
+        self.initialize()
+        verr = 2.0 * self.electrical.maxerr
+        terr = 2.0 * self.thermal.maxerr
+        while terr > self.thermal.maxerr or verr > self.electrical.maxerr:
+            verr = self.electrical.compute(self.tfreq)
+            terr = self.thermal.compute(1)
+        log.info("%s: thermoelectric computations finished (thermal err %g K, electrical err %g %%)",
+                 self.id, terr, verr)
+        if save:
+            log.warning("%s: saving to HDF5 skipped (h5py is not available)", self.id)
+
+    def get_total_current(self, nact=0):
+        """Total current through active region nact (mA)."""
+        return self.electrical.get_total_current(nact)
+
+    outTemperature = property(lambda self: self.thermal.outTemperature)
+    outHeatFlux = property(lambda self: self.thermal.outHeatFlux)
+    outThermalConductivity = property(lambda self: self.thermal.outThermalConductivity)
+    outVoltage = property(lambda self: self.electrical.outVoltage)
+    outCurrentDensity = property(lambda self: self.electrical.outCurrentDensity)
+    outHeat = property(lambda self: self.electrical.outHeat)
+    outConductivity = property(lambda self: self.electrical.outConductivity)
+
+
+class ThermoElectric2D(ThermoElectric):
+    _Thermal = Static2D
+    _Electrical = Shockley2D
+
+
+class ThermoElectricCyl(ThermoElectric):
+    _Thermal = StaticCyl
+    _Electrical = ShockleyCyl
+
+
+class ThermoElectric3D(ThermoElectric):
+    _Thermal = Static3D
+    _Electrical = Shockley3D
+
+
+__all__ = ["ThermoElectric2D", "ThermoElectricCyl", "ThermoElectric3D"]

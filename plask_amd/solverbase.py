@@ -81,9 +81,31 @@ def points_of(dest):
     return np.atleast_2d(np.asarray(list(dest), dtype=np.float64))
 
 
-def interp_nodes(axes, values, dest, fill):
-    """Linear interpolation of node values (natural order) on a rectilinear mesh."""
+def _same_axes(a, b):
+    return len(a) == len(b) and all(len(x) == len(y) and np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def _corner_mean(axes, values):
+    """Mean of the 2^d corner values of every element (natural order): linear
+    interpolation of node values at the element midpoints."""
+    v = np.asarray(values).reshape([len(a) for a in axes][::-1])
     d = len(axes)
+    acc = 0.0
+    for c in range(1 << d):
+        acc = acc + v[tuple(slice(1, None) if (c >> a) & 1 else slice(None, -1) for a in range(d))]
+    return (acc / (1 << d)).ravel()
+
+
+def interp_nodes(axes, values, dest, fill):
+    """Linear interpolation of node values (natural order) on a rectilinear mesh.
+    Destinations that are the mesh itself or its element-midpoint mesh are
+    evaluated directly (same values, no search)."""
+    d = len(axes)
+    if isinstance(dest, _mesh.Rectangular) and dest.dim == d:
+        if _same_axes(dest.axes, axes):
+            return np.array(values, dtype=np.float64)
+        if _same_axes(dest.axes, [0.5 * (a[1:] + a[:-1]) for a in axes]):
+            return _corner_mean(axes, values)
     vals = np.asarray(values).reshape([len(a) for a in axes][::-1]).transpose(*range(d)[::-1])
     f = RegularGridInterpolator(axes, vals, method="linear", bounds_error=False, fill_value=np.nan)
     out = f(points_of(dest))
@@ -97,6 +119,8 @@ def interp_elements(axes, values, dest, inside=None, fill=0.0):
     outside the midpoint range take the edge value; points outside the mesh take fill."""
     mids = [0.5 * (a[1:] + a[:-1]) for a in axes]
     d = len(axes)
+    if inside is None and isinstance(dest, _mesh.Rectangular) and dest.dim == d and _same_axes(dest.axes, mids):
+        return np.array(values, dtype=np.float64)
     vals = np.asarray(values)
     comp = vals.shape[1:] if vals.ndim > 1 else ()
     grid = vals.reshape([len(m) for m in mids][::-1] + list(comp)).transpose(*(list(range(d)[::-1]) +
@@ -212,8 +236,12 @@ class FemSolver:
         self._leaf = g.leaf_table(mids)
         leaves = g._leaves
         air = _material.Air()
-        self._mat = [leaves[k][0].material if k >= 0 else air for k in self._leaf]
-        self._roles = [leaves[k][0].roles if k >= 0 else set() for k in self._leaf]
+        leaf_mat = [lf[0].material for lf in leaves] + [air]
+        none = frozenset()
+        leaf_roles = [frozenset(lf[0].roles) for lf in leaves] + [none]
+        self._mat = [leaf_mat[k] for k in self._leaf]          # index -1: the air entry
+        self._roles = [leaf_roles[k] for k in self._leaf]
+        self._leaf_roles = leaf_roles
         self._axes = [np.asarray(a) for a in msh.axes]
         self._esize = [np.diff(a) for a in self._axes]
 

@@ -52,6 +52,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--lam", type=float, default=1.0, help="weight of the neighbour-difference loss term")
     ap.add_argument("--out", default="")
     ap.add_argument("--summary", default="")
     a = ap.parse_args(argv)
@@ -62,6 +63,7 @@ def main(argv=None):
     ctx = fem.FemContext()
     net = warmstart.WarmNet(shape, max_samples=a.train)
     net.set_params(warmstart.init_params(a.seed))
+    net.set_loss(a.lam)
     t0 = time.perf_counter()
     iters_train = []
     try:
@@ -102,15 +104,19 @@ def main(argv=None):
         _setup(ctx, p)
         st0 = ctx.solve_uniform(300.0, rtol=1e-8)
         T0 = ctx.solution()
-        tg = time.perf_counter()
-        x0 = ws.guess(p, p)
-        tg = time.perf_counter() - tg
-        T1, st1 = ctx.solve(x0=x0, init_value=300.0, rtol=1e-8)
-        _, st2 = ctx.solve(x0=x0, init_value=300.0, rtol=1e-8, precision=fem.PF_FP32_REFINE)
         st3 = ctx.solve_uniform(300.0, rtol=1e-8, precision=fem.PF_FP32_REFINE)
+        f = warmstart.features(p)
+        x0 = ws.guess(p, p)
+        tg = time.perf_counter()
+        ws.start_on_device(ctx, p, f)
+        tg = time.perf_counter() - tg
+        st1 = ctx.solve_warm(rtol=1e-8, init_value=300.0, checked=True)
+        T1 = ctx.solution()
+        ws.start_on_device(ctx, p, f)
+        st2 = ctx.solve_warm(rtol=1e-8, init_value=300.0, checked=True, precision=fem.PF_FP32_REFINE)
         rise = np.abs(T0 - 300.0).max()
         rows.append(dict(seed=9000 + j, it_uniform=st0.iters, it_net=st1.iters, it_net_fp32r=st2.iters,
-                         it_uniform_fp32r=st3.iters, guess_err=float(np.abs(x0 - T0).max() / rise),
+                         it_uniform_fp32r=st3.iters, fallback=int(st1.start), guess_err=float(np.abs(x0 - T0).max() / rise),
                          sol_diff=float(np.abs(T1 - T0).max() / rise), rel_resid=st1.rel_resid, guess_s=tg))
     it0 = np.array([r["it_uniform"] for r in rows])
     it1 = np.array([r["it_net"] for r in rows])
@@ -124,7 +130,7 @@ def main(argv=None):
                 held_iters_fp32r_net_mean=float(np.mean([r["it_net_fp32r"] for r in rows])),
                 held_guess_err_median=float(np.median([r["guess_err"] for r in rows])),
                 held_max_sol_diff=float(max(r["sol_diff"] for r in rows)),
-                held_fallbacks=int(np.sum(it1 > it0)))
+                held_fallbacks=int(sum(r["fallback"] for r in rows)))
     log(json.dumps(summ))
     if a.summary:
         with open(a.summary, "w") as fh:

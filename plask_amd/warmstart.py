@@ -16,7 +16,6 @@ caller in favour of the uniform (``inittemp``) start.
 import ctypes as C
 
 import numpy as np
-from scipy.interpolate import RegularGridInterpolator
 
 from . import fem, workloads
 
@@ -40,6 +39,7 @@ def _lib():
     L.pf_net_get_params.argtypes = [P, P, i64]
     L.pf_net_set_mask.argtypes = [P, P]
     L.pf_net_sample_set.argtypes = [P, I, P, P]
+    L.pf_net_set_loss.argtypes = [P, C.c_float]
     L.pf_net_predict.argtypes = [P, P, P]
     L.pf_net_predict_sample.argtypes = [P, I, P]
     L.pf_net_gradient.argtypes = [P, I, P, C.POINTER(C.c_double)]
@@ -119,6 +119,10 @@ class WarmNet:
         assert m.size == self.nv
         self._ck(self.L.pf_net_set_mask(self.h, _ptr(m)), "pf_net_set_mask")
 
+    def set_loss(self, lam):
+        """Weight of the neighbour-difference (gradient) term of the loss."""
+        self._ck(self.L.pf_net_set_loss(self.h, float(lam)), "pf_net_set_loss")
+
     def set_sample(self, idx, feat, target):
         feat, target = _f32(feat), _f32(target)
         assert feat.size == self.nv * NIN and target.size == self.nv
@@ -169,8 +173,9 @@ def _node_mean(ev, shape):
 
 def features(p):
     """(nodes, 8) float32 inputs in node order: log k_xy, log k_z (node means of
-    the element values), heat source / 1e16, Dirichlet mask, z / height,
-    log local dz, x and y relative to the centre."""
+    the element values), heat source / 1e16, Dirichlet mask, z / height, the
+    normalised vertical thermal resistance from the bottom, x and y relative to
+    the centre."""
     x, y, z = p["axes"]
     shape = (len(x), len(y), len(z))
     nx, ny, nz = shape
@@ -182,9 +187,18 @@ def features(p):
     m[p["dir_nodes"]] = 1.0
     f[..., 3] = m.reshape(nz, ny, nx)
     f[..., 4] = (z / z[-1])[:, None, None]
+    # vertical thermal resistance from the heat sink, sum dz / k_z along each
+    # element column, normalised by the column total (node: mean of its columns)
     dz = np.diff(z)
-    dzn = np.r_[dz[0], 0.5 * (dz[1:] + dz[:-1]), dz[-1]]
-    f[..., 5] = (np.log(dzn) / 3.0)[:, None, None]
+    r = np.cumsum(dz[:, None, None] / p["k_v"].reshape(nz - 1, ny - 1, nx - 1), axis=0)
+    r = np.concatenate([np.zeros((1, ny - 1, nx - 1)), r], axis=0) / r[-1][None]
+    rn = np.zeros((nz, ny, nx))
+    cnt = np.zeros((1, ny, nx))
+    for dy in (0, 1):
+        for dx in (0, 1):
+            rn[:, dy:dy + ny - 1, dx:dx + nx - 1] += r
+            cnt[:, dy:dy + ny - 1, dx:dx + nx - 1] += 1
+    f[..., 5] = rn / cnt
     f[..., 6] = (x / x[-1] - 0.5)[None, None, :]
     f[..., 7] = (y / y[-1] - 0.5)[None, :, None]
     return f.reshape(-1, NIN).astype(np.float32)
@@ -236,15 +250,32 @@ class WarmStart:
         """Prediction on the network grid (T in K)."""
         return 300.0 + target_scale(p_net) * self.net.predict(features(p_net)).astype(np.float64)
 
+    def start_on_device(self, ctx, p_net, feat=None):
+        """Predict on the network grid and prolong onto ctx's grid on the device
+        (the next ctx.solve_warm starts from it)."""
+        y = self.net.predict(features(p_net) if feat is None else feat)
+        ctx.prolong(y, p_net["axes"], offset=300.0, scale=target_scale(p_net))
+
     def guess(self, p_net, p):
         """Start vector for problem p from the prediction on p_net (the same stack on
         the network grid), trilinear in physical coordinates; Dirichlet values kept."""
-        Tc = self.guess_coarse(p_net)
-        xc, yc, zc = p_net["axes"]
-        f = RegularGridInterpolator((zc, yc, xc), Tc.reshape(len(zc), len(yc), len(xc)), bounds_error=False,
-                                    fill_value=None)
-        x, y, z = p["axes"]
-        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
-        T = f(np.stack([Z.ravel(), Y.ravel(), X.ravel()], axis=1))
+        return self.prolong(self.guess_coarse(p_net), p_net, p)
+
+    @staticmethod
+    def prolong(Tc, p_net, p):
+        """Trilinear interpolation of node values on the grid of p_net to the nodes of
+        p (separable: one axis at a time), Dirichlet values of p imposed."""
+        out = np.asarray(Tc, dtype=np.float64).reshape([len(a) for a in p_net["axes"]][::-1])
+        for ax in range(3):
+            src, dst = p_net["axes"][ax], p["axes"][ax]
+            i = np.clip(np.searchsorted(src, dst, side="right") - 1, 0, len(src) - 2)
+            w = np.clip((dst - src[i]) / (src[i + 1] - src[i]), 0.0, 1.0)
+            axis = 2 - ax                      # array axes are (z, y, x)
+            lo = np.take(out, i, axis=axis)
+            hi = np.take(out, i + 1, axis=axis)
+            shape = [1, 1, 1]
+            shape[axis] = len(dst)
+            out = lo + (hi - lo) * w.reshape(shape)
+        T = out.ravel()
         T[p["dir_nodes"]] = p["dir_vals"]
         return T

@@ -53,6 +53,28 @@ def test_apply_matches_oracle(shape):
     assert np.abs(y - y_ref).max() <= 1e-13 * scale, np.abs(y - y_ref).max() / scale
 
 
+@pytest.mark.parametrize("shape", SHAPES3)
+@pytest.mark.parametrize("kind,planes", [(0, 16), (1, 4), (1, 5), (1, 64)])
+def test_apply_kernels_match_oracle(shape, kind, planes):
+    """Both 3D operator kernels (tile blocks, row blocks with several plane
+    chunks) give the oracle's operator and residual."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem(shape, seed=7 + shape[0])
+    print("apply kernel", kind, planes, shape, flush=True)
+    c = _ctx(axes, kh, kv, q, dn, dv)
+    c.set_apply(kind, planes)
+    mask = np.zeros(g.nnodes, np.uint8)
+    mask[dn] = 1
+    x = np.random.default_rng(2).uniform(-1, 1, g.nnodes)
+    y_ref = O.apply(g, 0, kh, kv, mask, x)
+    y = c.apply(x)
+    assert np.abs(y - y_ref).max() <= 1e-13 * np.abs(y_ref).max()
+    A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
+    b = c.rhs()
+    assert np.abs(b - A.B).max() <= 1e-12 * np.abs(A.B).max()
+    T, st = c.solve(x0=np.full(g.nnodes, 300.0), rtol=1e-10)
+    assert st.rel_resid <= 1e-10
+
+
 @pytest.mark.parametrize("shape", [(5, 4, 6), (17, 9, 13), (96, 11, 21), (5, 4), (64, 33)])
 def test_rhs_matches_oracle(shape):
     cyl = len(shape) == 2
