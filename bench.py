@@ -54,14 +54,18 @@ def kernel_bytes(shape, m=8):
       (16 B/element); Dirichlet bitmask (N/8 B).
     line (partitioned, segments of m planes): read r, q, a, w, c'; write r, z
       (56 B/node); separator factors 3 x 8 B per separator per column.
-    apply_plain: read x, write y (16 B/node), 16 B/element, bitmask."""
+    apply_plain: read x, write y (16 B/node), 16 B/element, bitmask.
+    FP32 forms: half the bytes of vectors and element data (line: 16-plane segments)."""
     nx, ny, nz = shape
     N, E = nx * ny * nz, (nx - 1) * (ny - 1) * (nz - 1)
     mask = 8 * ((N + 63) // 64)
     P = (nz + m - 1) // m
+    P32 = (nz + 15) // 16
     return {"apply_fused": 48 * N + 16 * E + mask,
             "line": 56 * N + 24 * (P - 1) * nx * ny,
-            "apply_plain": 16 * N + 16 * E + mask}
+            "apply_plain": 16 * N + 16 * E + mask,
+            "apply_fused_fp32": 24 * N + 8 * E + mask,
+            "line_fp32": 28 * N + 12 * (P32 - 1) * nx * ny}
 
 
 def cpu_baseline(p, seconds, full_iters):
@@ -111,7 +115,7 @@ def learned(ctx, p, size, steps, warmup, T64):
 
     def step():
         ws.start_on_device(ctx, pc, feat)
-        return ctx.solve_warm(precision=PF_FP32_REFINE, rtol=1e-8, inner_rtol=1e-4, init_value=300.0, checked=True)
+        return ctx.solve_warm(precision=PF_FP32_REFINE, rtol=1e-8, inner_rtol=1e-5, init_value=300.0, checked=True)
 
     t, st = timed(step, steps, warmup)
     T = ctx.solution()
@@ -160,7 +164,7 @@ def main():
         return
 
     def step(prec, pc=PRECOND_LINE):
-        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-4 if prec else 0.0, precond=pc)
+        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-5 if prec else 0.0, precond=pc)
 
     t, st = timed(lambda: step(PF_FP64), args.steps, args.warmup)
     fp64 = dict(solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
@@ -189,17 +193,25 @@ def main():
         print("fp32+refine+learned start: %.2f solves/s, %.1f ms/solve, %d inner its" %
               (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
 
-    # roofline: one profiled FP64 solve (HIP events around each launch on the context stream)
-    ctx.set_profile(True)
-    ctx.solve_uniform(300.0, precision=PF_FP64, rtol=rtol)
-    ctx.set_profile(False)
-    (t_apply, t_line), (n_apply, n_line) = ctx.kernel_times()
+    # roofline: one profiled solve per precision -- HIP events around every kernel on the
+    # context stream, one synchronisation per 32-iteration chunk (the in-solve durations a
+    # kernel trace reports; launches after the stop are not counted)
+    def profiled(prec):
+        ctx.set_profile(True)
+        ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-5 if prec else 0.0)
+        ctx.set_profile(False)
+        (ta, tl), (na, nl) = ctx.kernel_times()
+        return ta, tl, int(na)
+    t_apply32, t_line32, n32 = profiled(PF_FP32_REFINE)
+    t_apply, t_line, n64 = profiled(PF_FP64)
     t_plain = ctx.time_apply(PF_FP64, reps=50)
     kb = kernel_bytes(size)
     kern = {
-        "apply_fused": dict(avg_us=1e6 * t_apply, launches=int(n_apply), bytes=kb["apply_fused"]),
-        "line": dict(avg_us=1e6 * t_line, launches=int(n_line), bytes=kb["line"]),
+        "apply_fused": dict(avg_us=1e6 * t_apply, launches=n64, bytes=kb["apply_fused"]),
+        "line": dict(avg_us=1e6 * t_line, launches=n64, bytes=kb["line"]),
         "apply_plain": dict(avg_us=1e6 * t_plain, bytes=kb["apply_plain"]),
+        "apply_fused_fp32": dict(avg_us=1e6 * t_apply32, launches=n32, bytes=kb["apply_fused_fp32"]),
+        "line_fp32": dict(avg_us=1e6 * t_line32, launches=n32, bytes=kb["line_fp32"]),
     }
     for k in kern.values():
         k["gbs"] = round(k["bytes"] / (1e-6 * k["avg_us"]) / 1e9, 1)

@@ -181,6 +181,11 @@ int pf_set_line_segment(pf_ctx* ctx, int seg);
  * (default for FP32, 16 planes).  A tuning hook, no reference counterpart. */
 int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes);
 
+/* Tile-block apply: xcd = 1 (default) gives each of the 8 XCDs a contiguous range of
+ * tiles (hardware block b runs on XCD b mod 8), so neighbouring tiles share halo rows in
+ * that XCD's L2; 0 = plain block order.  A tuning hook, no reference counterpart. */
+int pf_set_block_order(pf_ctx* ctx, int xcd);
+
 /* on = 1 (default): CG iterations are replayed in chunks from hipGraphs; 0: plain
  * launches (e.g. under tools that do not follow graph launches).  No reference
  * counterpart. */
@@ -198,6 +203,11 @@ int pf_stream_copy(pf_ctx* ctx, int64_t bytes, int reps, double* seconds);
 /* Profiling switch: when on, pf_solve launches each iteration's kernels bracketed by HIP
  * events (no graph) and accumulates per-kernel durations for pf_kernel_times. */
 int pf_set_profile(pf_ctx* ctx, int on);
+
+/* Average device time of the two kernels of one PCG iteration (fused operator apply;
+ * preconditioner update), each launched `reps` times back to back on the state of
+ * the last solve of that precision (the solution is restored).  For benchmarks. */
+int pf_time_kernels(pf_ctx* ctx, int precision, int precond, int reps, double* t_apply, double* t_update);
 
 /* Time `reps` back-to-back FP64 operator applies y = A x on the context stream with HIP
  * events; returns the average seconds per apply.  For benchmarks. */

@@ -44,10 +44,15 @@ int pf_net_get_params(pf_net* net, float* params, int64_t n);
 /* Per-node loss weights (0 excludes a node, e.g. Dirichlet nodes). */
 int pf_net_set_mask(pf_net* net, const float* mask);
 
-/* Loss weight lambda of the neighbour differences:
- * L = [sum m_v e_v^2 + lambda sum_edges (e_a - e_b)^2] / sum m_v, e = y - target,
- * edges = x, y, z neighbour pairs (default 0). */
+/* Loss, e = y - target, m = mask:
+ *   L = [sum m_v e_v^2 + lambda sum_{x,y edges} (e_a - e_b)^2] / sum m_v
+ *       + sum_{z edges} w_v (e_v - e_{v + nx ny})^2,
+ * with w from pf_net_sample_weights for that sample, else lambda / sum m_v.
+ * lambda defaults to 0. */
 int pf_net_set_loss(pf_net* net, float lambda);
+
+/* z-edge loss weights of stored sample idx ([v], edge v -- v + nx ny; host fp32). */
+int pf_net_sample_weights(pf_net* net, int idx, const float* wz);
 
 /* Training sample idx: inputs [v][8] and target [v] (host fp32). */
 int pf_net_sample_set(pf_net* net, int idx, const float* feat, const float* target);

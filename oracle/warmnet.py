@@ -112,9 +112,10 @@ def forward(p, x, rb=False):
                                       W=[W1, W2, W3, W4, W5], h=h)
 
 
-def loss_grad(p, x, t, mask=None, rb=False, lam=0.0):
-    """Loss [sum m (y - t)^2 + lam sum_edges (e_a - e_b)^2] / sum m (edges: x, y, z
-    neighbour pairs of e = y - t) and its gradient."""
+def loss_grad(p, x, t, mask=None, rb=False, lam=0.0, wz=None):
+    """Loss [sum m e^2 + lam sum_{x,y edges} (e_a - e_b)^2] / sum m + sum_{z edges}
+    w (e_a - e_b)^2 (e = y - t; w = wz[node below], or lam / sum m without wz) and its
+    gradient."""
     R = bf16 if rb else (lambda a: a)
     y, c = forward(p, x, rb)
     m = np.ones_like(t) if mask is None else mask
@@ -124,15 +125,19 @@ def loss_grad(p, x, t, mask=None, rb=False, lam=0.0):
     gy = 2.0 * m * e * inv
     for ax in range(3):
         d = np.diff(e, axis=ax)
-        loss += float(lam * (d * d).sum() * inv)
-        g = np.zeros_like(e)
         lo = [slice(None)] * 3
         hi = [slice(None)] * 3
         lo[ax] = slice(0, -1)
         hi[ax] = slice(1, None)
-        g[tuple(lo)] -= d
-        g[tuple(hi)] += d
-        gy += 2.0 * lam * g * inv
+        if ax == 0 and wz is not None:
+            w = wz[tuple(lo)]
+        else:
+            w = lam * inv
+        loss += float((w * d * d).sum())
+        g = np.zeros_like(e)
+        g[tuple(lo)] -= w * d
+        g[tuple(hi)] += w * d
+        gy += 2.0 * g
     W1, W2, W3, W4, W5 = c["W"]
     gh = (gy[..., None] * c["d1"]).reshape(-1, 32).sum(0)
     ghb = gy.sum()

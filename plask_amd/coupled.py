@@ -59,7 +59,7 @@ def build(seed=5, shape=(96, 96, 192), nlayers=12, junction=6, width=100.0, cont
     return te, dict(seed=seed, shape=shape, layers=t, sigma=sig, k0=k0, alpha=alpha, js=js, beta=beta, V=V)
 
 
-def run(te, outer=20, tfreq=6):
+def run(te, outer=20, tfreq=6, verbose=False):
     """outer x (electrical.compute(tfreq), thermal.compute(1)); returns timings."""
     te.invalidate()
     te.initialize()
@@ -74,6 +74,9 @@ def run(te, outer=20, tfreq=6):
         t_el += t1 - t0
         t_th += t2 - t1
         hist.append((float(verr), float(terr)))
+        if verbose:
+            print("outer %d: electrical err %.3g %% (%.2f s), thermal err %.3g K (%.2f s)" %
+                  (len(hist), verr, t1 - t0, terr, t2 - t1), file=sys.stderr, flush=True)
     return dict(electrical_s=t_el, thermal_s=t_th, total_s=t_el + t_th, history=hist,
                 current_mA=te.get_total_current(), Tmax=float(te.thermal._temps().max()))
 
@@ -82,7 +85,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--shape", default="96,96,192")
     ap.add_argument("--outer", type=int, default=20)
-    ap.add_argument("--rtol", type=float, default=1e-8, help="thermal PCG tolerance")
+    # the Joule heating raises T by ~0.07 K over 300 K: the FP64 relative residual of the
+    # thermal system bottoms out near 4e-8 (cancellation in B - A T), so ask for 1e-7
+    ap.add_argument("--rtol", type=float, default=1e-7, help="thermal PCG tolerance")
     a = ap.parse_args(argv)
     shape = tuple(int(v) for v in a.shape.split(","))
     out = {}
@@ -95,7 +100,7 @@ def main(argv=None):
         t0 = time.perf_counter()
         te.initialize()
         t_init = time.perf_counter() - t0
-        r = run(te, a.outer)
+        r = run(te, a.outer, verbose=True)
         r["init_s"] = t_init
         T[name] = te.thermal._temps().copy()
         out[name] = r

@@ -157,7 +157,8 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, unsi
 // T is the storage and arithmetic type of vectors and element data.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, bool FUSEP, bool DOT, int PFD = 1>
-__global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, const T* __restrict__ xin,
+__global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, int ytiles, int nb, int xcd,
+                                                  const T* __restrict__ xin,
                                                   const T* __restrict__ zin, const T* __restrict__ pin,
                                                   T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
@@ -171,13 +172,18 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
     if (FUSEP || DOT) {
         if (st->done) return;
     }
+    // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a contiguous
+    // range of logical tiles (adjacent y tiles of a z chunk share halo rows in its L2).
+    // Blocks past the last tile idle but still join the grid reduction.
+    const int L = xcd ? (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const bool live = L < nb;
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
-    const bool active = ty < BY;
-    const int jb = blockIdx.x * (BY - 1);
+    const bool active = live && ty < BY;
+    const int jb = (L % ytiles) * (BY - 1);
     const int j = jb - 1 + ty;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
-    const int kb = blockIdx.y * KC;
+    const int kb = live ? (L / ytiles) * KC : n2;
     const int ke = min(kb + KC, n2);
     const bool nvalid = active && j >= 0 && j < n1;
     const bool owned = nvalid && ty >= 1;
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, co
         t[3] = d1 - d0;  // delta delta
     };
 
-    const int P0 = (kb > 0) ? kb - 1 : 0;
+    const int P0 = !live ? n2 : (kb > 0) ? kb - 1 : 0;
     int buf = 0;
     T own_cur, own_next = T(0);
     bool dir_cur, dir_next = false;

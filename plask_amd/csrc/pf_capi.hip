@@ -70,11 +70,12 @@ struct pf_ctx {
     bool have_solution = false;
 
     // launch configuration
-    int BY = 8, KC = 8, KC2 = 16;
+    int BY[2] = {4, 4}, KC[2] = {8, 16}, KC2 = 16;  // [0] FP64, [1] FP32
     int pfd = 1;  // planes prefetched by the 3D apply (1 or 2)
     // 3D apply per precision ([0] FP64, [1] FP32): 0 tile blocks (BY x KC), 1 row blocks (KR planes)
-    int apply_kind[2] = {0, 1}, KR[2] = {16, 16};
+    int apply_kind[2] = {0, 0}, KR[2] = {16, 16};
     bool graphs = true;  // replay CG chunks from hipGraphs
+    int xcd_order = 1;   // tile apply: XCD-aware block order
     // line solve: partition length per precision (0: sequential whole-line kernel)
     int line_m = 0;  // requested (0: default 8 FP64 / 16 FP32, < 0: sequential)
     int pm64 = 8, pm32 = 16;
@@ -342,28 +343,28 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     ctx->have_coeffs = false;
     ctx->have_source = false;
     ctx->have_solution = false;
-    // launch shapes
+    // launch shapes: tile blocks of BY node rows (BY - 1 owned) x whole x rows, KC planes;
+    // measured best at 128 x 128 x 256: 4 rows, 8 planes (FP64) / 16 planes (FP32)
     if (dim == 3) {
-        ctx->BY = (int)std::max<int64_t>(2, std::min<int64_t>(16, 1024 / n0));
-        int ytiles = (int)((n1 + ctx->BY - 2) / (ctx->BY - 1));
-        int kc = (int)std::max<int64_t>(4, std::min<int64_t>(32, (n2 * ytiles) / 1024));
-        ctx->KC = kc;
+        const int by = (int)std::max<int64_t>(2, std::min<int64_t>(16, 512 / n0));
+        ctx->BY[0] = ctx->BY[1] = by;
+        ctx->KC[0] = (int)std::max<int64_t>(1, std::min<int64_t>(8, n2));
+        ctx->KC[1] = (int)std::max<int64_t>(1, std::min<int64_t>(16, n2));
     } else {
         ctx->KC2 = (int)std::max<int64_t>(4, std::min<int64_t>(64, n1 / 64));
     }
     ctx->upd_blocks = grid1(N, 256, 1024);
     size_t napply = 0;
     if (dim == 3) {
-        int ytiles = (int)((n1 + ctx->BY - 2) / (ctx->BY - 1));
-        napply = (size_t)ytiles * ((n2 + ctx->KC - 1) / ctx->KC);
-    } else {
-        napply = (size_t)((n1 + ctx->KC2 - 1) / ctx->KC2);
-    }
-    if (dim == 3) {
+        for (int pk = 0; pk < 2; ++pk) {
+            int ytiles = (int)((n1 + ctx->BY[pk] - 2) / (ctx->BY[pk] - 1));
+            napply = std::max(napply, (size_t)ytiles * ((n2 + ctx->KC[pk] - 1) / ctx->KC[pk]));
+        }
         // row-block apply: one block per row and chunk of KR >= 4 planes, grid rounded to 8
-        if (n0 > 1024) ctx->apply_kind[0] = ctx->apply_kind[1] = 0;
         const size_t nrow = (size_t)n1 * ((n2 + 3) / 4) + 8;
         napply = std::max(napply, nrow);
+    } else {
+        napply = (size_t)((n1 + ctx->KC2 - 1) / ctx->KC2);
     }
     ctx->npartials = 2 * std::max<size_t>(napply, 4096);
     if ((rc = dalloc(ctx, &ctx->partials, ctx->npartials))) return rc;
@@ -489,17 +490,18 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
                            axes3<T>(ctx), KR, nb, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
                            ctx->partials);
     } else if (ctx->dim == 3) {
-        const int BX = (int)ctx->n[0], BY = ctx->BY;
-        int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
-        int zt = (int)((ctx->n[2] + ctx->KC - 1) / ctx->KC);
+        const int BX = (int)ctx->n[0], BY = ctx->BY[pk], KC = ctx->KC[pk];
+        const int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
+        const int zt = (int)((ctx->n[2] + KC - 1) / KC);
+        const int nb = ytiles * zt, grid = (nb + 7) / 8 * 8;
         size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
         if (ctx->pfd == 2)
-            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
-                               axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
+                               axes3<T>(ctx), BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
                                ctx->partials);
         else
-            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(ytiles, zt), dim3(BX * BY), shm, ctx->stream,
-                               axes3<T>(ctx), BY, ctx->KC, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+            hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
+                               axes3<T>(ctx), BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
                                ctx->partials);
     } else {
         const int BX = (int)ctx->n[0];
@@ -651,29 +653,36 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     int chunk = check_every > 0 ? check_every : 32;
     chunk += chunk & 1;  // even: each graph replay starts on the same buffer parity
     if (ctx->profile) {
-        // event-timed launches, no graph
-        hipEvent_t ev[3];
+        // event-timed launches, no graph: events around every kernel of a chunk, one
+        // synchronisation per chunk; only iterations that ran (not the early exits after
+        // the stop) are counted
+        std::vector<hipEvent_t> ev(3 * (size_t)chunk);
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
+        int done_iters = 0;
         for (;;) {
             for (int c = 0; c < chunk; ++c) {
                 T* pin = pb[c & 1];
                 T* pout = pb[(c + 1) & 1];
-                hipEventRecord(ev[0], ctx->stream);
+                hipEventRecord(ev[3 * c], ctx->stream);
                 launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
-                hipEventRecord(ev[1], ctx->stream);
+                hipEventRecord(ev[3 * c + 1], ctx->stream);
                 update(pout);
-                hipEventRecord(ev[2], ctx->stream);
-                HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
-                HIPCK(hipStreamSynchronize(ctx->stream));
+                hipEventRecord(ev[3 * c + 2], ctx->stream);
+            }
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCK(hipStreamSynchronize(ctx->stream));
+            const int ran = std::min(chunk, ctx->hst->iters - done_iters);
+            for (int c = 0; c < ran; ++c) {
                 float t1 = 0.f, t2 = 0.f;
-                hipEventElapsedTime(&t1, ev[0], ev[1]);
-                hipEventElapsedTime(&t2, ev[1], ev[2]);
+                hipEventElapsedTime(&t1, ev[3 * c], ev[3 * c + 1]);
+                hipEventElapsedTime(&t2, ev[3 * c + 1], ev[3 * c + 2]);
                 ctx->kt[0] += 1e-3 * t1;
                 ctx->kt[1] += 1e-3 * t2;
                 ctx->kc[0] += 1;
                 ctx->kc[1] += 1;
-                if (ctx->hst->done) break;
             }
+            done_iters = ctx->hst->iters;
             if (ctx->hst->done) break;
         }
         for (auto& e : ev) hipEventDestroy(e);
@@ -1201,10 +1210,10 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes) {
         if (rows < 2 || rows * ctx->n[0] > 1024 || planes < 1)
             return fail(ctx, PF_EINVAL, "3D: need rows >= 2, rows * n0 <= 1024, planes >= 1");
         int ytiles = (int)((ctx->n[1] + rows - 2) / (rows - 1));
-        size_t need = 2 * (size_t)ytiles * ((ctx->n[2] + planes - 1) / planes);
+        size_t need = 2 * (((size_t)ytiles * ((ctx->n[2] + planes - 1) / planes) + 7) / 8 * 8);
         if (need > ctx->npartials) return fail(ctx, PF_EINVAL, "too many blocks for the reduction buffer");
-        ctx->BY = rows;
-        ctx->KC = planes;
+        ctx->BY[0] = ctx->BY[1] = rows;
+        ctx->KC[0] = ctx->KC[1] = planes;
     } else {
         if (planes < 1) return fail(ctx, PF_EINVAL, "planes >= 1");
         size_t need = 2 * (size_t)((ctx->n[1] + planes - 1) / planes);
@@ -1238,6 +1247,13 @@ int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes) {
         ctx->apply_kind[pk] = kind;
         if (kind == 1) ctx->KR[pk] = planes;
     }
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_set_block_order(pf_ctx* ctx, int xcd) {
+    if (!ctx) return PF_EINVAL;
+    ctx->xcd_order = xcd != 0;
     drop_graphs(ctx);
     return PF_OK;
 }
@@ -1301,6 +1317,84 @@ int pf_stream_copy(pf_ctx* ctx, int64_t bytes, int reps, double* seconds) {
     if (err != hipSuccess) return fail(ctx, PF_EDEVICE, std::string("pf_stream_copy: ") + hipGetErrorString(err));
     *seconds = 1e-3 * ms / reps;
     return PF_OK;
+}
+
+}  // extern "C"
+
+// Back-to-back launches of the two kernels of one PCG iteration on the state the
+// last solve of this precision left (as a kernel trace sees them inside a solve):
+// the fused operator apply (p = z + beta p_old, q = A p, (p, q), x += alpha p_old)
+// and the preconditioner update (r -= alpha q, z = M^-1 r, (r, z), (r, r)).  The
+// iteration is not advanced in any way that matters: x is saved and restored.
+template <typename T>
+static int time_kernels(pf_ctx* ctx, int precond, int reps, double* t_apply, double* t_update, T* x, T* r, T* p,
+                        T* p2, T* q, T* z, const T* dinv, const T* la, const T* lw, const T* lcp, const T* kh,
+                        const T* kv) {
+    const long long N = ctx->N;
+    T* save = reinterpret_cast<T*>(ctx->tmp);  // N doubles of scratch hold N values of T
+    HIPCK(hipMemcpyAsync(save, x, sizeof(T) * N, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    PcgState s = *ctx->hst;
+    s.done = 0;
+    s.status = 0;
+    s.iters = std::max(s.iters, 1);
+    s.maxit = 1 << 30;
+    s.stop2 = 0.;
+    if (!(s.rz_prev > 0.)) s.rz_prev = 1.;
+    if (!(s.rz > 0.)) s.rz = 1.;
+    *ctx->hst = s;
+    HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+    const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
+    const int pm = sizeof(T) == 8 ? ctx->pm64 : ctx->pm32;
+    const T* schur = (const T*)(sizeof(T) == 8 ? (const void*)ctx->schur : (const void*)ctx->schur32);
+    PartShape ps = part_shape<T>(pm, nplanes, ncol);
+    hipEvent_t e[3];
+    for (auto& ev : e) HIPCK(hipEventCreate(&ev));
+    launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, p2, q, nullptr, kh, kv, x);  // warm-up
+    hipEventRecord(e[0], ctx->stream);
+    for (int i = 0; i < reps; ++i) launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, p, p2, q, nullptr, kh, kv, x);
+    hipEventRecord(e[1], ctx->stream);
+    for (int i = 0; i < reps; ++i) {
+        if (precond == 2 && pm)
+            launch_part<T, false>(ctx, ps, ncol, nplanes, ncol, r, q, la, lw, lcp, schur, z);
+        else if (precond == 2)
+            hipLaunchKernelGGL((k_cg_line<T, false>), dim3((ncol + 63) / 64, 1), dim3(64), 0, ctx->stream, ncol,
+                               nplanes, (long long)ncol, nplanes, r, (const T*)q, la, lw, lcp, z, ctx->st,
+                               ctx->partials);
+        else
+            hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
+                               ctx->st, ctx->partials);
+    }
+    hipEventRecord(e[2], ctx->stream);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventSynchronize(e[2]));
+    float m1 = 0.f, m2 = 0.f;
+    hipEventElapsedTime(&m1, e[0], e[1]);
+    hipEventElapsedTime(&m2, e[1], e[2]);
+    for (auto& ev : e) hipEventDestroy(ev);
+    *t_apply = 1e-3 * m1 / reps;
+    *t_update = 1e-3 * m2 / reps;
+    HIPCK(hipMemcpyAsync(x, save, sizeof(T) * N, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->hst->done = 1;
+    HIPCK(hipMemcpyAsync(&ctx->st->done, &ctx->hst->done, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+}
+
+extern "C" {
+
+int pf_time_kernels(pf_ctx* ctx, int precision, int precond, int reps, double* t_apply, double* t_update) {
+    if (!ctx || reps <= 0 || !t_apply || !t_update) return PF_EINVAL;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "pf_time_kernels: run a solve of this precision first");
+    HIPCK(hipSetDevice(ctx->device));
+    const int pc = precond == 1 ? 1 : 2;
+    if (precision == PF_FP64)
+        return time_kernels<double>(ctx, pc, reps, t_apply, t_update, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv,
+                                    ctx->z, ctx->dinv, ctx->la, ctx->lw, ctx->lcp, ctx->kh, ctx->kv);
+    return time_kernels<float>(ctx, pc, reps, t_apply, t_update, ctx->x32, ctx->r32, ctx->p32, ctx->p2_32, ctx->q32,
+                               ctx->z32, ctx->dinv32, ctx->la32, ctx->lw32, ctx->lcp32, ctx->kh32, ctx->kv32);
 }
 
 int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds) {

@@ -55,6 +55,8 @@ struct pf_net {
     int max_samples = 0;
     bf16* dfeat = nullptr;
     float* dtarget = nullptr;
+    float* dwz = nullptr;            // per-sample z-edge loss weights (optional)
+    std::vector<char> has_wz;
     float* mask = nullptr;
     float mask_norm = 1.f;
     double* dloss = nullptr;
@@ -157,12 +159,12 @@ static void forward(pf_net* net) {
 }
 
 // backward pass against target t (fp32 per node); accumulates into net->g
-static void backward(pf_net* net, const float* t) {
+static void backward(pf_net* net, const float* t, const float* wz) {
     const Vol &V1 = net->L[0], &V2 = net->L[1], &V3 = net->L[2];
     Layer* Ly = net->layer;
     float* P = net->p;
     float* G = net->g;
-    hipLaunchKernelGGL(k_loss, dim3(gridn(V1.n(), 256, 2048)), dim3(256), 0, net->stream, V1, net->y, t, net->mask,
+    hipLaunchKernelGGL(k_loss, dim3(gridn(V1.n(), 256, 2048)), dim3(256), 0, net->stream, V1, net->y, t, net->mask, wz,
                        net->mask_norm, net->lambda, net->gy, net->dloss);
     hipLaunchKernelGGL(k_head_bwd, dim3(gridn(V1.n(), 256, 1024)), dim3(256), 0, net->stream, V1.n(), net->gy,
                        net->d1, P + net->h_off, net->g5, G + net->h_off, G + net->hb_off);
@@ -201,6 +203,10 @@ static int upload_feat(pf_net* net, const float* feat) {
     NCK(hipMemcpyAsync(net->fin, feat, sizeof(float) * n, hipMemcpyHostToDevice, net->stream));
     hipLaunchKernelGGL(k_f32_to_bf16, dim3(gridn(n)), dim3(256), 0, net->stream, n, net->fin, net->x);
     return PF_NET_OK;
+}
+
+static const float* sample_wz(pf_net* net, int idx) {
+    return (net->dwz && net->has_wz[idx]) ? net->dwz + (size_t)idx * net->L[0].n() : nullptr;
 }
 
 static void load_sample(pf_net* net, int idx) {
@@ -328,6 +334,21 @@ int pf_net_set_mask(pf_net* net, const float* mask) {
     return PF_NET_OK;
 }
 
+int pf_net_sample_weights(pf_net* net, int idx, const float* wz) {
+    if (!net || !wz) return PF_NET_EINVAL;
+    if (idx < 0 || idx >= net->max_samples) return nfail(net, PF_NET_EINVAL, "sample index out of range");
+    NCK(hipSetDevice(net->device));
+    const size_t n1 = net->L[0].n();
+    if (!net->dwz) {
+        int rc;
+        if ((rc = nalloc(net, &net->dwz, n1 * (size_t)net->max_samples))) return rc;
+        net->has_wz.assign(net->max_samples, 0);
+    }
+    NCK(hipMemcpy(net->dwz + (size_t)idx * n1, wz, sizeof(float) * n1, hipMemcpyHostToDevice));
+    net->has_wz[idx] = 1;
+    return PF_NET_OK;
+}
+
 int pf_net_set_loss(pf_net* net, float lambda) {
     if (!net || !(lambda >= 0.f)) return PF_NET_EINVAL;
     net->lambda = lambda;
@@ -380,7 +401,7 @@ int pf_net_gradient(pf_net* net, int idx, float* grads, double* loss) {
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, net->stream, 1LL, net->dloss);
     load_sample(net, idx);
     forward(net);
-    backward(net, net->dtarget + (size_t)idx * net->L[0].n());
+    backward(net, net->dtarget + (size_t)idx * net->L[0].n(), sample_wz(net, idx));
     NCK(hipGetLastError());
     NCK(hipMemcpyAsync(grads, net->g, sizeof(float) * net->nparams, hipMemcpyDeviceToHost, net->stream));
     double l = 0.;
@@ -400,7 +421,7 @@ int pf_net_train_batch(pf_net* net, const int* idx, int n, float lr, double* los
     for (int i = 0; i < n; ++i) {
         load_sample(net, idx[i]);
         forward(net);
-        backward(net, net->dtarget + (size_t)idx[i] * net->L[0].n());
+        backward(net, net->dtarget + (size_t)idx[i] * net->L[0].n(), sample_wz(net, idx[i]));
     }
     net->adam_t += 1;
     const float b1 = 0.9f, b2 = 0.999f;

@@ -260,13 +260,16 @@ __global__ void k_head(long long nv, const bf16* __restrict__ d, const float* __
     }
 }
 
-// Loss L = [sum_v m_v e_v^2 + lambda sum_edges (e_a - e_b)^2] / sum_v m_v with
-// e = y - t (m = mask, 1 when null; edges = the x, y, z neighbour pairs of the
-// node grid) and gy = dL/dy = [2 m e + 2 lambda sum_nbrs (e_v - e_nbr)] / sum m.
-// The gradient term weights errors that vary from node to node, which are the
-// ones that enlarge the residual of a start vector.
+// Loss, with e = y - t and m = mask (1 when null):
+//   L = [sum_v m_v e_v^2 + lambda sum_{x,y edges} (e_a - e_b)^2] / sum_v m_v
+//       + sum_{z edges} w_v (e_v - e_{v+z})^2
+// where the z-edge weight w_v is per node (edge v -- v + nx ny; null: lambda / sum m).
+// gy = dL/dy.  Neighbour differences weight the errors that enlarge the residual of a
+// start vector; weights proportional to the vertical conductances make the z term the
+// energy of the error in the dominant couplings.
 __global__ void k_loss(Vol o, const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ mask,
-                       float inv_norm, float lambda, float* __restrict__ gy, double* __restrict__ loss) {
+                       const float* __restrict__ wz, float inv_norm, float lambda, float* __restrict__ gy,
+                       double* __restrict__ loss) {
     __shared__ float red[8];
     const long long nv = o.n();
     float ll = 0.f;
@@ -274,30 +277,36 @@ __global__ void k_loss(Vol o, const float* __restrict__ y, const float* __restri
         const int x = (int)(v % o.nx), yy = (int)((v / o.nx) % o.ny), z = (int)(v / ((long long)o.nx * o.ny));
         const float m = mask ? mask[v] : 1.f;
         const float e = y[v] - t[v];
-        ll += m * e * e;
-        float lap = 0.f;
+        float g = 2.f * m * e * inv_norm;
+        float l = m * e * e * inv_norm;
+        const long long sx = 1, sy = o.nx, sz = (long long)o.nx * o.ny;
+        const float lx = lambda * inv_norm;
         if (lambda != 0.f) {
-            const long long sx = 1, sy = o.nx, sz = (long long)o.nx * o.ny;
-            if (x > 0) lap += e - (y[v - sx] - t[v - sx]);
+            if (x > 0) g += 2.f * lx * (e - (y[v - sx] - t[v - sx]));
             if (x < o.nx - 1) {
                 const float d = e - (y[v + sx] - t[v + sx]);
-                lap += d;
-                ll += lambda * d * d;
+                g += 2.f * lx * d;
+                l += lx * d * d;
             }
-            if (yy > 0) lap += e - (y[v - sy] - t[v - sy]);
+            if (yy > 0) g += 2.f * lx * (e - (y[v - sy] - t[v - sy]));
             if (yy < o.ny - 1) {
                 const float d = e - (y[v + sy] - t[v + sy]);
-                lap += d;
-                ll += lambda * d * d;
-            }
-            if (z > 0) lap += e - (y[v - sz] - t[v - sz]);
-            if (z < o.nz - 1) {
-                const float d = e - (y[v + sz] - t[v + sz]);
-                lap += d;
-                ll += lambda * d * d;
+                g += 2.f * lx * d;
+                l += lx * d * d;
             }
         }
-        gy[v] = 2.f * (m * e + lambda * lap) * inv_norm;
+        if (z > 0) {
+            const float w = wz ? wz[v - sz] : lx;
+            g += 2.f * w * (e - (y[v - sz] - t[v - sz]));
+        }
+        if (z < o.nz - 1) {
+            const float w = wz ? wz[v] : lx;
+            const float d = e - (y[v + sz] - t[v + sz]);
+            g += 2.f * w * d;
+            l += w * d * d;
+        }
+        gy[v] = g;
+        ll += l;
     }
     for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ll;
@@ -305,7 +314,7 @@ __global__ void k_loss(Vol o, const float* __restrict__ y, const float* __restri
     if (threadIdx.x == 0) {
         float s2 = 0.f;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s2 += red[k];
-        atomicAdd(loss, (double)s2 * inv_norm);
+        atomicAdd(loss, (double)s2);
     }
 }
 

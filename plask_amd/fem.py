@@ -23,7 +23,7 @@ EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
-    "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_solution_prolong",
+    "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
 ]
 
 
@@ -88,6 +88,8 @@ def load_library(path=LIB_PATH):
     L.pf_set_prefetch.argtypes = [P, I]
     L.pf_set_apply.argtypes = [P, I, I, I]
     L.pf_set_graphs.argtypes = [P, I]
+    L.pf_set_block_order.argtypes = [P, I]
+    L.pf_time_kernels.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pf_stream_copy.argtypes = [P, i64, I, C.POINTER(C.c_double)]
     L.pf_solution_prolong.argtypes = [P, P, P, I, P, I, P, I, C.c_double, C.c_double]
     _lib = L
@@ -246,6 +248,9 @@ class FemContext:
         precision PF_FP64, PF_FP32_REFINE or -1 (both)."""
         self._ck(self.L.pf_set_apply(self.h, int(precision), int(kind), int(planes)), "pf_set_apply")
 
+    def set_block_order(self, xcd):
+        self._ck(self.L.pf_set_block_order(self.h, 1 if xcd else 0), "pf_set_block_order")
+
     def set_graphs(self, on):
         self._ck(self.L.pf_set_graphs(self.h, 1 if on else 0), "pf_set_graphs")
 
@@ -269,6 +274,13 @@ class FemContext:
 
     def set_profile(self, on):
         self._ck(self.L.pf_set_profile(self.h, 1 if on else 0), "pf_set_profile")
+
+    def time_kernels(self, precision=PF_FP64, precond=PRECOND_LINE, reps=50):
+        """(apply, update) seconds per launch, back-to-back launches after a solve."""
+        a, u = C.c_double(0.), C.c_double(0.)
+        self._ck(self.L.pf_time_kernels(self.h, int(precision), int(precond), int(reps), C.byref(a), C.byref(u)),
+                 "pf_time_kernels")
+        return a.value, u.value
 
     def time_apply(self, precision=PF_FP64, reps=50):
         t = C.c_double(0.)

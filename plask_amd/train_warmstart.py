@@ -27,7 +27,7 @@ from . import fem, warmstart, workloads
 def _feat_worker(args):
     seed, shape = args
     p = workloads.thermal_3d(seed=seed, n=shape)
-    return seed, warmstart.features(p)
+    return seed, warmstart.features(p), warmstart.z_edge_conductance(p)
 
 
 def _setup(ctx, p):
@@ -52,7 +52,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--lam", type=float, default=1.0, help="weight of the neighbour-difference loss term")
+    ap.add_argument("--lam", type=float, default=1.0, help="weight of the x/y neighbour-difference loss term")
+    ap.add_argument("--mu", type=float, default=1.0,
+                    help="weight of the vertical-conductance energy term (relative to the target's own); 0: off")
     ap.add_argument("--out", default="")
     ap.add_argument("--summary", default="")
     a = ap.parse_args(argv)
@@ -68,14 +70,17 @@ def main(argv=None):
     iters_train = []
     try:
         seeds = list(range(4000, 4000 + a.train))
-        for i, (seed, f) in enumerate(pool.imap(_feat_worker, [(s, shape) for s in seeds], chunksize=4)):
+        for i, (seed, f, cond) in enumerate(pool.imap(_feat_worker, [(s, shape) for s in seeds], chunksize=4)):
             p = workloads.thermal_3d(seed=seed, n=shape)
             _setup(ctx, p)
             st = ctx.solve_uniform(300.0, rtol=1e-8)
             T = ctx.solution()
             if i == 0:
                 net.set_mask(warmstart.dirichlet_mask(p))
-            net.set_sample(i, f, (T - 300.0) / warmstart.target_scale(p))
+            t = (T - 300.0) / warmstart.target_scale(p)
+            net.set_sample(i, f, t)
+            if a.mu > 0:
+                net.set_sample_weights(i, warmstart.z_edge_weights(p, t, a.mu, conductance=cond))
             iters_train.append(st.iters)
             if (i + 1) % 200 == 0:
                 log("data %d/%d (%.0f s)" % (i + 1, a.train, time.perf_counter() - t0))
@@ -115,6 +120,8 @@ def main(argv=None):
         ws.start_on_device(ctx, p, f)
         st2 = ctx.solve_warm(rtol=1e-8, init_value=300.0, checked=True, precision=fem.PF_FP32_REFINE)
         rise = np.abs(T0 - 300.0).max()
+        if (j + 1) % 50 == 0:
+            log("held-out %d/%d" % (j + 1, a.held))
         rows.append(dict(seed=9000 + j, it_uniform=st0.iters, it_net=st1.iters, it_net_fp32r=st2.iters,
                          it_uniform_fp32r=st3.iters, fallback=int(st1.start), guess_err=float(np.abs(x0 - T0).max() / rise),
                          sol_diff=float(np.abs(T1 - T0).max() / rise), rel_resid=st1.rel_resid, guess_s=tg))

@@ -40,6 +40,7 @@ def _lib():
     L.pf_net_set_mask.argtypes = [P, P]
     L.pf_net_sample_set.argtypes = [P, I, P, P]
     L.pf_net_set_loss.argtypes = [P, C.c_float]
+    L.pf_net_sample_weights.argtypes = [P, I, P]
     L.pf_net_predict.argtypes = [P, P, P]
     L.pf_net_predict_sample.argtypes = [P, I, P]
     L.pf_net_gradient.argtypes = [P, I, P, C.POINTER(C.c_double)]
@@ -128,6 +129,11 @@ class WarmNet:
         assert feat.size == self.nv * NIN and target.size == self.nv
         self._ck(self.L.pf_net_sample_set(self.h, int(idx), _ptr(feat), _ptr(target)), "pf_net_sample_set")
 
+    def set_sample_weights(self, idx, wz):
+        wz = _f32(wz)
+        assert wz.size == self.nv
+        self._ck(self.L.pf_net_sample_weights(self.h, int(idx), _ptr(wz)), "pf_net_sample_weights")
+
     def predict(self, feat):
         feat = _f32(feat)
         assert feat.size == self.nv * NIN
@@ -215,6 +221,36 @@ def target_scale(p):
     a = p["active"]
     r = float(np.sum(p["thick"][:a] / p["kz_layers"][:a])) * 1e-6   # m^2 K / W
     return max(power / area * r, 1e-6)
+
+
+def z_edge_conductance(p):
+    """Vertical conductance k_z dx dy / dz of the z edge above each node (mean of its up
+    to 4 element columns; 0 on the top plane), node order."""
+    x, y, z = p["axes"]
+    nx, ny, nz = len(x), len(y), len(z)
+    kz = p["k_v"].reshape(nz - 1, ny - 1, nx - 1)
+    area = np.outer(np.diff(y), np.diff(x))[None]
+    g = kz * area / np.diff(z)[:, None, None]
+    acc = np.zeros((nz - 1, ny, nx))
+    cnt = np.zeros((1, ny, nx))
+    for dy in (0, 1):
+        for dx in (0, 1):
+            acc[:, dy:dy + ny - 1, dx:dx + nx - 1] += g
+            cnt[:, dy:dy + ny - 1, dx:dx + nx - 1] += 1
+    w = np.zeros((nz, ny, nx))
+    w[:-1] = acc / cnt
+    return w.ravel()
+
+
+def z_edge_weights(p, target, mu=1.0, conductance=None):
+    """Per-node weights of the vertical neighbour differences of the loss: the z-edge
+    conductances scaled so that the target's own weighted energy is mu."""
+    x, y, z = p["axes"]
+    shape = (len(z), len(y), len(x))
+    w = (z_edge_conductance(p) if conductance is None else conductance).reshape(shape)
+    t = np.asarray(target, dtype=np.float64).reshape(shape)
+    energy = float((w[:-1] * np.diff(t, axis=0) ** 2).sum())
+    return (mu * w / max(energy, 1e-300)).ravel().astype(np.float32)
 
 
 def dirichlet_mask(p):

@@ -33,11 +33,14 @@ def test_forward_and_gradient_match_oracle():
     net.set_params(p)
     net.set_loss(0.5)
     net.set_sample(0, x.reshape(-1, 8), t.ravel())
+    wz = rng.random((nz, ny, nx)).astype(np.float32) * 1e-3
+    net.set_sample_weights(0, wz.ravel())
     y_gpu = net.predict_sample(0)
     y_host = net.predict(x.reshape(-1, 8))
     assert np.array_equal(y_gpu, y_host)
     loss_gpu, g_gpu = net.gradient(0)
-    loss, g, y = W.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rb=True, lam=0.5)
+    loss, g, y = W.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rb=True, lam=0.5,
+                             wz=wz.astype(np.float64))
     ey = _rel(y_gpu, y.ravel())
     print("forward rel err %.2e, loss %.6g vs %.6g" % (ey, loss_gpu, loss), flush=True)
     assert ey <= 1e-2

@@ -47,15 +47,16 @@ def test_backward_matches_finite_differences():
     t = rng.standard_normal((8, 8, 8))
     mask = (rng.random((8, 8, 8)) > 0.2).astype(float)
     lam = 0.7
-    loss, g, _ = W.loss_grad(p, x, t, mask, lam=lam)
+    wz = rng.random((8, 8, 8))
+    loss, g, _ = W.loss_grad(p, x, t, mask, lam=lam, wz=wz)
     idx = np.r_[rng.choice(p.size - 33, 60, replace=False), np.arange(p.size - 33, p.size)]
     eps = 1e-6
     for i in idx:
         pp = p.copy()
         pp[i] += eps
-        lp = W.loss_grad(pp, x, t, mask, lam=lam)[0]
+        lp = W.loss_grad(pp, x, t, mask, lam=lam, wz=wz)[0]
         pp[i] -= 2 * eps
-        lm = W.loss_grad(pp, x, t, mask, lam=lam)[0]
+        lm = W.loss_grad(pp, x, t, mask, lam=lam, wz=wz)[0]
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
 
