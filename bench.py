@@ -9,7 +9,8 @@ the device) to ||B - A T|| <= 1e-8 ||B - A T_init||, with the element data, sour
 and Dirichlet set already resident in HBM.  The solution stays on the device.
 
 Printed JSON line: value = solves/s of FP64 PCG with the exact line (layer-axis)
-preconditioner; extra fields give the FP32-operator + FP64-refinement form, the
+preconditioner; extra fields give the FP32-operator forms (reliable updates, and
+restarted FP64 iterative refinement), the
 plain Jacobi-PCG of the comparison, the learned-warm-start form when trained weights
 are present, the roofline of the dominant kernel (HIP events around each launch of
 an extra profiled solve, on the context stream) and the CPU path timed on this host
@@ -27,7 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from plask_amd import workloads  # noqa: E402
-from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PRECOND_JACOBI, PRECOND_LINE, FemContext  # noqa: E402
+from plask_amd.fem import (PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PRECOND_JACOBI, PRECOND_LINE,  # noqa: E402
+                           FemContext)
+
 
 HBM_PEAK_GBS = 8000.0
 # Full-solve iteration count of the CPU path (serial IC(0)-CG on the 14-diagonal
@@ -37,6 +40,9 @@ HBM_PEAK_GBS = 8000.0
 CPU_FULL_ITERS = {(128, 128, 256): 161}
 WEIGHTS = os.path.join(ROOT, "plask_amd", "data", "warmstart_64x64x128.npz")
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+# relative reduction asked of each FP32 run: restarted refinement needs deep inner solves,
+# the reliable-update form replaces the residual early and keeps the search direction
+INNER = {PF_FP64: 0.0, PF_FP32_REFINE: 1e-5, PF_FP32_RELIABLE: 1e-2}
 
 
 def make_ctx(p, profile=False):
@@ -101,7 +107,7 @@ def timed(fn, steps, warmup):
 
 
 def learned(ctx, p, size, steps, warmup, T64):
-    """FP32+refinement PCG from the warm-start network guess (prediction on the
+    """FP32 reliable-update PCG from the warm-start network guess (prediction on the
     64x64x128 grid of the same stack, trilinear onto the solve grid); the network
     forward pass and the interpolation are inside the timed step."""
     from plask_amd import warmstart
@@ -115,7 +121,8 @@ def learned(ctx, p, size, steps, warmup, T64):
 
     def step():
         ws.start_on_device(ctx, pc, feat)
-        return ctx.solve_warm(precision=PF_FP32_REFINE, rtol=1e-8, inner_rtol=1e-5, init_value=300.0, checked=True)
+        return ctx.solve_warm(precision=PF_FP32_RELIABLE, rtol=1e-8, inner_rtol=INNER[PF_FP32_RELIABLE],
+                              init_value=300.0, checked=True)
 
     t, st = timed(step, steps, warmup)
     T = ctx.solution()
@@ -128,6 +135,33 @@ def learned(ctx, p, size, steps, warmup, T64):
     return dict(solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters, outer=st.outer, rel_resid=st.rel_resid,
                 fell_back_to_uniform_start=bool(st.start),
                 network_and_prolong_ms=1e3 * t_net, host_features_ms_untimed=1e3 * t_feat, max_rel_diff_vs_fp64=float(np.abs(T - T64).max() / np.abs(T64).max()))
+
+
+def config1(steps, with_cpu):
+    """2D cylindrical stack, 200 (r) x 400 (z) nodes (plask_amd.workloads.thermal_2d_cyl, seed 1):
+    GPU FP64 line-PCG solve to 1e-8 vs the reference method on the CPU (assembly + band
+    Cholesky, kd = 201, serial restatement), with the max relative difference."""
+    p = workloads.thermal_2d_cyl(seed=1)
+    c = FemContext(0)
+    c.set_grid(p["axes"], cylindrical=True)
+    c.set_coeffs(p["k_h"], p["k_v"])
+    c.set_source(p["q"])
+    c.set_dirichlet(p["dir_nodes"], p["dir_vals"])
+    t, st = timed(lambda: c.solve_uniform(300.0, rtol=1e-8), steps, 1)
+    T = c.solution()
+    out = dict(workload="thermal2d_cyl_200x400", gpu_ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
+    if with_cpu:
+        import oracle as O
+        g = O.Grid(p["axes"], cyl=True)
+        t0 = time.perf_counter()
+        A = O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_BAND, p["dir_nodes"], p["dir_vals"])
+        Tr = O.band_cholesky_solve(A)
+        tc = time.perf_counter() - t0
+        out.update(cpu_band_cholesky_ms=1e3 * tc, cpu_cores=1, kd=int(A.kd),
+                   max_rel_diff_vs_band_cholesky=float(np.abs(T - Tr).max() / np.abs(Tr).max()),
+                   speedup_vs_cpu=tc / t)
+    c.close()
+    return out
 
 
 def main():
@@ -164,7 +198,7 @@ def main():
         return
 
     def step(prec, pc=PRECOND_LINE):
-        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-5 if prec else 0.0, precond=pc)
+        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=INNER[prec] or None, precond=pc)
 
     t, st = timed(lambda: step(PF_FP64), args.steps, args.warmup)
     fp64 = dict(solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
@@ -176,21 +210,27 @@ def main():
     jac = dict(solves_per_s=1.0 / tj, ms=1e3 * tj, iters=stj.iters, rel_resid=stj.rel_resid)
     print("fp64 jacobi: %.2f solves/s, %d iterations" % (jac["solves_per_s"], stj.iters), file=sys.stderr, flush=True)
 
-    mixed = None
-    if not args.no_mixed:
-        t, st = timed(lambda: step(PF_FP32_REFINE), args.steps, args.warmup)
+    def mixed_form(prec, name):
+        t, st = timed(lambda: step(prec), args.steps, args.warmup)
         T32 = ctx.solution()
-        mixed = dict(solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters, outer=st.outer, rel_resid=st.rel_resid,
-                     max_rel_diff_vs_fp64=float(np.abs(T32 - T64).max() / np.abs(T64).max()),
-                     speedup_vs_fp64=fp64["ms"] / (1e3 * t))
-        print("fp32+refine: %.2f solves/s, %.1f ms/solve, %d inner its, %d outer, rel resid %.2e" %
-              (mixed["solves_per_s"], mixed["ms"], st.iters, st.outer, st.rel_resid), file=sys.stderr, flush=True)
+        m = dict(method=name, inner_rtol=INNER[prec], solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters,
+                 outer=st.outer, rel_resid=st.rel_resid,
+                 max_rel_diff_vs_fp64=float(np.abs(T32 - T64).max() / np.abs(T64).max()),
+                 speedup_vs_fp64=fp64["ms"] / (1e3 * t))
+        print("%s: %.2f solves/s, %.1f ms/solve, %d inner its, %d outer, rel resid %.2e" %
+              (name, m["solves_per_s"], m["ms"], st.iters, st.outer, st.rel_resid), file=sys.stderr, flush=True)
+        return m
+
+    mixed = mixed_refine = None
+    if not args.no_mixed:
+        mixed = mixed_form(PF_FP32_RELIABLE, "fp32 operator, reliable updates (fp64 residual replacement)")
+        mixed_refine = mixed_form(PF_FP32_REFINE, "fp32 operator, restarted fp64 iterative refinement")
 
     warm = None
     if not args.no_mixed and os.path.exists(WEIGHTS):
         warm = learned(ctx, p, size, args.steps, args.warmup, T64)
         warm["speedup_vs_fp64"] = fp64["ms"] / warm["ms"]
-        print("fp32+refine+learned start: %.2f solves/s, %.1f ms/solve, %d inner its" %
+        print("fp32 reliable + learned start: %.2f solves/s, %.1f ms/solve, %d inner its" %
               (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
 
     # roofline: one profiled solve per precision -- HIP events around every kernel on the
@@ -198,11 +238,11 @@ def main():
     # kernel trace reports; launches after the stop are not counted)
     def profiled(prec):
         ctx.set_profile(True)
-        ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=1e-5 if prec else 0.0)
+        ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=INNER[prec] or None)
         ctx.set_profile(False)
         (ta, tl), (na, nl) = ctx.kernel_times()
         return ta, tl, int(na)
-    t_apply32, t_line32, n32 = profiled(PF_FP32_REFINE)
+    t_apply32, t_line32, n32 = profiled(PF_FP32_RELIABLE)
     t_apply, t_line, n64 = profiled(PF_FP64)
     t_plain = ctx.time_apply(PF_FP64, reps=50)
     kb = kernel_bytes(size)
@@ -242,6 +282,9 @@ def main():
                          "solve to the same stopping rule" % (c["iters_sampled"], c["s_per_iter"], full),
                "assemble_s": round(c["assemble_s"], 3), "s_per_iter": c["s_per_iter"]}
 
+    c1 = config1(args.steps, not args.no_cpu)
+    print("config 1 (2D cyl): %.2f ms GPU" % c1["gpu_ms"], file=sys.stderr, flush=True)
+
     line = {
         "metric": "3D thermal solves/s to 1e-8 rel. residual (4.2M DOF); operator-apply HBM GB/s",
         "value": round(fp64["solves_per_s"], 4),
@@ -260,8 +303,10 @@ def main():
                    "residual": "||B-AT||/||B-AT_init||, T_init = 300 K",
                    "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]},
         "mixed_precision": mixed,
+        "mixed_precision_restarted": mixed_refine,
         "learned_warm_start": warm,
         "fp64_jacobi_pcg": jac,
+        "config1_2d_cyl": c1,
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,

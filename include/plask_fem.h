@@ -67,6 +67,8 @@ typedef struct pf_ctx pf_ctx;
 /* precision of a solve */
 #define PF_FP64 0         /* FP64 vectors and operator */
 #define PF_FP32_REFINE 1  /* FP32 operator + inner PCG, FP64 outer residual / update */
+#define PF_FP32_RELIABLE 2 /* as PF_FP32_REFINE, but one FP32 Krylov run: at each FP64 residual
+                              replacement the search direction is kept (reliable updates) */
 
 /* start vector */
 #define PF_X0_GIVEN 0 /* x_inout holds the start vector (its Dirichlet entries are overwritten) */
@@ -90,7 +92,8 @@ typedef struct {
     int x0;           /* PF_X0_* */
     int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
     double rtol;      /* stop when ||B - A X||_2 <= rtol * (denominator chosen by `norm`) */
-    double inner_rtol;/* PF_FP32_REFINE: inner relative reduction per refinement step */
+    double inner_rtol;/* FP32 forms: relative reduction asked of each FP32 run (0: 1e-4 for
+                         PF_FP32_REFINE, 1e-2 for PF_FP32_RELIABLE) */
     int max_outer;    /* PF_FP32_REFINE: max refinement steps */
     int check_every;  /* iterations between host convergence checks (0 = default 32) */
     int norm;         /* PF_NORM_* */

@@ -42,7 +42,7 @@ struct PcgState {
     double rr;       // (r, r)
     double stop2;    // stop when rr <= stop2
     double bb;       // (b, b)
-    double spare;
+    double beta0;    // beta of the first iteration: 0, or (r,z)/(r,z)_prev when a run resumes
     int iters;
     int maxit;
     int done;    // 1: converged or stopped
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     const int BX = g.n0;
     const int PX = BX + 1;  // padded row: column BX holds zeros
     T* xs = reinterpret_cast<T*>(smem_raw);       // [2][BY+1][PX]
-    T* oc = xs + 2 * (BY + 1) * PX;               // [4][BY][PX], corner outputs at column tx+1
+    T* oc = xs + 2 * (BY + 1) * PX;               // [2][4][BY][PX], corner outputs at column tx+1
     if (FUSEP || DOT) {
         if (st->done) return;
     }
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     double beta = 0.;
     T alpha_prev = T(0);
     if (FUSEP) {
-        beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+        beta = (st->iters == 0) ? st->beta0 : st->rz / st->rz_prev;
         alpha_prev = T(st->alpha);
     }
 
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     }
     // zero the padding column of both plane buffers and of the corner outputs
     for (int r = tid; r < 2 * (BY + 1); r += blockDim.x) xs[r * PX + BX] = T(0);
-    for (int r = tid; r < 4 * BY; r += blockDim.x) oc[r * PX] = T(0);
+    for (int r = tid; r < 8 * BY; r += blockDim.x) oc[r * PX] = T(0);
 
     // Global loads of one step, issued one step ahead (software prefetch): node values
     // of plane k (own row and, for the last thread row, row j+1) and the element data
@@ -303,12 +303,32 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     T acc[4] = {T(0), T(0), T(0), T(0)};
     double dot = 0.;
 
+    // One barrier per plane: the corner contributions of plane P go to one of two
+    // corner buffers (alternating by step) after the barrier of step P and are
+    // gathered after the barrier of step P + 1.
+    T own_prev = T(0);
+    bool dir_prev = false;
+    int obuf = 0;
+    auto gather = [&](int Pg, int ob, T own_g, bool dir_g) {
+        const T* o = oc + ob * 4 * BY * PX;
+        T s = o[(0 * BY + ty) * PX + tx + 1] + o[(1 * BY + ty) * PX + tx] + o[(2 * BY + ty - 1) * PX + tx + 1] +
+              o[(3 * BY + ty - 1) * PX + tx];
+        long long node = (long long)Pg * plane + (long long)j * n0 + tx;
+        T out;
+        if (MODE == MODE_OP) {
+            out = dir_g ? own_g : s;
+        } else {
+            out = dir_g ? T(0) : (F[node] - s);
+        }
+        y[node] = out;
+        if (DOT && !dir_g) dot += (double)own_g * (double)out;
+    };
     for (int P = P0; P < ke; ++P) {
         const bool has_next = (P + 1) < n2;
         T bot[4] = {T(0), T(0), T(0), T(0)}, top[4] = {T(0), T(0), T(0), T(0)};
+        T a = T(0), b = T(0), c = T(0);
         if (has_next) {
             lstore(P + 1, buf ^ 1, R1, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
-            T a = T(0), b = T(0), c = T(0);
             if (evalid) {
                 const T hz = g.hz[P];
                 a = R1.kxy * (gxf * hz);
@@ -321,63 +341,56 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
             } else if (P + 2 < n2 && P + 2 <= ke) {
                 gload(P + 2, R1);  // prefetch the next step
             }
-            __syncthreads();
-            if (active) {
-                plane_tf(buf ^ 1, xt_next);
-                // z stage: w[pq sigma] = T + B, w[pq delta] = T - B, scaled by the eigenvalues
-                const T ab = a + b, ac = a + c, bc = b + c, abc3 = (ab + c) * T(1. / 3.);
-                // components: 0 = ss, 1 = ds, 2 = sd, 3 = dd (x,y); z sigma / delta
-                T ws1 = (xt_next[1] + xt_cur[1]) * (T(3) * a);   // d s s
-                T ws2 = (xt_next[2] + xt_cur[2]) * (T(3) * b);   // s d s
-                T ws3 = (xt_next[3] + xt_cur[3]) * ab;           // d d s
-                T wd0 = (xt_next[0] - xt_cur[0]) * (T(3) * c);   // s s d
-                T wd1 = (xt_next[1] - xt_cur[1]) * ac;           // d s d
-                T wd2 = (xt_next[2] - xt_cur[2]) * bc;           // s d d
-                T wd3 = (xt_next[3] - xt_cur[3]) * abc3;         // d d d
-                bot[0] = -wd0;
-                top[0] = wd0;
-                bot[1] = ws1 - wd1;
-                top[1] = ws1 + wd1;
-                bot[2] = ws2 - wd2;
-                top[2] = ws2 + wd2;
-                bot[3] = ws3 - wd3;
-                top[3] = ws3 + wd3;
-            }
         }
-        if (P >= kb) {
+        __syncthreads();
+        // outputs of plane P - 1, whose corner contributions were written in the last step
+        if (P - 1 >= kb && owned) gather(P - 1, obuf ^ 1, own_prev, dir_prev);
+        if (has_next && active) {
+            plane_tf(buf ^ 1, xt_next);
+            // z stage: w[pq sigma] = T + B, w[pq delta] = T - B, scaled by the eigenvalues
+            const T ab = a + b, ac = a + c, bc = b + c, abc3 = (ab + c) * T(1. / 3.);
+            // components: 0 = ss, 1 = ds, 2 = sd, 3 = dd (x,y); z sigma / delta
+            T ws1 = (xt_next[1] + xt_cur[1]) * (T(3) * a);   // d s s
+            T ws2 = (xt_next[2] + xt_cur[2]) * (T(3) * b);   // s d s
+            T ws3 = (xt_next[3] + xt_cur[3]) * ab;           // d d s
+            T wd0 = (xt_next[0] - xt_cur[0]) * (T(3) * c);   // s s d
+            T wd1 = (xt_next[1] - xt_cur[1]) * ac;           // d s d
+            T wd2 = (xt_next[2] - xt_cur[2]) * bc;           // s d d
+            T wd3 = (xt_next[3] - xt_cur[3]) * abc3;         // d d d
+            bot[0] = -wd0;
+            top[0] = wd0;
+            bot[1] = ws1 - wd1;
+            top[1] = ws1 + wd1;
+            bot[2] = ws2 - wd2;
+            top[2] = ws2 + wd2;
+            bot[3] = ws3 - wd3;
+            top[3] = ws3 + wd3;
+        }
+        if (P >= kb && active) {
             // plane P is complete: inverse in-plane transform -> 4 corner outputs
-            if (active) {
-                T t0 = acc[0] + bot[0], t1 = acc[1] + bot[1], t2 = acc[2] + bot[2], t3 = acc[3] + bot[3];
-                T as = t0 - t2, ad = t1 - t3;  // row j   (y sigma - y delta)
-                T bs = t0 + t2, bd = t1 + t3;  // row j+1
-                oc[(0 * BY + ty) * PX + tx + 1] = as - ad;  // corner (0,0)
-                oc[(1 * BY + ty) * PX + tx + 1] = as + ad;  // corner (1,0)
-                oc[(2 * BY + ty) * PX + tx + 1] = bs - bd;  // corner (0,1)
-                oc[(3 * BY + ty) * PX + tx + 1] = bs + bd;  // corner (1,1)
-            }
-            __syncthreads();
-            if (owned) {
-                T s = oc[(0 * BY + ty) * PX + tx + 1] + oc[(1 * BY + ty) * PX + tx] +
-                      oc[(2 * BY + ty - 1) * PX + tx + 1] + oc[(3 * BY + ty - 1) * PX + tx];
-                long long node = (long long)P * plane + (long long)j * n0 + tx;
-                T out;
-                if (MODE == MODE_OP) {
-                    out = dir_cur ? own_cur : s;
-                } else {
-                    out = dir_cur ? T(0) : (F[node] - s);
-                }
-                y[node] = out;
-                if (DOT && !dir_cur) dot += (double)own_cur * (double)out;
-            }
+            T* o = oc + obuf * 4 * BY * PX;
+            T t0 = acc[0] + bot[0], t1 = acc[1] + bot[1], t2 = acc[2] + bot[2], t3 = acc[3] + bot[3];
+            T as = t0 - t2, ad = t1 - t3;  // row j   (y sigma - y delta)
+            T bs = t0 + t2, bd = t1 + t3;  // row j+1
+            o[(0 * BY + ty) * PX + tx + 1] = as - ad;  // corner (0,0)
+            o[(1 * BY + ty) * PX + tx + 1] = as + ad;  // corner (1,0)
+            o[(2 * BY + ty) * PX + tx + 1] = bs - bd;  // corner (0,1)
+            o[(3 * BY + ty) * PX + tx + 1] = bs + bd;  // corner (1,1)
         }
-        for (int a = 0; a < 4; ++a) {
-            acc[a] = top[a];
-            xt_cur[a] = xt_next[a];
+        for (int q = 0; q < 4; ++q) {
+            acc[q] = top[q];
+            xt_cur[q] = xt_next[q];
         }
+        own_prev = own_cur;
+        dir_prev = dir_cur;
         own_cur = own_next;
         dir_cur = dir_next;
         buf ^= 1;
+        obuf ^= 1;
     }
+    // drain: outputs of the last plane
+    __syncthreads();
+    if (ke - 1 >= kb && owned) gather(ke - 1, obuf ^ 1, own_prev, dir_prev);
     if (DOT) {
         double v[1] = {dot}, tot[1];
         if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
@@ -439,7 +452,7 @@ __global__ __launch_bounds__(512) void k_apply3r(Axes3<T> g, int KR, int nb, con
         double beta = 0.;
         T alpha_prev = T(0);
         if (FUSEP) {
-            beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+            beta = (st->iters == 0) ? st->beta0 : st->rz / st->rz_prev;
             alpha_prev = T(st->alpha);
         }
         const T c48 = T(1e-6 / 48.);
@@ -668,7 +681,7 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
     double beta = 0.;
     T alpha_prev = T(0);
     if (FUSEP) {
-        beta = (st->iters == 0) ? 0. : st->rz / st->rz_prev;
+        beta = (st->iters == 0) ? st->beta0 : st->rz / st->rz_prev;
         alpha_prev = T(st->alpha);
     }
 
@@ -807,9 +820,13 @@ __device__ inline void finish_update(PcgState* st, const double (&tot)[2]) {
 }
 
 // start of a CG run: scalars from (r,z), (r,r)
+// Start of a CG run.  beta0 > 0 on entry marks a resumed run (the residual was replaced
+// and the previous direction kept): it then holds the previous (r, z) in the current
+// scaling and becomes the first beta.
 __device__ inline void finish_start(PcgState* st, const double (&tot)[2]) {
     st->rz = tot[0];
     st->rz_prev = tot[0];
+    st->beta0 = st->beta0 > 0. ? tot[0] / st->beta0 : 0.;
     st->rr = tot[1];
     st->iters = 0;
     st->status = 0;

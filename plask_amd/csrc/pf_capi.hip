@@ -251,6 +251,12 @@ int64_t pf_num_nodes(const pf_ctx* ctx) { return ctx ? ctx->N : 0; }
 int64_t pf_num_elements(const pf_ctx* ctx) { return ctx ? ctx->E : 0; }
 void* pf_solution_device(pf_ctx* ctx) { return ctx ? (void*)ctx->x : nullptr; }
 
+// LDS bytes of the FP64 3D tile kernel: two x tiles of (rows + 1) x (n0 + 1) and two
+// corner-output buffers of 4 x rows x (n0 + 1)
+static size_t tile_lds(int rows, int n0) {
+    return sizeof(double) * (size_t)(2 * (rows + 1) + 8 * rows) * (size_t)(n0 + 1);
+}
+
 int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, const double* ax1, int64_t n1,
                 const double* ax2, int64_t n2) {
     if (!ctx) return PF_EINVAL;
@@ -350,6 +356,9 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
         ctx->BY[0] = ctx->BY[1] = by;
         ctx->KC[0] = (int)std::max<int64_t>(1, std::min<int64_t>(8, n2));
         ctx->KC[1] = (int)std::max<int64_t>(1, std::min<int64_t>(16, n2));
+        // long x rows: the tile kernel's LDS would pass 64 KiB, use row blocks
+        const int kind = tile_lds(by, (int)n0) > 65536 ? 1 : 0;
+        ctx->apply_kind[0] = ctx->apply_kind[1] = kind;
     } else {
         ctx->KC2 = (int)std::max<int64_t>(4, std::min<int64_t>(64, n1 / 64));
     }
@@ -494,7 +503,7 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
         const int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
         const int zt = (int)((ctx->n[2] + KC - 1) / KC);
         const int nb = ytiles * zt, grid = (nb + 7) / 8 * 8;
-        size_t shm = sizeof(T) * (size_t)(2 * (BY + 1) * (BX + 1) + 4 * BY * (BX + 1));
+        const size_t shm = tile_lds(BY, BX) / sizeof(double) * sizeof(T);
         if (ctx->pfd == 2)
             hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
                                axes3<T>(ctx), BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
@@ -600,11 +609,12 @@ static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long 
 template <typename T>
 static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* la,
                   const T* lw, const T* lcp, const T* kh, const T* kv, double stop2, int maxit, int check_every,
-                  hipGraphExec_t* gcache, int* iters_out, int* status_out) {
+                  hipGraphExec_t* gcache, int* iters_out, int* status_out, double resume_rz = 0.) {
     const long long N = ctx->N;
     PcgState init{};
     init.stop2 = stop2;
     init.maxit = maxit;
+    init.beta0 = resume_rz;  // > 0: keep the direction already in p (see finish_start)
     *ctx->hst = init;  // pinned staging: the host does not touch it again before the next sync
     HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
     const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
@@ -638,8 +648,9 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
                                ctx->st, ctx->partials);
     };
-    // both direction buffers start at zero (beta = 0 on the first iteration)
-    hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p, T(0));
+    // a fresh run starts both direction buffers at zero (beta = 0 on the first iteration);
+    // a resumed run keeps the previous direction in p
+    if (!(resume_rz > 0.)) hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p, T(0));
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, p2, T(0));
     HIPCK(hipGetLastError());
     // iteration t reads p_old from pb[t % 2] and writes p_new to pb[(t + 1) % 2]
@@ -722,6 +733,12 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     *iters_out = ctx->hst->iters;
     *status_out = ctx->hst->status;
     return PF_OK;
+}
+
+// dst = f * src (FP32; src may alias dst)
+__global__ void k_scale_f32(long long n, const float* src, float f, float* dst) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = f * src[i];
 }
 
 // ||M(F - A0 x)||^2 + 0 (Dirichlet rows of x hold the values exactly) into r; returns sqrt
@@ -837,6 +854,54 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
                 status = stt;
                 break;
             }
+        }
+    } else if (opts->precision == PF_FP32_RELIABLE) {
+        // One FP32 Krylov run with residual replacement (reliable updates): whenever the
+        // FP32 recursive residual has dropped by `inner`, the correction is added to x in
+        // FP64, the residual is recomputed in FP64 and CG continues with the previous
+        // direction, rescaled to the new residual's units, instead of restarting.  The
+        // direction is kept only while the true residual stays close to the recursive one
+        // (within a factor 3): the FP32 operator's error grows with the reduction asked of
+        // one run, so `inner` should be modest (1e-2 to 1e-3) for this form.
+        const double inner = opts->inner_rtol > 0. ? opts->inner_rtol : 1e-2;
+        const int max_outer = opts->max_outer > 0 ? opts->max_outer : 50;
+        double rn_prev = 0., rz_last = 0.;
+        int last_it = 0;
+        bool keep = false;  // resume with the kept direction (else a fresh start)
+        while (rn > rtol * bnorm && outer < max_outer && total_iters < maxit) {
+            double s = 1. / rn;
+            hipLaunchKernelGGL(k_scale_to_f32, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->r, s, ctx->r32);
+            hipLaunchKernelGGL((k_fill<float>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x32, 0.f);
+            double resume = 0.;
+            if (keep && rz_last > 0.) {
+                const double f = rn_prev / rn;  // old scale over new scale
+                const float* src = (last_it & 1) ? ctx->p2_32 : ctx->p32;
+                hipLaunchKernelGGL(k_scale_f32, dim3(gb), dim3(256), 0, ctx->stream, N, src, (float)f, ctx->p32);
+                resume = rz_last * f * f;
+            }
+            int it = 0, stt = 0;
+            double goal = std::max(inner, 0.5 * rtol * bnorm / rn);
+            if ((rc = run_cg<float>(ctx, precond, ctx->x32, ctx->r32, ctx->p32, ctx->p2_32, ctx->q32, ctx->z32,
+                                    ctx->dinv32, ctx->la32, ctx->lw32, ctx->lcp32, ctx->kh32, ctx->kv32, goal * goal,
+                                    maxit - total_iters, opts->check_every, &ctx->graph32, &it, &stt, resume)))
+                return rc;
+            total_iters += it;
+            last_it = it;
+            rz_last = ctx->hst->rz_prev;  // (r, z) the last direction was built from
+            const double rec = std::sqrt(ctx->hst->rr);  // recursive residual, units of rn
+            rn_prev = rn;
+            hipLaunchKernelGGL((k_add_correction<float>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->x, ctx->x32, rn);
+            HIPCK(hipGetLastError());
+            outer++;
+            if ((rc = true_residual(ctx, &rn))) return rc;
+            if (stt < 0) {
+                status = stt;
+                break;
+            }
+            // a direction is kept only if the replaced residual is close to the recursive
+            // one; otherwise it is no longer conjugate to the new residual's error and the
+            // next run restarts from the preconditioned residual
+            keep = it > 0 && rn < 0.5 * rn_prev && rn <= 3. * rec * rn_prev;
         }
     } else {
         return fail(ctx, PF_EINVAL, "unknown precision");
@@ -1209,6 +1274,8 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes) {
     if (ctx->dim == 3) {
         if (rows < 2 || rows * ctx->n[0] > 1024 || planes < 1)
             return fail(ctx, PF_EINVAL, "3D: need rows >= 2, rows * n0 <= 1024, planes >= 1");
+        if (tile_lds(rows, (int)ctx->n[0]) > 65536)
+            return fail(ctx, PF_EINVAL, "3D: rows * n0 too large for the tile kernel's LDS (64 KiB)");
         int ytiles = (int)((ctx->n[1] + rows - 2) / (rows - 1));
         size_t need = 2 * (((size_t)ytiles * ((ctx->n[2] + planes - 1) / planes) + 7) / 8 * 8);
         if (need > ctx->npartials) return fail(ctx, PF_EINVAL, "too many blocks for the reduction buffer");
@@ -1236,8 +1303,10 @@ int pf_set_line_segment(pf_ctx* ctx, int seg) {
 int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes) {
     if (!ctx) return PF_EINVAL;
     if (kind != 0 && kind != 1) return fail(ctx, PF_EINVAL, "apply kind 0 (tile blocks) or 1 (row blocks)");
-    if (precision != PF_FP64 && precision != PF_FP32_REFINE && precision != -1)
+    if (precision != PF_FP64 && precision != PF_FP32_REFINE && precision != PF_FP32_RELIABLE && precision != -1)
         return fail(ctx, PF_EINVAL, "precision PF_FP64, PF_FP32_REFINE or -1 (both)");
+    if (kind == 0 && ctx->dim == 3 && tile_lds(ctx->BY[0], (int)ctx->n[0]) > 65536)
+        return fail(ctx, PF_EINVAL, "tile blocks: rows * n0 too large for the kernel's LDS (64 KiB)");
     if (kind == 1) {
         if (ctx->dim == 3 && ctx->n[0] > 1024) return fail(ctx, PF_EINVAL, "row blocks need n0 <= 1024");
         if (planes < 4) return fail(ctx, PF_EINVAL, "row blocks: planes >= 4");

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libplaskfem.so")
 PF_OK, PF_EINVAL, PF_ESTATE, PF_EDEVICE, PF_ESHAPE, PF_ENOCONV, PF_EBREAK, PF_ENONFINITE = 0, -1, -2, -3, -4, -5, -6, -8
 PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
 PF_THERMAL, PF_ELECTRICAL = 0, 1
-PF_FP64, PF_FP32_REFINE = 0, 1
+PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE = 0, 1, 2
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
 PRECOND_JACOBI, PRECOND_LINE = 1, 2
@@ -175,7 +175,7 @@ class FemContext:
         self._ck(self.L.pf_dirichlet_set(self.h, _p(self._dn), _p(self._dv), len(self._dn)), "pf_dirichlet_set")
 
     # -- compute -------------------------------------------------------------
-    def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, max_outer=50,
+    def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=None, max_outer=50,
               check_every=32, keep_on_device=False, raise_noconv=True, norm=PF_NORM_INIT, init_value=None,
               precond=PRECOND_LINE):
         """Solve; the relative residual is ||B - A X|| / ||B - A X_init|| (norm=PF_NORM_INIT,
@@ -183,7 +183,7 @@ class FemContext:
         if init_value is None:
             init_value = float(np.ravel(x0)[0]) if x0 is not None else 0.0
         o = SolveOpts(precision, precond, PF_X0_GIVEN if x0 is not None else PF_X0_ZERO, int(maxit), float(rtol),
-                      float(inner_rtol), int(max_outer), int(check_every), int(norm), float(init_value))
+                      float(inner_rtol or 0.0), int(max_outer), int(check_every), int(norm), float(init_value))
         st = SolveStats()
         if x0 is not None:
             x = _f64(np.array(x0, dtype=np.float64, copy=True))
@@ -194,21 +194,21 @@ class FemContext:
             self._ck(rc, "pf_solve")
         return x, st
 
-    def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
+    def solve_warm(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=None, check_every=32,
                    norm=PF_NORM_INIT, init_value=0.0, precond=PRECOND_LINE, checked=False):
         """Start from the solution already on the device (no host transfers); checked:
         fall back to the uniform init_value start if that has the smaller residual."""
         o = SolveOpts(precision, precond, PF_X0_DEVICE_CHECKED if checked else PF_X0_DEVICE, int(maxit), float(rtol),
-                      float(inner_rtol), 50, int(check_every), int(norm), float(init_value))
+                      float(inner_rtol or 0.0), 50, int(check_every), int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
         self._ck(rc, "pf_solve")
         return st
 
-    def solve_uniform(self, init_value, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=1e-4, check_every=32,
+    def solve_uniform(self, init_value, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=None, check_every=32,
                       norm=PF_NORM_INIT, max_outer=50, precond=PRECOND_LINE):
         """Start from init_value everywhere (device fill); the solution stays on the device."""
-        o = SolveOpts(precision, precond, PF_X0_UNIFORM, int(maxit), float(rtol), float(inner_rtol), int(max_outer),
+        o = SolveOpts(precision, precond, PF_X0_UNIFORM, int(maxit), float(rtol), float(inner_rtol or 0.0), int(max_outer),
                       int(check_every), int(norm), float(init_value))
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))

@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from plask_amd.fem import PF_FP32_REFINE, PF_FP64, PF_THERMAL, PF_ELECTRICAL, PRECOND_JACOBI, PRECOND_LINE, FemContext
+from plask_amd.fem import (PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PF_THERMAL, PF_ELECTRICAL, PRECOND_JACOBI,
+                           PRECOND_LINE, FemContext)
 
 pytestmark = pytest.mark.gpu
 
@@ -87,7 +88,7 @@ def test_rhs_matches_oracle(shape):
 
 
 @pytest.mark.parametrize("shape,cyl", [((9, 7, 11), False), ((24, 20, 30), False), ((40, 60), True), ((31, 45), False)])
-@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE])
+@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE])
 @pytest.mark.parametrize("precond", [PRECOND_LINE, PRECOND_JACOBI])
 def test_solve_matches_band_cholesky(shape, cyl, precision, precond):
     g, axes, kh, kv, q, dn, dv = _rand_problem(shape, cyl=cyl, seed=3)
@@ -120,7 +121,8 @@ def test_line_schedules_same_preconditioner(shape, cyl, seg):
     c.set_line_segment(-1)
     T_seq, st_seq = c.solve(x0=np.full(g.nnodes, 300.0), precision=PF_FP64, rtol=1e-10, precond=PRECOND_LINE)
     c.set_line_segment(seg)
-    for precision in (PF_FP64, PF_FP32_REFINE):
+    for precision in (PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE):
+        print("  precision", precision, flush=True)
         T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-10, precond=PRECOND_LINE)
         assert st.rel_resid <= 1e-10
         rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
