@@ -86,9 +86,11 @@ typedef struct pf_ctx pf_ctx;
 #define PF_NORM_B_FULL 2 /* ||B|| including the Dirichlet rows (B exactly as assembled by the reference) */
 
 typedef struct {
-    int precision;    /* PF_FP64 | PF_FP32_REFINE */
+    int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE */
     int precond;      /* 1 = Jacobi (NSPCG 'jac'); 2 = line block-Jacobi along the last (layer) axis
-                         (the tridiagonal part of A per column, cf. NSPCG 'ljac'); 0 = default (2) */
+                         (the tridiagonal part of A per column, cf. NSPCG 'ljac'); 3 = multigrid
+                         V-cycle (3D only: in-plane semicoarsening, damped line smoother; no
+                         reference counterpart, see pf_set_multigrid); 0 = default (2) */
     int x0;           /* PF_X0_* */
     int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
     double rtol;      /* stop when ||B - A X||_2 <= rtol * (denominator chosen by `norm`) */
@@ -150,6 +152,11 @@ int pf_apply(pf_ctx* ctx, int precision, const void* x_dev, void* y_dev);
 /* Host-buffer convenience: y = A x (FP64). */
 int pf_apply_host(pf_ctx* ctx, const double* x, double* y);
 
+/* Host-buffer test hook: z = M^-1 r for one preconditioner (1 Jacobi, 2 line, 3 one
+ * multigrid V-cycle) in the given precision (r rounded to FP32 for the FP32 forms).
+ * Overwrites the device residual and direction vectors of the last solve. */
+int pf_precond_apply_host(pf_ctx* ctx, int precision, int precond, const double* r_in, double* z_out);
+
 /* The eliminated load vector B (host, N values). */
 int pf_rhs_host(pf_ctx* ctx, double* b_out);
 
@@ -178,10 +185,26 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes);
  * one-thread-per-line kernel.  No counterpart in the reference (a tuning hook). */
 int pf_set_line_segment(pf_ctx* ctx, int seg);
 
+/* Multigrid preconditioner (precond = 3, 3D): levels coarsen the two in-plane axes by
+ * two (an axis stops at 3 nodes), keep every plane of the layer axis, and use the same
+ * Q1 operator with area-weighted mean conductivities; the smoother is the exact line
+ * solve along the layer axis damped by omega (0 < omega <= 0.66, default 0.65: the line
+ * preconditioned operator has eigenvalues <= 3, so the smoother converges), one sweep
+ * before and after each coarse correction and coarse_sweeps (1-64, default 2) on the
+ * coarsest level; max_levels limits the level count (0 = no limit).  Needs the
+ * partitioned line solve (pf_set_line_segment >= 0).  The levels are built on the first
+ * multigrid solve after a grid change.  No counterpart in the reference (NSPCG offers
+ * no multigrid); a solver option. */
+int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omega);
+
+/* Number of multigrid levels built (the solve grid included; 0 before the first
+ * multigrid solve). */
+int pf_multigrid_levels(const pf_ctx* ctx);
+
 /* 3D operator kernel for one precision (PF_FP64, PF_FP32_REFINE, or -1 for both):
- * kind 0 = tile blocks (shape from pf_set_launch; default for FP64), kind 1 = row
- * blocks, one block per node row and chunk of `planes` >= 4 planes along axis 2
- * (default for FP32, 16 planes).  A tuning hook, no reference counterpart. */
+ * kind 0 = tile blocks (shape from pf_set_launch; the default), kind 1 = row blocks,
+ * one block per node row and chunk of `planes` >= 4 planes along axis 2 (the default
+ * when axis 0 is too long for a tile block).  A tuning hook, no reference counterpart. */
 int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes);
 
 /* Tile-block apply: xcd = 1 (default) gives each of the 8 XCDs a contiguous range of

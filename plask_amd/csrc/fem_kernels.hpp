@@ -30,7 +30,7 @@
 
 namespace pf {
 
-enum { MODE_OP = 0, MODE_RES = 1 };
+enum { MODE_OP = 0, MODE_RES = 1, MODE_RESG = 2 };  // MODE_RESG: MODE_RES, skipped after the stop
 
 // Scalars of one PCG solve, resident on the device.  Every kernel of the
 // iteration reads them; the last block of a reduction kernel writes them.
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     const int PX = BX + 1;  // padded row: column BX holds zeros
     T* xs = reinterpret_cast<T*>(smem_raw);       // [2][BY+1][PX]
     T* oc = xs + 2 * (BY + 1) * PX;               // [2][4][BY][PX], corner outputs at column tx+1
-    if (FUSEP || DOT) {
+    if (FUSEP || DOT || MODE == MODE_RESG) {
         if (st->done) return;
     }
     // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a contiguous
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(512) void k_apply3r(Axes3<T> g, int KR, int nb, con
     const int PX = n0 + 1;
     T* xs = reinterpret_cast<T*>(smem_raw);  // [3][PX]: rows j-1, j, j+1 of the incoming plane
     T* oc = xs + 3 * PX;                     // [PX]: corner contributions for node i + 1
-    if (FUSEP || DOT) {
+    if (FUSEP || DOT || MODE == MODE_RESG) {
         if (st->done) return;
     }
     const int q8 = gridDim.x >> 3;
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
     const int PX = n0 + 1;
     T* xs = reinterpret_cast<T*>(smem_raw);  // [2][PX]
     T* oc = xs + 2 * PX;                     // [2][PX], corner outputs at column tx+1
-    if (FUSEP || DOT) {
+    if (FUSEP || DOT || MODE == MODE_RESG) {
         if (st->done) return;
     }
     const int tx = threadIdx.x;
@@ -1211,71 +1211,57 @@ struct PartLB {
     static constexpr int value = (M * (int)sizeof(T) <= 64) ? 1024 : 512;
 };
 
-// One CG update with the partitioned line solve.  Thread (cl, seg) of a block
-// owns planes [seg m, seg m + m) of column blockIdx.x * CB + cl (lanes run over
-// columns, so every load is a contiguous row segment).  Steps: r -= alpha q
-// (skipped for START); interior solve y = T_j^-1 r; the separator right-hand
-// sides g = r_s - T_sI y go to LDS; one thread per column solves S z_s = g; each
-// interior is solved again with the separator values moved to its right-hand
-// side.  (r, z) and (r, r) are reduced over the grid.
-template <typename T, int M, bool START>
-__global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
-    int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q,
-    const T* __restrict__ la, const T* __restrict__ lw, const T* __restrict__ lcp, const T* __restrict__ schur,
-    T* __restrict__ z, PcgState* __restrict__ st, double* __restrict__ partials) {
-    if (!START && st->done) return;
-    extern __shared__ __align__(16) unsigned char part_smem[];
+// Placement of one thread in a partitioned line block: column blockIdx.x * CB + cl,
+// planes [seg m, seg m + m) (lanes run over columns, so every load is a contiguous
+// row segment).  The last plane of every segment but the final one is a separator.
+struct PartPos {
+    int t, cl, seg, col, len, lint;
+    bool act, sep;
+    long long base;
+};
+
+__device__ inline PartPos part_pos(int ncol, int n, int P, int CB, long long plane, int m) {
+    PartPos q;
+    q.t = threadIdx.x;
+    q.cl = q.t % CB;
+    q.seg = q.t / CB;
+    q.col = blockIdx.x * CB + q.cl;
+    q.act = q.seg < P && q.col < ncol;
+    const int k0 = q.seg * m;
+    q.len = q.act ? min(m, n - k0) : 0;
+    q.sep = q.act && q.seg < P - 1;
+    q.lint = q.sep ? q.len - 1 : q.len;
+    q.base = (long long)k0 * plane + q.col;
+    return q;
+}
+
+// z = T^-1 r for the thread's segment (values in registers): interior solve
+// y = T_j^-1 r; the separator right-hand sides g = r_s - T_sI y go to LDS; one thread
+// per column solves S z_s = g; each interior is solved again with the separator
+// values moved to its right-hand side.  Every thread of the block calls it (two
+// barriers); lds holds 6 * blockDim.x values.
+template <typename T, int M>
+__device__ inline void part_solve(const PartPos& q, int P, int CB, int ncol, const T* __restrict__ schur, T* lds,
+                                  const T (&rv)[M], const T (&av)[M], const T (&wv)[M], const T (&cv)[M], T (&zv)[M]) {
     const int S = blockDim.x;
-    T* s_yf = reinterpret_cast<T*>(part_smem);
+    T* s_yf = lds;
     T* s_g = s_yf + S;
     T* s_eh = s_g + S;
     T* s_sa = s_eh + S;
     T* s_sw = s_sa + S;
     T* s_sc = s_sw + S;
-    const int t = threadIdx.x;
-    const int cl = t % CB, seg = t / CB;
-    const int col = blockIdx.x * CB + cl;
-    const bool act = seg < P && col < ncol;
-    const int k0 = seg * M;
-    const int len = act ? min(M, n - k0) : 0;
-    const bool sep = act && seg < P - 1;
-    const int lint = sep ? len - 1 : len;
-    const long long base = (long long)k0 * plane + col;
-    T rv[M], av[M], wv[M], cv[M], qv[M];
-#pragma unroll
-    for (int u = 0; u < M; ++u) {
-        if (u < len) {
-            const long long node = base + (long long)u * plane;
-            rv[u] = r[node];
-            av[u] = la[node];
-            wv[u] = lw[node];
-            cv[u] = lcp[node];
-            if (!START) qv[u] = q[node];
-        }
-    }
-    if (sep) {
-        s_sa[t] = schur[(size_t)seg * ncol + col];
-        s_sw[t] = schur[(size_t)(P - 1 + seg) * ncol + col];
-        s_sc[t] = schur[(size_t)(2 * (P - 1) + seg) * ncol + col];
-    }
-    const T alpha = START ? T(0) : T(st->alpha);
-    double rr = 0., rz = 0.;
-#pragma unroll
-    for (int u = 0; u < M; ++u) {
-        if (u < len) {
-            if (!START) {
-                rv[u] = rv[u] - alpha * qv[u];
-                r[base + (long long)u * plane] = rv[u];
-            }
-            rr += (double)rv[u] * (double)rv[u];
-        }
+    const int t = q.t, cl = q.cl, seg = q.seg;
+    if (q.sep) {
+        s_sa[t] = schur[(size_t)seg * ncol + q.col];
+        s_sw[t] = schur[(size_t)(P - 1 + seg) * ncol + q.col];
+        s_sc[t] = schur[(size_t)(2 * (P - 1) + seg) * ncol + q.col];
     }
     // interior solve with the separators at zero
     T d[M];
     T dp = T(0);
 #pragma unroll
     for (int u = 0; u < M; ++u) {
-        if (u < lint) {
+        if (u < q.lint) {
             const T y = rv[u] - av[u] * dp;
             d[u] = y * wv[u];
             dp = d[u];
@@ -1284,9 +1270,9 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
     T zn = T(0), yl = T(0);
 #pragma unroll
     for (int u = M - 1; u >= 0; --u) {
-        if (u < lint) {
+        if (u < q.lint) {
             const T zz = d[u] - cv[u] * zn;
-            if (u == lint - 1) yl = zz;
+            if (u == q.lint - 1) yl = zz;
             d[u] = zz;
             zn = zz;
         }
@@ -1294,16 +1280,16 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
     T rs = T(0), elo = T(0), ehi = T(0);
 #pragma unroll
     for (int u = 0; u < M; ++u)
-        if (sep && u == len - 1) {
+        if (q.sep && u == q.len - 1) {
             rs = rv[u];
             elo = av[u];
             ehi = cv[u];
         }
-    s_yf[t] = act ? d[0] : T(0);
-    s_g[t] = sep ? rs - elo * yl : T(0);
-    s_eh[t] = sep ? ehi : T(0);
+    s_yf[t] = q.act ? d[0] : T(0);
+    s_g[t] = q.sep ? rs - elo * yl : T(0);
+    s_eh[t] = q.sep ? ehi : T(0);
     __syncthreads();
-    if (seg == 0 && col < ncol) {
+    if (seg == 0 && q.col < ncol) {
         T dq = T(0);
         for (int j = 0; j < P - 1; ++j) {
             const int o = j * CB + cl;
@@ -1321,17 +1307,17 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
         }
     }
     __syncthreads();
-    const T lo = (act && seg > 0) ? s_g[t - CB] : T(0);
-    const T elp = (act && seg > 0) ? s_eh[t - CB] : T(0);
-    const T hi = sep ? s_g[t] : T(0);
+    const T lo = (q.act && seg > 0) ? s_g[t - CB] : T(0);
+    const T elp = (q.act && seg > 0) ? s_eh[t - CB] : T(0);
+    const T hi = q.sep ? s_g[t] : T(0);
     // interior solve with the separator values on the right-hand side
     dp = T(0);
 #pragma unroll
     for (int u = 0; u < M; ++u) {
-        if (u < lint) {
+        if (u < q.lint) {
             T rhs = rv[u];
             if (u == 0) rhs -= elp * lo;
-            if (u == lint - 1) rhs -= elo * hi;
+            if (u == q.lint - 1) rhs -= elo * hi;
             const T y = rhs - av[u] * dp;
             d[u] = y * wv[u];
             dp = d[u];
@@ -1340,24 +1326,119 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
     zn = T(0);
 #pragma unroll
     for (int u = M - 1; u >= 0; --u) {
-        if (u < lint) {
+        if (u < q.lint) {
             const T zz = d[u] - cv[u] * zn;
             d[u] = zz;
             zn = zz;
         }
     }
 #pragma unroll
+    for (int u = 0; u < M; ++u) zv[u] = u < q.lint ? d[u] : hi;
+}
+
+// One CG update with the partitioned line solve: r -= alpha q (skipped for START),
+// z = T^-1 r (part_solve), and (r, z), (r, r) reduced over the grid.
+template <typename T, int M, bool START>
+__global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
+    int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q,
+    const T* __restrict__ la, const T* __restrict__ lw, const T* __restrict__ lcp, const T* __restrict__ schur,
+    T* __restrict__ z, PcgState* __restrict__ st, double* __restrict__ partials) {
+    if (!START && st->done) return;
+    extern __shared__ __align__(16) unsigned char part_smem[];
+    const PartPos pp = part_pos(ncol, n, P, CB, plane, M);
+    T rv[M], av[M], wv[M], cv[M], qv[M];
+#pragma unroll
     for (int u = 0; u < M; ++u) {
-        if (u < len) {
-            const T zz = u < lint ? d[u] : hi;
-            z[base + (long long)u * plane] = zz;
-            rz += (double)rv[u] * (double)zz;
+        if (u < pp.len) {
+            const long long node = pp.base + (long long)u * plane;
+            rv[u] = r[node];
+            av[u] = la[node];
+            wv[u] = lw[node];
+            cv[u] = lcp[node];
+            if (!START) qv[u] = q[node];
+        }
+    }
+    const T alpha = START ? T(0) : T(st->alpha);
+    double rr = 0., rz = 0.;
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < pp.len) {
+            if (!START) {
+                rv[u] = rv[u] - alpha * qv[u];
+                r[pp.base + (long long)u * plane] = rv[u];
+            }
+            rr += (double)rv[u] * (double)rv[u];
+        }
+    }
+    T zv[M];
+    part_solve<T, M>(pp, P, CB, ncol, schur, reinterpret_cast<T*>(part_smem), rv, av, wv, cv, zv);
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < pp.len) {
+            z[pp.base + (long long)u * plane] = zv[u];
+            rz += (double)rv[u] * (double)zv[u];
         }
     }
     double v[2] = {rz, rr}, tot[2];
     if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
         if (START) finish_start(st, tot);
         else finish_update(st, tot);
+    }
+}
+
+// Damped line-Jacobi sweep of the multigrid smoother, same partitioned line solve:
+// xout = xin + omega T^-1 r (xin null: start from zero; xin may be xout).
+// UPD: the PCG residual update r -= alpha q is applied (and stored) first.
+// FIN 1 / 2: this sweep yields the preconditioned PCG residual z = xout, and
+// (rp, z), (rp, rp) of the PCG residual rp end the iteration (1) or start a run (2).
+// Exits at once after the stop (st->done).
+template <typename T, int M, bool UPD, int FIN>
+__global__ __launch_bounds__((PartLB<T, M>::value)) void k_mg_smooth(
+    int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q, const T* xin,
+    T* xout, const T* __restrict__ rp, T omega, const T* __restrict__ la, const T* __restrict__ lw,
+    const T* __restrict__ lcp, const T* __restrict__ schur, PcgState* __restrict__ st,
+    double* __restrict__ partials) {
+    if (st->done) return;
+    extern __shared__ __align__(16) unsigned char part_smem[];
+    const PartPos pp = part_pos(ncol, n, P, CB, plane, M);
+    T rv[M], av[M], wv[M], cv[M];
+    const T alpha = UPD ? T(st->alpha) : T(0);
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < pp.len) {
+            const long long node = pp.base + (long long)u * plane;
+            rv[u] = r[node];
+            av[u] = la[node];
+            wv[u] = lw[node];
+            cv[u] = lcp[node];
+            if (UPD) {
+                rv[u] = rv[u] - alpha * q[node];
+                r[node] = rv[u];
+            }
+        }
+    }
+    T zv[M];
+    part_solve<T, M>(pp, P, CB, ncol, schur, reinterpret_cast<T*>(part_smem), rv, av, wv, cv, zv);
+    double rz = 0., rr = 0.;
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        if (u < pp.len) {
+            const long long node = pp.base + (long long)u * plane;
+            const T xv = (xin ? xin[node] : T(0)) + omega * zv[u];
+            xout[node] = xv;
+            if (FIN) {
+                const double g = (double)(UPD ? rv[u] : rp[node]);
+                rz += g * (double)xv;
+                rr += g * g;
+            }
+        }
+    }
+    if (FIN) {
+        double v[2] = {rz, rr}, tot[2];
+        if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+            if (FIN == 2) finish_start(st, tot);
+            else finish_update(st, tot);
+        }
     }
 }
 

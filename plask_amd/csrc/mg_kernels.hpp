@@ -1,0 +1,145 @@
+// Grid-transfer and coarse-level set-up kernels of the semicoarsening multigrid
+// preconditioner (3D).  The coarse levels coarsen the two in-plane axes only (every
+// other node, plus the last one); the layer axis keeps all its planes, so every level
+// has the same columns structure and the exact line solve along the layer axis is
+// the smoother on every level.
+//
+// Transfer tables per in-plane axis, from a finer level to the next coarser one:
+//   cidx[I]          fine index of coarse node I (increasing, first 0, last n_f - 1)
+//   tm[i], tw[i]     fine node i lies in coarse interval [tm[i], tm[i] + 1] at relative
+//                    position tw[i] in [0, 1) (linear interpolation weight of the right
+//                    node; 0 for nodes that are coarse nodes)
+// Prolongation P = I_z (x) P_y (x) P_x is the Q1 interpolation of the coarse field at the
+// fine nodes; restriction is its transpose; both skip Dirichlet nodes (the coarse
+// correction is zero there).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fem_kernels.hpp"
+
+namespace pf {
+
+struct MgTransfer {
+    int n0f, n1f, n0c, n1c, n2;
+    const int* cx;  // [n0c]
+    const int* cy;  // [n1c]
+    const int* mx;  // [n0f]
+    const int* my;  // [n1f]
+};
+
+// weight of fine node i (left coarse index m, position w) in coarse node I
+template <typename T>
+__device__ __forceinline__ T tweight(int m, T w, int I) {
+    return m == I ? T(1) - w : (m + 1 == I ? w : T(0));
+}
+
+// b_c = R r_f (transpose of the interpolation), zero at coarse Dirichlet nodes
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __restrict__ wx,
+                                                     const T* __restrict__ wy, const T* __restrict__ rf,
+                                                     T* __restrict__ bc, const uint64_t* __restrict__ dmask_c,
+                                                     const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const long long planec = (long long)tr.n0c * tr.n1c, planef = (long long)tr.n0f * tr.n1f;
+    const long long Nc = planec * tr.n2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        if (dbit(dmask_c, idx)) {
+            bc[idx] = T(0);
+            continue;
+        }
+        const int I = (int)(idx % tr.n0c);
+        const int J = (int)((idx / tr.n0c) % tr.n1c);
+        const long long k = idx / planec;
+        const int ilo = I > 0 ? tr.cx[I - 1] + 1 : 0, ihi = I < tr.n0c - 1 ? tr.cx[I + 1] - 1 : tr.cx[I];
+        const int jlo = J > 0 ? tr.cy[J - 1] + 1 : 0, jhi = J < tr.n1c - 1 ? tr.cy[J + 1] - 1 : tr.cy[J];
+        const T* src = rf + k * planef;
+        T s = T(0);
+        for (int j = jlo; j <= jhi; ++j) {
+            const T wj = tweight(tr.my[j], wy[j], J);
+            T sx = T(0);
+            for (int i = ilo; i <= ihi; ++i) sx += tweight(tr.mx[i], wx[i], I) * src[(long long)j * tr.n0f + i];
+            s += wj * sx;
+        }
+        bc[idx] = s;
+    }
+}
+
+// x_f += P x_c at the free fine nodes
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong_add(MgTransfer tr, const T* __restrict__ wx,
+                                                        const T* __restrict__ wy, const T* __restrict__ xc,
+                                                        T* __restrict__ xf, const uint64_t* __restrict__ dmask_f,
+                                                        const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const long long planec = (long long)tr.n0c * tr.n1c, planef = (long long)tr.n0f * tr.n1f;
+    const long long Nf = planef * tr.n2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nf;
+         idx += (long long)gridDim.x * blockDim.x) {
+        if (dbit(dmask_f, idx)) continue;
+        const int i = (int)(idx % tr.n0f);
+        const int j = (int)((idx / tr.n0f) % tr.n1f);
+        const long long k = idx / planef;
+        const int I = tr.mx[i], J = tr.my[j];
+        const int I1 = min(I + 1, tr.n0c - 1), J1 = min(J + 1, tr.n1c - 1);
+        const T a = wx[i], b = wy[j];
+        const T* c = xc + k * planec;
+        const T lo = (T(1) - a) * c[(long long)J * tr.n0c + I] + a * c[(long long)J * tr.n0c + I1];
+        const T hi = (T(1) - a) * c[(long long)J1 * tr.n0c + I] + a * c[(long long)J1 * tr.n0c + I1];
+        xf[idx] += (T(1) - b) * lo + b * hi;
+    }
+}
+
+// Coarse element conductivities: coarse element (I, J, k) covers the fine elements
+// [cx[I], cx[I+1]) x [cy[J], cy[J+1]) of element layer k; area-weighted means.
+__global__ __launch_bounds__(256) void k_mg_coarsen_k(MgTransfer tr, const double* __restrict__ hxf,
+                                                      const double* __restrict__ hyf, const double* __restrict__ khf,
+                                                      const double* __restrict__ kvf, double* __restrict__ khc,
+                                                      double* __restrict__ kvc) {
+    const int e0c = tr.n0c - 1, e1c = tr.n1c - 1, e0f = tr.n0f - 1, e1f = tr.n1f - 1;
+    const long long Ec = (long long)e0c * e1c * (tr.n2 - 1);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < Ec; e += (long long)gridDim.x * blockDim.x) {
+        const int I = (int)(e % e0c);
+        const int J = (int)((e / e0c) % e1c);
+        const long long k = e / ((long long)e0c * e1c);
+        const double* kh = khf + k * e0f * e1f;
+        const double* kv = kvf + k * e0f * e1f;
+        double sa = 0., sh = 0., sv = 0.;
+        for (int ey = tr.cy[J]; ey < tr.cy[J + 1]; ++ey)
+            for (int ex = tr.cx[I]; ex < tr.cx[I + 1]; ++ex) {
+                const double a = hxf[ex] * hyf[ey];
+                const long long f = (long long)ey * e0f + ex;
+                sa += a;
+                sh += a * kh[f];
+                sv += a * kv[f];
+            }
+        khc[e] = sh / sa;
+        kvc[e] = sv / sa;
+    }
+}
+
+// Coarse Dirichlet mask: a coarse node is fixed when the fine node at its position is.
+__global__ __launch_bounds__(256) void k_mg_coarsen_mask(MgTransfer tr, const uint64_t* __restrict__ dmf,
+                                                         uint64_t* __restrict__ dmc) {
+    const long long planec = (long long)tr.n0c * tr.n1c, planef = (long long)tr.n0f * tr.n1f;
+    const long long Nc = planec * tr.n2;
+    const long long words = (Nc + 63) / 64;
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (long long)gridDim.x * blockDim.x) {
+        uint64_t bits = 0;
+        for (int b = 0; b < 64; ++b) {
+            const long long idx = w * 64 + b;
+            if (idx >= Nc) break;
+            const int I = (int)(idx % tr.n0c);
+            const int J = (int)((idx / tr.n0c) % tr.n1c);
+            const long long k = idx / planec;
+            const long long f = k * planef + (long long)tr.cy[J] * tr.n0f + tr.cx[I];
+            if (dbit(dmf, f)) bits |= (uint64_t)1 << b;
+        }
+        dmc[w] = bits;
+    }
+}
+
+}  // namespace pf

@@ -12,11 +12,13 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/plask_fem.h"
 #include "fem_kernels.hpp"
 #include "batch_kernels.hpp"
+#include "mg_kernels.hpp"
 
 using namespace pf;
 
@@ -28,6 +30,30 @@ struct DevBuf {
 };
 
 }  // namespace
+
+// One coarse level of the semicoarsening multigrid preconditioner (3D): the in-plane
+// axes coarsened, all planes of the layer axis kept.  Device arrays; the level above
+// is the next finer one (level 0 is the solve grid itself).
+struct MgLevel {
+    int n0 = 0, n1 = 0;
+    long long N = 0, E = 0;
+    double *hx = nullptr, *hy = nullptr;
+    float *hx32 = nullptr, *hy32 = nullptr;
+    double *kh = nullptr, *kv = nullptr;
+    float *kh32 = nullptr, *kv32 = nullptr;
+    uint64_t* dmask = nullptr;
+    // transfer from the finer level (tables described in mg_kernels.hpp)
+    int *cx = nullptr, *cy = nullptr, *mx = nullptr, *my = nullptr;
+    double *wx = nullptr, *wy = nullptr;
+    float *wx32 = nullptr, *wy32 = nullptr;
+    // partitioned line factors (same partition lengths as the solve grid)
+    double *la = nullptr, *lw = nullptr, *lcp = nullptr, *schur = nullptr;
+    float *la32 = nullptr, *lw32 = nullptr, *lcp32 = nullptr, *schur32 = nullptr;
+    int schur_m64 = -1, schur_m32 = -1;
+    // right-hand side, iterate and residual of the level
+    double *b = nullptr, *x = nullptr, *r = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+};
 
 struct pf_ctx {
     int device = 0;
@@ -82,6 +108,14 @@ struct pf_ctx {
     double* schur = nullptr;  // Thomas factors of the separator systems
     float* schur32 = nullptr;
     int upd_blocks = 1024;
+    // multigrid preconditioner (3D): coarse levels, smoother residual of level 0, tuning
+    std::vector<MgLevel> mg;
+    bool mg_built = false;
+    double *mg_r0 = nullptr;
+    float *mg_r0_32 = nullptr;
+    int mg_max_levels = 0;      // 0: coarsen until both in-plane axes have <= 3 nodes
+    int mg_coarse_sweeps = 2;   // smoothing sweeps on the coarsest level
+    double mg_omega = 0.65;     // damping of the line-Jacobi smoother
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
     int graph_chunk = 0, graph_precond = 0;
@@ -139,6 +173,32 @@ static void drop_graphs(pf_ctx* ctx) {
     if (ctx->graph64) hipGraphExecDestroy(ctx->graph64);
     if (ctx->graph32) hipGraphExecDestroy(ctx->graph32);
     ctx->graph64 = ctx->graph32 = nullptr;
+}
+
+static void dfree(pf_ctx* ctx, void** ptr) {
+    if (!*ptr) return;
+    hipFree(*ptr);
+    auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), *ptr);
+    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    *ptr = nullptr;
+}
+
+// release the multigrid levels (grid change)
+static void mg_free(pf_ctx* ctx) {
+    for (MgLevel& L : ctx->mg) {
+        void** ptrs[] = {(void**)&L.hx,    (void**)&L.hy,    (void**)&L.hx32,    (void**)&L.hy32,  (void**)&L.kh,
+                         (void**)&L.kv,    (void**)&L.kh32,  (void**)&L.kv32,    (void**)&L.dmask, (void**)&L.cx,
+                         (void**)&L.cy,    (void**)&L.mx,    (void**)&L.my,      (void**)&L.wx,    (void**)&L.wy,
+                         (void**)&L.wx32,  (void**)&L.wy32,  (void**)&L.la,      (void**)&L.lw,    (void**)&L.lcp,
+                         (void**)&L.schur, (void**)&L.la32,  (void**)&L.lw32,    (void**)&L.lcp32, (void**)&L.schur32,
+                         (void**)&L.b,     (void**)&L.x,     (void**)&L.r,       (void**)&L.b32,   (void**)&L.x32,
+                         (void**)&L.r32};
+        for (void** q : ptrs) dfree(ctx, q);
+    }
+    ctx->mg.clear();
+    dfree(ctx, (void**)&ctx->mg_r0);
+    dfree(ctx, (void**)&ctx->mg_r0_32);
+    ctx->mg_built = false;
 }
 
 // Launch shape of the partitioned line kernel: P segments of m planes per column,
@@ -275,6 +335,7 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     }
     if (geom == PF_GEOM_CYLINDRICAL && ax0[0] < 0.) return fail(ctx, PF_EINVAL, "radius must be non-negative");
     drop_graphs(ctx);
+    mg_free(ctx);
     ctx->dim = dim;
     ctx->geom = geom;
     for (int a = 0; a < 3; ++a) {
@@ -521,6 +582,8 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
     }
 }
 
+static int mg_refresh(pf_ctx* ctx);
+
 // F, Dinv (FP64/FP32) and ||B||: B = M (F - A0 xD) + xD, xD = Dirichlet values
 static int prepare(pf_ctx* ctx) {
     if (!ctx->dirty) return PF_OK;
@@ -584,6 +647,7 @@ static int prepare(pf_ctx* ctx) {
     HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
     HIPCK(hipStreamSynchronize(ctx->stream));
     ctx->bnorm = std::sqrt(ctx->hst->bb);  // free rows
+    if (int rc = mg_refresh(ctx)) return rc;
     ctx->dirty = false;
     return PF_OK;
 }
@@ -605,6 +669,330 @@ static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long 
 #undef PF_PART
 }
 
+// ---------------------------------------------------------------------------
+// Semicoarsening multigrid preconditioner (3D).  Level 0 is the solve grid; each
+// coarser level keeps every other node of the two in-plane axes (and the last) and
+// every plane of the layer axis.  Coarse operators are the same Q1 operator on the
+// coarse grid with area-weighted mean conductivities; the smoother is the damped
+// exact line solve along the layer axis (the line preconditioner of the single-level
+// PCG), one sweep before and one after the coarse correction, and
+// mg_coarse_sweeps sweeps on the coarsest level.  Restriction is the transpose of the
+// Q1 interpolation, so the V-cycle is a symmetric positive definite preconditioner
+// when omega * lambda_max(M_line^-1 A) < 2.  Element by element, the column-block part
+// of a Q1 box matrix with diagonal conductivity is kx'/3 + ky'/3 (x) M_z + kz'/9 S_z and
+// the ratio of the two matrices' eigenvalues is at most 3 (reached for kx' >> ky', kz'),
+// so lambda_max <= 3 for any element shapes: omega <= 0.66 is accepted, 0.65 default.
+// ---------------------------------------------------------------------------
+
+static MgTransfer mg_transfer(pf_ctx* ctx, size_t l) {
+    const MgLevel& L = ctx->mg[l];
+    MgTransfer tr;
+    tr.n0f = l ? ctx->mg[l - 1].n0 : (int)ctx->n[0];
+    tr.n1f = l ? ctx->mg[l - 1].n1 : (int)ctx->n[1];
+    tr.n0c = L.n0;
+    tr.n1c = L.n1;
+    tr.n2 = (int)ctx->n[2];
+    tr.cx = L.cx;
+    tr.cy = L.cy;
+    tr.mx = L.mx;
+    tr.my = L.my;
+    return tr;
+}
+
+// coarse element data, Dirichlet masks and line factors of every level (after a
+// change of the coefficients or of the Dirichlet set)
+static int mg_refresh(pf_ctx* ctx) {
+    if (!ctx->mg_built) return PF_OK;
+    if (!ctx->pm64 || !ctx->pm32) {  // sequential line solve selected: levels dropped
+        mg_free(ctx);
+        return PF_OK;
+    }
+    const int n2 = (int)ctx->n[2];
+    int rc;
+    for (size_t l = 0; l < ctx->mg.size(); ++l) {
+        MgLevel& L = ctx->mg[l];
+        const MgLevel* F = l ? &ctx->mg[l - 1] : nullptr;
+        const MgTransfer tr = mg_transfer(ctx, l);
+        hipLaunchKernelGGL(k_mg_coarsen_k, dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, tr,
+                           F ? F->hx : ctx->hx, F ? F->hy : ctx->hy, F ? F->kh : ctx->kh, F ? F->kv : ctx->kv, L.kh,
+                           L.kv);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, L.E,
+                           L.kh, L.kh32);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, L.E,
+                           L.kv, L.kv32);
+        hipLaunchKernelGGL(k_mg_coarsen_mask, dim3(grid1((L.N + 63) / 64, 256, 4096)), dim3(256), 0, ctx->stream, tr,
+                           F ? F->dmask : ctx->dmask, L.dmask);
+        const int ncol = L.n0 * L.n1;
+        const PartShape s64 = part_shape<double>(ctx->pm64, n2, ncol), s32 = part_shape<float>(ctx->pm32, n2, ncol);
+        if (!s64.m || !s32.m) return fail(ctx, PF_EINVAL, "multigrid: line partition does not fit a coarse level");
+        if (L.schur_m64 != s64.m) {
+            if ((rc = dalloc(ctx, &L.schur, 3 * (size_t)std::max(s64.P - 1, 1) * ncol))) return rc;
+            L.schur_m64 = s64.m;
+        }
+        if (L.schur_m32 != s32.m) {
+            if ((rc = dalloc(ctx, &L.schur32, 3 * (size_t)std::max(s32.P - 1, 1) * ncol))) return rc;
+            L.schur_m32 = s32.m;
+        }
+        const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz};
+        const int cb = grid1(ncol, 64, 1 << 20);
+        hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                           s64.m, L.la, L.lw, L.lcp, L.schur);
+        hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                           s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+    }
+    HIPCK(hipGetLastError());
+    return PF_OK;
+}
+
+template <typename T> static int upload(pf_ctx* ctx, T** dst, const std::vector<T>& v) {
+    int rc;
+    if ((rc = dalloc(ctx, dst, v.size()))) return rc;
+    HIPCK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+// coarse index list, per-fine-node interval and weight, coarse coordinates of one axis
+static void mg_coarsen_axis(const std::vector<double>& f, bool on, std::vector<int>& ci, std::vector<int>& m,
+                            std::vector<double>& w, std::vector<double>& c) {
+    const int n = (int)f.size();
+    ci.clear();
+    for (int i = 0; i < n; i += on ? 2 : 1) ci.push_back(i);
+    if (ci.back() != n - 1) ci.push_back(n - 1);
+    m.assign(n, 0);
+    w.assign(n, 0.);
+    for (size_t I = 0; I + 1 < ci.size(); ++I)
+        for (int i = ci[I]; i < ci[I + 1]; ++i) {
+            m[i] = (int)I;
+            w[i] = (f[i] - f[ci[I]]) / (f[ci[I + 1]] - f[ci[I]]);
+        }
+    m[n - 1] = (int)ci.size() - 1;
+    w[n - 1] = 0.;
+    c.clear();
+    for (int i : ci) c.push_back(f[i]);
+}
+
+// levels of the current grid: an in-plane axis is coarsened while it has more than 3
+// nodes; at most mg_max_levels levels in all (0: no limit)
+static int mg_build(pf_ctx* ctx) {
+    if (ctx->dim != 3) return fail(ctx, PF_EINVAL, "the multigrid preconditioner is 3D only");
+    if (!ctx->pm64 || !ctx->pm32)
+        return fail(ctx, PF_EINVAL, "the multigrid preconditioner needs the partitioned line solve");
+    mg_free(ctx);
+    int rc;
+    const int n2 = (int)ctx->n[2];
+    std::vector<double> fx = ctx->ax[0], fy = ctx->ax[1];
+    while (ctx->mg_max_levels == 0 || (int)ctx->mg.size() + 1 < ctx->mg_max_levels) {
+        const bool cxa = fx.size() > 3, cya = fy.size() > 3;
+        if (!cxa && !cya) break;
+        std::vector<int> cix, ciy, mx, my;
+        std::vector<double> wx, wy, cxv, cyv;
+        mg_coarsen_axis(fx, cxa, cix, mx, wx, cxv);
+        mg_coarsen_axis(fy, cya, ciy, my, wy, cyv);
+        ctx->mg.emplace_back();
+        MgLevel& L = ctx->mg.back();
+        L.n0 = (int)cxv.size();
+        L.n1 = (int)cyv.size();
+        L.N = (long long)L.n0 * L.n1 * n2;
+        L.E = (long long)(L.n0 - 1) * (L.n1 - 1) * (n2 - 1);
+        std::vector<double> hx(L.n0 - 1), hy(L.n1 - 1);
+        for (int i = 0; i + 1 < L.n0; ++i) hx[i] = cxv[i + 1] - cxv[i];
+        for (int j = 0; j + 1 < L.n1; ++j) hy[j] = cyv[j + 1] - cyv[j];
+        if ((rc = upload(ctx, &L.cx, cix)) || (rc = upload(ctx, &L.cy, ciy)) || (rc = upload(ctx, &L.mx, mx)) ||
+            (rc = upload(ctx, &L.my, my)) || (rc = upload(ctx, &L.wx, wx)) || (rc = upload(ctx, &L.wy, wy)) ||
+            (rc = upload(ctx, &L.wx32, std::vector<float>(wx.begin(), wx.end()))) ||
+            (rc = upload(ctx, &L.wy32, std::vector<float>(wy.begin(), wy.end()))) || (rc = upload(ctx, &L.hx, hx)) ||
+            (rc = upload(ctx, &L.hy, hy)) || (rc = upload(ctx, &L.hx32, std::vector<float>(hx.begin(), hx.end()))) ||
+            (rc = upload(ctx, &L.hy32, std::vector<float>(hy.begin(), hy.end()))))
+            return rc;
+        if ((rc = dalloc(ctx, &L.kh, L.E)) || (rc = dalloc(ctx, &L.kv, L.E)) || (rc = dalloc(ctx, &L.kh32, L.E)) ||
+            (rc = dalloc(ctx, &L.kv32, L.E)) || (rc = dalloc(ctx, &L.dmask, (L.N + 63) / 64)) ||
+            (rc = dalloc(ctx, &L.la, L.N)) || (rc = dalloc(ctx, &L.lw, L.N)) || (rc = dalloc(ctx, &L.lcp, L.N)) ||
+            (rc = dalloc(ctx, &L.la32, L.N)) || (rc = dalloc(ctx, &L.lw32, L.N)) ||
+            (rc = dalloc(ctx, &L.lcp32, L.N)) || (rc = dalloc(ctx, &L.b, L.N)) || (rc = dalloc(ctx, &L.x, L.N)) ||
+            (rc = dalloc(ctx, &L.r, L.N)) || (rc = dalloc(ctx, &L.b32, L.N)) || (rc = dalloc(ctx, &L.x32, L.N)) ||
+            (rc = dalloc(ctx, &L.r32, L.N)))
+            return rc;
+        fx = cxv;
+        fy = cyv;
+    }
+    if ((rc = dalloc(ctx, &ctx->mg_r0, ctx->N)) || (rc = dalloc(ctx, &ctx->mg_r0_32, ctx->N))) return rc;
+    ctx->mg_built = true;
+    return mg_refresh(ctx);
+}
+
+// one level's operands in precision T (level 0: the solve grid, b = the PCG residual,
+// x = the preconditioned residual z)
+template <typename T>
+struct MgView {
+    Axes3<T> g;
+    const T *kh, *kv, *la, *lw, *lcp, *schur, *wx, *wy;
+    const uint64_t* dmask;
+    T *b, *x, *r;
+    int pm;
+};
+
+template <typename T> static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0);
+template <> MgView<double> mg_view<double>(pf_ctx* ctx, int l, double* r0, double* z0) {
+    MgView<double> v;
+    v.pm = ctx->pm64;
+    if (l == 0) {
+        v.g = axes3<double>(ctx);
+        v.kh = ctx->kh;
+        v.kv = ctx->kv;
+        v.la = ctx->la;
+        v.lw = ctx->lw;
+        v.lcp = ctx->lcp;
+        v.schur = ctx->schur;
+        v.wx = v.wy = nullptr;
+        v.dmask = ctx->dmask;
+        v.b = r0;
+        v.x = z0;
+        v.r = ctx->mg_r0;
+        return v;
+    }
+    const MgLevel& L = ctx->mg[l - 1];
+    v.g = Axes3<double>{L.n0, L.n1, (int)ctx->n[2], L.hx, L.hy, ctx->hz};
+    v.kh = L.kh;
+    v.kv = L.kv;
+    v.la = L.la;
+    v.lw = L.lw;
+    v.lcp = L.lcp;
+    v.schur = L.schur;
+    v.wx = L.wx;
+    v.wy = L.wy;
+    v.dmask = L.dmask;
+    v.b = L.b;
+    v.x = L.x;
+    v.r = L.r;
+    return v;
+}
+template <> MgView<float> mg_view<float>(pf_ctx* ctx, int l, float* r0, float* z0) {
+    MgView<float> v;
+    v.pm = ctx->pm32;
+    if (l == 0) {
+        v.g = axes3<float>(ctx);
+        v.kh = ctx->kh32;
+        v.kv = ctx->kv32;
+        v.la = ctx->la32;
+        v.lw = ctx->lw32;
+        v.lcp = ctx->lcp32;
+        v.schur = ctx->schur32;
+        v.wx = v.wy = nullptr;
+        v.dmask = ctx->dmask;
+        v.b = r0;
+        v.x = z0;
+        v.r = ctx->mg_r0_32;
+        return v;
+    }
+    const MgLevel& L = ctx->mg[l - 1];
+    v.g = Axes3<float>{L.n0, L.n1, (int)ctx->n[2], L.hx32, L.hy32, ctx->hz32};
+    v.kh = L.kh32;
+    v.kv = L.kv32;
+    v.la = L.la32;
+    v.lw = L.lw32;
+    v.lcp = L.lcp32;
+    v.schur = L.schur32;
+    v.wx = L.wx32;
+    v.wy = L.wy32;
+    v.dmask = L.dmask;
+    v.b = L.b32;
+    v.x = L.x32;
+    v.r = L.r32;
+    return v;
+}
+
+// r = b - A x on level l (skipped after the stop)
+template <typename T>
+static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
+    if (l == 0) {
+        launch_apply<T, MODE_RESG, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv);
+        return;
+    }
+    const int n0 = v.g.n0, n1 = v.g.n1, n2 = v.g.n2;
+    int BY = std::max(2, std::min(16, 512 / n0));
+    while (BY > 2 && (BY * n0 > 1024 || tile_lds(BY, n0) > 65536)) --BY;
+    const int KC = 8;
+    const int ytiles = (n1 + BY - 2) / (BY - 1);
+    const int nb = ytiles * ((n2 + KC - 1) / KC), grid = (nb + 7) / 8 * 8;
+    const size_t shm = tile_lds(BY, n0) / sizeof(double) * sizeof(T);
+    hipLaunchKernelGGL((k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream, v.g,
+                       BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr, (const T*)nullptr,
+                       (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv, ctx->st, ctx->partials);
+}
+
+// one smoothing sweep on a level: xout = xin + omega M_line^-1 r (see k_mg_smooth)
+template <typename T, int M>
+static void mg_smooth_m(pf_ctx* ctx, dim3 grid, int threads, size_t lds, const PartShape& ps, int ncol, int n,
+                        long long plane, bool upd, int fin, T* r, const T* q, const T* xin, T* xout, const T* rp,
+                        T omega, const MgView<T>& v) {
+#define PF_MGS(U, F)                                                                                              \
+    hipLaunchKernelGGL((k_mg_smooth<T, M, U, F>), grid, dim3(threads), lds, ctx->stream, ncol, n, ps.P, ps.CB,    \
+                       plane, r, q, xin, xout, rp, omega, v.la, v.lw, v.lcp, v.schur, ctx->st, ctx->partials)
+    if (upd) {
+        if (fin == 0) PF_MGS(true, 0);
+        else PF_MGS(true, 1);
+    } else {
+        if (fin == 0) PF_MGS(false, 0);
+        else if (fin == 1) PF_MGS(false, 1);
+        else PF_MGS(false, 2);
+    }
+#undef PF_MGS
+}
+
+template <typename T>
+static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, const T* q, const T* xin, T* xout,
+                      const T* rp) {
+    const int ncol = v.g.n0 * v.g.n1, n = v.g.n2;
+    const PartShape ps = part_shape<T>(v.pm, n, ncol);
+    const dim3 grid((ncol + ps.CB - 1) / ps.CB);
+    const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
+    const T om = T(ctx->mg_omega);
+    switch (ps.m) {
+        case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
+        case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
+        case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
+        case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
+    }
+}
+
+// z = V(r): one V-cycle.  upd: apply r -= alpha q first (inside the first sweep);
+// fin: 1 the last sweep ends the PCG iteration, 2 it starts a PCG run.
+template <typename T>
+static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
+    const int L = (int)ctx->mg.size() + 1;
+    for (int l = 0; l < L; ++l) {
+        const MgView<T> v = mg_view<T>(ctx, l, r, z);
+        const bool last = l == L - 1;
+        const int sweeps = last ? ctx->mg_coarse_sweeps : 1;
+        for (int s = 0; s < sweeps; ++s) {
+            const int f = (l == 0 && last && s == sweeps - 1) ? fin : 0;
+            if (s == 0) {
+                mg_smooth<T>(ctx, v, l == 0 && upd, f, v.b, q, nullptr, v.x, r);
+            } else {
+                mg_residual<T>(ctx, l, v);
+                mg_smooth<T>(ctx, v, false, f, v.r, nullptr, v.x, v.x, r);
+            }
+        }
+        if (!last) {
+            mg_residual<T>(ctx, l, v);
+            const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
+            const MgTransfer tr = mg_transfer(ctx, (size_t)l);
+            hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0, ctx->stream,
+                               tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
+        }
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        const MgView<T> v = mg_view<T>(ctx, l, r, z);
+        const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
+        const MgTransfer tr = mg_transfer(ctx, (size_t)l);
+        const long long nf = (long long)v.g.n0 * v.g.n1 * v.g.n2;
+        hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid1(nf, 256, 4096)), dim3(256), 0, ctx->stream, tr, c.wx,
+                           c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
+        mg_residual<T>(ctx, l, v);
+        mg_smooth<T>(ctx, v, false, l == 0 ? fin : 0, v.r, nullptr, v.x, v.x, r);
+    }
+}
+
 // One CG run on (x, r already holding b - A x).  T selects FP64 or FP32 vectors.
 template <typename T>
 static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z, const T* dinv, const T* la,
@@ -621,7 +1009,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
     const long long plane = ncol;
     const dim3 lgrid((ncol + 63) / 64, 1);
-    const bool line = precond == 2;
+    const bool line = precond == 2, mg = precond == 3;
     const int pm = sizeof(T) == 8 ? ctx->pm64 : ctx->pm32;
     const T* schur = (const T*)(sizeof(T) == 8 ? (const void*)ctx->schur : (const void*)ctx->schur32);
     PartShape ps = part_shape<T>(pm, nplanes, ncol);
@@ -632,14 +1020,18 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             hipLaunchKernelGGL((k_cg_line<T, true>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
                                r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
     };
-    if (line)
+    if (mg)
+        mg_vcycle<T>(ctx, r, nullptr, z, false, 2);
+    else if (line)
         line_start();
     else
         hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
                            ctx->partials);
     auto update = [&](T* pout) {
         (void)pout;
-        if (line && pm)
+        if (mg)
+            mg_vcycle<T>(ctx, r, q, z, true, 1);
+        else if (line && pm)
             launch_part<T, false>(ctx, ps, ncol, nplanes, plane, r, q, la, lw, lcp, schur, z);
         else if (line)
             hipLaunchKernelGGL((k_cg_line<T, false>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
@@ -662,6 +1054,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
         update(pout);
     };
     int chunk = check_every > 0 ? check_every : 32;
+    if (mg) chunk = std::min(chunk, 4);  // few, expensive iterations: check the stop often
     chunk += chunk & 1;  // even: each graph replay starts on the same buffer parity
     if (ctx->profile) {
         // event-timed launches, no graph: events around every kernel of a chunk, one
@@ -798,7 +1191,8 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     }
     if (!(bnorm > 0.)) bnorm = 1.;
     const double rtol = opts->rtol > 0. ? opts->rtol : 1e-8;
-    const int precond = opts->precond == 1 ? 1 : 2;  // 1 Jacobi, 2 (default) line
+    const int precond = opts->precond == 1 ? 1 : opts->precond == 3 ? 3 : 2;  // 1 Jacobi, 2 (default) line, 3 multigrid
+    if (precond == 3 && !ctx->mg_built && (rc = mg_build(ctx))) return rc;
     const int maxit = opts->maxit > 0 ? opts->maxit : 100000;
     int total_iters = 0, outer = 0, status = 0;
     double rn = 0.;
@@ -961,6 +1355,54 @@ int pf_apply_host(pf_ctx* ctx, const double* x, double* y) {
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(ctx->stream));
     HIPCK(hipMemcpy(y, ctx->qv, sizeof(double) * ctx->N, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+int pf_precond_apply_host(pf_ctx* ctx, int precision, int precond, const double* r_in, double* z_out) {
+    if (!ctx || !r_in || !z_out) return PF_EINVAL;
+    if (precision != PF_FP64 && precision != PF_FP32_REFINE && precision != PF_FP32_RELIABLE)
+        return fail(ctx, PF_EINVAL, "unknown precision");
+    if (precond != 1 && precond != 2 && precond != 3) return fail(ctx, PF_EINVAL, "precond must be 1, 2 or 3");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = prepare(ctx))) return rc;
+    if (precond == 3 && !ctx->mg_built && (rc = mg_build(ctx))) return rc;
+    const long long N = ctx->N;
+    const int gb = grid1(N, 256, 4096);
+    PcgState init{};
+    init.maxit = 1;
+    *ctx->hst = init;
+    HIPCK(hipMemcpyAsync(ctx->st, ctx->hst, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->tmp, r_in, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    const int ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+    const int nplanes = (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]);
+    auto run = [&](auto* r, auto* z, auto* dinv, auto* la, auto* lw, auto* lcp, auto* schur, int pm) {
+        using T = std::remove_pointer_t<decltype(r)>;
+        if (precond == 3) {
+            mg_vcycle<T>(ctx, r, nullptr, z, false, 2);
+        } else if (precond == 2 && pm) {
+            launch_part<T, true>(ctx, part_shape<T>(pm, nplanes, ncol), ncol, nplanes, ncol, r, nullptr, la, lw, lcp,
+                                 schur, z);
+        } else if (precond == 2) {
+            hipLaunchKernelGGL((k_cg_line<T, true>), dim3((ncol + 63) / 64), dim3(64), 0, ctx->stream, ncol, nplanes,
+                               (long long)ncol, nplanes, r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
+        } else {
+            hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z,
+                               ctx->st, ctx->partials);
+        }
+    };
+    if (precision == PF_FP64) {
+        HIPCK(hipMemcpyAsync(ctx->r, ctx->tmp, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+        run(ctx->r, ctx->z, ctx->dinv, ctx->la, ctx->lw, ctx->lcp, ctx->schur, ctx->pm64);
+        HIPCK(hipMemcpyAsync(ctx->tmp, ctx->z, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->tmp, ctx->r32);
+        run(ctx->r32, ctx->z32, ctx->dinv32, ctx->la32, ctx->lw32, ctx->lcp32, ctx->schur32, ctx->pm32);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->z32, ctx->tmp);
+    }
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(z_out, ctx->tmp, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
     return PF_OK;
 }
 
@@ -1299,6 +1741,23 @@ int pf_set_line_segment(pf_ctx* ctx, int seg) {
     if (!ctx->N) return PF_OK;
     return line_config(ctx);
 }
+
+int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omega) {
+    if (!ctx) return PF_EINVAL;
+    if (max_levels < 0 || max_levels > 32) return fail(ctx, PF_EINVAL, "max_levels must be 0 (no limit) to 32");
+    if (coarse_sweeps < 1 || coarse_sweeps > 64) return fail(ctx, PF_EINVAL, "coarse_sweeps must be 1 to 64");
+    if (!(omega > 0. && omega <= 0.66)) return fail(ctx, PF_EINVAL, "omega must be in (0, 0.66]");
+    ctx->mg_max_levels = max_levels;
+    ctx->mg_coarse_sweeps = coarse_sweeps;
+    ctx->mg_omega = omega;
+    HIPCK(hipSetDevice(ctx->device));
+    if (ctx->stream) HIPCK(hipStreamSynchronize(ctx->stream));
+    mg_free(ctx);  // rebuilt by the next multigrid solve
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_multigrid_levels(const pf_ctx* ctx) { return ctx && ctx->mg_built ? (int)ctx->mg.size() + 1 : 0; }
 
 int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes) {
     if (!ctx) return PF_EINVAL;

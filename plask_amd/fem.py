@@ -17,13 +17,14 @@ PF_THERMAL, PF_ELECTRICAL = 0, 1
 PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE = 0, 1, 2
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
-PRECOND_JACOBI, PRECOND_LINE = 1, 2
+PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
     "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
+    "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host",
 ]
 
 
@@ -92,6 +93,9 @@ def load_library(path=LIB_PATH):
     L.pf_time_kernels.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pf_stream_copy.argtypes = [P, i64, I, C.POINTER(C.c_double)]
     L.pf_solution_prolong.argtypes = [P, P, P, I, P, I, P, I, C.c_double, C.c_double]
+    L.pf_set_multigrid.argtypes = [P, I, I, C.c_double]
+    L.pf_multigrid_levels.argtypes = [P]
+    L.pf_precond_apply_host.argtypes = [P, I, I, P, P]
     _lib = L
     return L
 
@@ -247,6 +251,23 @@ class FemContext:
         """3D operator kernel: kind 0 tile blocks, 1 row blocks of `planes` planes;
         precision PF_FP64, PF_FP32_REFINE or -1 (both)."""
         self._ck(self.L.pf_set_apply(self.h, int(precision), int(kind), int(planes)), "pf_set_apply")
+
+    def set_multigrid(self, max_levels=0, coarse_sweeps=2, omega=0.65):
+        """Multigrid preconditioner (precond=PRECOND_MG, 3D) options; levels are rebuilt
+        by the next multigrid solve."""
+        self._ck(self.L.pf_set_multigrid(self.h, int(max_levels), int(coarse_sweeps), float(omega)),
+                 "pf_set_multigrid")
+
+    def precond_apply(self, r, precision=PF_FP64, precond=PRECOND_LINE):
+        """z = M^-1 r for one preconditioner (test hook; host buffers)."""
+        r = _f64(r)
+        z = np.zeros_like(r)
+        self._ck(self.L.pf_precond_apply_host(self.h, int(precision), int(precond), _p(r), _p(z)),
+                 "pf_precond_apply_host")
+        return z
+
+    def multigrid_levels(self):
+        return int(self.L.pf_multigrid_levels(self.h))
 
     def set_block_order(self, xcd):
         self._ck(self.L.pf_set_block_order(self.h, 1 if xcd else 0), "pf_set_block_order")
