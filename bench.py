@@ -8,13 +8,14 @@ A step is one complete solve from the reference start field (inittemp = 300 K, m
 the device) to ||B - A T|| <= 1e-8 ||B - A T_init||, with the element data, source
 and Dirichlet set already resident in HBM.  The solution stays on the device.
 
-Printed JSON line: value = solves/s of FP64 PCG with the exact line (layer-axis)
-preconditioner; extra fields give the FP32-operator forms (reliable updates, and
-restarted FP64 iterative refinement), the
-plain Jacobi-PCG of the comparison, the learned-warm-start form when trained weights
-are present, the roofline of the dominant kernel (HIP events around each launch of
-an extra profiled solve, on the context stream) and the CPU path timed on this host
-(serial IC(0)-CG on the reference's 14-diagonal storage, bounded sample).
+Printed JSON line: value = solves/s of FP64 PCG preconditioned by the multigrid V-cycle
+(in-plane semicoarsening, exact line smoother along the layer axis).  Extra fields give
+the single-level forms of the same solve (exact line preconditioner, Jacobi), the
+FP32-operator forms (FP64 refinement or reliable updates), the learned-warm-start form
+when trained weights are present, per-kernel-class timings of profiled solves (HIP events
+around each launch on the context stream), the roofline of the kernel class with the
+largest share of the headline solve, and the CPU path timed on this host (serial IC(0)-CG
+on the reference's 14-diagonal storage, bounded sample).
 """
 import argparse
 import json
@@ -28,8 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from plask_amd import workloads  # noqa: E402
-from plask_amd.fem import (PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PRECOND_JACOBI, PRECOND_LINE,  # noqa: E402
-                           FemContext)
+from plask_amd.fem import (KCAT_APPLY, KCAT_MG_COARSE, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH,  # noqa: E402
+                           KCAT_MG_TRANSFER, KCAT_PRECOND, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64,
+                           PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG, FemContext)
 
 
 HBM_PEAK_GBS = 8000.0
@@ -43,6 +45,9 @@ TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 # relative reduction asked of each FP32 run: restarted refinement needs deep inner solves,
 # the reliable-update form replaces the residual early and keeps the search direction
 INNER = {PF_FP64: 0.0, PF_FP32_REFINE: 1e-5, PF_FP32_RELIABLE: 1e-2}
+# with the multigrid preconditioner each iteration reduces the residual ~10x, so a kept
+# direction pays off only over a longer run
+INNER_MG_RELIABLE = 1e-3
 
 
 def make_ctx(p, profile=False):
@@ -61,17 +66,27 @@ def kernel_bytes(shape, m=8):
     line (partitioned, segments of m planes): read r, q, a, w, c'; write r, z
       (56 B/node); separator factors 3 x 8 B per separator per column.
     apply_plain: read x, write y (16 B/node), 16 B/element, bitmask.
+    mg_residual (V-cycle, solve grid): r = b - A x: read x, b; write r (24 B/node),
+      16 B/element, bitmask.
+    mg_smooth (V-cycle, solve grid): damped line sweep, first (read r, q, a, w, c';
+      write r, x) or last (read r_s, x, r, a, w, c'; write x): 56 B/node + separators.
+    mg_transfer: restriction (read fine r, write coarse b) and prolongation (read coarse x,
+      read and write fine x), averaged: 12 B per fine node + 8 B per coarse node.
     FP32 forms: half the bytes of vectors and element data (line: 16-plane segments)."""
     nx, ny, nz = shape
     N, E = nx * ny * nz, (nx - 1) * (ny - 1) * (nz - 1)
     mask = 8 * ((N + 63) // 64)
     P = (nz + m - 1) // m
     P32 = (nz + 15) // 16
+    Nc = (nx // 2 + 1 if nx > 3 else nx) * (ny // 2 + 1 if ny > 3 else ny) * nz
     return {"apply_fused": 48 * N + 16 * E + mask,
             "line": 56 * N + 24 * (P - 1) * nx * ny,
             "apply_plain": 16 * N + 16 * E + mask,
             "apply_fused_fp32": 24 * N + 8 * E + mask,
-            "line_fp32": 28 * N + 12 * (P32 - 1) * nx * ny}
+            "line_fp32": 28 * N + 12 * (P32 - 1) * nx * ny,
+            "mg_residual": 24 * N + 16 * E + mask,
+            "mg_smooth": 56 * N + 24 * (P - 1) * nx * ny,
+            "mg_transfer": 12 * N + 8 * Nc}
 
 
 def cpu_baseline(p, seconds, full_iters):
@@ -107,7 +122,7 @@ def timed(fn, steps, warmup):
 
 
 def learned(ctx, p, size, steps, warmup, T64):
-    """FP32 reliable-update PCG from the warm-start network guess (prediction on the
+    """FP32 + FP64-refinement multigrid PCG from the warm-start network guess (prediction on the
     64x64x128 grid of the same stack, trilinear onto the solve grid); the network
     forward pass and the interpolation are inside the timed step."""
     from plask_amd import warmstart
@@ -121,8 +136,8 @@ def learned(ctx, p, size, steps, warmup, T64):
 
     def step():
         ws.start_on_device(ctx, pc, feat)
-        return ctx.solve_warm(precision=PF_FP32_RELIABLE, rtol=1e-8, inner_rtol=INNER[PF_FP32_RELIABLE],
-                              init_value=300.0, checked=True)
+        return ctx.solve_warm(precision=PF_FP32_REFINE, rtol=1e-8, inner_rtol=INNER[PF_FP32_REFINE],
+                              init_value=300.0, checked=True, precond=PRECOND_MG)
 
     t, st = timed(step, steps, warmup)
     T = ctx.solution()
@@ -134,22 +149,25 @@ def learned(ctx, p, size, steps, warmup, T64):
     t_feat = time.perf_counter() - t0
     return dict(solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters, outer=st.outer, rel_resid=st.rel_resid,
                 fell_back_to_uniform_start=bool(st.start),
-                network_and_prolong_ms=1e3 * t_net, host_features_ms_untimed=1e3 * t_feat, max_rel_diff_vs_fp64=float(np.abs(T - T64).max() / np.abs(T64).max()))
+                network_and_prolong_ms=1e3 * t_net, host_features_ms_untimed=1e3 * t_feat, max_rel_diff_vs_fp64_mg=float(np.abs(T - T64).max() / np.abs(T64).max()))
 
 
 def config1(steps, with_cpu):
     """2D cylindrical stack, 200 (r) x 400 (z) nodes (plask_amd.workloads.thermal_2d_cyl, seed 1):
-    GPU FP64 line-PCG solve to 1e-8 vs the reference method on the CPU (assembly + band
-    Cholesky, kd = 201, serial restatement), with the max relative difference."""
+    GPU FP64 multigrid-PCG solve to 1e-8 (and the line-PCG form) vs the reference method on
+    the CPU (assembly + band Cholesky, kd = 201, serial restatement), with the max relative
+    difference."""
     p = workloads.thermal_2d_cyl(seed=1)
     c = FemContext(0)
     c.set_grid(p["axes"], cylindrical=True)
     c.set_coeffs(p["k_h"], p["k_v"])
     c.set_source(p["q"])
     c.set_dirichlet(p["dir_nodes"], p["dir_vals"])
-    t, st = timed(lambda: c.solve_uniform(300.0, rtol=1e-8), steps, 1)
+    t_line, st_line = timed(lambda: c.solve_uniform(300.0, rtol=1e-8, precond=PRECOND_LINE), steps, 1)
+    t, st = timed(lambda: c.solve_uniform(300.0, rtol=1e-8, precond=PRECOND_MG), steps, 1)
     T = c.solution()
-    out = dict(workload="thermal2d_cyl_200x400", gpu_ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
+    out = dict(workload="thermal2d_cyl_200x400", gpu_ms=1e3 * t, method="fp64 pcg, multigrid V-cycle",
+               iters=st.iters, rel_resid=st.rel_resid, gpu_ms_line_pcg=1e3 * t_line, iters_line_pcg=st_line.iters)
     if with_cpu:
         import oracle as O
         g = O.Grid(p["axes"], cyl=True)
@@ -174,8 +192,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="plain kernel launches instead of hipGraph replay")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed headline solves (for kernel-trace runs): no comparison forms, "
+                         "profiled solves, CPU baseline or 2D case")
     ap.add_argument("--counters", action="store_true",
-                    help="short run for hardware-counter profiles: stream-copy calibration, one FP64 solve")
+                    help="short run for hardware-counter profiles: stream-copy calibration, one FP64 "
+                         "multigrid solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the solve path has no multi-GPU "
@@ -193,72 +215,91 @@ def main():
         cb = 1 << 30
         t = ctx.stream_copy(cb, reps=3)
         print("stream copy %d bytes: %.1f us, %.0f GB/s" % (cb, 1e6 * t, 2 * cb / t / 1e9), file=sys.stderr)
-        st = ctx.solve_uniform(300.0, rtol=rtol)
+        st = ctx.solve_uniform(300.0, rtol=rtol, precond=PRECOND_MG)
         print(json.dumps({"counters_run": True, "copy_bytes": cb, "iters": st.iters}))
         return
 
-    def step(prec, pc=PRECOND_LINE):
-        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=INNER[prec] or None, precond=pc)
+    def step(prec, pc, inner=None):
+        return ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=inner, precond=pc)
 
-    t, st = timed(lambda: step(PF_FP64), args.steps, args.warmup)
-    fp64 = dict(solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, rel_resid=st.rel_resid)
-    print("fp64: %.2f solves/s, %.1f ms/solve, %d iterations, rel resid %.2e" %
-          (fp64["solves_per_s"], fp64["ms"], st.iters, st.rel_resid), file=sys.stderr, flush=True)
-    T64 = ctx.solution()
+    def form(name, prec, pc, inner=None, steps=args.steps, warmup=args.warmup):
+        t, st = timed(lambda: step(prec, pc, inner), steps, warmup)
+        T = ctx.solution()
+        f = dict(method=name, solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, outer=st.outer,
+                 rel_resid=st.rel_resid)
+        if prec != PF_FP64:
+            f["inner_rtol"] = inner
+        if T64 is not None:
+            f["max_rel_diff_vs_fp64_mg"] = float(np.abs(T - T64).max() / np.abs(T64).max())
+            f["speedup_vs_fp64_mg"] = fp64["ms"] / f["ms"]
+        print("%s: %.2f solves/s, %.2f ms/solve, %d iterations (%d outer), rel resid %.2e" %
+              (name, f["solves_per_s"], f["ms"], st.iters, st.outer, st.rel_resid), file=sys.stderr, flush=True)
+        return f, T
 
-    tj, stj = timed(lambda: step(PF_FP64, PRECOND_JACOBI), 1, 1)
-    jac = dict(solves_per_s=1.0 / tj, ms=1e3 * tj, iters=stj.iters, rel_resid=stj.rel_resid)
-    print("fp64 jacobi: %.2f solves/s, %d iterations" % (jac["solves_per_s"], stj.iters), file=sys.stderr, flush=True)
-
-    def mixed_form(prec, name):
-        t, st = timed(lambda: step(prec), args.steps, args.warmup)
-        T32 = ctx.solution()
-        m = dict(method=name, inner_rtol=INNER[prec], solves_per_s=1.0 / t, ms=1e3 * t, inner_iters=st.iters,
-                 outer=st.outer, rel_resid=st.rel_resid,
-                 max_rel_diff_vs_fp64=float(np.abs(T32 - T64).max() / np.abs(T64).max()),
-                 speedup_vs_fp64=fp64["ms"] / (1e3 * t))
-        print("%s: %.2f solves/s, %.1f ms/solve, %d inner its, %d outer, rel resid %.2e" %
-              (name, m["solves_per_s"], m["ms"], st.iters, st.outer, st.rel_resid), file=sys.stderr, flush=True)
-        return m
-
-    mixed = mixed_refine = None
-    if not args.no_mixed:
-        mixed = mixed_form(PF_FP32_RELIABLE, "fp32 operator, reliable updates (fp64 residual replacement)")
-        mixed_refine = mixed_form(PF_FP32_REFINE, "fp32 operator, restarted fp64 iterative refinement")
-
+    # headline: FP64 PCG preconditioned by the multigrid V-cycle
+    T64 = None
+    fp64, T64 = form("fp64 pcg, multigrid V-cycle (in-plane semicoarsening, exact line smoother)", PF_FP64,
+                     PRECOND_MG)
+    fp64.pop("inner_rtol", None)
+    if args.headline_only:
+        print(json.dumps({"metric": "3D thermal solves/s to 1e-8 rel. residual (4.2M DOF)", "value":
+                          round(fp64["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": round(fp64["ms"], 3), "headline_only": True,
+                          "config": {"workload": "thermal3d_%dx%dx%d" % size, "method": fp64["method"],
+                                     "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]}}))
+        return
+    # the single-level forms of the same solve, for comparison
+    line64, _ = form("fp64 pcg, exact line preconditioner", PF_FP64, PRECOND_LINE)
+    jac, _ = form("fp64 pcg, Jacobi preconditioner", PF_FP64, PRECOND_JACOBI, steps=1, warmup=1)
+    mixed = {}
     warm = None
-    if not args.no_mixed and os.path.exists(WEIGHTS):
-        warm = learned(ctx, p, size, args.steps, args.warmup, T64)
-        warm["speedup_vs_fp64"] = fp64["ms"] / warm["ms"]
-        print("fp32 reliable + learned start: %.2f solves/s, %.1f ms/solve, %d inner its" %
-              (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
+    if not args.no_mixed:
+        mixed["mg_fp32_refine"], _ = form("fp32 operator + fp64 refinement, multigrid", PF_FP32_REFINE, PRECOND_MG,
+                                          INNER[PF_FP32_REFINE])
+        mixed["mg_fp32_reliable"], _ = form("fp32 operator, reliable updates, multigrid", PF_FP32_RELIABLE,
+                                            PRECOND_MG, INNER_MG_RELIABLE)
+        mixed["line_fp32_reliable"], _ = form("fp32 operator, reliable updates, line preconditioner",
+                                              PF_FP32_RELIABLE, PRECOND_LINE, INNER[PF_FP32_RELIABLE])
+        if os.path.exists(WEIGHTS):
+            warm = learned(ctx, p, size, args.steps, args.warmup, T64)
+            warm["speedup_vs_fp64_mg"] = fp64["ms"] / warm["ms"]
+            print("learned start + fp32 refinement, multigrid: %.2f solves/s, %.2f ms/solve, %d inner its" %
+                  (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
+    best = min(mixed.values(), key=lambda f: f["ms"]) if mixed else None
 
-    # roofline: one profiled solve per precision -- HIP events around every kernel on the
-    # context stream, one synchronisation per 32-iteration chunk (the in-solve durations a
-    # kernel trace reports; launches after the stop are not counted)
-    def profiled(prec):
+    # roofline: profiled solves -- HIP events around every kernel class on the context
+    # stream, one synchronisation per chunk (the in-solve durations a kernel trace reports;
+    # launches after the stop are not counted)
+    def profiled(prec, pc, inner=None):
         ctx.set_profile(True)
-        ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=INNER[prec] or None)
+        st = ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=inner, precond=pc)
         ctx.set_profile(False)
-        (ta, tl), (na, nl) = ctx.kernel_times()
-        return ta, tl, int(na)
-    t_apply32, t_line32, n32 = profiled(PF_FP32_RELIABLE)
-    t_apply, t_line, n64 = profiled(PF_FP64)
-    t_plain = ctx.time_apply(PF_FP64, reps=50)
+        t, n = ctx.kernel_times(6)
+        return t, n, st.iters
     kb = kernel_bytes(size)
-    kern = {
-        "apply_fused": dict(avg_us=1e6 * t_apply, launches=n64, bytes=kb["apply_fused"]),
-        "line": dict(avg_us=1e6 * t_line, launches=n64, bytes=kb["line"]),
-        "apply_plain": dict(avg_us=1e6 * t_plain, bytes=kb["apply_plain"]),
-        "apply_fused_fp32": dict(avg_us=1e6 * t_apply32, launches=n32, bytes=kb["apply_fused_fp32"]),
-        "line_fp32": dict(avg_us=1e6 * t_line32, launches=n32, bytes=kb["line_fp32"]),
-    }
+    kern = {}
+    t, n, its = profiled(PF_FP64, PRECOND_MG)
+    names = {KCAT_APPLY: "apply_fused", KCAT_MG_RESIDUAL: "mg_residual", KCAT_MG_SMOOTH: "mg_smooth",
+             KCAT_MG_TRANSFER: "mg_transfer"}
+    for cat, name in names.items():
+        kern[name] = dict(avg_us=1e6 * t[cat], launches=int(n[cat]), bytes=kb[name])
+    kern["mg_vcycle"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]))
+    kern["mg_coarse_levels"] = dict(avg_us=1e6 * t[KCAT_MG_COARSE], launches=int(n[KCAT_MG_COARSE]))
+    t, n, _ = profiled(PF_FP32_REFINE, PRECOND_MG, INNER[PF_FP32_REFINE])
+    kern["apply_fused_fp32"] = dict(avg_us=1e6 * t[KCAT_APPLY], launches=int(n[KCAT_APPLY]),
+                                    bytes=kb["apply_fused_fp32"])
+    kern["mg_vcycle_fp32"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]))
+    t, n, _ = profiled(PF_FP64, PRECOND_LINE)
+    kern["line"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]), bytes=kb["line"])
+    kern["apply_plain"] = dict(avg_us=1e6 * ctx.time_apply(PF_FP64, reps=50), bytes=kb["apply_plain"])
     for k in kern.values():
-        k["gbs"] = round(k["bytes"] / (1e-6 * k["avg_us"]) / 1e9, 1)
+        if "bytes" in k and k["avg_us"] > 0:
+            k["gbs"] = round(k["bytes"] / (1e-6 * k["avg_us"]) / 1e9, 1)
         k["avg_us"] = round(k["avg_us"], 2)
     t_copy = ctx.stream_copy(1 << 30, reps=5)
     copy_gbs = 2 * (1 << 30) / t_copy / 1e9
-    dom = "apply_fused" if t_apply >= t_line else "line"
+    # dominant kernel class of the headline solve: largest total time per solve
+    dom = max(names.values(), key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     k = kern[dom]
     traffic = None
     if os.path.exists(TRAFFIC):
@@ -269,6 +310,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": k["bytes"], "avg_launch_us": k["avg_us"],
+                "time_share_of_solve": round(1e-3 * k["avg_us"] * k["launches"] / fp64["ms"], 3),
                 "stream_copy_gbs": round(copy_gbs, 1)}
 
     cpu = None
@@ -298,14 +340,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded layered stack, SplitMix64)",
-        "config": {"workload": "thermal3d_%dx%dx%d" % size, "method": "fp64 pcg, exact line preconditioner",
+        "config": {"workload": "thermal3d_%dx%dx%d" % size, "method": fp64["method"],
                    "dof": N, "elements": E, "rtol": rtol,
                    "residual": "||B-AT||/||B-AT_init||, T_init = 300 K",
                    "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]},
-        "mixed_precision": mixed,
-        "mixed_precision_restarted": mixed_refine,
-        "learned_warm_start": warm,
+        "fp64_line_pcg": line64,
         "fp64_jacobi_pcg": jac,
+        "mixed_precision": best,
+        "mixed_precision_forms": mixed,
+        "learned_warm_start": warm,
         "config1_2d_cyl": c1,
         "kernels": kern,
         "roofline": roofline,

@@ -89,8 +89,8 @@ typedef struct {
     int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE */
     int precond;      /* 1 = Jacobi (NSPCG 'jac'); 2 = line block-Jacobi along the last (layer) axis
                          (the tridiagonal part of A per column, cf. NSPCG 'ljac'); 3 = multigrid
-                         V-cycle (3D only: in-plane semicoarsening, damped line smoother; no
-                         reference counterpart, see pf_set_multigrid); 0 = default (2) */
+                         V-cycle (in-plane semicoarsening, damped line smoother; no reference
+                         counterpart, see pf_set_multigrid); 0 = default (2) */
     int x0;           /* PF_X0_* */
     int maxit;        /* max CG iterations (sum of inner iterations for PF_FP32_REFINE) */
     double rtol;      /* stop when ||B - A X||_2 <= rtol * (denominator chosen by `norm`) */
@@ -167,9 +167,19 @@ void* pf_solution_device(pf_ctx* ctx);
 int64_t pf_num_nodes(const pf_ctx* ctx);
 int64_t pf_num_elements(const pf_ctx* ctx);
 
-/* Average duration (s) of each kernel class over the last pf_solve, measured with
- * HIP events on the context stream: what[0] = operator apply + (p,Ap),
- * what[1] = vector update + (r,z); counts[] receives launch counts.  For benchmarks. */
+/* Kernel classes of pf_kernel_times */
+#define PF_KCAT_APPLY 0       /* fused operator apply + (p, Ap) */
+#define PF_KCAT_PRECOND 1     /* residual update + preconditioner + (r, z): the line / Jacobi kernel or a whole V-cycle */
+#define PF_KCAT_MG_RESIDUAL 2 /* V-cycle on the solve grid: smoother residual applies */
+#define PF_KCAT_MG_SMOOTH 3   /* V-cycle on the solve grid: damped line sweeps */
+#define PF_KCAT_MG_TRANSFER 4 /* V-cycle: restriction / prolongation between the solve grid and level 1 */
+#define PF_KCAT_MG_COARSE 5   /* V-cycle: all work on the coarse levels (one record per V-cycle) */
+#define PF_KCAT_COUNT 6
+
+/* Average duration (s) of each kernel class (PF_KCAT_*, n <= 8 entries) over the
+ * iterations of the last pf_solve run with profiling on (pf_set_profile), measured with
+ * HIP events around each launch on the context stream; counts[] receives the number of
+ * timed launches per class.  For benchmarks. */
 int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n);
 
 /* Launch shape of the operator kernels (for tuning): 3D rows = thread rows per block
@@ -185,8 +195,9 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes);
  * one-thread-per-line kernel.  No counterpart in the reference (a tuning hook). */
 int pf_set_line_segment(pf_ctx* ctx, int seg);
 
-/* Multigrid preconditioner (precond = 3, 3D): levels coarsen the two in-plane axes by
- * two (an axis stops at 3 nodes), keep every plane of the layer axis, and use the same
+/* Multigrid preconditioner (precond = 3): levels coarsen the in-plane axes (3D: axes 0
+ * and 1; 2D: axis 0) by two (an axis stops at 3 nodes), keep every plane of the layer
+ * axis, and use the same
  * Q1 operator with area-weighted mean conductivities; the smoother is the exact line
  * solve along the layer axis damped by omega (0 < omega <= 0.66, default 0.65: the line
  * preconditioned operator has eigenvalues <= 3, so the smoother converges), one sweep

@@ -8,11 +8,12 @@ Dirichlet elimination `matrix.hpp:111-145`).  This file states, with SciPy spars
 matrices, what the GPU kernels compute so that one V-cycle can be compared entry by
 entry:
 
-* levels: each in-plane axis keeps nodes 0, 2, 4, ... and its last node while it has
-  more than 3 nodes; the layer axis keeps all planes;
+* levels: each in-plane axis (3D: axes 0 and 1; 2D: axis 0) keeps nodes 0, 2, 4, ...
+  and its last node while it has more than 3 nodes; the layer axis keeps all planes;
 * coarse operator: the oracle's eliminated Q1 operator on the coarse grid with
-  area-weighted mean conductivities (``orc_assemble``); a coarse node is Dirichlet when
-  the fine node at its position is;
+  area-weighted mean conductivities (2D: width-weighted, times the mid radius in
+  cylindrical geometry) (``orc_assemble``); a coarse node is Dirichlet when the fine node
+  at its position is;
 * transfer: P = I_z (x) P_y (x) P_x (linear interpolation along each in-plane axis),
   masked to zero at Dirichlet nodes on both sides; restriction R = P^T;
 * smoother: x <- x + omega M^-1 (b - A x), M the tridiagonal part of A along the layer
@@ -86,16 +87,18 @@ def interp_1d(m, w, nc):
 
 
 class Level:
-    def __init__(self, axes, kh, kv, dmask):
+    def __init__(self, axes, kh, kv, dmask, cyl=False):
         self.axes = [np.asarray(a, dtype=np.float64) for a in axes]
+        self.cyl = bool(cyl)
         self.n = [len(a) for a in self.axes]
         self.N = int(np.prod(self.n))
-        self.plane = self.n[0] * self.n[1]
+        self.plane = self.n[0] * self.n[1] if len(self.n) == 3 else self.n[0]
         self.kh = np.asarray(kh, dtype=np.float64)
         self.kv = np.asarray(kv, dtype=np.float64)
         self.dmask = np.asarray(dmask, dtype=bool)
         dn = np.nonzero(self.dmask)[0].astype(np.int64)
-        A = assemble(Grid(self.axes), 0, self.kh, self.kv, np.zeros(self.kh.size), ST_DIA, dn, np.zeros(len(dn)))
+        A = assemble(Grid(self.axes, cyl=self.cyl), 0, self.kh, self.kv, np.zeros(self.kh.size), ST_DIA, dn,
+                     np.zeros(len(dn)))
         self.A = csr(A)
         coo = self.A.tocoo()
         keep = (coo.row == coo.col) | (np.abs(coo.row - coo.col) == self.plane)
@@ -103,9 +106,37 @@ class Level:
         self.Mlu = spl.splu(self.M)
 
 
+def coarse_level_2d(L, max_n=3):
+    """2D: axis 0 coarsened, axis 1 (the layer axis) kept; conductivity means weighted by
+    element width, times the mid radius in cylindrical geometry."""
+    if L.n[0] <= max_n:
+        return None, None
+    cx, mx, wx, ax = coarsen_axis(L.axes[0], True)
+    n1 = L.n[1]
+    e0f, e0c = L.n[0] - 1, len(cx) - 1
+    w = np.diff(L.axes[0])
+    if L.cyl:
+        w = w * 0.5 * (L.axes[0][1:] + L.axes[0][:-1])
+    khf = L.kh.reshape(n1 - 1, e0f)
+    kvf = L.kv.reshape(n1 - 1, e0f)
+    khc = np.zeros((n1 - 1, e0c))
+    kvc = np.zeros((n1 - 1, e0c))
+    for I in range(e0c):
+        a = w[cx[I]:cx[I + 1]]
+        khc[:, I] = (khf[:, cx[I]:cx[I + 1]] * a).sum(axis=1) / a.sum()
+        kvc[:, I] = (kvf[:, cx[I]:cx[I + 1]] * a).sum(axis=1) / a.sum()
+    cd = L.dmask.reshape(n1, L.n[0])[:, cx].ravel()
+    C = Level([ax, L.axes[1]], khc.ravel(), kvc.ravel(), cd, cyl=L.cyl)
+    P = sp.kron(sp.identity(n1), interp_1d(mx, wx, len(cx)))
+    P = (sp.diags((~L.dmask).astype(float)) @ P @ sp.diags((~C.dmask).astype(float))).tocsr()
+    return C, P
+
+
 def coarse_level(L, max_n=3):
     """Next coarser level of L and the masked interpolation P (fine x coarse); None when
-    neither in-plane axis has more than ``max_n`` nodes."""
+    no in-plane axis has more than ``max_n`` nodes."""
+    if len(L.n) == 2:
+        return coarse_level_2d(L, max_n)
     onx, ony = L.n[0] > max_n, L.n[1] > max_n
     if not (onx or ony):
         return None, None
@@ -134,13 +165,14 @@ def coarse_level(L, max_n=3):
 
 
 class VCycle:
-    """Levels of a 3D grid (natural node order, axis 0 fastest) and the V-cycle z = V r."""
+    """Levels of a 2D or 3D grid (natural node order, axis 0 fastest; the last axis is the
+    layer axis) and the V-cycle z = V r."""
 
-    def __init__(self, axes, kh, kv, dnodes, omega=0.65, coarse_sweeps=2, max_levels=0):
+    def __init__(self, axes, kh, kv, dnodes, omega=0.65, coarse_sweeps=2, max_levels=0, cyl=False):
         N = int(np.prod([len(a) for a in axes]))
         dmask = np.zeros(N, bool)
         dmask[np.asarray(dnodes, dtype=np.int64)] = True
-        self.levels = [Level(axes, kh, kv, dmask)]
+        self.levels = [Level(axes, kh, kv, dmask, cyl=cyl)]
         self.P = []
         while max_levels == 0 or len(self.levels) < max_levels:
             C, P = coarse_level(self.levels[-1])

@@ -59,7 +59,7 @@ std::vector<double> FemCore::solution() {
 void FemCore::solve(double init_value) {
     pf_solve_opts o{};
     o.precision = iter.precision;
-    o.precond = iter.precond;
+    o.precond = iter.precond ? iter.precond : 3;
     o.x0 = PF_X0_DEVICE;
     o.maxit = iter.maxit;
     o.rtol = iter.maxerr;

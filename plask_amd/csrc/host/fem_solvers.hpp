@@ -22,9 +22,10 @@ enum LogLevel { LOG_CRITICAL_ERROR = 0, LOG_ERROR, LOG_ERROR_DETAIL, LOG_WARNING
 struct IterParams {
     int maxit = 100000;
     double maxerr = 1e-8;
-    int precond = 2;    // 1 Jacobi, 2 line (layer-axis) block Jacobi
-    int precision = 0;  // PF_FP64 or PF_FP32_REFINE
-    double inner_rtol = 1e-4;
+    int precond = 0;    // 0 automatic (multigrid), 1 Jacobi, 2 line (layer-axis) block Jacobi,
+                        // 3 multigrid V-cycle
+    int precision = 0;  // PF_FP64, PF_FP32_REFINE or PF_FP32_RELIABLE
+    double inner_rtol = 0.;  // FP32 forms: reduction per FP32 run (0: the library default)
     int noconv = 1;     // 0 error, 1 warning, 2 continue (NoConvergenceBehavior)
     bool converged = true;
     int iters = 0;

@@ -1,8 +1,9 @@
 // Grid-transfer and coarse-level set-up kernels of the semicoarsening multigrid
-// preconditioner (3D).  The coarse levels coarsen the two in-plane axes only (every
-// other node, plus the last one); the layer axis keeps all its planes, so every level
-// has the same columns structure and the exact line solve along the layer axis is
-// the smoother on every level.
+// preconditioner.  The coarse levels coarsen the in-plane axes only (every other node,
+// plus the last one; in 2D the one in-plane axis 0, with a trivial one-node axis 1 in
+// the tables below); the layer axis keeps all its planes, so every level has the same
+// column structure and the exact line solve along the layer axis is the smoother on
+// every level.
 //
 // Transfer tables per in-plane axis, from a finer level to the next coarser one:
 //   cidx[I]          fine index of coarse node I (increasing, first 0, last n_f - 1)
@@ -115,6 +116,32 @@ __global__ __launch_bounds__(256) void k_mg_coarsen_k(MgTransfer tr, const doubl
                 sh += a * kh[f];
                 sv += a * kv[f];
             }
+        khc[e] = sh / sa;
+        kvc[e] = sv / sa;
+    }
+}
+
+// 2D: coarse element (I, k) covers the fine elements [cx[I], cx[I+1]) of element row k;
+// means weighted by the element width (Cartesian) or width x mid radius (cylindrical,
+// the weight the element matrix carries).
+__global__ __launch_bounds__(256) void k_mg_coarsen_k2d(MgTransfer tr, const double* __restrict__ hxf,
+                                                        const double* __restrict__ rmidf,
+                                                        const double* __restrict__ khf,
+                                                        const double* __restrict__ kvf, double* __restrict__ khc,
+                                                        double* __restrict__ kvc) {
+    const int e0c = tr.n0c - 1, e0f = tr.n0f - 1;
+    const long long Ec = (long long)e0c * (tr.n2 - 1);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < Ec; e += (long long)gridDim.x * blockDim.x) {
+        const int I = (int)(e % e0c);
+        const long long k = e / e0c;
+        double sa = 0., sh = 0., sv = 0.;
+        for (int ex = tr.cx[I]; ex < tr.cx[I + 1]; ++ex) {
+            const double a = hxf[ex] * (rmidf ? rmidf[ex] : 1.);
+            const long long f = k * e0f + ex;
+            sa += a;
+            sh += a * khf[f];
+            sv += a * kvf[f];
+        }
         khc[e] = sh / sa;
         kvc[e] = sv / sa;
     }

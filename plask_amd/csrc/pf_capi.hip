@@ -31,14 +31,15 @@ struct DevBuf {
 
 }  // namespace
 
-// One coarse level of the semicoarsening multigrid preconditioner (3D): the in-plane
-// axes coarsened, all planes of the layer axis kept.  Device arrays; the level above
-// is the next finer one (level 0 is the solve grid itself).
+// One coarse level of the semicoarsening multigrid preconditioner: the in-plane axes
+// coarsened (3D: axes 0 and 1; 2D: axis 0, with n1 = 1), all planes of the layer axis
+// kept.  Device arrays; the level above is the next finer one (level 0 is the solve
+// grid itself).
 struct MgLevel {
     int n0 = 0, n1 = 0;
     long long N = 0, E = 0;
-    double *hx = nullptr, *hy = nullptr;
-    float *hx32 = nullptr, *hy32 = nullptr;
+    double *hx = nullptr, *hy = nullptr, *rmid = nullptr;  // hy: 3D only; rmid: 2D cylindrical
+    float *hx32 = nullptr, *hy32 = nullptr, *rmid32 = nullptr;
     double *kh = nullptr, *kv = nullptr;
     float *kh32 = nullptr, *kv32 = nullptr;
     uint64_t* dmask = nullptr;
@@ -120,10 +121,18 @@ struct pf_ctx {
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
     int graph_chunk = 0, graph_precond = 0;
 
-    // profiling
+    // profiling: HIP event pairs around kernels of profiled CG chunks, summed per
+    // category (PF_KCAT_*) over the iterations that ran
     bool profile = false;
-    double kt[2] = {0., 0.};
-    int64_t kc[2] = {0, 0};
+    int prof_iter = -1;  // iteration index inside a profiled chunk, -1 outside
+    struct ProfRec {
+        int a, b, cat, iter;
+    };
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<ProfRec> prof_recs;
+    size_t prof_used = 0;
+    double kt[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    int64_t kc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     // Picard-loop state
     double *k0h = nullptr, *k0v = nullptr, *ah = nullptr, *av = nullptr;
@@ -175,6 +184,39 @@ static void drop_graphs(pf_ctx* ctx) {
     ctx->graph64 = ctx->graph32 = nullptr;
 }
 
+// Profiled chunks: prof_open records a start event for one kernel category (returns a
+// handle, -1 when not profiling), prof_close its end event; prof_collect (after a stream
+// synchronisation) adds the durations of the records of iterations < ran.
+static int prof_open(pf_ctx* ctx, int cat) {
+    if (!ctx->profile || ctx->prof_iter < 0) return -1;
+    while (ctx->prof_used + 2 > ctx->prof_ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->prof_ev.push_back(e);
+    }
+    pf_ctx::ProfRec r{(int)ctx->prof_used, (int)ctx->prof_used + 1, cat, ctx->prof_iter};
+    ctx->prof_used += 2;
+    hipEventRecord(ctx->prof_ev[r.a], ctx->stream);
+    ctx->prof_recs.push_back(r);
+    return (int)ctx->prof_recs.size() - 1;
+}
+
+static void prof_close(pf_ctx* ctx, int h) {
+    if (h >= 0) hipEventRecord(ctx->prof_ev[ctx->prof_recs[h].b], ctx->stream);
+}
+
+static void prof_collect(pf_ctx* ctx, int ran) {
+    for (const auto& r : ctx->prof_recs) {
+        if (r.iter >= ran) continue;
+        float t = 0.f;
+        hipEventElapsedTime(&t, ctx->prof_ev[r.a], ctx->prof_ev[r.b]);
+        ctx->kt[r.cat] += 1e-3 * t;
+        ctx->kc[r.cat] += 1;
+    }
+    ctx->prof_recs.clear();
+    ctx->prof_used = 0;
+}
+
 static void dfree(pf_ctx* ctx, void** ptr) {
     if (!*ptr) return;
     hipFree(*ptr);
@@ -192,7 +234,7 @@ static void mg_free(pf_ctx* ctx) {
                          (void**)&L.wx32,  (void**)&L.wy32,  (void**)&L.la,      (void**)&L.lw,    (void**)&L.lcp,
                          (void**)&L.schur, (void**)&L.la32,  (void**)&L.lw32,    (void**)&L.lcp32, (void**)&L.schur32,
                          (void**)&L.b,     (void**)&L.x,     (void**)&L.r,       (void**)&L.b32,   (void**)&L.x32,
-                         (void**)&L.r32};
+                         (void**)&L.r32,   (void**)&L.rmid,  (void**)&L.rmid32};
         for (void** q : ptrs) dfree(ctx, q);
     }
     ctx->mg.clear();
@@ -298,6 +340,7 @@ void pf_ctx_destroy(pf_ctx* ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
+    for (hipEvent_t e : ctx->prof_ev) hipEventDestroy(e);
     for (void* p : ctx->allocs) hipFree(p);
     if (ctx->st) hipFree(ctx->st);
     if (ctx->hst) hipHostFree(ctx->hst);
@@ -670,9 +713,9 @@ static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long 
 }
 
 // ---------------------------------------------------------------------------
-// Semicoarsening multigrid preconditioner (3D).  Level 0 is the solve grid; each
-// coarser level keeps every other node of the two in-plane axes (and the last) and
-// every plane of the layer axis.  Coarse operators are the same Q1 operator on the
+// Semicoarsening multigrid preconditioner.  Level 0 is the solve grid; each coarser
+// level keeps every other node of the in-plane axes (3D: axes 0 and 1; 2D: axis 0) and
+// the last one, and every plane of the layer axis.  Coarse operators are the same Q1 operator on the
 // coarse grid with area-weighted mean conductivities; the smoother is the damped
 // exact line solve along the layer axis (the line preconditioner of the single-level
 // PCG), one sweep before and one after the coarse correction, and
@@ -684,14 +727,17 @@ static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long 
 // so lambda_max <= 3 for any element shapes: omega <= 0.66 is accepted, 0.65 default.
 // ---------------------------------------------------------------------------
 
+// planes along the layer (line) axis: axis 2 in 3D, axis 1 in 2D
+static int mg_planes(const pf_ctx* ctx) { return (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]); }
+
 static MgTransfer mg_transfer(pf_ctx* ctx, size_t l) {
     const MgLevel& L = ctx->mg[l];
     MgTransfer tr;
     tr.n0f = l ? ctx->mg[l - 1].n0 : (int)ctx->n[0];
-    tr.n1f = l ? ctx->mg[l - 1].n1 : (int)ctx->n[1];
+    tr.n1f = l ? ctx->mg[l - 1].n1 : (ctx->dim == 3 ? (int)ctx->n[1] : 1);
     tr.n0c = L.n0;
     tr.n1c = L.n1;
-    tr.n2 = (int)ctx->n[2];
+    tr.n2 = mg_planes(ctx);
     tr.cx = L.cx;
     tr.cy = L.cy;
     tr.mx = L.mx;
@@ -707,15 +753,20 @@ static int mg_refresh(pf_ctx* ctx) {
         mg_free(ctx);
         return PF_OK;
     }
-    const int n2 = (int)ctx->n[2];
+    const int n2 = mg_planes(ctx);
     int rc;
     for (size_t l = 0; l < ctx->mg.size(); ++l) {
         MgLevel& L = ctx->mg[l];
         const MgLevel* F = l ? &ctx->mg[l - 1] : nullptr;
         const MgTransfer tr = mg_transfer(ctx, l);
-        hipLaunchKernelGGL(k_mg_coarsen_k, dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, tr,
-                           F ? F->hx : ctx->hx, F ? F->hy : ctx->hy, F ? F->kh : ctx->kh, F ? F->kv : ctx->kv, L.kh,
-                           L.kv);
+        if (ctx->dim == 3)
+            hipLaunchKernelGGL(k_mg_coarsen_k, dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, tr,
+                               F ? F->hx : ctx->hx, F ? F->hy : ctx->hy, F ? F->kh : ctx->kh, F ? F->kv : ctx->kv,
+                               L.kh, L.kv);
+        else
+            hipLaunchKernelGGL(k_mg_coarsen_k2d, dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, tr,
+                               F ? F->hx : ctx->hx, F ? F->rmid : ctx->rmid, F ? F->kh : ctx->kh,
+                               F ? F->kv : ctx->kv, L.kh, L.kv);
         hipLaunchKernelGGL((k_convert<float, double>), dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, L.E,
                            L.kh, L.kh32);
         hipLaunchKernelGGL((k_convert<float, double>), dim3(grid1(L.E, 256, 4096)), dim3(256), 0, ctx->stream, L.E,
@@ -733,12 +784,20 @@ static int mg_refresh(pf_ctx* ctx) {
             if ((rc = dalloc(ctx, &L.schur32, 3 * (size_t)std::max(s32.P - 1, 1) * ncol))) return rc;
             L.schur_m32 = s32.m;
         }
-        const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz};
         const int cb = grid1(ncol, 64, 1 << 20);
-        hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                           s64.m, L.la, L.lw, L.lcp, L.schur);
-        hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                           s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+        if (ctx->dim == 3) {
+            const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz};
+            hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                               s64.m, L.la, L.lw, L.lcp, L.schur);
+            hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                               s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+        } else {
+            const Axes2<double> g{L.n0, n2, L.hx, ctx->hy, L.rmid};
+            hipLaunchKernelGGL((k_part_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                               s64.m, L.la, L.lw, L.lcp, L.schur);
+            hipLaunchKernelGGL((k_part_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
+                               s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+        }
     }
     HIPCK(hipGetLastError());
     return PF_OK;
@@ -747,6 +806,7 @@ static int mg_refresh(pf_ctx* ctx) {
 template <typename T> static int upload(pf_ctx* ctx, T** dst, const std::vector<T>& v) {
     int rc;
     if ((rc = dalloc(ctx, dst, v.size()))) return rc;
+    if (v.empty()) return PF_OK;
     HIPCK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return PF_OK;
 }
@@ -771,16 +831,17 @@ static void mg_coarsen_axis(const std::vector<double>& f, bool on, std::vector<i
     for (int i : ci) c.push_back(f[i]);
 }
 
-// levels of the current grid: an in-plane axis is coarsened while it has more than 3
-// nodes; at most mg_max_levels levels in all (0: no limit)
+// levels of the current grid: an in-plane axis (3D: 0 and 1, 2D: 0) is coarsened while
+// it has more than 3 nodes; at most mg_max_levels levels in all (0: no limit)
 static int mg_build(pf_ctx* ctx) {
-    if (ctx->dim != 3) return fail(ctx, PF_EINVAL, "the multigrid preconditioner is 3D only");
     if (!ctx->pm64 || !ctx->pm32)
         return fail(ctx, PF_EINVAL, "the multigrid preconditioner needs the partitioned line solve");
     mg_free(ctx);
     int rc;
-    const int n2 = (int)ctx->n[2];
-    std::vector<double> fx = ctx->ax[0], fy = ctx->ax[1];
+    const bool three = ctx->dim == 3;
+    const int n2 = mg_planes(ctx);
+    // 2D: axis 1 is the layer axis; the in-plane y axis of the tables is a single node
+    std::vector<double> fx = ctx->ax[0], fy = three ? ctx->ax[1] : std::vector<double>{0.};
     while (ctx->mg_max_levels == 0 || (int)ctx->mg.size() + 1 < ctx->mg_max_levels) {
         const bool cxa = fx.size() > 3, cya = fy.size() > 3;
         if (!cxa && !cya) break;
@@ -793,10 +854,15 @@ static int mg_build(pf_ctx* ctx) {
         L.n0 = (int)cxv.size();
         L.n1 = (int)cyv.size();
         L.N = (long long)L.n0 * L.n1 * n2;
-        L.E = (long long)(L.n0 - 1) * (L.n1 - 1) * (n2 - 1);
-        std::vector<double> hx(L.n0 - 1), hy(L.n1 - 1);
+        L.E = (long long)(L.n0 - 1) * (three ? L.n1 - 1 : 1) * (n2 - 1);
+        std::vector<double> hx(L.n0 - 1), hy(L.n1 - 1), rm;
         for (int i = 0; i + 1 < L.n0; ++i) hx[i] = cxv[i + 1] - cxv[i];
         for (int j = 0; j + 1 < L.n1; ++j) hy[j] = cyv[j + 1] - cyv[j];
+        if (ctx->geom == PF_GEOM_CYLINDRICAL) {
+            for (int i = 0; i + 1 < L.n0; ++i) rm.push_back(0.5 * (cxv[i] + cxv[i + 1]));
+            if ((rc = upload(ctx, &L.rmid, rm)) || (rc = upload(ctx, &L.rmid32, std::vector<float>(rm.begin(), rm.end()))))
+                return rc;
+        }
         if ((rc = upload(ctx, &L.cx, cix)) || (rc = upload(ctx, &L.cy, ciy)) || (rc = upload(ctx, &L.mx, mx)) ||
             (rc = upload(ctx, &L.my, my)) || (rc = upload(ctx, &L.wx, wx)) || (rc = upload(ctx, &L.wy, wy)) ||
             (rc = upload(ctx, &L.wx32, std::vector<float>(wx.begin(), wx.end()))) ||
@@ -824,80 +890,111 @@ static int mg_build(pf_ctx* ctx) {
 // x = the preconditioned residual z)
 template <typename T>
 struct MgView {
-    Axes3<T> g;
+    Axes3<T> g;   // 3D levels
+    Axes2<T> g2;  // 2D levels
+    int ncol, planes;
     const T *kh, *kv, *la, *lw, *lcp, *schur, *wx, *wy;
     const uint64_t* dmask;
     T *b, *x, *r;
     int pm;
 };
 
-template <typename T> static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0);
-template <> MgView<double> mg_view<double>(pf_ctx* ctx, int l, double* r0, double* z0) {
-    MgView<double> v;
-    v.pm = ctx->pm64;
+// FP64 / FP32 members of the context and of a level
+template <typename T> struct MgPick;
+template <> struct MgPick<double> {
+    static int pm(const pf_ctx* c) { return c->pm64; }
+    static const double* kh(const pf_ctx* c) { return c->kh; }
+    static const double* kv(const pf_ctx* c) { return c->kv; }
+    static const double* la(const pf_ctx* c) { return c->la; }
+    static const double* lw(const pf_ctx* c) { return c->lw; }
+    static const double* lcp(const pf_ctx* c) { return c->lcp; }
+    static const double* schur(const pf_ctx* c) { return c->schur; }
+    static double* r0(const pf_ctx* c) { return c->mg_r0; }
+    static const double* hy(const pf_ctx* c) { return c->hy; }
+    static const double* hz(const pf_ctx* c) { return c->hz; }
+    static const MgLevel& lv(const pf_ctx* c, int l) { return c->mg[l - 1]; }
+    static const double* hx(const MgLevel& L) { return L.hx; }
+    static const double* hyl(const MgLevel& L) { return L.hy; }
+    static const double* rmid(const MgLevel& L) { return L.rmid; }
+    static const double* kh(const MgLevel& L) { return L.kh; }
+    static const double* kv(const MgLevel& L) { return L.kv; }
+    static const double* la(const MgLevel& L) { return L.la; }
+    static const double* lw(const MgLevel& L) { return L.lw; }
+    static const double* lcp(const MgLevel& L) { return L.lcp; }
+    static const double* schur(const MgLevel& L) { return L.schur; }
+    static const double* wx(const MgLevel& L) { return L.wx; }
+    static const double* wy(const MgLevel& L) { return L.wy; }
+    static double* b(const MgLevel& L) { return L.b; }
+    static double* x(const MgLevel& L) { return L.x; }
+    static double* r(const MgLevel& L) { return L.r; }
+};
+template <> struct MgPick<float> {
+    static int pm(const pf_ctx* c) { return c->pm32; }
+    static const float* kh(const pf_ctx* c) { return c->kh32; }
+    static const float* kv(const pf_ctx* c) { return c->kv32; }
+    static const float* la(const pf_ctx* c) { return c->la32; }
+    static const float* lw(const pf_ctx* c) { return c->lw32; }
+    static const float* lcp(const pf_ctx* c) { return c->lcp32; }
+    static const float* schur(const pf_ctx* c) { return c->schur32; }
+    static float* r0(const pf_ctx* c) { return c->mg_r0_32; }
+    static const float* hy(const pf_ctx* c) { return c->hy32; }
+    static const float* hz(const pf_ctx* c) { return c->hz32; }
+    static const MgLevel& lv(const pf_ctx* c, int l) { return c->mg[l - 1]; }
+    static const float* hx(const MgLevel& L) { return L.hx32; }
+    static const float* hyl(const MgLevel& L) { return L.hy32; }
+    static const float* rmid(const MgLevel& L) { return L.rmid32; }
+    static const float* kh(const MgLevel& L) { return L.kh32; }
+    static const float* kv(const MgLevel& L) { return L.kv32; }
+    static const float* la(const MgLevel& L) { return L.la32; }
+    static const float* lw(const MgLevel& L) { return L.lw32; }
+    static const float* lcp(const MgLevel& L) { return L.lcp32; }
+    static const float* schur(const MgLevel& L) { return L.schur32; }
+    static const float* wx(const MgLevel& L) { return L.wx32; }
+    static const float* wy(const MgLevel& L) { return L.wy32; }
+    static float* b(const MgLevel& L) { return L.b32; }
+    static float* x(const MgLevel& L) { return L.x32; }
+    static float* r(const MgLevel& L) { return L.r32; }
+};
+
+template <typename T>
+static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
+    using S = MgPick<T>;
+    MgView<T> v{};
+    v.pm = S::pm(ctx);
+    v.planes = mg_planes(ctx);
     if (l == 0) {
-        v.g = axes3<double>(ctx);
-        v.kh = ctx->kh;
-        v.kv = ctx->kv;
-        v.la = ctx->la;
-        v.lw = ctx->lw;
-        v.lcp = ctx->lcp;
-        v.schur = ctx->schur;
+        if (ctx->dim == 3) v.g = axes3<T>(ctx);
+        else v.g2 = axes2<T>(ctx);
+        v.ncol = (int)(ctx->dim == 3 ? ctx->n[0] * ctx->n[1] : ctx->n[0]);
+        v.kh = S::kh(ctx);
+        v.kv = S::kv(ctx);
+        v.la = S::la(ctx);
+        v.lw = S::lw(ctx);
+        v.lcp = S::lcp(ctx);
+        v.schur = S::schur(ctx);
         v.wx = v.wy = nullptr;
         v.dmask = ctx->dmask;
         v.b = r0;
         v.x = z0;
-        v.r = ctx->mg_r0;
+        v.r = S::r0(ctx);
         return v;
     }
-    const MgLevel& L = ctx->mg[l - 1];
-    v.g = Axes3<double>{L.n0, L.n1, (int)ctx->n[2], L.hx, L.hy, ctx->hz};
-    v.kh = L.kh;
-    v.kv = L.kv;
-    v.la = L.la;
-    v.lw = L.lw;
-    v.lcp = L.lcp;
-    v.schur = L.schur;
-    v.wx = L.wx;
-    v.wy = L.wy;
+    const MgLevel& L = S::lv(ctx, l);
+    if (ctx->dim == 3) v.g = Axes3<T>{L.n0, L.n1, v.planes, S::hx(L), S::hyl(L), S::hz(ctx)};
+    else v.g2 = Axes2<T>{L.n0, v.planes, S::hx(L), S::hy(ctx), S::rmid(L)};
+    v.ncol = L.n0 * L.n1;
+    v.kh = S::kh(L);
+    v.kv = S::kv(L);
+    v.la = S::la(L);
+    v.lw = S::lw(L);
+    v.lcp = S::lcp(L);
+    v.schur = S::schur(L);
+    v.wx = S::wx(L);
+    v.wy = S::wy(L);
     v.dmask = L.dmask;
-    v.b = L.b;
-    v.x = L.x;
-    v.r = L.r;
-    return v;
-}
-template <> MgView<float> mg_view<float>(pf_ctx* ctx, int l, float* r0, float* z0) {
-    MgView<float> v;
-    v.pm = ctx->pm32;
-    if (l == 0) {
-        v.g = axes3<float>(ctx);
-        v.kh = ctx->kh32;
-        v.kv = ctx->kv32;
-        v.la = ctx->la32;
-        v.lw = ctx->lw32;
-        v.lcp = ctx->lcp32;
-        v.schur = ctx->schur32;
-        v.wx = v.wy = nullptr;
-        v.dmask = ctx->dmask;
-        v.b = r0;
-        v.x = z0;
-        v.r = ctx->mg_r0_32;
-        return v;
-    }
-    const MgLevel& L = ctx->mg[l - 1];
-    v.g = Axes3<float>{L.n0, L.n1, (int)ctx->n[2], L.hx32, L.hy32, ctx->hz32};
-    v.kh = L.kh32;
-    v.kv = L.kv32;
-    v.la = L.la32;
-    v.lw = L.lw32;
-    v.lcp = L.lcp32;
-    v.schur = L.schur32;
-    v.wx = L.wx32;
-    v.wy = L.wy32;
-    v.dmask = L.dmask;
-    v.b = L.b32;
-    v.x = L.x32;
-    v.r = L.r32;
+    v.b = S::b(L);
+    v.x = S::x(L);
+    v.r = S::r(L);
     return v;
 }
 
@@ -908,11 +1005,26 @@ static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
         launch_apply<T, MODE_RESG, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv);
         return;
     }
+    if (ctx->dim == 2) {
+        // one block per chunk of KC rows of the layer axis, one thread per node of axis 0;
+        // short chunks on the (latency-bound) coarse levels
+        const int n0 = v.g2.n0, n1 = v.g2.n1;
+        int KC = 64;
+        while (KC > 1 && (n1 + KC - 1) / KC < 256) KC >>= 1;
+        hipLaunchKernelGGL((k_apply2d<T, MODE_RESG, false, false>), dim3((n1 + KC - 1) / KC), dim3(n0),
+                           sizeof(T) * 4 * (size_t)(n0 + 1), ctx->stream, v.g2, KC, (const T*)v.x, (const T*)nullptr,
+                           (const T*)nullptr, (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv,
+                           ctx->st, ctx->partials);
+        return;
+    }
     const int n0 = v.g.n0, n1 = v.g.n1, n2 = v.g.n2;
     int BY = std::max(2, std::min(16, 512 / n0));
     while (BY > 2 && (BY * n0 > 1024 || tile_lds(BY, n0) > 65536)) --BY;
-    const int KC = 8;
     const int ytiles = (n1 + BY - 2) / (BY - 1);
+    // coarse levels are latency-bound: shorter z-marches (down to 1 plane per block)
+    // until the grid has about 4 blocks per CU
+    int KC = 8;
+    while (KC > 1 && (long long)ytiles * ((n2 + KC - 1) / KC) < 1024) KC >>= 1;
     const int nb = ytiles * ((n2 + KC - 1) / KC), grid = (nb + 7) / 8 * 8;
     const size_t shm = tile_lds(BY, n0) / sizeof(double) * sizeof(T);
     hipLaunchKernelGGL((k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream, v.g,
@@ -942,8 +1054,14 @@ static void mg_smooth_m(pf_ctx* ctx, dim3 grid, int threads, size_t lds, const P
 template <typename T>
 static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, const T* q, const T* xin, T* xout,
                       const T* rp) {
-    const int ncol = v.g.n0 * v.g.n1, n = v.g.n2;
-    const PartShape ps = part_shape<T>(v.pm, n, ncol);
+    const int ncol = v.ncol, n = v.planes;
+    PartShape ps = part_shape<T>(v.pm, n, ncol);
+    // small coarse levels: fewer columns per block, so that the few columns spread over
+    // at least 64 blocks (larger levels keep full blocks, which measured faster)
+    while (ps.CB > 4 && (ncol + ps.CB - 1) / ps.CB < 64) {
+        ps.CB >>= 1;
+        ps.threads = (ps.CB * ps.P + 63) / 64 * 64;
+    }
     const dim3 grid((ncol + ps.CB - 1) / ps.CB);
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
     const T om = T(ctx->mg_omega);
@@ -960,36 +1078,57 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
 template <typename T>
 static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
     const int L = (int)ctx->mg.size() + 1;
+    // profiled chunks: level-0 kernels by kind, all coarse-level work as one record
+    auto cat = [](int l, int kind) { return l == 0 ? kind : -1; };
+    auto open = [&](int c) { return c >= 0 ? prof_open(ctx, c) : -1; };
+    int hc = -1;
     for (int l = 0; l < L; ++l) {
         const MgView<T> v = mg_view<T>(ctx, l, r, z);
         const bool last = l == L - 1;
         const int sweeps = last ? ctx->mg_coarse_sweeps : 1;
+        if (l == 1) hc = prof_open(ctx, PF_KCAT_MG_COARSE);
         for (int s = 0; s < sweeps; ++s) {
             const int f = (l == 0 && last && s == sweeps - 1) ? fin : 0;
-            if (s == 0) {
-                mg_smooth<T>(ctx, v, l == 0 && upd, f, v.b, q, nullptr, v.x, r);
-            } else {
+            if (s > 0) {
+                const int h = open(cat(l, PF_KCAT_MG_RESIDUAL));
                 mg_residual<T>(ctx, l, v);
-                mg_smooth<T>(ctx, v, false, f, v.r, nullptr, v.x, v.x, r);
+                prof_close(ctx, h);
             }
+            const int h = open(cat(l, PF_KCAT_MG_SMOOTH));
+            if (s == 0)
+                mg_smooth<T>(ctx, v, l == 0 && upd, f, v.b, q, nullptr, v.x, r);
+            else
+                mg_smooth<T>(ctx, v, false, f, v.r, nullptr, v.x, v.x, r);
+            prof_close(ctx, h);
         }
         if (!last) {
+            int h = open(cat(l, PF_KCAT_MG_RESIDUAL));
             mg_residual<T>(ctx, l, v);
+            prof_close(ctx, h);
             const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
             const MgTransfer tr = mg_transfer(ctx, (size_t)l);
+            h = open(cat(l, PF_KCAT_MG_TRANSFER));
             hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0, ctx->stream,
                                tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
+            prof_close(ctx, h);
         }
     }
     for (int l = L - 2; l >= 0; --l) {
+        if (l == 0) prof_close(ctx, hc);
         const MgView<T> v = mg_view<T>(ctx, l, r, z);
         const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
         const MgTransfer tr = mg_transfer(ctx, (size_t)l);
-        const long long nf = (long long)v.g.n0 * v.g.n1 * v.g.n2;
+        const long long nf = (long long)v.ncol * v.planes;
+        int h = open(cat(l, PF_KCAT_MG_TRANSFER));
         hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid1(nf, 256, 4096)), dim3(256), 0, ctx->stream, tr, c.wx,
                            c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
+        prof_close(ctx, h);
+        h = open(cat(l, PF_KCAT_MG_RESIDUAL));
         mg_residual<T>(ctx, l, v);
+        prof_close(ctx, h);
+        h = open(cat(l, PF_KCAT_MG_SMOOTH));
         mg_smooth<T>(ctx, v, false, l == 0 ? fin : 0, v.r, nullptr, v.x, v.x, r);
+        prof_close(ctx, h);
     }
 }
 
@@ -1060,36 +1199,27 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
         // event-timed launches, no graph: events around every kernel of a chunk, one
         // synchronisation per chunk; only iterations that ran (not the early exits after
         // the stop) are counted
-        std::vector<hipEvent_t> ev(3 * (size_t)chunk);
-        for (auto& e : ev) HIPCK(hipEventCreate(&e));
         int done_iters = 0;
         for (;;) {
             for (int c = 0; c < chunk; ++c) {
                 T* pin = pb[c & 1];
                 T* pout = pb[(c + 1) & 1];
-                hipEventRecord(ev[3 * c], ctx->stream);
+                ctx->prof_iter = c;
+                int h = prof_open(ctx, PF_KCAT_APPLY);
                 launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
-                hipEventRecord(ev[3 * c + 1], ctx->stream);
+                prof_close(ctx, h);
+                h = prof_open(ctx, PF_KCAT_PRECOND);
                 update(pout);
-                hipEventRecord(ev[3 * c + 2], ctx->stream);
+                prof_close(ctx, h);
             }
+            ctx->prof_iter = -1;
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(ctx->hst, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
             HIPCK(hipStreamSynchronize(ctx->stream));
-            const int ran = std::min(chunk, ctx->hst->iters - done_iters);
-            for (int c = 0; c < ran; ++c) {
-                float t1 = 0.f, t2 = 0.f;
-                hipEventElapsedTime(&t1, ev[3 * c], ev[3 * c + 1]);
-                hipEventElapsedTime(&t2, ev[3 * c + 1], ev[3 * c + 2]);
-                ctx->kt[0] += 1e-3 * t1;
-                ctx->kt[1] += 1e-3 * t2;
-                ctx->kc[0] += 1;
-                ctx->kc[1] += 1;
-            }
+            prof_collect(ctx, std::min(chunk, ctx->hst->iters - done_iters));
             done_iters = ctx->hst->iters;
             if (ctx->hst->done) break;
         }
-        for (auto& e : ev) hipEventDestroy(e);
     } else if (!ctx->graphs) {
         // plain launches, one state read-back per chunk
         for (;;) {
@@ -1157,8 +1287,10 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     const long long N = ctx->N;
     const int gb = grid1(N, 256, 4096);
     auto t0 = std::chrono::steady_clock::now();
-    ctx->kt[0] = ctx->kt[1] = 0.;
-    ctx->kc[0] = ctx->kc[1] = 0;
+    for (int i = 0; i < 8; ++i) {
+        ctx->kt[i] = 0.;
+        ctx->kc[i] = 0;
+    }
     // start vector with the Dirichlet values
     if (opts->x0 == PF_X0_GIVEN) {
         if (!x_inout) return fail(ctx, PF_EINVAL, "x_inout required with PF_X0_GIVEN");
@@ -1426,7 +1558,7 @@ int pf_rhs_host(pf_ctx* ctx, double* b_out) {
 
 int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
     if (!ctx) return PF_EINVAL;
-    for (int i = 0; i < n && i < 2; ++i) {
+    for (int i = 0; i < n && i < 8; ++i) {
         if (seconds) seconds[i] = ctx->kc[i] ? ctx->kt[i] / ctx->kc[i] : 0.;
         if (counts) counts[i] = ctx->kc[i];
     }

@@ -18,6 +18,7 @@ PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE = 0, 1, 2
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
 PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3
+KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KCAT_MG_COARSE = range(6)
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
@@ -235,10 +236,11 @@ class FemContext:
         self._ck(self.L.pf_rhs_host(self.h, _p(b)), "pf_rhs_host")
         return b
 
-    def kernel_times(self):
-        s = np.zeros(2)
-        c = np.zeros(2, dtype=np.int64)
-        self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), 2), "pf_kernel_times")
+    def kernel_times(self, n=2):
+        """(average seconds, timed launches) per kernel class KCAT_* of the last profiled solve."""
+        s = np.zeros(n)
+        c = np.zeros(n, dtype=np.int64)
+        self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), int(n)), "pf_kernel_times")
         return s, c
 
     def set_launch(self, rows, planes):

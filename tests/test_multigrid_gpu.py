@@ -105,7 +105,7 @@ def test_mg_iterations_on_layered_stack():
     c.close()
 
 
-def test_mg_needs_partitioned_line_solve_and_3d():
+def test_mg_needs_partitioned_line_solve():
     g, axes, kh, kv, q, dn, dv = _rand_problem((9, 7, 11), seed=2)
     c = _ctx(axes, kh, kv, q, dn, dv)
     c.set_line_segment(-1)
@@ -115,10 +115,57 @@ def test_mg_needs_partitioned_line_solve_and_3d():
     T, st = c.solve(x0=np.full(g.nnodes, 300.0), rtol=1e-10, precond=PRECOND_MG)
     assert st.rel_resid <= 1e-10
     c.close()
-    c2 = FemContext()
-    c2.set_grid([np.linspace(0, 1, 9), np.linspace(0, 1, 7)])
-    c2.set_coeffs(np.ones(48), np.ones(48))
-    c2.set_dirichlet(np.arange(9), np.zeros(9))
-    with pytest.raises(FemError):
-        c2.solve(x0=np.zeros(63), precond=PRECOND_MG)
-    c2.close()
+
+
+def _rand_problem_2d(shape, cyl, seed):
+    rng = np.random.default_rng(seed)
+    axes = [np.cumsum(np.r_[0.0, rng.uniform(0.05, 2.0, n - 1)]) for n in shape]
+    g = O.Grid(axes, cyl=cyl)
+    E = g.nelems
+    kh = rng.uniform(0.5, 150.0, E)
+    kv = rng.uniform(0.5, 150.0, E)
+    q = rng.uniform(0, 1e16, E)
+    dn = np.r_[np.arange(shape[0]), rng.choice(np.arange(shape[0], g.nnodes), 3, replace=False)]
+    dv = rng.uniform(280, 320, len(dn))
+    return g, axes, kh, kv, q, dn.astype(np.int64), dv
+
+
+def _ctx2(axes, cyl, kh, kv, q, dn, dv):
+    c = FemContext()
+    c.set_grid(axes, cylindrical=cyl)
+    c.set_coeffs(kh, kv)
+    c.set_source(q)
+    c.set_dirichlet(dn, dv)
+    return c
+
+
+@pytest.mark.parametrize("shape,cyl", [((40, 60), True), ((31, 45), False), ((5, 300), False), ((200, 41), True)])
+def test_vcycle_2d_matches_oracle(shape, cyl):
+    """2D (axis 0 coarsened, axis 1 the layer axis): one V-cycle vs the CPU statement."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem_2d(shape, cyl, seed=shape[0])
+    print("vcycle 2d", shape, cyl, flush=True)
+    c = _ctx2(axes, cyl, kh, kv, q, dn, dv)
+    vc = MG.VCycle(axes, kh, kv, dn, cyl=cyl)
+    r = np.random.default_rng(6).uniform(-1, 1, g.nnodes)
+    r[dn] = 0.0
+    z_ref = vc.apply(r)
+    z = c.precond_apply(r, PF_FP64, PRECOND_MG)
+    assert c.multigrid_levels() == len(vc.levels)
+    scale = np.abs(z_ref).max()
+    assert np.abs(z - z_ref).max() <= 1e-10 * scale, np.abs(z - z_ref).max() / scale
+    c.close()
+
+
+@pytest.mark.parametrize("shape,cyl", [((40, 60), True), ((31, 45), False), ((5, 1100), False), ((200, 41), True)])
+@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_RELIABLE])
+def test_mg_solve_2d_matches_band_cholesky(shape, cyl, precision):
+    g, axes, kh, kv, q, dn, dv = _rand_problem_2d(shape, cyl, seed=7)
+    print("mg solve 2d", shape, cyl, precision, flush=True)
+    A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
+    T_ref = O.band_cholesky_solve(A)
+    c = _ctx2(axes, cyl, kh, kv, q, dn, dv)
+    T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-10, precond=PRECOND_MG)
+    assert st.rel_resid <= 1e-10
+    rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
+    assert rel <= 1e-7, (precision, rel)
+    c.close()

@@ -154,3 +154,53 @@ def test_vcycle_spd_on_random_anisotropic_grid():
 
     assert min_eig(0.65) > 0.0
     assert min_eig(0.75) < 0.0
+
+
+@pytest.mark.parametrize("shape", [(9, 5), (12, 7), (4, 6)])
+def test_coarse_operator_is_galerkin_2d(shape):
+    """2D Cartesian: same nested-space identity with axis 0 coarsened."""
+    rng = np.random.default_rng(sum(shape))
+    axes = [np.cumsum(np.r_[0.0, rng.uniform(0.1, 2.0, n - 1)]) for n in shape]
+    kh = np.repeat(rng.uniform(0.5, 150.0, shape[1] - 1), shape[0] - 1)
+    kv = np.repeat(rng.uniform(0.5, 150.0, shape[1] - 1), shape[0] - 1)
+    vc = MG.VCycle(axes, kh, kv, np.arange(shape[0]))
+    assert len(vc.levels) >= 2
+    for l, P in enumerate(vc.P):
+        F, Cl = vc.levels[l], vc.levels[l + 1]
+        G = (P.T @ F.A @ P).toarray()
+        Ac = Cl.A.toarray()
+        free = ~Cl.dmask
+        scale = np.abs(Ac[np.ix_(free, free)]).max()
+        assert np.abs(G[np.ix_(free, free)] - Ac[np.ix_(free, free)]).max() <= 1e-12 * scale
+
+
+def test_cylindrical_coarse_conductivity_keeps_radial_conductance():
+    """Cylindrical: the coarse conductivity times the coarse mid radius and width equals the
+    sum of k r h over the fine elements it covers (the weight the element matrix carries)."""
+    axes = [np.array([0.0, 1.0, 3.0, 4.0, 6.0]), np.array([0.0, 1.0, 2.0])]
+    kh = np.arange(1.0, 9.0)
+    vc = MG.VCycle(axes, kh, 2.0 * kh, np.arange(5), cyl=True)
+    C = vc.levels[1]
+    assert C.n == [3, 3]
+    r = 0.5 * (axes[0][1:] + axes[0][:-1])
+    h = np.diff(axes[0])
+    want = (kh[0] * r[0] * h[0] + kh[1] * r[1] * h[1]) / (1.5 * 3.0)   # coarse element [0, 3]: r 1.5, h 3
+    assert C.kh[0] == pytest.approx(want, rel=1e-14)
+
+
+def test_vcycle_spd_2d_cylindrical():
+    rng = np.random.default_rng(5)
+    shape = (11, 9)
+    axes = [np.cumsum(np.r_[0.0, rng.uniform(0.05, 2.0, n - 1)]) for n in shape]
+    E = (shape[0] - 1) * (shape[1] - 1)
+    kh = rng.uniform(0.5, 150.0, E)
+    kv = rng.uniform(0.5, 150.0, E)
+    vc = MG.VCycle(axes, kh, kv, np.r_[np.arange(shape[0]), [30, 47]], cyl=True)
+    idx = np.nonzero(~vc.levels[0].dmask)[0]
+    Z = np.zeros((len(idx), len(idx)))
+    for c, j in enumerate(idx):
+        e = np.zeros(vc.levels[0].N)
+        e[j] = 1.0
+        Z[:, c] = vc.apply(e)[idx]
+    assert np.abs(Z - Z.T).max() <= 1e-12 * np.abs(Z).max()
+    assert np.linalg.eigvalsh(0.5 * (Z + Z.T)).min() > 0.0
