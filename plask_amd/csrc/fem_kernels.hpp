@@ -212,47 +212,57 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     for (int r = tid; r < 8 * BY; r += blockDim.x) oc[r * PX] = T(0);
 
     // Global loads of one step, issued one step ahead (software prefetch): node values
-    // of plane k (own row and, for the last thread row, row j+1) and the element data
-    // of layer k-1.  With one 1024-thread block per CU nothing else hides HBM latency.
+    // of plane k (own row and, for the last thread row, row j+1), the element data and
+    // the height of layer k-1, and (residual modes) the load vector of plane k.  The
+    // loads are branch-free -- lanes outside the grid read a clamped valid address and
+    // ignore the value -- so that the compiler's wait counts before the next step's use
+    // cover only this group and the prefetch stays in flight across the barrier.
+    const uint32_t* dmask32 = reinterpret_cast<const uint32_t*>(dmask);
+    const int jc = min(max(j, 0), n1 - 1), jxc = min(max(j + 1, 0), n1 - 1);
+    const int ejc = min(max(j, 0), n1 - 2), etx = min(tx, n0 - 2);
     struct Raw {
-        T z, p, zx, px, kxy, kz, x;
-        bool d, dx;
+        T z, p, zx, px, kxy, kz, x, f, hz;
+        uint32_t dw, dwx;
     };
     auto gload = [&](int k, Raw& R) {
-        if (nvalid) {
-            const long long node = (long long)k * plane + (long long)j * n0 + tx;
+        const int kc = min(k, n2 - 1);
+        const long long node = (long long)kc * plane + (long long)jc * n0 + tx;
+        if (FUSEP) {
+            R.z = zin[node];
+            R.p = pin[node];
+            R.x = xsol[node];
+        } else {
+            R.z = xin[node];
+        }
+        if (MODE != MODE_OP) R.f = F[node];
+        R.dw = dmask32[node >> 5];
+        if (ty == BY - 1) {
+            const long long nx = (long long)kc * plane + (long long)jxc * n0 + tx;
             if (FUSEP) {
-                R.z = zin[node];
-                R.p = pin[node];
-                if (owned && k >= kb && k < ke) R.x = xsol[node];
+                R.zx = zin[nx];
+                R.px = pin[nx];
             } else {
-                R.z = xin[node];
+                R.zx = xin[nx];
             }
-            R.d = dbit(dmask, node);
+            R.dwx = dmask32[nx >> 5];
         }
-        if (xrow) {
-            const long long node = (long long)k * plane + (long long)(j + 1) * n0 + tx;
-            if (FUSEP) {
-                R.zx = zin[node];
-                R.px = pin[node];
-            } else {
-                R.zx = xin[node];
-            }
-            R.dx = dbit(dmask, node);
-        }
-        if (evalid && k >= 1) {
-            const long long e = (long long)(k - 1) * (n0 - 1) * (n1 - 1) + (long long)j * (n0 - 1) + tx;
-            R.kxy = kh[e];
-            R.kz = kv[e];
-        }
+        const int ek = max(kc - 1, 0);
+        const long long e = (long long)ek * (n0 - 1) * (n1 - 1) + (long long)ejc * (n0 - 1) + etx;
+        R.kxy = kh[e];
+        R.kz = kv[e];
+        R.hz = g.hz[ek];
+    };
+    auto bit = [&](uint32_t w, int k, int jj) {
+        const long long node = (long long)k * plane + (long long)jj * n0 + tx;
+        return ((w >> (node & 31)) & 1u) != 0u;
     };
     // plane k values into xs[buf] (masked for MODE_OP), p_new stored for owned nodes
-    auto lstore = [&](int k, int buf, const Raw& R, T& own, bool& own_dir, bool store_p) {
+    auto lstore = [&](int k, int buf, const Raw& R, T& own, bool& own_dir, T& own_f, bool store_p) {
         T v = T(0);
         bool d = false;
         if (nvalid) {
             v = FUSEP ? R.z + T(beta) * R.p : R.z;
-            d = R.d;
+            d = bit(R.dw, k, j);
             if (FUSEP && store_p) {
                 const long long node = (long long)k * plane + (long long)j * n0 + tx;
                 pout[node] = v;
@@ -261,12 +271,13 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         }
         own = v;
         own_dir = d;
+        own_f = R.f;
         if (active) xs[(buf * (BY + 1) + ty) * PX + tx] = (MODE == MODE_OP && d) ? T(0) : v;
         if (active && ty == BY - 1) {
             T w = T(0);
             if (xrow) {
                 w = FUSEP ? R.zx + T(beta) * R.px : R.zx;
-                if (MODE == MODE_OP && R.dx) w = T(0);
+                if (MODE == MODE_OP && bit(R.dwx, k, j + 1)) w = T(0);
             }
             xs[(buf * (BY + 1) + BY) * PX + tx] = w;
         }
@@ -285,18 +296,20 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
 
     const int P0 = !live ? n2 : (kb > 0) ? kb - 1 : 0;
     int buf = 0;
-    T own_cur, own_next = T(0);
+    T own_cur, own_next = T(0), f_cur = T(0), f_next = T(0);
     bool dir_cur, dir_next = false;
     // PFD planes in flight: R1 = plane P + 1, R2 = plane P + 2 (PFD == 2)
     Raw R1, R2;
-    R1.z = R1.p = R1.zx = R1.px = R1.kxy = R1.kz = R1.x = T(0);
-    R1.d = R1.dx = false;
+    R1.z = R1.p = R1.zx = R1.px = R1.kxy = R1.kz = R1.x = R1.f = R1.hz = T(0);
+    R1.dw = R1.dwx = 0u;
     R2 = R1;
     __syncthreads();
-    gload(P0, R1);
-    lstore(P0, 0, R1, own_cur, dir_cur, FUSEP && P0 >= kb && owned);
-    if (P0 + 1 < n2) gload(P0 + 1, R1);
-    if (PFD == 2 && P0 + 2 < n2 && P0 + 2 <= ke) gload(P0 + 2, R2);
+    if (live) {
+        gload(P0, R1);
+        lstore(P0, 0, R1, own_cur, dir_cur, f_cur, FUSEP && P0 >= kb && owned);
+        gload(P0 + 1, R1);
+        if (PFD == 2) gload(P0 + 2, R2);
+    }
     __syncthreads();
     T xt_cur[4], xt_next[4] = {T(0), T(0), T(0), T(0)};
     if (active) plane_tf(0, xt_cur);
@@ -306,10 +319,10 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     // One barrier per plane: the corner contributions of plane P go to one of two
     // corner buffers (alternating by step) after the barrier of step P and are
     // gathered after the barrier of step P + 1.
-    T own_prev = T(0);
+    T own_prev = T(0), f_prev = T(0);
     bool dir_prev = false;
     int obuf = 0;
-    auto gather = [&](int Pg, int ob, T own_g, bool dir_g) {
+    auto gather = [&](int Pg, int ob, T own_g, bool dir_g, T f_g) {
         const T* o = oc + ob * 4 * BY * PX;
         T s = o[(0 * BY + ty) * PX + tx + 1] + o[(1 * BY + ty) * PX + tx] + o[(2 * BY + ty - 1) * PX + tx + 1] +
               o[(3 * BY + ty - 1) * PX + tx];
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         if (MODE == MODE_OP) {
             out = dir_g ? own_g : s;
         } else {
-            out = dir_g ? T(0) : (F[node] - s);
+            out = dir_g ? T(0) : (f_g - s);
         }
         y[node] = out;
         if (DOT && !dir_g) dot += (double)own_g * (double)out;
@@ -328,23 +341,24 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         T bot[4] = {T(0), T(0), T(0), T(0)}, top[4] = {T(0), T(0), T(0), T(0)};
         T a = T(0), b = T(0), c = T(0);
         if (has_next) {
-            lstore(P + 1, buf ^ 1, R1, own_next, dir_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
+            lstore(P + 1, buf ^ 1, R1, own_next, dir_next, f_next, FUSEP && (P + 1) >= kb && (P + 1) < ke && owned);
             if (evalid) {
-                const T hz = g.hz[P];
+                const T hz = R1.hz;
                 a = R1.kxy * (gxf * hz);
                 b = R1.kxy * (gyf * hz);
                 c = R1.kz * (gzf / hz);
             }
-            if (PFD == 2) {
-                R1 = R2;
-                if (P + 3 < n2 && P + 3 <= ke) gload(P + 3, R2);  // prefetch two steps ahead
-            } else if (P + 2 < n2 && P + 2 <= ke) {
-                gload(P + 2, R1);  // prefetch the next step
-            }
+        }
+        // prefetch (clamped planes past the chunk's top halo are read and unused)
+        if (PFD == 2) {
+            R1 = R2;
+            gload(P + 3, R2);
+        } else {
+            gload(P + 2, R1);
         }
         __syncthreads();
         // outputs of plane P - 1, whose corner contributions were written in the last step
-        if (P - 1 >= kb && owned) gather(P - 1, obuf ^ 1, own_prev, dir_prev);
+        if (P - 1 >= kb && owned) gather(P - 1, obuf ^ 1, own_prev, dir_prev, f_prev);
         if (has_next && active) {
             plane_tf(buf ^ 1, xt_next);
             // z stage: w[pq sigma] = T + B, w[pq delta] = T - B, scaled by the eigenvalues
@@ -383,14 +397,16 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         }
         own_prev = own_cur;
         dir_prev = dir_cur;
+        f_prev = f_cur;
         own_cur = own_next;
         dir_cur = dir_next;
+        f_cur = f_next;
         buf ^= 1;
         obuf ^= 1;
     }
     // drain: outputs of the last plane
     __syncthreads();
-    if (ke - 1 >= kb && owned) gather(ke - 1, obuf ^ 1, own_prev, dir_prev);
+    if (ke - 1 >= kb && owned) gather(ke - 1, obuf ^ 1, own_prev, dir_prev, f_prev);
     if (DOT) {
         double v[1] = {dot}, tot[1];
         if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
