@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 from plask_amd import workloads  # noqa: E402
 from plask_amd.fem import (KCAT_APPLY, KCAT_MG_COARSE, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH,  # noqa: E402
-                           KCAT_MG_TRANSFER, KCAT_PRECOND, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64,
+                           KCAT_MG_TRANSFER, KCAT_PRECOND, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PF_FP64_PREC32,
                            PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG, FemContext)
 
 
@@ -44,7 +44,7 @@ WEIGHTS = os.path.join(ROOT, "plask_amd", "data", "warmstart_64x64x128.npz")
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 # relative reduction asked of each FP32 run: restarted refinement needs deep inner solves,
 # the reliable-update form replaces the residual early and keeps the search direction
-INNER = {PF_FP64: 0.0, PF_FP32_REFINE: 1e-5, PF_FP32_RELIABLE: 1e-2}
+INNER = {PF_FP64: 0.0, PF_FP32_REFINE: 1e-5, PF_FP32_RELIABLE: 1e-2, PF_FP64_PREC32: 0.0}
 # with the multigrid preconditioner each iteration reduces the residual ~10x, so a kept
 # direction pays off only over a longer run
 INNER_MG_RELIABLE = 1e-3
@@ -227,7 +227,7 @@ def main():
         T = ctx.solution()
         f = dict(method=name, solves_per_s=1.0 / t, ms=1e3 * t, iters=st.iters, outer=st.outer,
                  rel_resid=st.rel_resid)
-        if prec != PF_FP64:
+        if prec in (PF_FP32_REFINE, PF_FP32_RELIABLE):
             f["inner_rtol"] = inner
         if T64 is not None:
             f["max_rel_diff_vs_fp64_mg"] = float(np.abs(T - T64).max() / np.abs(T64).max())
@@ -258,6 +258,8 @@ def main():
                                           INNER[PF_FP32_REFINE])
         mixed["mg_fp32_reliable"], _ = form("fp32 operator, reliable updates, multigrid", PF_FP32_RELIABLE,
                                             PRECOND_MG, INNER_MG_RELIABLE)
+        mixed["mg_fp64_cg_fp32_precond"], _ = form("fp64 cg, multigrid preconditioner in fp32", PF_FP64_PREC32,
+                                                   PRECOND_MG)
         mixed["line_fp32_reliable"], _ = form("fp32 operator, reliable updates, line preconditioner",
                                               PF_FP32_RELIABLE, PRECOND_LINE, INNER[PF_FP32_RELIABLE])
         if os.path.exists(WEIGHTS):

@@ -69,6 +69,8 @@ typedef struct pf_ctx pf_ctx;
 #define PF_FP32_REFINE 1  /* FP32 operator + inner PCG, FP64 outer residual / update */
 #define PF_FP32_RELIABLE 2 /* as PF_FP32_REFINE, but one FP32 Krylov run: at each FP64 residual
                               replacement the search direction is kept (reliable updates) */
+#define PF_FP64_PREC32 3   /* FP64 vectors, operator and CG; the preconditioner (line solve or
+                              multigrid V-cycle) applied in FP32 */
 
 /* start vector */
 #define PF_X0_GIVEN 0 /* x_inout holds the start vector (its Dirichlet entries are overwritten) */
@@ -86,7 +88,7 @@ typedef struct pf_ctx pf_ctx;
 #define PF_NORM_B_FULL 2 /* ||B|| including the Dirichlet rows (B exactly as assembled by the reference) */
 
 typedef struct {
-    int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE */
+    int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE | PF_FP64_PREC32 */
     int precond;      /* 1 = Jacobi (NSPCG 'jac'); 2 = line block-Jacobi along the last (layer) axis
                          (the tridiagonal part of A per column, cf. NSPCG 'ljac'); 3 = multigrid
                          V-cycle (in-plane semicoarsening, damped line smoother; no reference

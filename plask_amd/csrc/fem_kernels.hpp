@@ -906,6 +906,42 @@ __global__ __launch_bounds__(256) void k_cg_start(long long n, const T* __restri
     if (grid_reduce<2>(v, partials, &st->counter[1], tot)) finish_start(st, tot);
 }
 
+// FP64 PCG with an FP32 preconditioner, before it: r -= alpha q (FP64, skipped when
+// upd is false) and the FP32 copy r32 of the residual.  Exits after the stop.
+__global__ __launch_bounds__(256) void k_cg_r_to_f32(long long n, bool upd, double* __restrict__ r,
+                                                     const double* __restrict__ q, float* __restrict__ r32,
+                                                     const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const double alpha = upd ? st->alpha : 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double ri = r[i];
+        if (upd) {
+            ri -= alpha * q[i];
+            r[i] = ri;
+        }
+        r32[i] = (float)ri;
+    }
+}
+
+// ... and after it: z = z32 in FP64, (r, z), (r, r); fin 1 ends the iteration, 2 starts a run
+__global__ __launch_bounds__(256) void k_cg_z_from_f32(long long n, int fin, const double* __restrict__ r,
+                                                       const float* __restrict__ z32, double* __restrict__ z,
+                                                       PcgState* __restrict__ st, double* __restrict__ partials) {
+    if (st->done) return;
+    double rz = 0., rr = 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double ri = r[i], zi = (double)z32[i];
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (grid_reduce<2>(v, partials, &st->counter[1], tot)) {
+        if (fin == 2) finish_start(st, tot);
+        else finish_update(st, tot);
+    }
+}
+
 // sum of squares of a vector -> st->bb
 template <typename T>
 __global__ __launch_bounds__(256) void k_norm2(long long n, const T* __restrict__ v_, PcgState* __restrict__ st,

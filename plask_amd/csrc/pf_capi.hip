@@ -51,6 +51,7 @@ struct MgLevel {
     double *la = nullptr, *lw = nullptr, *lcp = nullptr, *schur = nullptr;
     float *la32 = nullptr, *lw32 = nullptr, *lcp32 = nullptr, *schur32 = nullptr;
     int schur_m64 = -1, schur_m32 = -1;
+    int pm64 = 0, pm32 = 0;  // partition lengths of this level's line solve
     // right-hand side, iterate and residual of the level
     double *b = nullptr, *x = nullptr, *r = nullptr;
     float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr;
@@ -119,7 +120,8 @@ struct pf_ctx {
     double mg_omega = 0.65;     // damping of the line-Jacobi smoother
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
-    int graph_chunk = 0, graph_precond = 0;
+    int graph_chunk[2] = {0, 0}, graph_key[2] = {-1, -1};  // what graph64 / graph32 hold
+    bool prec32 = false;  // FP64 PCG with the preconditioner applied in FP32 (PF_FP64_PREC32)
 
     // profiling: HIP event pairs around kernels of profiled CG chunks, summed per
     // category (PF_KCAT_*) over the iterations that ran
@@ -182,6 +184,7 @@ static void drop_graphs(pf_ctx* ctx) {
     if (ctx->graph64) hipGraphExecDestroy(ctx->graph64);
     if (ctx->graph32) hipGraphExecDestroy(ctx->graph32);
     ctx->graph64 = ctx->graph32 = nullptr;
+    ctx->graph_key[0] = ctx->graph_key[1] = -1;
 }
 
 // Profiled chunks: prof_open records a start event for one kernel category (returns a
@@ -774,8 +777,12 @@ static int mg_refresh(pf_ctx* ctx) {
         hipLaunchKernelGGL(k_mg_coarsen_mask, dim3(grid1((L.N + 63) / 64, 256, 4096)), dim3(256), 0, ctx->stream, tr,
                            F ? F->dmask : ctx->dmask, L.dmask);
         const int ncol = L.n0 * L.n1;
+        // same segment lengths as the solve grid (longer segments on the small levels, with
+        // fewer separators in the serial Schur solve, measured slower: fewer waves per CU)
         const PartShape s64 = part_shape<double>(ctx->pm64, n2, ncol), s32 = part_shape<float>(ctx->pm32, n2, ncol);
         if (!s64.m || !s32.m) return fail(ctx, PF_EINVAL, "multigrid: line partition does not fit a coarse level");
+        L.pm64 = s64.m;
+        L.pm32 = s32.m;
         if (L.schur_m64 != s64.m) {
             if ((rc = dalloc(ctx, &L.schur, 3 * (size_t)std::max(s64.P - 1, 1) * ncol))) return rc;
             L.schur_m64 = s64.m;
@@ -913,6 +920,7 @@ template <> struct MgPick<double> {
     static const double* hy(const pf_ctx* c) { return c->hy; }
     static const double* hz(const pf_ctx* c) { return c->hz; }
     static const MgLevel& lv(const pf_ctx* c, int l) { return c->mg[l - 1]; }
+    static int pm(const MgLevel& L) { return L.pm64; }
     static const double* hx(const MgLevel& L) { return L.hx; }
     static const double* hyl(const MgLevel& L) { return L.hy; }
     static const double* rmid(const MgLevel& L) { return L.rmid; }
@@ -940,6 +948,7 @@ template <> struct MgPick<float> {
     static const float* hy(const pf_ctx* c) { return c->hy32; }
     static const float* hz(const pf_ctx* c) { return c->hz32; }
     static const MgLevel& lv(const pf_ctx* c, int l) { return c->mg[l - 1]; }
+    static int pm(const MgLevel& L) { return L.pm32; }
     static const float* hx(const MgLevel& L) { return L.hx32; }
     static const float* hyl(const MgLevel& L) { return L.hy32; }
     static const float* rmid(const MgLevel& L) { return L.rmid32; }
@@ -980,6 +989,7 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
         return v;
     }
     const MgLevel& L = S::lv(ctx, l);
+    v.pm = S::pm(L);
     if (ctx->dim == 3) v.g = Axes3<T>{L.n0, L.n1, v.planes, S::hx(L), S::hyl(L), S::hz(ctx)};
     else v.g2 = Axes2<T>{L.n0, v.planes, S::hx(L), S::hy(ctx), S::rmid(L)};
     v.ncol = L.n0 * L.n1;
@@ -1053,7 +1063,7 @@ static void mg_smooth_m(pf_ctx* ctx, dim3 grid, int threads, size_t lds, const P
 
 template <typename T>
 static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, const T* q, const T* xin, T* xout,
-                      const T* rp) {
+                      const T* rp, T omega = T(-1)) {
     const int ncol = v.ncol, n = v.planes;
     PartShape ps = part_shape<T>(v.pm, n, ncol);
     // small coarse levels: fewer columns per block, so that the few columns spread over
@@ -1064,7 +1074,7 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
     }
     const dim3 grid((ncol + ps.CB - 1) / ps.CB);
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
-    const T om = T(ctx->mg_omega);
+    const T om = omega > T(0) ? omega : T(ctx->mg_omega);
     switch (ps.m) {
         case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
         case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
@@ -1149,6 +1159,22 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const long long plane = ncol;
     const dim3 lgrid((ncol + 63) / 64, 1);
     const bool line = precond == 2, mg = precond == 3;
+    // FP64 CG with the preconditioner in FP32: r -> r32, z32 = M^-1 r32, z = z32 + dots
+    const bool p32 = sizeof(T) == 8 && ctx->prec32;
+    auto precond32 = [&](bool start) {
+        if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL(k_cg_r_to_f32, dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, !start,
+                               (double*)r, (const double*)q, ctx->r32, ctx->st);
+            if (mg) {
+                mg_vcycle<float>(ctx, ctx->r32, nullptr, ctx->z32, false, 0);
+            } else {  // exact line solve in FP32 (the smoother kernel with omega = 1)
+                const MgView<float> v0 = mg_view<float>(ctx, 0, ctx->r32, ctx->z32);
+                mg_smooth<float>(ctx, v0, false, 0, ctx->r32, nullptr, nullptr, ctx->z32, nullptr, 1.f);
+            }
+            hipLaunchKernelGGL(k_cg_z_from_f32, dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, start ? 2 : 1,
+                               (const double*)r, (const float*)ctx->z32, (double*)z, ctx->st, ctx->partials);
+        }
+    };
     const int pm = sizeof(T) == 8 ? ctx->pm64 : ctx->pm32;
     const T* schur = (const T*)(sizeof(T) == 8 ? (const void*)ctx->schur : (const void*)ctx->schur32);
     PartShape ps = part_shape<T>(pm, nplanes, ncol);
@@ -1159,7 +1185,9 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             hipLaunchKernelGGL((k_cg_line<T, true>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
                                r, (const T*)nullptr, la, lw, lcp, z, ctx->st, ctx->partials);
     };
-    if (mg)
+    if (p32)
+        precond32(true);
+    else if (mg)
         mg_vcycle<T>(ctx, r, nullptr, z, false, 2);
     else if (line)
         line_start();
@@ -1168,7 +1196,9 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
                            ctx->partials);
     auto update = [&](T* pout) {
         (void)pout;
-        if (mg)
+        if (p32)
+            precond32(false);
+        else if (mg)
             mg_vcycle<T>(ctx, r, q, z, true, 1);
         else if (line && pm)
             launch_part<T, false>(ctx, ps, ncol, nplanes, plane, r, q, la, lw, lcp, schur, z);
@@ -1230,7 +1260,11 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             if (ctx->hst->done) break;
         }
     } else {
-        if (!*gcache || ctx->graph_chunk != chunk || ctx->graph_precond != precond) {
+        // the cached graph of this precision holds one chunk of iterations for one
+        // preconditioner set-up; anything else is captured anew
+        const int gi = sizeof(T) == 8 ? 0 : 1;
+        const int key = precond + (p32 ? 16 : 0);
+        if (!*gcache || ctx->graph_chunk[gi] != chunk || ctx->graph_key[gi] != key) {
             if (*gcache) hipGraphExecDestroy(*gcache);
             *gcache = nullptr;
             hipGraph_t graph;
@@ -1239,8 +1273,8 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
             HIPCK(hipStreamEndCapture(ctx->stream, &graph));
             HIPCK(hipGraphInstantiate(gcache, graph, nullptr, nullptr, 0));
             hipGraphDestroy(graph);
-            ctx->graph_chunk = chunk;
-            ctx->graph_precond = precond;
+            ctx->graph_chunk[gi] = chunk;
+            ctx->graph_key[gi] = key;
         }
         for (;;) {
             HIPCK(hipGraphLaunch(*gcache, ctx->stream));
@@ -1341,15 +1375,19 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
             start = 1;
         }
     }
-    if (opts->precision == PF_FP64) {
+    if (opts->precision == PF_FP64 || opts->precision == PF_FP64_PREC32) {
+        if (opts->precision == PF_FP64_PREC32 && (precond == 1 || !ctx->pm32))
+            return fail(ctx, PF_EINVAL, "PF_FP64_PREC32 needs the partitioned line or the multigrid preconditioner");
         // restarts only if the recursive residual drifted from the true one
         for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
             int it = 0, stt = 0;
             double stop2 = (rtol * bnorm) * (rtol * bnorm);
-            if ((rc = run_cg<double>(ctx, precond, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->la,
-                                     ctx->lw, ctx->lcp, ctx->kh, ctx->kv, stop2,
-                                     maxit - total_iters, opts->check_every, &ctx->graph64, &it, &stt)))
-                return rc;
+            ctx->prec32 = opts->precision == PF_FP64_PREC32;
+            rc = run_cg<double>(ctx, precond, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->la,
+                                ctx->lw, ctx->lcp, ctx->kh, ctx->kv, stop2, maxit - total_iters, opts->check_every,
+                                &ctx->graph64, &it, &stt);
+            ctx->prec32 = false;
+            if (rc) return rc;
             total_iters += it;
             status = stt;
             if ((rc = true_residual(ctx, &rn))) return rc;

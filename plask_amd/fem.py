@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libplaskfem.so")
 PF_OK, PF_EINVAL, PF_ESTATE, PF_EDEVICE, PF_ESHAPE, PF_ENOCONV, PF_EBREAK, PF_ENONFINITE = 0, -1, -2, -3, -4, -5, -6, -8
 PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
 PF_THERMAL, PF_ELECTRICAL = 0, 1
-PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE = 0, 1, 2
+PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64_PREC32 = 0, 1, 2, 3
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
 PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3

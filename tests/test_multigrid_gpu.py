@@ -5,8 +5,8 @@ import pytest
 
 import oracle as O
 from oracle import multigrid as MG
-from plask_amd.fem import (PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PRECOND_LINE, PRECOND_MG, FemContext,
-                           FemError)
+from plask_amd.fem import (PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PF_FP64_PREC32, PRECOND_JACOBI, PRECOND_LINE,
+                           PRECOND_MG, FemContext, FemError)
 
 pytestmark = pytest.mark.gpu
 
@@ -64,7 +64,7 @@ def test_vcycle_matches_oracle(shape):
 
 
 @pytest.mark.parametrize("shape", [(9, 7, 11), (24, 20, 30), (33, 40, 70), (10, 6, 257), (128, 20, 12), (5, 4, 6)])
-@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE])
+@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64_PREC32])
 def test_mg_solve_matches_band_cholesky(shape, precision):
     g, axes, kh, kv, q, dn, dv = _rand_problem(shape, seed=11)
     print("mg solve", shape, precision, flush=True)
@@ -168,4 +168,42 @@ def test_mg_solve_2d_matches_band_cholesky(shape, cyl, precision):
     assert st.rel_resid <= 1e-10
     rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
     assert rel <= 1e-7, (precision, rel)
+    c.close()
+
+
+@pytest.mark.parametrize("shape", [(9, 7, 11), (24, 20, 30), (10, 6, 257)])
+def test_fp32_preconditioner_in_fp64_cg(shape):
+    """PF_FP64_PREC32: FP64 CG, line or multigrid preconditioner applied in FP32 -- same
+    FP64 solution; the Jacobi preconditioner is refused for this mode."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem(shape, seed=13)
+    print("fp64 cg, fp32 preconditioner", shape, flush=True)
+    A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
+    T_ref = O.band_cholesky_solve(A)
+    c = _ctx(axes, kh, kv, q, dn, dv)
+    for pc in (PRECOND_LINE, PRECOND_MG):
+        T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=PF_FP64_PREC32, rtol=1e-10, precond=pc)
+        assert st.rel_resid <= 1e-10
+        assert np.abs(T - T_ref).max() / np.abs(T_ref).max() <= 1e-7
+    with pytest.raises(FemError):
+        c.solve(x0=np.full(g.nnodes, 300.0), precision=PF_FP64_PREC32, precond=PRECOND_JACOBI)
+    c.close()
+
+
+def test_graph_cache_follows_the_preconditioner():
+    """Cached iteration graphs are kept per precision and preconditioner: alternating
+    FP64 / FP32 solves with different preconditioners give each form's own result."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem((24, 20, 30), seed=17)
+    print("graph cache", flush=True)
+    A = O.assemble(g, 0, kh, kv, q, O.ST_BAND, dn, dv)
+    T_ref = O.band_cholesky_solve(A)
+    c = _ctx(axes, kh, kv, q, dn, dv)
+    seq = [(PF_FP64, PRECOND_LINE), (PF_FP32_REFINE, PRECOND_MG), (PF_FP64, PRECOND_MG), (PF_FP64, PRECOND_JACOBI),
+           (PF_FP32_REFINE, PRECOND_LINE), (PF_FP64, PRECOND_MG), (PF_FP64_PREC32, PRECOND_MG), (PF_FP64, PRECOND_LINE)]
+    iters = {}
+    for prec, pc in seq:
+        T, st = c.solve(x0=np.full(g.nnodes, 300.0), precision=prec, rtol=1e-10, precond=pc)
+        assert np.abs(T - T_ref).max() / np.abs(T_ref).max() <= 1e-7, (prec, pc)
+        iters.setdefault((prec, pc), []).append(st.iters)
+    for k, v in iters.items():
+        assert max(v) - min(v) <= 1, (k, v)
     c.close()
