@@ -312,7 +312,7 @@ int pf_get_currents(pf_ctx* ctx, double* cur, double* heat);
  * law beta.hpp:43-46) -- e.g. voltage/temperature sweeps over device variants -- in one
  * persistent kernel launch replayed from a hipGraph.  Each workgroup takes instances
  * from a largest-first work queue and runs the whole Picard loop on the device: junction
- * update, line-preconditioned CG (warm start), currents and loop error.  The reference
+ * update, line- or multigrid-preconditioned CG (warm start), currents and loop error.  The reference
  * runs such sweeps as separate processes (gui/plugins/launcher_batch.py:16). */
 typedef struct pf_batch pf_batch;
 
@@ -324,6 +324,8 @@ typedef struct {
     int stable;     /* CONVERGENCE_STABLE */
     int block;      /* threads per workgroup 256/512/1024; 0 = autotuned or 1024 */
     int nwg;        /* persistent workgroups; 0 = autotuned or one per CU */
+    int precond;    /* 2 (or 0) = line preconditioner, 3 = multigrid V-cycle (axis 0 coarsened per
+                       instance, damped line smoother, omega 0.65, 2 coarsest sweeps) */
 } pf_batch_opts;
 
 typedef struct {

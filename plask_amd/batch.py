@@ -8,7 +8,7 @@ from .fem import FemError, load_library
 
 class BatchOpts(C.Structure):
     _fields_ = [("maxloops", C.c_int), ("maxit", C.c_int), ("maxerr", C.c_double), ("rtol", C.c_double),
-                ("stable", C.c_int), ("block", C.c_int), ("nwg", C.c_int)]
+                ("stable", C.c_int), ("block", C.c_int), ("nwg", C.c_int), ("precond", C.c_int)]
 
 
 class BatchStat(C.Structure):
@@ -76,8 +76,9 @@ class ShockleyBatch:
         except Exception:
             pass
 
-    def opts(self, maxloops=30, maxit=100000, maxerr=0.05, rtol=1e-8, stable=False, block=0, nwg=0):
-        return BatchOpts(maxloops, maxit, maxerr, rtol, 1 if stable else 0, block, nwg)
+    def opts(self, maxloops=30, maxit=100000, maxerr=0.05, rtol=1e-8, stable=False, block=0, nwg=0, precond=2):
+        """precond 2: line preconditioner, 3: multigrid V-cycle per instance."""
+        return BatchOpts(maxloops, maxit, maxerr, rtol, 1 if stable else 0, block, nwg, int(precond))
 
     def autotune(self, **kw):
         o = self.opts(**kw)

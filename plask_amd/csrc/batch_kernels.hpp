@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mg_kernels.hpp"
+
 namespace pf {
 
 struct BatchInst {
@@ -20,7 +22,34 @@ struct BatchInst {
     int jlo, jhi;                // junction element rows [jlo, jhi); jlo == jhi: none
     double jheight, beta, js;
     int cyl;
+    int nlev;     // multigrid levels below the instance grid (coarse levels 1..nlev)
+    int lev_off;  // first of them in the level table
     int pad;
+};
+
+// One coarse multigrid level of an instance (axis 0 coarsened, every row of axis 1 kept);
+// offsets into the batch's level pools.  The transfer tables describe the step from the
+// next finer level: cx[I] = fine index of coarse node I; fine node i lies in coarse
+// interval [mx[i], mx[i] + 1] with right weight wx[i].
+struct BatchLevel {
+    int n0;
+    int cx_off, mx_off;  // into the int tables (cx) and the int / double tables (mx, wx)
+    int h0_off;          // element widths and mid radii of this level
+    long long node_off;  // level node pool (b, x, r, line factors)
+    long long elem_off;  // level element pool (conductivities)
+    long long mask_off;  // bit index of node 0 in the level Dirichlet mask
+};
+
+// Pools of the coarse levels of all instances (see BatchLevel).
+struct BatchMg {
+    const BatchLevel* lev;
+    const int* cx;
+    const int* mx;
+    const double* wx;
+    const double* h0;
+    const double* rm;
+    const uint64_t* mask;
+    double *kh, *kv, *b, *x, *r, *la, *lw, *lc;
 };
 
 struct BatchStats {
@@ -38,6 +67,9 @@ struct BatchParams {
     int maxloops, maxit;
     double maxerr, rtol;
     int stable;
+    int precond;  // 2 line, 3 multigrid V-cycle
+    double omega; // multigrid smoother damping
+    int coarse_sweeps;
     int pad;
 };
 
@@ -101,6 +133,166 @@ __device__ inline double row2d(const BatchInst& I, const double* __restrict__ h0
     return RES ? -s : s;
 }
 
+// One level of an instance as the workgroup sees it (level 0: the instance grid).
+struct LView {
+    int n0, n1, cyl;
+    const double* h0;  // element widths along axis 0 (and mid radii rm)
+    const double* h1;  // element heights along axis 1
+    const double* rm;
+    double* kh;
+    double* kv;
+    const uint64_t* mask;
+    long long mbase;  // bit index of node 0 in mask
+    double *la, *lw, *lc;
+};
+
+__device__ __forceinline__ bool lbit(const LView& V, long long node) { return dbit(V.mask, V.mbase + node); }
+
+// Node-centric operator row on a level (same element matrices as row2d):
+// RES = false: (M A0 M x + (I - M) x)_node, RES = true: (M (0 - A0 x))_node.
+template <bool RES>
+__device__ inline double row2d_l(const LView& V, const double* __restrict__ x, int i, int k) {
+    const long long node = (long long)k * V.n0 + i;
+    const bool di = lbit(V, node);
+    const double xi = x[node];
+    if (di) return RES ? 0. : xi;
+    double s = 0.;
+    for (int c = 0; c < 4; ++c) {
+        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
+        if (ei < 0 || ek < 0 || ei >= V.n0 - 1 || ek >= V.n1 - 1) continue;
+        const long long e = (long long)ek * (V.n0 - 1) + ei;
+        const double w = V.h0[ei], h = V.h1[ek];
+        const double r = V.cyl ? V.rm[ei] : 1.;
+        const double kx = r * V.kh[e] * h / w, ky = r * V.kv[e] * w / h;
+        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
+        for (int b = 0; b < 4; ++b) {
+            const int bx = b & 1, by = b >> 1;
+            const long long nn = (long long)(ek + by) * V.n0 + ei + bx;
+            if (!RES && lbit(V, nn)) continue;
+            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
+            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
+            s += (kx * Sx * My + ky * Mx * Sy) * x[nn];
+        }
+    }
+    return RES ? -s : s;
+}
+
+// Thomas factors of the tridiagonal part of the level's matrix along axis 1 (per column)
+template <int BS>
+__device__ inline void line_factors_l(const LView& V) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < V.n0; i += BS) {
+        double cprev = 0., cpprev = 0.;
+        for (int k = 0; k < V.n1; ++k) {
+            const long long node = (long long)k * V.n0 + i;
+            double b = 0., c = 0.;
+            for (int cc = 0; cc < 4; ++cc) {
+                const int ei = i - 1 + (cc & 1), ek = k - 1 + (cc >> 1);
+                if (ei < 0 || ek < 0 || ei >= V.n0 - 1 || ek >= V.n1 - 1) continue;
+                const long long e = (long long)ek * (V.n0 - 1) + ei;
+                const double w = V.h0[ei], h = V.h1[ek];
+                const double rr = V.cyl ? V.rm[ei] : 1.;
+                const double kx1 = rr * V.kh[e] * h / w, ky1 = rr * V.kv[e] * w / h;
+                b += (kx1 + ky1) / 3.;
+                if (ek == k) c += (kx1 - 2. * ky1) / 6.;
+            }
+            const bool dk = lbit(V, node);
+            if (dk) b = 1.;
+            if (dk || k == V.n1 - 1 || lbit(V, node + V.n0)) c = 0.;
+            const double a = cprev, w = 1. / (b - a * cpprev), cp = c * w;
+            V.la[node] = a;
+            V.lw[node] = w;
+            V.lc[node] = cp;
+            cprev = c;
+            cpprev = cp;
+        }
+    }
+    __syncthreads();
+}
+
+// xout = xin + om M^-1 rhs (xin null: from zero; xin may be xout; rhs may be tmp),
+// the line solve per column with the forward sweep kept in tmp
+template <int BS>
+__device__ inline void line_sweep_l(const LView& V, const double* rhs, double* tmp, const double* xin, double* xout,
+                                    double om) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < V.n0; i += BS) {
+        double d = 0.;
+        for (int k = 0; k < V.n1; ++k) {
+            const long long node = (long long)k * V.n0 + i;
+            d = (rhs[node] - V.la[node] * d) * V.lw[node];
+            tmp[node] = d;
+        }
+        double zn = 0.;
+        for (int k = V.n1 - 1; k >= 0; --k) {
+            const long long node = (long long)k * V.n0 + i;
+            zn = tmp[node] - V.lc[node] * zn;
+            xout[node] = (xin ? xin[node] : 0.) + om * zn;
+        }
+    }
+    __syncthreads();
+}
+
+// r = b - A x on a level (zero at Dirichlet nodes, where b is zero)
+template <int BS>
+__device__ inline void residual_l(const LView& V, const double* b, const double* x, double* r) {
+    const long long N = (long long)V.n0 * V.n1;
+    for (long long n = threadIdx.x; n < N; n += BS) r[n] = b[n] + row2d_l<true>(V, x, (int)(n % V.n0), (int)(n / V.n0));
+    __syncthreads();
+}
+
+// b_c = P^T r_f (coarse Dirichlet nodes: 0) and x_f += P x_c (fine Dirichlet nodes skipped)
+template <int BS>
+__device__ inline void restrict_l(const LView& F, const LView& Cv, const int* cx, const int* mx, const double* wx,
+                                  const double* rf, double* bc) {
+    const long long Nc = (long long)Cv.n0 * Cv.n1;
+    for (long long n = threadIdx.x; n < Nc; n += BS) {
+        const int I = (int)(n % Cv.n0), k = (int)(n / Cv.n0);
+        double s = 0.;
+        if (!lbit(Cv, n)) {
+            const int ilo = I > 0 ? cx[I - 1] + 1 : 0, ihi = I < Cv.n0 - 1 ? cx[I + 1] - 1 : cx[I];
+            for (int i = ilo; i <= ihi; ++i) s += tweight(mx[i], wx[i], I) * rf[(long long)k * F.n0 + i];
+        }
+        bc[n] = s;
+    }
+    __syncthreads();
+}
+
+template <int BS>
+__device__ inline void prolong_l(const LView& F, const LView& Cv, const int* mx, const double* wx, const double* xc,
+                                 double* xf) {
+    const long long Nf = (long long)F.n0 * F.n1;
+    for (long long n = threadIdx.x; n < Nf; n += BS) {
+        if (lbit(F, n)) continue;
+        const int i = (int)(n % F.n0), k = (int)(n / F.n0);
+        const int I = mx[i], I1 = min(I + 1, Cv.n0 - 1);
+        const double a = wx[i];
+        xf[n] += (1. - a) * xc[(long long)k * Cv.n0 + I] + a * xc[(long long)k * Cv.n0 + I1];
+    }
+    __syncthreads();
+}
+
+// coarse conductivities of a level from the next finer one: width (x mid radius) means
+template <int BS>
+__device__ inline void coarsen_k_l(const LView& F, const LView& Cv, const int* cx) {
+    const long long Ec = (long long)(Cv.n0 - 1) * (Cv.n1 - 1);
+    for (long long e = threadIdx.x; e < Ec; e += BS) {
+        const int I = (int)(e % (Cv.n0 - 1));
+        const long long k = e / (Cv.n0 - 1);
+        double sa = 0., sh = 0., sv = 0.;
+        for (int ex = cx[I]; ex < cx[I + 1]; ++ex) {
+            const double a = F.h0[ex] * (F.cyl ? F.rm[ex] : 1.);
+            const long long f = k * (F.n0 - 1) + ex;
+            sa += a;
+            sh += a * F.kh[f];
+            sv += a * F.kv[f];
+        }
+        Cv.kh[e] = sh / sa;
+        Cv.kv[e] = sv / sa;
+    }
+    __syncthreads();
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __restrict__ insts, const int* __restrict__ order,
                                                          int ninst, unsigned int* __restrict__ queue, BatchParams prm,
@@ -111,7 +303,8 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                                                          double* __restrict__ P, double* __restrict__ Q,
                                                          double* __restrict__ Z, double* __restrict__ LA,
                                                          double* __restrict__ LW, double* __restrict__ LC,
-                                                         double* __restrict__ CUR, BatchStats* __restrict__ stats) {
+                                                         double* __restrict__ CUR, BatchStats* __restrict__ stats,
+                                                         BatchMg mg) {
     __shared__ double red[16];
     __shared__ int s_q;
     const int tid = threadIdx.x;
@@ -138,6 +331,50 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
         double* sv = kv + I.elem_off;
         const bool has_j = I.jhi > I.jlo;
         BatchStats S{};
+        // multigrid: level views (0 = this grid) and their b / x / scratch vectors; on the
+        // instance grid b is the CG residual, x the preconditioned residual z, the
+        // scratch q (free between the update and the next operator apply)
+        const bool mgon = prm.precond == 3;
+        const LView V0{n0, n1, I.cyl, h0 + I.h0_off, h1 + I.h1_off, rm + I.h0_off, sh, sv, dmask, I.node_off, la, lw, lc};
+        auto lview = [&](int l) -> LView {
+            if (l == 0) return V0;
+            const BatchLevel& B = mg.lev[I.lev_off + l - 1];
+            return LView{B.n0, n1, I.cyl, mg.h0 + B.h0_off, h1 + I.h1_off, mg.rm + B.h0_off, mg.kh + B.elem_off,
+                         mg.kv + B.elem_off, mg.mask, B.mask_off, mg.la + B.node_off, mg.lw + B.node_off,
+                         mg.lc + B.node_off};
+        };
+        auto lb = [&](int l) { return l ? mg.b + mg.lev[I.lev_off + l - 1].node_off : r; };
+        auto lx = [&](int l) { return l ? mg.x + mg.lev[I.lev_off + l - 1].node_off : z; };
+        auto lt = [&](int l) { return l ? mg.r + mg.lev[I.lev_off + l - 1].node_off : q; };
+        // z = V r: damped line sweep before and after each coarse correction, coarse sweeps
+        // on the coarsest level (the V-cycle of the single-instance solver)
+        auto vcycle = [&]() {
+            const int L = I.nlev + 1;
+            const double om = prm.omega;
+            for (int l = 0; l < L; ++l) {
+                const LView V = lview(l);
+                double *bl = lb(l), *xl = lx(l), *tl = lt(l);
+                line_sweep_l<BS>(V, bl, tl, nullptr, xl, om);
+                if (l == L - 1) {
+                    for (int sw = 1; sw < prm.coarse_sweeps; ++sw) {
+                        residual_l<BS>(V, bl, xl, tl);
+                        line_sweep_l<BS>(V, tl, tl, xl, xl, om);
+                    }
+                } else {
+                    residual_l<BS>(V, bl, xl, tl);
+                    const BatchLevel& B = mg.lev[I.lev_off + l];
+                    restrict_l<BS>(V, lview(l + 1), mg.cx + B.cx_off, mg.mx + B.mx_off, mg.wx + B.mx_off, tl, lb(l + 1));
+                }
+            }
+            for (int l = L - 2; l >= 0; --l) {
+                const LView V = lview(l);
+                double *bl = lb(l), *xl = lx(l), *tl = lt(l);
+                const BatchLevel& B = mg.lev[I.lev_off + l];
+                prolong_l<BS>(V, lview(l + 1), mg.mx + B.mx_off, mg.wx + B.mx_off, lx(l + 1), xl);
+                residual_l<BS>(V, bl, xl, tl);
+                line_sweep_l<BS>(V, tl, tl, xl, xl, om);
+            }
+        };
         // denominator: residual of the start field (Dirichlet values, zero elsewhere)
         for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
         __syncthreads();
@@ -164,40 +401,26 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                 }
                 __syncthreads();
             }
-            // line factors along axis 1 (tridiagonal part of A per column)
-            for (int i = tid; i < n0; i += BS) {
-                double cprev = 0., cpprev = 0.;
-                for (int k = 0; k < n1; ++k) {
-                    const long long node = (long long)k * n0 + i;
-                    double b = 0., c = 0.;
-                    for (int cc = 0; cc < 4; ++cc) {
-                        const int ei = i - 1 + (cc & 1), ek = k - 1 + (cc >> 1);
-                        if (ei < 0 || ek < 0 || ei >= n0 - 1 || ek >= n1 - 1) continue;
-                        const long long e = (long long)ek * (n0 - 1) + ei;
-                        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
-                        const double rr = I.cyl ? rm[I.h0_off + ei] : 1.;
-                        const double kx1 = rr * sh[e] * h / w, ky1 = rr * sv[e] * w / h;
-                        b += (kx1 + ky1) / 3.;
-                        if (ek == k) c += (kx1 - 2. * ky1) / 6.;
-                    }
-                    const bool dk = dbit(dmask, I.node_off + node);
-                    if (dk) b = 1.;
-                    if (dk || k == n1 - 1 || dbit(dmask, I.node_off + node + n0)) c = 0.;
-                    const double a = cprev, w = 1. / (b - a * cpprev), cp = c * w;
-                    la[node] = a;
-                    lw[node] = w;
-                    lc[node] = cp;
-                    cprev = c;
-                    cpprev = cp;
+            // line factors along axis 1 (tridiagonal part of A per column); with the
+            // multigrid preconditioner also the coarse conductivities and line factors
+            line_factors_l<BS>(V0);
+            if (mgon)
+                for (int l = 1; l <= I.nlev; ++l) {
+                    const LView F = lview(l - 1), Cv = lview(l);
+                    coarsen_k_l<BS>(F, Cv, mg.cx + mg.lev[I.lev_off + l - 1].cx_off);
+                    line_factors_l<BS>(Cv);
                 }
-            }
-            __syncthreads();
             // CG from the current potential
             for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
             __syncthreads();
             double rz = 0., rr = 0.;
             auto precond = [&]() {
                 double acc = 0.;
+                if (mgon) {
+                    vcycle();
+                    for (long long n = tid; n < N; n += BS) acc += r[n] * z[n];
+                    return wg_sum(acc, red);
+                }
                 for (int i = tid; i < n0; i += BS) {
                     double d = 0.;
                     for (int k = 0; k < n1; ++k) {

@@ -2143,6 +2143,8 @@ struct pf_batch {
     double *x = nullptr, *x0 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *la = nullptr,
            *lw = nullptr, *lc = nullptr, *cur = nullptr;
     BatchStats* dstats = nullptr;
+    // multigrid levels of all instances (pools described by BatchLevel)
+    BatchMg mg{};
     std::vector<long long> node_off;
     int tuned_block = 0, tuned_nwg = 0;
     std::vector<void*> allocs;
@@ -2276,6 +2278,63 @@ int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, con
         }
         dpos += ndir[i];
     }
+    // multigrid levels per instance: axis 0 coarsened while it has more than 3 nodes, all
+    // rows of axis 1 kept; the Dirichlet set is fixed, so the coarse masks are made here
+    std::vector<BatchLevel> levs;
+    std::vector<int> tcx, tmx;
+    std::vector<double> twx, lh0, lrm;
+    std::vector<uint64_t> lmask;
+    long long lnode = 0, lelem = 0, lbits = 0;
+    {
+        long long ax0_pos = 0;
+        for (int i = 0; i < ninst; ++i) {
+            BatchInst& I = inst[i];
+            const int m1 = n1[i];
+            std::vector<double> f(ax0 + ax0_pos, ax0 + ax0_pos + n0[i]);
+            ax0_pos += n0[i];
+            std::vector<char> fm((size_t)n0[i] * m1, 0);
+            for (long long nd = 0; nd < (long long)n0[i] * m1; ++nd) {
+                const long long g = I.node_off + nd;
+                fm[nd] = (char)((mask[g >> 6] >> (g & 63)) & 1ull);
+            }
+            I.lev_off = (int)levs.size();
+            I.nlev = 0;
+            while (f.size() > 3) {
+                std::vector<int> ci, mm;
+                std::vector<double> ww, cc;
+                mg_coarsen_axis(f, true, ci, mm, ww, cc);
+                BatchLevel B{};
+                B.n0 = (int)cc.size();
+                B.cx_off = (int)tcx.size();
+                tcx.insert(tcx.end(), ci.begin(), ci.end());
+                B.mx_off = (int)tmx.size();
+                tmx.insert(tmx.end(), mm.begin(), mm.end());
+                twx.insert(twx.end(), ww.begin(), ww.end());
+                B.h0_off = (int)lh0.size();
+                for (int k = 0; k + 1 < B.n0; ++k) {
+                    lh0.push_back(cc[k + 1] - cc[k]);
+                    lrm.push_back(0.5 * (cc[k] + cc[k + 1]));
+                }
+                B.node_off = lnode;
+                B.elem_off = lelem;
+                B.mask_off = lbits;
+                const int nf = (int)f.size();
+                std::vector<char> cm((size_t)B.n0 * m1, 0);
+                for (int k = 0; k < m1; ++k)
+                    for (int c = 0; c < B.n0; ++c) cm[(size_t)k * B.n0 + c] = fm[(size_t)k * nf + ci[c]];
+                lmask.resize((size_t)((lbits + (long long)B.n0 * m1 + 63) / 64), 0ull);
+                for (long long nd = 0; nd < (long long)B.n0 * m1; ++nd)
+                    if (cm[nd]) lmask[(lbits + nd) >> 6] |= 1ull << ((lbits + nd) & 63);
+                lnode += (long long)B.n0 * m1;
+                lelem += (long long)(B.n0 - 1) * (m1 - 1);
+                lbits += (long long)B.n0 * m1;
+                levs.push_back(B);
+                I.nlev++;
+                f = cc;
+                fm.swap(cm);
+            }
+        }
+    }
     // largest instances first (longest-processing-time order for the work queue)
     std::vector<int> order(ninst);
     for (int i = 0; i < ninst; ++i) order[i] = i;
@@ -2295,6 +2354,34 @@ int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, con
         (rc = balloc(b, &b->lw, N)) || (rc = balloc(b, &b->lc, N)) || (rc = balloc(b, &b->cur, 2 * E)) ||
         (rc = balloc(b, &b->dstats, ninst)))
         return rc;
+    {
+        BatchLevel* dlev = nullptr;
+        int *dcx = nullptr, *dmx = nullptr;
+        double *dwx = nullptr, *dh0 = nullptr, *drm = nullptr;
+        uint64_t* dlm = nullptr;
+        BatchMg& M = b->mg;
+        if ((rc = balloc(b, &dlev, levs.size())) || (rc = balloc(b, &dcx, tcx.size())) ||
+            (rc = balloc(b, &dmx, tmx.size())) || (rc = balloc(b, &dwx, twx.size())) ||
+            (rc = balloc(b, &dh0, lh0.size())) || (rc = balloc(b, &drm, lrm.size())) ||
+            (rc = balloc(b, &dlm, lmask.size())) || (rc = balloc(b, &M.kh, lelem)) || (rc = balloc(b, &M.kv, lelem)) ||
+            (rc = balloc(b, &M.b, lnode)) || (rc = balloc(b, &M.x, lnode)) || (rc = balloc(b, &M.r, lnode)) ||
+            (rc = balloc(b, &M.la, lnode)) || (rc = balloc(b, &M.lw, lnode)) || (rc = balloc(b, &M.lc, lnode)))
+            return rc;
+        if (!levs.empty()) BHIPCK(hipMemcpy(dlev, levs.data(), sizeof(BatchLevel) * levs.size(), hipMemcpyHostToDevice));
+        if (!tcx.empty()) BHIPCK(hipMemcpy(dcx, tcx.data(), sizeof(int) * tcx.size(), hipMemcpyHostToDevice));
+        if (!tmx.empty()) BHIPCK(hipMemcpy(dmx, tmx.data(), sizeof(int) * tmx.size(), hipMemcpyHostToDevice));
+        if (!twx.empty()) BHIPCK(hipMemcpy(dwx, twx.data(), sizeof(double) * twx.size(), hipMemcpyHostToDevice));
+        if (!lh0.empty()) BHIPCK(hipMemcpy(dh0, lh0.data(), sizeof(double) * lh0.size(), hipMemcpyHostToDevice));
+        if (!lrm.empty()) BHIPCK(hipMemcpy(drm, lrm.data(), sizeof(double) * lrm.size(), hipMemcpyHostToDevice));
+        if (!lmask.empty()) BHIPCK(hipMemcpy(dlm, lmask.data(), sizeof(uint64_t) * lmask.size(), hipMemcpyHostToDevice));
+        M.lev = dlev;
+        M.cx = dcx;
+        M.mx = dmx;
+        M.wx = dwx;
+        M.h0 = dh0;
+        M.rm = drm;
+        M.mask = dlm;
+    }
     BHIPCK(hipMemcpy(b->dinst, inst.data(), sizeof(BatchInst) * ninst, hipMemcpyHostToDevice));
     BHIPCK(hipMemcpy(b->dorder, order.data(), sizeof(int) * ninst, hipMemcpyHostToDevice));
     BHIPCK(hipMemcpy(b->h0, h0.data(), sizeof(double) * h0.size(), hipMemcpyHostToDevice));
@@ -2319,7 +2406,7 @@ static void batch_launch(pf_batch* b, int block, int nwg, const BatchParams& prm
 #define PF_BATCH_LAUNCH(BS)                                                                                         \
     hipLaunchKernelGGL((k_batch_shockley2d<BS>), dim3(nwg), dim3(BS), 0, b->stream, b->dinst, b->dorder, b->ninst, \
                        b->dqueue, prm, b->h0, b->h1, b->rm, b->kh, b->kv, b->dmask, b->x, b->r, b->p, b->q, b->z,   \
-                       b->la, b->lw, b->lc, b->cur, b->dstats)
+                       b->la, b->lw, b->lc, b->cur, b->dstats, b->mg)
     if (block == 256) PF_BATCH_LAUNCH(256);
     else if (block == 512) PF_BATCH_LAUNCH(512);
     else PF_BATCH_LAUNCH(1024);
@@ -2328,7 +2415,7 @@ static void batch_launch(pf_batch* b, int block, int nwg, const BatchParams& prm
 
 static bool same_prm(const BatchParams& a, const BatchParams& c) {
     return a.maxloops == c.maxloops && a.maxit == c.maxit && a.maxerr == c.maxerr && a.rtol == c.rtol &&
-           a.stable == c.stable;
+           a.stable == c.stable && a.precond == c.precond && a.omega == c.omega && a.coarse_sweeps == c.coarse_sweeps;
 }
 
 static int batch_run(pf_batch* b, int block, int nwg, const BatchParams& prm, double* seconds) {
@@ -2369,7 +2456,7 @@ int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int*
         return PF_ESTATE;
     }
     BHIPCK(hipSetDevice(b->device));
-    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, 0};
+    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, o->precond == 3 ? 3 : 2, 0.65, 2, 0};
     int props_cu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) props_cu = pr.multiProcessorCount;
@@ -2414,7 +2501,8 @@ int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_
     }
     int nwg = o->nwg ? o->nwg : (b->tuned_nwg ? b->tuned_nwg : std::min(b->ninst, props_cu));
     BatchParams prm{o->maxloops > 0 ? o->maxloops : 30, o->maxit > 0 ? o->maxit : 100000,
-                    o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable, 0};
+                    o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable,
+                    o->precond == 3 ? 3 : 2, 0.65, 2, 0};
     int rc = batch_run(b, block, nwg, prm, seconds);
     if (rc) return rc;
     if (V_out) BHIPCK(hipMemcpy(V_out, b->x, sizeof(double) * b->N, hipMemcpyDeviceToHost));

@@ -22,12 +22,12 @@ def _oracle(p, loops, maxerr):
     return O.electrical_compute(g, sh, sv, p["dir_nodes"], p["dir_vals"], [J], loops, maxerr=maxerr)
 
 
-@pytest.mark.parametrize("block", [256, 1024])
-def test_batch_matches_oracle_small(block):
+@pytest.mark.parametrize("block,precond", [(256, 2), (1024, 2), (256, 3), (1024, 3)])
+def test_batch_matches_oracle_small(block, precond):
     insts = workloads.shockley_cyl_batch(n_inst=6, seed0=3000, nr_range=(12, 40), nz_range=(30, 70))
     b = ShockleyBatch(insts)
-    print("batch", [len(p["axes"][0]) * len(p["axes"][1]) for p in insts], flush=True)
-    V, stats, t = b.solve(maxloops=8, maxerr=1e-30, rtol=1e-13, block=block)
+    print("batch", precond, [len(p["axes"][0]) * len(p["axes"][1]) for p in insts], flush=True)
+    V, stats, t = b.solve(maxloops=8, maxerr=1e-30, rtol=1e-13, block=block, precond=precond)
     for p, v, s in zip(insts, V, stats):
         Vr, cur, heat, sig, loops, err = _oracle(p, 8, 0.0)
         assert s["loops"] == 8 and s["status"] in (0, 2), s
@@ -42,3 +42,17 @@ def test_batch_autotune_and_convergence():
     V, stats, t2 = b.solve(maxloops=30)
     assert all(s["status"] in (0, 2) for s in stats)
     assert sum(s["status"] == 0 for s in stats) >= len(stats) // 2
+
+
+def test_batch_multigrid_fewer_iterations_same_potentials():
+    """The per-instance multigrid V-cycle: the same potentials as the line preconditioner
+    (to the CG tolerance) with far fewer CG iterations on larger instances."""
+    insts = workloads.shockley_cyl_batch(n_inst=8, seed0=3200, nr_range=(60, 160), nz_range=(100, 300))
+    b = ShockleyBatch(insts)
+    print("batch mg vs line", flush=True)
+    V2, s2, t2 = b.solve(maxloops=5, maxerr=1e-30, rtol=1e-11, precond=2)
+    V3, s3, t3 = b.solve(maxloops=5, maxerr=1e-30, rtol=1e-11, precond=3)
+    for v2, v3, a, c in zip(V2, V3, s2, s3):
+        assert a["status"] in (0, 2) and c["status"] in (0, 2), (a, c)
+        assert np.abs(v3 - v2).max() <= 1e-8 * np.abs(v2).max()
+        assert c["iters"] * 5 <= a["iters"], (c["iters"], a["iters"])
