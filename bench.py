@@ -182,6 +182,59 @@ def config1(steps, with_cpu):
     return out
 
 
+def config3(cpu_seconds):
+    """256 uneven 2D cylindrical Shockley instances (plask_amd.workloads.shockley_cyl_batch,
+    seeds 3000+i; 50x100 to 400x800 nodes): one persistent batched launch per solve
+    (multigrid-preconditioned CG inside each workgroup, Picard loop <= 30 on the device),
+    autotuned workgroup size / count.  CPU reference path (assembly + band Cholesky per
+    Picard loop, serial oracle) on a bounded sample: one loop timed on a few instances, a
+    cost model t = c N kd^2 (band Cholesky, kd = n_r + 1) fitted to them and summed over
+    the loops every instance ran on the GPU."""
+    import oracle as O
+    from plask_amd.batch import ShockleyBatch
+    insts = workloads.shockley_cyl_batch(n_inst=256)
+    nr = np.array([len(p["axes"][0]) for p in insts])
+    nz = np.array([len(p["axes"][1]) for p in insts])
+    b = ShockleyBatch(insts)
+    blk, nwg, t_tune = b.autotune(maxloops=30)
+    _, stats, t = b.solve(want_v=False, maxloops=30)
+    loops = np.array([s["loops"] for s in stats])
+    its = np.array([s["iters"] for s in stats])
+    status = np.array([s["status"] for s in stats])
+    b.close()
+    # CPU sample: one Picard loop (assembly + band Cholesky) on instances of increasing size
+    order = np.argsort(nr * nz)
+    cpu = []
+    spent = 0.0
+    for i in order[::32]:
+        if spent > cpu_seconds:
+            break
+        p = insts[i]
+        r, z = p["axes"]
+        g = O.Grid([r, z], cyl=True)
+        E0 = len(r) - 1
+        elems = np.arange(p["jlo"] * E0, p["jhi"] * E0)
+        bot = [[e % E0 + p["jlo"] * len(r), e % E0 + 1 + p["jlo"] * len(r)] for e in elems]
+        top = [[e % E0 + p["jhi"] * len(r), e % E0 + 1 + p["jhi"] * len(r)] for e in elems]
+        J = dict(elems=elems, bottom=bot, top=top, height=z[p["jhi"]] - z[p["jlo"]], beta=p["beta"], js=p["js"])
+        t0 = time.perf_counter()
+        O.electrical_compute(g, p["sh"].copy(), p["sv"].copy(), p["dir_nodes"], p["dir_vals"], [J], 1)
+        dt = time.perf_counter() - t0
+        spent += dt
+        cpu.append((int(i), float(dt)))
+        print("config 3 CPU sample: instance %d (%d x %d nodes): %.3f s per loop" % (i, nr[i], nz[i], dt),
+              file=sys.stderr, flush=True)
+    work = lambda i: float(nr[i] * nz[i]) * float(nr[i] + 1) ** 2
+    c = sum(dt for _, dt in cpu) / sum(work(i) for i, _ in cpu)
+    cpu_total = c * sum(loops[i] * work(i) for i in range(len(insts)))
+    return dict(workload="shockley_cyl_batch_256", instances=len(insts), nodes_total=int((nr * nz).sum()),
+                gpu_s=t, block=blk, workgroups=nwg, autotune_s=t_tune, picard_loops_mean=float(loops.mean()),
+                picard_loops_max=int(loops.max()), cg_iters_total=int(its.sum()),
+                converged=int((status == 0).sum()), stopped_at_maxloops=int((status == 2).sum()),
+                cpu_sample=[dict(instance=i, seconds_per_loop=dt) for i, dt in cpu],
+                cpu_estimate_s=cpu_total, cpu_cores=1, speedup_vs_cpu=cpu_total / t)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="plain kernel launches instead of hipGraph replay")
+    ap.add_argument("--config3", action="store_true",
+                    help="the batched 2D Shockley workload (256 instances) instead of the 3D benchmark")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed headline solves (for kernel-trace runs): no comparison forms, "
                          "profiled solves, CPU baseline or 2D case")
@@ -203,6 +258,9 @@ def main():
         print("bench.py: this build runs on one GPU only (the solve path has no multi-GPU "
               "decomposition); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config3:
+        print(json.dumps(config3(args.cpu_seconds)))
+        return
     size = tuple(int(v) for v in args.size.split(","))
     p = workloads.thermal_3d(seed=2, n=size)
     N = int(np.prod(size))

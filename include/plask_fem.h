@@ -324,8 +324,8 @@ typedef struct {
     int stable;     /* CONVERGENCE_STABLE */
     int block;      /* threads per workgroup 256/512/1024; 0 = autotuned or 1024 */
     int nwg;        /* persistent workgroups; 0 = autotuned or one per CU */
-    int precond;    /* 2 (or 0) = line preconditioner, 3 = multigrid V-cycle (axis 0 coarsened per
-                       instance, damped line smoother, omega 0.65, 2 coarsest sweeps) */
+    int precond;    /* 3 (or 0, the default) = multigrid V-cycle (axis 0 coarsened per instance,
+                       damped line smoother, omega 0.65, 2 coarsest sweeps); 2 = line preconditioner */
 } pf_batch_opts;
 
 typedef struct {

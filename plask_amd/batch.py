@@ -76,8 +76,8 @@ class ShockleyBatch:
         except Exception:
             pass
 
-    def opts(self, maxloops=30, maxit=100000, maxerr=0.05, rtol=1e-8, stable=False, block=0, nwg=0, precond=2):
-        """precond 2: line preconditioner, 3: multigrid V-cycle per instance."""
+    def opts(self, maxloops=30, maxit=100000, maxerr=0.05, rtol=1e-8, stable=False, block=0, nwg=0, precond=3):
+        """precond 3: multigrid V-cycle per instance (the default), 2: line preconditioner."""
         return BatchOpts(maxloops, maxit, maxerr, rtol, 1 if stable else 0, block, nwg, int(precond))
 
     def autotune(self, **kw):

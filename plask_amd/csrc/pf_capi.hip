@@ -2456,7 +2456,7 @@ int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int*
         return PF_ESTATE;
     }
     BHIPCK(hipSetDevice(b->device));
-    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, o->precond == 3 ? 3 : 2, 0.65, 2, 0};
+    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, o->precond == 2 ? 2 : 3, 0.65, 2, 0};
     int props_cu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) props_cu = pr.multiProcessorCount;
@@ -2502,7 +2502,7 @@ int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_
     int nwg = o->nwg ? o->nwg : (b->tuned_nwg ? b->tuned_nwg : std::min(b->ninst, props_cu));
     BatchParams prm{o->maxloops > 0 ? o->maxloops : 30, o->maxit > 0 ? o->maxit : 100000,
                     o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable,
-                    o->precond == 3 ? 3 : 2, 0.65, 2, 0};
+                    o->precond == 2 ? 2 : 3, 0.65, 2, 0};
     int rc = batch_run(b, block, nwg, prm, seconds);
     if (rc) return rc;
     if (V_out) BHIPCK(hipMemcpy(V_out, b->x, sizeof(double) * b->N, hipMemcpyDeviceToHost));
