@@ -16,6 +16,14 @@ template <typename T> static py::array_t<T> to_np(const std::vector<T>& v) {
     return a;
 }
 
+// Array arguments: one contiguous copy (numpy arrays, or any sequence numpy converts)
+// instead of pybind11's element-by-element sequence conversion, which costs ~20 ns per
+// value (~70 ms for two 1.7M-element conductivity arrays).
+template <typename T> using arr = py::array_t<T, py::array::c_style | py::array::forcecast>;
+template <typename T> static std::vector<T> vec(const arr<T>& a) {
+    return std::vector<T>(a.data(), a.data() + a.size());
+}
+
 PYBIND11_MODULE(_core, m) {
     m.doc() = "C++ solver cores over the gfx950 FEM C ABI";
     py::register_exception<ComputationError>(m, "ComputationError", PyExc_ArithmeticError);
@@ -35,7 +43,10 @@ PYBIND11_MODULE(_core, m) {
 
     py::class_<FemCore>(m, "FemCore")
         .def("set_mesh", &FemCore::set_mesh)
-        .def("set_dirichlet", &FemCore::set_dirichlet)
+        .def("set_dirichlet",
+             [](FemCore& s, const arr<int64_t>& nodes, const arr<double>& values) {
+                 s.set_dirichlet(vec(nodes), vec(values));
+             })
         .def("solution", [](FemCore& s) { return to_np(s.solution()); })
         .def("invalidate", &FemCore::invalidate)
         .def_property_readonly("initialized", &FemCore::initialized)
@@ -48,7 +59,9 @@ PYBIND11_MODULE(_core, m) {
     py::class_<ThermalCore, FemCore>(m, "ThermalCore")
         .def(py::init<int, bool, int, std::string>(), py::arg("dim"), py::arg("cylindrical") = false,
              py::arg("device") = 0, py::arg("id") = "thermal")
-        .def("set_conductivity", &ThermalCore::set_conductivity)
+        .def("set_conductivity",
+             [](ThermalCore& s, const arr<double>& k0h, const arr<double>& k0v, const arr<double>& ah,
+                const arr<double>& av) { s.set_conductivity(vec(k0h), vec(k0v), vec(ah), vec(av)); })
         .def("set_conductivity_callback",
              [](ThermalCore& s, py::function f) {
                  s.set_conductivity_callback([f](const double* t, double* kh, double* kv, int64_t n) {
@@ -62,7 +75,7 @@ PYBIND11_MODULE(_core, m) {
                      std::copy(b.data(), b.data() + n, kv);
                  });
              })
-        .def("set_heat", &ThermalCore::set_heat)
+        .def("set_heat", [](ThermalCore& s, const arr<double>& q) { s.set_heat(vec(q)); })
         .def("initialize", &ThermalCore::initialize)
         .def("compute", &ThermalCore::compute, py::arg("loops") = 0)
         .def_readwrite("inittemp", &ThermalCore::inittemp)
@@ -74,10 +87,21 @@ PYBIND11_MODULE(_core, m) {
     py::class_<ElectricalCore, FemCore>(m, "ElectricalCore")
         .def(py::init<int, bool, int, std::string>(), py::arg("dim"), py::arg("cylindrical") = false,
              py::arg("device") = 0, py::arg("id") = "electrical")
-        .def("set_conductivity", &ElectricalCore::set_conductivity)
-        .def("set_junctions", &ElectricalCore::set_junctions)
-        .def("set_junction_params", &ElectricalCore::set_junction_params)
-        .def("set_flags", &ElectricalCore::set_flags)
+        .def("set_conductivity",
+             [](ElectricalCore& s, const arr<double>& sh, const arr<double>& sv) { s.set_conductivity(vec(sh), vec(sv)); })
+        .def("set_junctions",
+             [](ElectricalCore& s, const arr<int64_t>& elems, int ncorner, const arr<int64_t>& bottom,
+                const arr<int64_t>& top, const arr<int>& junction, const arr<double>& heights) {
+                 s.set_junctions(vec(elems), ncorner, vec(bottom), vec(top), vec(junction), vec(heights));
+             })
+        .def("set_junction_params",
+             [](ElectricalCore& s, const arr<double>& beta, const arr<double>& js) {
+                 s.set_junction_params(vec(beta), vec(js));
+             })
+        .def("set_flags",
+             [](ElectricalCore& s, const arr<unsigned char>& select, const arr<unsigned char>& noheat) {
+                 s.set_flags(vec(select), vec(noheat));
+             })
         .def("set_start_cond", &ElectricalCore::set_start_cond)
         .def("initialize", &ElectricalCore::initialize)
         .def("compute", &ElectricalCore::compute, py::arg("loops") = 0)

@@ -53,9 +53,22 @@ class Rectangular:
         return type(self)(*[0.5 * (a[1:] + a[:-1]) for a in self.axes])
 
     def points(self):
-        """All mesh points, natural order (axis 0 fastest), shape (N, dim)."""
-        g = np.meshgrid(*self.axes, indexing="ij")
-        return np.stack([x.transpose(*range(self.dim)[::-1]).ravel() for x in g], axis=1)
+        """All mesh points, natural order (axis 0 fastest), shape (N, dim), read-only.
+        Kept with the mesh (the axis arrays are not modified in place; assigning a new
+        axis array recomputes them)."""
+        key = tuple((id(a), len(a)) for a in self.axes)
+        cached = getattr(self, "_points", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        n = [len(a) for a in self.axes]
+        pts = np.empty((int(np.prod(n)), self.dim))
+        for a in range(self.dim):
+            shape = [1] * self.dim
+            shape[self.dim - 1 - a] = n[a]   # axis 0 varies fastest
+            pts[:, a] = np.broadcast_to(self.axes[a].reshape(shape), n[::-1]).ravel()
+        pts.flags.writeable = False
+        self._points = (key, pts)
+        return pts
 
     def __iter__(self):
         return iter(self.points())
@@ -123,24 +136,23 @@ class Boundary:
         """Node indices (natural order) of this place."""
         n = [len(a) for a in mesh.axes]
         d = mesh.dim
-        idx = np.arange(int(np.prod(n)), dtype=np.int64).reshape(n[::-1])  # [k][j][i] or [j][i]
         vert = d - 1
-        if self.kind == "bottom":
-            sl = [slice(None)] * d
-            sl[0] = 0
-            return idx[tuple(sl)].ravel()
-        if self.kind == "top":
-            sl = [slice(None)] * d
-            sl[0] = -1
-            return idx[tuple(sl)].ravel()
-        if self.kind in ("left", "right", "back", "front"):
-            ax = 0 if self.kind in ("left", "right") else 1
-            if d == 3:
+        if self.kind in ("bottom", "top", "left", "right", "back", "front"):
+            # the nodes with index 0 (bottom, left, back) or n - 1 (top, right, front) along
+            # one axis, in natural order (axis 0 fastest)
+            if self.kind in ("bottom", "top"):
+                ax = vert
+            elif d == 3:
                 ax = {"back": 0, "front": 0, "left": 1, "right": 1}[self.kind]
-            pos = 0 if self.kind in ("left", "back") else -1
-            sl = [slice(None)] * d
-            sl[d - 1 - ax] = pos
-            return idx[tuple(sl)].ravel()
+            else:
+                ax = 0
+            pos = 0 if self.kind in ("bottom", "left", "back") else n[ax] - 1
+            strides = np.cumprod([1] + n[:-1])
+            out = np.array([pos * strides[ax]], dtype=np.int64)
+            for a in range(d - 1, -1, -1):   # outer axes first
+                if a != ax:
+                    out = (out[:, None] + strides[a] * np.arange(n[a], dtype=np.int64)[None, :]).ravel()
+            return out
         if self.kind == "predicate":
             fn, vec = self.arg if isinstance(self.arg, tuple) else (self.arg, False)
             pts = mesh.points()
