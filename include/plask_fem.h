@@ -139,6 +139,12 @@ int pf_source_set(pf_ctx* ctx, const double* q_elem, int64_t nelem);
  * boundary conditions).  A node listed twice takes the last value. */
 int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n);
 
+/* Nodal load added to the element load vector before the Dirichlet elimination: the
+ * boundary heat-flux and radiation terms of setMatrix (therm2d.cpp:139-173 with
+ * :232-240, :256-265; cylindrical :358-368, :394-406; therm3d.cpp:141-168 with :242-249,
+ * :261-268), evaluated by the caller.  Repeated nodes are summed; n = 0 clears it.  Kept until replaced. */
+int pf_load_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n);
+
 /* Solve the eliminated system A X = B.  x_inout: host array of N node values
  * (start vector in, solution out); may be NULL with PF_X0_ZERO / PF_X0_DEVICE to
  * leave the solution on the device (pf_get_solution copies it out). */

@@ -1596,6 +1596,13 @@ __global__ void k_scatter(long long m, const long long* __restrict__ nodes, cons
         x[nodes[i]] = vals[i];
 }
 
+// x[node] += val (nodes distinct)
+__global__ void k_scatter_add(long long m, const long long* __restrict__ nodes, const double* __restrict__ vals,
+                              double* __restrict__ x) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+        x[nodes[i]] += vals[i];
+}
+
 // x += s * (double) d
 template <typename T>
 __global__ void k_add_correction(long long n, double* __restrict__ x, const T* __restrict__ d, double s) {

@@ -41,6 +41,7 @@ void FemCore::set_mesh(const std::vector<std::vector<double>>& axes) {
           "mesh");
     N_ = pf_num_nodes(ctx_);
     E_ = pf_num_elements(ctx_);
+    axes_ = axes;
     have_mesh_ = true;
     initialized_ = false;
 }
@@ -120,6 +121,159 @@ void ThermalCore::set_heat(const std::vector<double>& q) {
     check(pf_source_set(ctx_, q.data(), E_), "heat");
 }
 
+void ThermalCore::set_boundary_loads(const std::vector<int64_t>& flux_nodes, const std::vector<double>& flux,
+                                     const std::vector<int64_t>& rad_nodes, const std::vector<double>& emissivity,
+                                     const std::vector<double>& ambient) {
+    if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
+    if (flux_nodes.size() != flux.size() || rad_nodes.size() != emissivity.size() || rad_nodes.size() != ambient.size())
+        throw std::invalid_argument(id + ": boundary node and value lists differ in length");
+    const double nan = std::nan("");
+    if (flux_nodes.empty() && rad_nodes.empty()) {  // no conditions: nothing to evaluate per loop
+        if (have_flux_ || have_rad_) check(pf_load_set(ctx_, nullptr, nullptr, 0), "boundary load");
+        have_flux_ = have_rad_ = false;
+        flux_.clear();
+        emis_.clear();
+        ramb_.clear();
+        bc_elems_.clear();
+        return;
+    }
+    have_flux_ = !flux_nodes.empty();
+    have_rad_ = !rad_nodes.empty();
+    flux_.assign(have_flux_ ? N_ : 0, nan);
+    emis_.assign(have_rad_ ? N_ : 0, nan);
+    ramb_.assign(have_rad_ ? N_ : 0, nan);
+    std::vector<char> touched(N_, 0);
+    auto node_ok = [&](int64_t n) {
+        if (n < 0 || n >= N_) throw std::invalid_argument(id + ": boundary node out of range");
+    };
+    for (size_t q = 0; q < flux_nodes.size(); ++q) {
+        const int64_t n = flux_nodes[q];
+        node_ok(n);
+        if (std::isnan(flux_[n])) flux_[n] = flux[q];
+        touched[n] = 1;
+    }
+    for (size_t q = 0; q < rad_nodes.size(); ++q) {
+        const int64_t n = rad_nodes[q];
+        node_ok(n);
+        if (std::isnan(emis_[n])) {
+            emis_[n] = emissivity[q];
+            ramb_[n] = ambient[q];
+        }
+        touched[n] = 1;
+    }
+    // elements touching a node under a condition (natural order, axis 0 fastest)
+    bc_elems_.clear();
+    const int64_t n0 = (int64_t)axes_[0].size(), n1 = (int64_t)axes_[1].size();
+    const int64_t m0 = n0 - 1, m1 = n1 - 1;
+    std::vector<char> ein(E_, 0);
+    for (int64_t n = 0; n < N_; ++n) {
+        if (!touched[n]) continue;
+        const int64_t i = n % n0, j = (n / n0) % n1, k = dim_ == 3 ? n / (n0 * n1) : 0;
+        const int64_t m2 = dim_ == 3 ? (int64_t)axes_[2].size() - 1 : 1;
+        for (int64_t dk = (dim_ == 3 ? -1 : 0); dk <= 0; ++dk)
+            for (int64_t dj = -1; dj <= 0; ++dj)
+                for (int64_t di = -1; di <= 0; ++di) {
+                    const int64_t ei = i + di, ej = j + dj, ek = k + dk;
+                    if (ei < 0 || ei >= m0 || ej < 0 || ej >= m1 || ek < 0 || (dim_ == 3 && ek >= m2)) continue;
+                    ein[ei + m0 * (ej + m1 * ek)] = 1;
+                }
+    }
+    for (int64_t e = 0; e < E_; ++e)
+        if (ein[e]) bc_elems_.push_back(e);
+}
+
+// The flux and radiation parts of setMatrix (therm2d.cpp:139-173, :232-240, :256-265;
+// cylindrical :358-368, :394-406; therm3d.cpp:141-168, :242-249, :261-268): an element
+// side whose nodes all carry a condition adds -len/2 * value per node (2D, cylindrical
+// weights r -/+ len/6 along r and the edge radius on r = const sides) or -area/4 * value
+// (3D; as in the reference the four values of any wall go to the element's local nodes
+// 0-3).  Radiation's value is emissivity * SB * (T^4 - T_ambient^4) at the node's
+// current temperature in 2D; in 3D the reference reads the temperature of the mesh node
+// whose index is the wall node's local number (0-7), kept here.
+void ThermalCore::boundary_load() {
+    const double SB = 5.670373e-8;  // plask/phys/constants.hpp:41
+    std::vector<double> T;
+    if (have_rad_) T = solution();
+    std::vector<double> F(N_, 0.);
+    std::vector<char> seen(N_, 0);
+    std::vector<int64_t> nodes;
+    auto add = [&](int64_t n, double v) {
+        if (!seen[n]) {
+            seen[n] = 1;
+            nodes.push_back(n);
+        }
+        F[n] += v;
+    };
+    // condition value at node n; radiation uses the temperature of node nt
+    auto value = [&](int kind, int64_t n, int64_t nt, double& v) -> bool {
+        if (kind == 0) {
+            if (!have_flux_ || std::isnan(flux_[n])) return false;
+            v = flux_[n];
+        } else {
+            if (!have_rad_ || std::isnan(emis_[n])) return false;
+            double a = ramb_[n], t = T[nt];
+            a *= a;
+            t *= t;
+            v = emis_[n] * SB * (t * t - a * a);
+        }
+        return true;
+    };
+    const std::vector<double>& ax0 = axes_[0];
+    const std::vector<double>& ax1 = axes_[1];
+    const int64_t n0 = (int64_t)ax0.size(), n1 = (int64_t)ax1.size();
+    const int64_t m0 = n0 - 1, m1 = n1 - 1;
+    for (int kind = 0; kind < 2; ++kind) {
+        if ((kind == 0 && !have_flux_) || (kind == 1 && !have_rad_)) continue;
+        for (int64_t e : bc_elems_) {
+            if (dim_ == 2) {
+                const int64_t i = e % m0, j = e / m0;
+                const int64_t lolo = i + n0 * j, uplo = lolo + 1, upup = uplo + n0, loup = lolo + n0;
+                const double w = ax0[i + 1] - ax0[i], h = ax1[j + 1] - ax1[j];
+                const double r = 0.5 * (ax0[i] + ax0[i + 1]);
+                // sides in the reference's order: bottom, right, top, left
+                const int64_t sa[4] = {lolo, uplo, upup, loup}, sb[4] = {uplo, upup, loup, lolo};
+                const double len[4] = {w, h, w, h};
+                for (int side = 0; side < 4; ++side) {
+                    double va, vb;
+                    if (!value(kind, sa[side], sa[side], va) || !value(kind, sb[side], sb[side], vb)) continue;
+                    const double L = len[side];
+                    auto weight = [&](int64_t a, int64_t b) {
+                        if (!cyl_) return 1.;
+                        if (side == 3) return ax0[i];      // left: r of the side
+                        if (side == 1) return ax0[i + 1];  // right
+                        return r + (a < b ? -L / 6. : L / 6.);
+                    };
+                    add(sa[side], -0.5e-6 * L * va * weight(sa[side], sb[side]));
+                    add(sb[side], -0.5e-6 * L * vb * weight(sb[side], sa[side]));
+                }
+            } else {
+                const int64_t i = e % m0, j = (e / m0) % m1, k = e / (m0 * m1);
+                const std::vector<double>& ax2 = axes_[2];
+                int64_t idx[8];
+                for (int l = 0; l < 8; ++l)
+                    idx[l] = (i + (l & 1)) + n0 * ((j + ((l >> 1) & 1)) + n1 * (k + (l >> 2)));
+                const double dx = ax0[i + 1] - ax0[i], dy = ax1[j + 1] - ax1[j], dz = ax2[k + 1] - ax2[k];
+                static const int walls[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 2, 4, 6},
+                                                {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}};
+                const double areas[3] = {dx * dy, dy * dz, dz * dx};
+                for (int side = 0; side < 6; ++side) {
+                    double v[4];
+                    bool all = true;
+                    // the reference passes the wall's local node number where the radiation
+                    // term reads the temperature, i.e. the temperature of mesh node 0-7
+                    for (int q = 0; q < 4 && all; ++q)
+                        all = value(kind, idx[walls[side][q]], std::min<int64_t>(walls[side][q], N_ - 1), v[q]);
+                    if (!all) continue;
+                    for (int q = 0; q < 4; ++q) add(idx[q], -0.25e-12 * areas[side / 2] * v[q]);
+                }
+            }
+        }
+    }
+    std::vector<double> vals(nodes.size());
+    for (size_t q = 0; q < nodes.size(); ++q) vals[q] = F[nodes[q]];
+    check(pf_load_set(ctx_, nodes.data(), vals.data(), (int64_t)nodes.size()), "boundary load");
+}
+
 // onInitialize: temperatures = inittemp, loopno = 0 (therm3d.cpp:71-80)
 void ThermalCore::initialize() {
     if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
@@ -146,6 +300,8 @@ double ThermalCore::compute(int loops) {
         } else if (tdep_) {
             check(pf_coeffs_update(ctx_), "conductivity");
         }
+        // boundary loads at the loop's start temperatures (radiation changes every loop)
+        if (have_rad_ || (have_flux_ && loop == 0)) boundary_load();
         solve(inittemp);
         check(pf_solution_change(ctx_, &err, &maxT), "loop error");
         if (err > toterr) toterr = err;

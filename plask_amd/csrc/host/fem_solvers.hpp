@@ -72,6 +72,7 @@ class FemCore {
     int64_t N_ = 0, E_ = 0;
     bool initialized_ = false;
     bool have_mesh_ = false;
+    std::vector<std::vector<double>> axes_;
 };
 
 // ThermalFem2DSolver / ThermalFem3DSolver compute loop (solvers/thermal/static/therm2d.cpp:443-502,
@@ -86,6 +87,12 @@ class ThermalCore : public FemCore {
     // callback(T_avg[E], kh[E], kv[E]) for materials the device law cannot express
     void set_conductivity_callback(std::function<void(const double*, double*, double*, int64_t)> cb);
     void set_heat(const std::vector<double>& q);  // W/m^3 per element (empty = none)
+    // Boundary heat flux (W/m^2, outward positive) and radiation (emissivity, ambient K)
+    // per node; a node listed twice keeps its first value (BoundaryConditions::getValue,
+    // boundary_conditions.hpp:182-186).  Empty lists clear them.
+    void set_boundary_loads(const std::vector<int64_t>& flux_nodes, const std::vector<double>& flux,
+                            const std::vector<int64_t>& rad_nodes, const std::vector<double>& emissivity,
+                            const std::vector<double>& ambient);
     double compute(int loops);
     void initialize();
 
@@ -95,8 +102,12 @@ class ThermalCore : public FemCore {
     double toterr = 0., maxT = 0.;
 
   private:
+    void boundary_load();  // pf_load_set from the boundary values and the current temperatures
     std::vector<double> k0h_, k0v_, ah_, av_;
     bool tdep_ = false;
+    std::vector<double> flux_, emis_, ramb_;  // per node, NaN: no condition
+    std::vector<int64_t> bc_elems_;           // elements with a node under a flux / radiation condition
+    bool have_flux_ = false, have_rad_ = false;
     std::function<void(const double*, double*, double*, int64_t)> kcb_;
 };
 

@@ -76,6 +76,11 @@ PYBIND11_MODULE(_core, m) {
                  });
              })
         .def("set_heat", [](ThermalCore& s, const arr<double>& q) { s.set_heat(vec(q)); })
+        .def("set_boundary_loads",
+             [](ThermalCore& s, const arr<int64_t>& flux_nodes, const arr<double>& flux, const arr<int64_t>& rad_nodes,
+                const arr<double>& emissivity, const arr<double>& ambient) {
+                 s.set_boundary_loads(vec(flux_nodes), vec(flux), vec(rad_nodes), vec(emissivity), vec(ambient));
+             })
         .def("initialize", &ThermalCore::initialize)
         .def("compute", &ThermalCore::compute, py::arg("loops") = 0)
         .def_readwrite("inittemp", &ThermalCore::inittemp)

@@ -80,6 +80,10 @@ struct pf_ctx {
     double* dvals = nullptr;
     int64_t nd = 0;
     double dvals_sq = 0.;
+    // nodal load added to the element load vector (boundary heat flux / radiation)
+    long long* lnodes = nullptr;
+    double* lvals = nullptr;
+    int64_t nl = 0;
     // vectors
     double *p2 = nullptr;
     float *p2_32 = nullptr;
@@ -571,6 +575,41 @@ int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int6
     return PF_OK;
 }
 
+int pf_load_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (n < 0 || (n > 0 && (!nodes || !vals))) return fail(ctx, PF_EINVAL, "bad load list");
+    if (n == 0 && ctx->nl == 0) return PF_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    // one entry per node, repeated nodes summed in list order
+    std::vector<std::pair<long long, int64_t>> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (nodes[i] < 0 || nodes[i] >= ctx->N) return fail(ctx, PF_EINVAL, "load node out of range");
+        if (!std::isfinite(vals[i])) return fail(ctx, PF_EINVAL, "load value not finite");
+        order[(size_t)i] = {(long long)nodes[i], i};
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<long long> un;
+    std::vector<double> uv;
+    for (const auto& o : order) {
+        if (un.empty() || un.back() != o.first) {
+            un.push_back(o.first);
+            uv.push_back(0.);
+        }
+        uv.back() += vals[o.second];
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->lnodes, std::max<size_t>(1, un.size())))) return rc;
+    if ((rc = dalloc(ctx, &ctx->lvals, std::max<size_t>(1, un.size())))) return rc;
+    if (!un.empty()) {
+        HIPCK(hipMemcpy(ctx->lnodes, un.data(), sizeof(long long) * un.size(), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->lvals, uv.data(), sizeof(double) * uv.size(), hipMemcpyHostToDevice));
+    }
+    ctx->nl = (int64_t)un.size();
+    ctx->dirty = true;
+    return PF_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -681,6 +720,9 @@ static int prepare(pf_ctx* ctx) {
         // electrical: no load vector (electr2d.cpp:268-317); thermal without source: zero
         hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->F, 0.);
     }
+    if (ctx->nl)
+        hipLaunchKernelGGL(k_scatter_add, dim3(grid1(ctx->nl)), dim3(256), 0, ctx->stream, (long long)ctx->nl,
+                           ctx->lnodes, ctx->lvals, ctx->F);
     // xD into tmp, then r = M (F - A0 xD)
     hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->tmp, 0.);
     if (ctx->nd)

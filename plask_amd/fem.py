@@ -22,7 +22,7 @@ KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KC
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
-    "pf_dirichlet_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
+    "pf_dirichlet_set", "pf_load_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host",
@@ -71,6 +71,7 @@ def load_library(path=LIB_PATH):
     L.pf_coeffs_set.argtypes = [P, I, P, P, i64]
     L.pf_source_set.argtypes = [P, P, i64]
     L.pf_dirichlet_set.argtypes = [P, P, P, i64]
+    L.pf_load_set.argtypes = [P, P, P, i64]
     L.pf_solve.argtypes = [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]
     L.pf_get_solution.argtypes = [P, P]
     L.pf_apply.argtypes = [P, I, P, P]
@@ -178,6 +179,13 @@ class FemContext:
         self._dn = np.ascontiguousarray(nodes, dtype=np.int64)
         self._dv = _f64(np.broadcast_to(values, self._dn.shape))
         self._ck(self.L.pf_dirichlet_set(self.h, _p(self._dn), _p(self._dv), len(self._dn)), "pf_dirichlet_set")
+
+    def set_load(self, nodes, values):
+        """Nodal load added to the element load vector (boundary terms); repeated nodes
+        are summed; empty clears it."""
+        self._ln = np.ascontiguousarray(nodes, dtype=np.int64)
+        self._lv = _f64(np.broadcast_to(values, self._ln.shape))
+        self._ck(self.L.pf_load_set(self.h, _p(self._ln), _p(self._lv), len(self._ln)), "pf_load_set")
 
     # -- compute -------------------------------------------------------------
     def solve(self, x0=None, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=None, max_outer=50,

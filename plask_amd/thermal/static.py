@@ -13,6 +13,65 @@ from .. import solverbase as _sb
 from .._core import ThermalCore
 
 
+class _BcValue:
+    """Value of a convection / radiation boundary condition (therm_python.cpp:25-86):
+    ``Cls(first, ambient)``; also built from a dict {first: ..., 'ambient': ...}
+    (ambient defaults to 300 K) or a (first, ambient) pair; item access by key."""
+    FIRST = ""
+
+    def __init__(self, first, ambient=300.0):
+        setattr(self, self.FIRST, float(first))
+        self.ambient = float(ambient)
+
+    @classmethod
+    def of(cls, v):
+        if isinstance(v, cls):
+            return v
+        if isinstance(v, dict):
+            return cls(v[cls.FIRST], v.get("ambient", 300.0))
+        if isinstance(v, (tuple, list)) and len(v) == 2:
+            return cls(*v)
+        raise TypeError("%s boundary value must be %s(%s, ambient), a dict or a pair" %
+                        (cls.__name__, cls.__name__, cls.FIRST))
+
+    def __getitem__(self, key):
+        if key in (self.FIRST, "ambient"):
+            return getattr(self, key)
+        raise KeyError(key)
+
+    def __setitem__(self, key, value):
+        if key not in (self.FIRST, "ambient"):
+            raise KeyError(key)
+        setattr(self, key, float(value))
+
+    def __repr__(self):
+        return "{'%s': %r, 'ambient': %r}" % (self.FIRST, getattr(self, self.FIRST), self.ambient)
+
+    def __str__(self):
+        return "%s (%sK)" % (getattr(self, self.FIRST), self.ambient)
+
+
+class Convection(_BcValue):
+    """Convective boundary condition value: coeff (W/m^2/K), ambient (K)."""
+    FIRST = "coeff"
+
+
+class Radiation(_BcValue):
+    """Radiative boundary condition value: emissivity (-), ambient (K)."""
+    FIRST = "emissivity"
+
+
+def _bc_lists(bcs, mesh, geometry, convert):
+    """Nodes and converted values of a condition list, in list order (the first place
+    holding a node gives its value, BoundaryConditions::getValue)."""
+    nodes, vals = [], []
+    for place, value in bcs:
+        nd = np.asarray(place.nodes(mesh, geometry), dtype=np.int64)
+        nodes.append(nd)
+        vals.extend([convert(value)] * len(nd))
+    return (np.concatenate(nodes) if nodes else np.zeros(0, np.int64)), vals
+
+
 class _StaticBase(_sb.FemSolver):
     _core_class = ThermalCore
     inHeat = _sb.ReceiverAttr(0.0)
@@ -74,10 +133,9 @@ class _StaticBase(_sb.FemSolver):
     def compute(self, loops=0):
         """Run thermal calculations; returns the maximum temperature update (K)."""
         self.initialize()
-        for bc in (self.heatflux_boundary, self.convection_boundary, self.radiation_boundary):
-            if len(bc):
-                raise NotImplementedError("%s: heat flux / convection / radiation boundary conditions are not "
-                                          "implemented on the GPU path" % self.id)
+        if len(self.convection_boundary):
+            raise NotImplementedError("%s: convection boundary conditions change the operator and are not "
+                                      "implemented on the GPU path (heat flux and radiation are)" % self.id)
         c = self._core
         c.inittemp = float(self.inittemp)
         c.maxerr = float(self.maxerr)
@@ -85,6 +143,11 @@ class _StaticBase(_sb.FemSolver):
         c.set_heat(np.ascontiguousarray(q, dtype=np.float64))
         nodes, vals = self.temperature_boundary.resolve(self.mesh, self.geometry)
         c.set_dirichlet(nodes.astype(np.int64), vals)
+        fn, fv = _bc_lists(self.heatflux_boundary, self.mesh, self.geometry, float)
+        rn, rv = _bc_lists(self.radiation_boundary, self.mesh, self.geometry, Radiation.of)
+        c.set_boundary_loads(fn, np.asarray(fv, dtype=np.float64), rn,
+                             np.array([v.emissivity for v in rv], dtype=np.float64),
+                             np.array([v.ambient for v in rv], dtype=np.float64))
         err = c.compute(int(loops))
         self._T = None
         return err

@@ -1,0 +1,189 @@
+"""Heat-flux and radiation boundary conditions through thermal.static on the GPU, against
+the CPU statement of the reference's boundary loads (oracle/bc.py) and band Cholesky."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import bc as BC
+from plask_amd import geometry, material, mesh
+from plask_amd.thermal.static import Convection, Radiation, Static2D, Static3D, StaticCyl
+
+pytestmark = pytest.mark.gpu
+
+
+def _stack_2d(cls, cyl, seed=7, nx=17, nlayers=5, kscale=1.0):
+    rng = np.random.default_rng(seed)
+    stack = geometry.Stack2D()
+    for i in range(nlayers):
+        t = float(rng.uniform(0.3, 2.0))
+        k = kscale * float(np.exp(rng.uniform(0, 4)))
+        stack.append(geometry.Rectangle(15., t, material.Const(thermk=(k, 0.5 * k), name="B%d" % i)))
+    s = cls("therm")
+    s.geometry = geometry.Cylindrical(stack) if cyl else geometry.Cartesian2D(stack)
+    s.mesh = mesh.Rectangular2D.DivideGenerator(prediv=(nx - 1, 3))
+    s.temperature_boundary.append(s.mesh.Bottom(), 300.)
+    s.iterative.maxerr = 1e-13
+    return s
+
+
+def _stack_3d(seed=8, shape=(7, 6, 9), kscale=1.0):
+    rng = np.random.default_rng(seed)
+    stack = geometry.Stack3D(xcenter=0.0, ycenter=0.0)
+    ts = []
+    for i in range(3):
+        t = float(rng.uniform(0.5, 2.0))
+        k = kscale * float(np.exp(rng.uniform(0, 4)))
+        stack.append(geometry.Cuboid(12., 10., t, material.Const(thermk=(k, 0.5 * k), name="C%d" % i)))
+        ts.append(t)
+    z = np.concatenate([[0.0], np.cumsum(ts)])
+    zz = np.unique(np.concatenate([np.linspace(z[i], z[i + 1], 4) for i in range(3)]))
+    s = Static3D("therm3")
+    s.geometry = geometry.Cartesian3D(stack)
+    s.mesh = mesh.Rectangular3D(np.linspace(-6., 6., shape[0]), np.linspace(-5., 5., shape[1]), zz)
+    s.temperature_boundary.append(s.mesh.Bottom(), 300.)
+    s.iterative.maxerr = 1e-13
+    return s
+
+
+def _oracle_system(s, cyl=False):
+    msh = s.mesh
+    g = O.Grid(msh.axes, cyl=cyl)
+    k = np.array([m.thermk(300.) for m in s._mat])
+    k = k if k.ndim == 2 else np.column_stack([k, k])
+    dn, dv = s.temperature_boundary.resolve(msh, s.geometry)
+    A = O.assemble(g, 0, k[:, 0], k[:, 1], np.zeros(g.nelems), O.ST_BAND, dn, dv)
+    return A, dn
+
+
+def _nodes(s, place):
+    return place.nodes(s.mesh, s.geometry)
+
+
+@pytest.mark.parametrize("cls,cyl", [(Static2D, False), (StaticCyl, True)])
+def test_heat_flux_2d_vs_oracle(cls, cyl):
+    s = _stack_2d(cls, cyl)
+    print("heat flux 2d", cls.__name__, flush=True)
+    top, right = s.mesh.Top(), s.mesh.Right()
+    s.heatflux_boundary.append(top, 3.0e5)
+    s.heatflux_boundary.append(right, -1.0e5)   # the shared corner keeps the first value
+    s.initialize()
+    s.compute(1)
+    T = s._core.solution()
+    A, dn = _oracle_system(s, cyl)
+    F = BC.load_2d(s.mesh.axes, flux=BC.first_values([(_nodes(s, top), 3.0e5), (_nodes(s, right), -1.0e5)]),
+                   cyl=cyl)
+    T_ref = O.band_cholesky_solve(A, BC.with_load(A.B, F, dn))
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 0.1   # the load matters
+
+
+def test_heat_flux_3d_vs_oracle():
+    s = _stack_3d()
+    print("heat flux 3d", flush=True)
+    top, left = s.mesh.Top(), s.mesh.Left()
+    s.heatflux_boundary.append(top, -2.0e6)
+    s.heatflux_boundary.append(left, 5.0e5)
+    s.initialize()
+    s.compute(1)
+    T = s._core.solution()
+    A, dn = _oracle_system(s)
+    F = BC.load_3d(s.mesh.axes, flux=BC.first_values([(_nodes(s, top), -2.0e6), (_nodes(s, left), 5.0e5)]))
+    T_ref = O.band_cholesky_solve(A, BC.with_load(A.B, F, dn))
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 0.1
+
+
+def _picard(A, dn, load, T0, loops):
+    T = T0.copy()
+    for _ in range(loops):
+        T = O.band_cholesky_solve(A, BC.with_load(A.B, load(T), dn))
+    return T
+
+
+@pytest.mark.parametrize("cls,cyl", [(Static2D, False), (StaticCyl, True)])
+def test_radiation_2d_picard_vs_oracle(cls, cyl):
+    s = _stack_2d(cls, cyl, kscale=0.02)
+    print("radiation 2d", cls.__name__, flush=True)
+    top = s.mesh.Top()
+    s.radiation_boundary.append(top, Radiation(0.9, 900.))
+    s.maxerr = 0.0   # run every loop
+    s.initialize()
+    s.compute(3)
+    T = s._core.solution()
+    A, dn = _oracle_system(s, cyl)
+    rad = BC.first_values([(_nodes(s, top), (0.9, 900.))])
+    T_ref = _picard(A, dn, lambda t: BC.load_2d(s.mesh.axes, radiation=rad, T=t, cyl=cyl),
+                    np.full(A.rank, 300.), 3)
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 1.0
+
+
+def test_radiation_3d_picard_vs_oracle():
+    s = _stack_3d(kscale=0.02)
+    print("radiation 3d", flush=True)
+    top = s.mesh.Top()
+    s.radiation_boundary.append(top, {"emissivity": 0.7, "ambient": 1200.})
+    s.maxerr = 0.0
+    s.initialize()
+    s.compute(2)
+    T = s._core.solution()
+    A, dn = _oracle_system(s)
+    rad = BC.first_values([(_nodes(s, top), (0.7, 1200.))])
+    T_ref = _picard(A, dn, lambda t: BC.load_3d(s.mesh.axes, radiation=rad, T=t), np.full(A.rank, 300.), 2)
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 1.0
+
+
+def test_boundary_loads_clear():
+    s = _stack_2d(Static2D, False)
+    s.initialize()
+    s.compute(1)
+    T0 = s._core.solution()
+    s.heatflux_boundary.append(s.mesh.Top(), 1.0e6)
+    s.compute(1)
+    assert np.abs(s._core.solution() - T0).max() > 0.1
+    s.heatflux_boundary.clear()
+    s.compute(1)
+    assert np.abs(s._core.solution() - T0).max() <= 1e-9 * 300.
+
+
+def test_convection_is_refused():
+    s = _stack_2d(Static2D, False)
+    s.convection_boundary.append(s.mesh.Top(), Convection(10., 300.))
+    with pytest.raises(NotImplementedError):
+        s.compute(1)
+
+
+def test_condition_values():
+    r = Radiation.of({"emissivity": 0.5})
+    assert r["ambient"] == 300.0 and r.emissivity == 0.5
+    c = Convection.of((4.0, 350.0))
+    c["coeff"] = 5.0
+    assert c.coeff == 5.0 and c["ambient"] == 350.0
+    assert repr(c) == "{'coeff': 5.0, 'ambient': 350.0}"
+    with pytest.raises(KeyError):
+        c["emissivity"]
+
+
+def test_load_set_through_the_c_abi():
+    """pf_load_set: repeated nodes are summed into the right-hand side, Dirichlet rows
+    keep their values, and an empty list removes the load."""
+    from plask_amd.fem import FemContext
+    axes = [np.linspace(0, 4, 5), np.linspace(0, 3, 4), np.linspace(0, 2, 6)]
+    E = 4 * 3 * 5
+    c = FemContext()
+    c.set_grid(axes)
+    c.set_coeffs(np.full(E, 3.0), np.full(E, 1.5))
+    c.set_source(np.full(E, 1e15))
+    dn = np.arange(20)
+    c.set_dirichlet(dn, np.full(20, 300.0))
+    b0 = c.rhs()
+    c.set_load([45, 61, 45, 3], [1.0, 2.0, 0.5, 7.0])
+    d = c.rhs() - b0
+    want = np.zeros_like(d)
+    want[45], want[61] = 1.5, 2.0   # node 3 is a Dirichlet node
+    assert np.allclose(d, want, rtol=0, atol=1e-12)
+    assert d[3] == 0.0
+    c.set_load([], [])
+    assert np.array_equal(c.rhs(), b0)
+    c.close()
