@@ -209,7 +209,7 @@ int pf_set_line_segment(pf_ctx* ctx, int seg);
  * Q1 operator with area-weighted mean conductivities; the smoother is the exact line
  * solve along the layer axis damped by omega (0 < omega <= 0.66, default 0.65: the line
  * preconditioned operator has eigenvalues <= 3, so the smoother converges), one sweep
- * before and after each coarse correction and coarse_sweeps (1-64, default 2) on the
+ * before and after each coarse correction and coarse_sweeps (1-64, default 1) on the
  * coarsest level; max_levels limits the level count (0 = no limit).  Needs the
  * partitioned line solve (pf_set_line_segment >= 0).  The levels are built on the first
  * multigrid solve after a grid change.  No counterpart in the reference (NSPCG offers

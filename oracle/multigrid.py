@@ -168,7 +168,7 @@ class VCycle:
     """Levels of a 2D or 3D grid (natural node order, axis 0 fastest; the last axis is the
     layer axis) and the V-cycle z = V r."""
 
-    def __init__(self, axes, kh, kv, dnodes, omega=0.65, coarse_sweeps=2, max_levels=0, cyl=False):
+    def __init__(self, axes, kh, kv, dnodes, omega=0.65, coarse_sweeps=1, max_levels=0, cyl=False):
         N = int(np.prod([len(a) for a in axes]))
         dmask = np.zeros(N, bool)
         dmask[np.asarray(dnodes, dtype=np.int64)] = True

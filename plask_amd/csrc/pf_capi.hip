@@ -120,7 +120,7 @@ struct pf_ctx {
     double *mg_r0 = nullptr;
     float *mg_r0_32 = nullptr;
     int mg_max_levels = 0;      // 0: coarsen until both in-plane axes have <= 3 nodes
-    int mg_coarse_sweeps = 2;   // smoothing sweeps on the coarsest level
+    int mg_coarse_sweeps = 1;   // smoothing sweeps on the coarsest level
     double mg_omega = 0.65;     // damping of the line-Jacobi smoother
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;

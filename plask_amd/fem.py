@@ -262,7 +262,7 @@ class FemContext:
         precision PF_FP64, PF_FP32_REFINE or -1 (both)."""
         self._ck(self.L.pf_set_apply(self.h, int(precision), int(kind), int(planes)), "pf_set_apply")
 
-    def set_multigrid(self, max_levels=0, coarse_sweeps=2, omega=0.65):
+    def set_multigrid(self, max_levels=0, coarse_sweeps=1, omega=0.65):
         """Multigrid preconditioner (precond=PRECOND_MG, 3D) options; levels are rebuilt
         by the next multigrid solve."""
         self._ck(self.L.pf_set_multigrid(self.h, int(max_levels), int(coarse_sweeps), float(omega)),
