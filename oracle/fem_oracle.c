@@ -403,8 +403,12 @@ int64_t orc_dia_pcg(int64_t rank, int ndiag, const int64_t* offsets, const doubl
         }
         if (rz <= 0.) { status = -7; break; }
         if (it >= maxit) { *rel_out = rn / bn; status = -1; break; }
+        /* first direction p = z: p is uninitialised memory then, and 0 * (NaN garbage) is NaN */
         double beta = (it == 0) ? 0. : rz / rz_old;
-        for (int64_t i = 0; i < rank; ++i) p[i] = z[i] + beta * p[i];
+        if (it == 0)
+            memcpy(p, z, sizeof(double) * rank);
+        else
+            for (int64_t i = 0; i < rank; ++i) p[i] = z[i] + beta * p[i];
         orc_dia_mult(rank, ndiag, offsets, data, p, q);
         double pap = dot(rank, p, q);
         if (pap <= 0.) { status = -6; break; }
