@@ -209,6 +209,7 @@ class _ShockleyBase(_sb.FemSolver):
         c.stable = (str(self.convergence).lower() == "stable")
         T = self.inTemperature(self._element_mesh())
         sh, sv = self._load_conductivities(T)
+        sh[self._excl], sv[self._excl] = _sb.MASKED_K   # masked-out (empty) elements
         c.set_conductivity(sh, sv)
         # beta, js per junction element at the junction mid-row temperature (electr2d.cpp:255-256)
         beta = np.empty(len(self._jelems))
@@ -229,6 +230,9 @@ class _ShockleyBase(_sb.FemSolver):
             js[q] = j(float(T[ti])) if callable(j) else float(j)
         c.set_junction_params(beta, js)
         nodes, vals = self.voltage_boundary.resolve(self.mesh, self.geometry)
+        if len(self._isolated):  # nodes of the masked-out region: held at 0 V, coupled to nothing
+            nodes = np.concatenate([nodes, self._isolated])
+            vals = np.concatenate([vals, np.zeros(len(self._isolated))])
         c.set_dirichlet(nodes.astype(np.int64), vals)
         err = c.compute(int(loops))
         self._V = None
@@ -270,7 +274,9 @@ class _ShockleyBase(_sb.FemSolver):
             sh[self._act > 0] = self._start_cond[0]
         else:
             sh, sv = self._core.conductivity_h(), self._core.conductivity_v()
-        return _sb.nearest_elements(self._axes, np.column_stack([sh, sv]), mesh)
+        k = np.column_stack([sh, sv])
+        k[self._excl] = np.nan   # not in the masked mesh
+        return _sb.nearest_elements(self._axes, k, mesh)
 
     # -- integrals -------------------------------------------------------------------
     def get_total_current(self, nact=0):
@@ -333,7 +339,7 @@ class _ShockleyBase(_sb.FemSolver):
         """Energy of the electrostatic field (J) (getTotalEnergy, electr2d.cpp:574-615)."""
         T = self.inTemperature(self._element_mesh())
         eps = np.array([mm.eps(float(t)) if mm is not None else 1.0 for mm, t in zip(self._mat, T)])
-        w = eps * self._grad2() * self._element_volume()
+        w = np.where(self._excl, 0.0, eps * self._grad2() * self._element_volume())
         if self.DIM == 2 and self.CYL:
             r = np.tile(0.5 * (self._axes[0][1:] + self._axes[0][:-1]), len(self._axes[1]) - 1)
             return float(2.0 * math.pi * 0.5e-18 * EPSILON0 * np.sum(w * r))

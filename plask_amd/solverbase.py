@@ -17,6 +17,10 @@ except ImportError as e:  # the compute path is the compiled core only
 
 log = logging.getLogger("plask_amd")
 
+# Conductivity of masked-out (empty) elements: zero in-plane and 1e-30 vertical (the device
+# operator takes vertical > 0); the held nodes of the masked region couple at 1e-30.
+MASKED_K = (0.0, 1e-30)
+
 # plask/log/log.hpp levels used by the solvers -> Python logging
 _LEVELS = {0: logging.CRITICAL, 1: logging.ERROR, 2: logging.ERROR, 3: logging.WARNING, 4: logging.WARNING,
            5: logging.INFO, 6: logging.INFO, 7: logging.DEBUG, 8: logging.DEBUG, 9: logging.DEBUG}
@@ -163,10 +167,45 @@ class FemSolver:
         self._mesh = None
         self._msh = None
         self._core = None
-        self.algorithm = "iterative"
-        self.empty_elements = "default"
+        # the reference's defaults: Cholesky in 2D (fem_solver.hpp:32), iterative in 3D
+        # (therm3d.cpp:32, electr3d.cpp:36); the GPU always iterates, the choice only sets
+        # what empty_elements='default' means (fem_solver.hpp:182-188)
+        self._algorithm = "cholesky" if self.DIM == 2 else "iterative"
+        self._empty = "default"
         self._iter = _core.IterParams()
         self._initialized = False
+
+    @property
+    def algorithm(self):
+        return self._algorithm
+
+    @algorithm.setter
+    def algorithm(self, v):
+        v = str(v).lower()
+        if v not in ("cholesky", "gauss", "iterative"):
+            raise ValueError("%s: algorithm must be 'cholesky', 'gauss' or 'iterative'" % self.id)
+        self._algorithm = v
+        self.invalidate()
+
+    @property
+    def empty_elements(self):
+        """'default', 'include' or 'exclude' (EmptyElementsHandling, fem_solver.hpp:139-176)."""
+        return self._empty
+
+    @empty_elements.setter
+    def empty_elements(self, v):
+        if isinstance(v, bool):
+            v = "include" if v else "exclude"
+        v = {"included": "include", "excluded": "exclude"}.get(str(v).lower(), str(v).lower())
+        if v not in ("default", "include", "exclude"):
+            raise ValueError("%s: empty_elements must be 'default', 'include' or 'exclude'" % self.id)
+        self._empty = v
+        self.invalidate()
+
+    def _excludes_empty(self):
+        """Whether elements of empty materials (air) leave the mesh: 'exclude', or
+        'default' with a direct algorithm (setupMaskedMesh, fem_solver.hpp:182-188)."""
+        return self._empty == "exclude" or (self._empty == "default" and self._algorithm != "iterative")
 
     # -- geometry / mesh ------------------------------------------------------
     @property
@@ -211,9 +250,6 @@ class FemSolver:
             raise ValueError("%s: no geometry" % self.id)
         if self.mesh is None:
             raise ValueError("%s: no mesh" % self.id)
-        if self.empty_elements in ("exclude", "excluded"):
-            raise NotImplementedError("%s: empty_elements='exclude' (masked mesh) is not supported by the GPU "
-                                      "path; the full mesh is used with 'include'" % self.id)
         msh = self.mesh
         if self._core is None:
             self._core = self._core_class(self.DIM, self.CYL, 0, self.id)
@@ -244,6 +280,27 @@ class FemSolver:
         self._leaf_roles = leaf_roles
         self._axes = [np.asarray(a) for a in msh.axes]
         self._esize = [np.diff(a) for a in self._axes]
+        # Masked mesh (EMPTY_ELEMENTS_EXCLUDED): elements of empty materials carry no
+        # conductivity, heat or field energy, and nodes that only such elements touch are
+        # held at a fixed value; the solution on the other nodes is the masked mesh's.
+        n_el = len(self._mat)
+        if self._excludes_empty():
+            leaf_empty = np.array([lf[0].material is None or getattr(lf[0].material, "kind", "") == "empty"
+                                   for lf in leaves] + [True])
+            self._excl = leaf_empty[self._leaf]
+            if self._excl.all():
+                raise ValueError("%s: every element is empty" % self.id)
+            m = [len(a) - 1 for a in self._axes][::-1]
+            kept = np.zeros([k + 2 for k in m], bool)
+            kept[tuple(slice(1, -1) for _ in m)] = ~self._excl.reshape(m)
+            touched = np.zeros([k + 1 for k in m], bool)
+            for c in range(1 << len(m)):
+                off = [(c >> a) & 1 for a in range(len(m))]
+                touched |= kept[tuple(slice(o, o + k + 1) for o, k in zip(off, m))]
+            self._isolated = np.nonzero(~touched.ravel())[0].astype(np.int64)
+        else:
+            self._excl = np.zeros(n_el, bool)
+            self._isolated = np.zeros(0, np.int64)
 
     def _thickness(self):
         """Per element the thickness of the same-material run in its column

@@ -101,6 +101,7 @@ class _StaticBase(_sb.FemSolver):
         laws = [m.thermk_law() if m is not None else None for m in self._mat]
         if all(l is not None for l in laws):
             L = np.array(laws)
+            L[self._excl, 0], L[self._excl, 1] = _sb.MASKED_K   # masked-out (empty) elements
             c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
         else:
             mats = self._mat
@@ -110,11 +111,14 @@ class _StaticBase(_sb.FemSolver):
                 groups.setdefault(id(m), (m, []))[1].append(e)
             groups = [(m, np.array(ix)) for m, ix in groups.values()]
 
+            excl = self._excl
+
             def kfun(tavg):
-                kh = np.empty(len(tavg))
-                kv = np.empty(len(tavg))
+                kh = np.full(len(tavg), _sb.MASKED_K[0])
+                kv = np.full(len(tavg), _sb.MASKED_K[1])
                 for m, ix in groups:
-                    if m is None:
+                    ix = ix[~excl[ix]]
+                    if len(ix) and m is None:
                         raise ValueError("%s: element without material" % self.id)
                     for e in ix:
                         k = m.thermk(float(tavg[e]), float(h[e]))
@@ -139,9 +143,13 @@ class _StaticBase(_sb.FemSolver):
         c = self._core
         c.inittemp = float(self.inittemp)
         c.maxerr = float(self.maxerr)
-        q = self.inHeat(self._element_mesh())
-        c.set_heat(np.ascontiguousarray(q, dtype=np.float64))
+        q = np.array(self.inHeat(self._element_mesh()), dtype=np.float64)
+        q[self._excl] = 0.0
+        c.set_heat(q)
         nodes, vals = self.temperature_boundary.resolve(self.mesh, self.geometry)
+        if len(self._isolated):  # nodes of the masked-out region hold the initial temperature
+            nodes = np.concatenate([nodes, self._isolated])
+            vals = np.concatenate([vals, np.full(len(self._isolated), float(self.inittemp))])
         c.set_dirichlet(nodes.astype(np.int64), vals)
         fn, fv = _bc_lists(self.heatflux_boundary, self.mesh, self.geometry, float)
         rn, rv = _bc_lists(self.radiation_boundary, self.mesh, self.geometry, Radiation.of)
@@ -192,7 +200,9 @@ class _StaticBase(_sb.FemSolver):
     def outThermalConductivity(self, mesh, interpolation="nearest"):
         """Element conductivities (in-plane, vertical) at the element temperatures."""
         self.initialize()
-        return _sb.nearest_elements(self._axes, self._element_k(), mesh)
+        k = self._element_k()
+        k[self._excl] = np.nan   # not in the masked mesh
+        return _sb.nearest_elements(self._axes, k, mesh)
 
     def outHeatFlux(self, mesh, interpolation="linear"):
         """Heat flux -k grad T (W/m^2); uses the leaf height for thermk as the reference

@@ -177,6 +177,71 @@ def test_shockley3d_computations():
     assert abs(s.get_total_heat() - correct_current * 1.) < 0.5e-3
 
 
+def test_shockley3d_computations_excluded():
+    """shockley3d.py:71-73: the same answers with the empty (air) elements beside the
+    700 um contacts taken out of the mesh."""
+    s, S = _shockley3d()
+    s.empty_elements = "exclude"
+    s.initialize()
+    assert s._excl.any() and len(s._isolated) > 0
+    s.compute(1000)
+    correct_current = 1e-9 * S * s.js * (math.exp(s.beta) - 1)
+    assert abs(s.get_total_current() - correct_current) < 0.5e-3
+    capacitance = eps0 * material.GaAs().eps() * S / 0.02
+    assert abs(s.get_capacitance() - capacitance) < 0.5e-2
+    assert abs(s.get_total_heat() - correct_current * 1.) < 0.5e-3
+
+
+def _slab_2d(extra_air_rows, cls=Static2D):
+    """Two-layer slab 10 um wide with a heated strip; extra_air_rows element rows of air
+    above the stack (mesh beyond the geometry)."""
+    stack = geometry.Stack2D()
+    stack.append(geometry.Rectangle(10., 1.5, material.Const(thermk=(3.0, 2.0), name="S0")))
+    stack.append(geometry.Rectangle(10., 0.8, material.Const(thermk=(40.0, 30.0), name="S1")))
+    s = cls("slab")
+    s.geometry = geometry.Cylindrical(stack) if cls is StaticCyl else geometry.Cartesian2D(stack)
+    x = np.linspace(0., 10., 11)
+    z = np.r_[np.linspace(0., 1.5, 4), np.linspace(1.5, 2.3, 3)[1:]]
+    z = np.r_[z, 2.3 + 0.5 * np.arange(1, extra_air_rows + 1)]
+    s.mesh = mesh.Rectangular2D(x, z)
+    s.temperature_boundary.append(s.mesh.Bottom(), 300.)
+    s.iterative.maxerr = 1e-13
+    em = s.mesh.element_mesh().points()
+    s.inHeat = np.where((em[:, 1] < 1.5) & (em[:, 1] > 0.5) & (em[:, 0] < 4.0), 2e15, 0.0)
+    return s
+
+
+@pytest.mark.parametrize("cls", [Static2D, StaticCyl])
+def test_masked_air_rows_match_the_smaller_mesh(cls):
+    """Air rows above the stack, masked out (the 2D default: algorithm 'cholesky' ->
+    empty elements excluded, fem_solver.hpp:182-188), give the temperatures of the mesh
+    that stops at the stack top; with 'include' the air conducts and they differ."""
+    ref = _slab_2d(0, cls)
+    assert ref.algorithm == "cholesky"
+    ref.compute(1)
+    T0 = ref.outTemperature(ref.mesh)
+    s = _slab_2d(3, cls)
+    s.compute(1)
+    n_stack = len(ref.mesh.axes[0]) * len(ref.mesh.axes[1])
+    T = s._core.solution()
+    assert np.abs(T[:n_stack] - T0).max() <= 1e-9 * T0.max()
+    assert np.abs(T0 - 300.).max() > 1.0
+    s2 = _slab_2d(3, cls)
+    s2.empty_elements = "include"
+    s2.compute(1)
+    assert np.abs(s2._core.solution()[:n_stack] - T0).max() > 1e-6
+
+
+def test_algorithm_defaults():
+    assert Static2D("a").algorithm == "cholesky" and StaticCyl("b").algorithm == "cholesky"
+    assert Static3D("c").algorithm == "iterative" and Shockley3D("d").algorithm == "iterative"
+    s = Static2D("e")
+    s.empty_elements = True
+    assert s.empty_elements == "include"
+    with pytest.raises(ValueError):
+        s.empty_elements = "sometimes"
+
+
 def test_shockley3d_computations_temp():
     s, S = _shockley3d()
     s.beta = lambda T: math.log(T * 70)
