@@ -207,6 +207,24 @@ class Space:
             idx[inside] = k
         return idx
 
+    def leaf_table_grid(self, axes):
+        """leaf_table of the points of a rectilinear grid (natural order, axis 0
+        fastest), from per-axis interval tests: a leaf is a box."""
+        axes = [np.asarray(a, dtype=float) for a in axes]
+        shape = [len(a) for a in axes][::-1]
+        idx = np.full(shape, -1, dtype=np.int64)
+        d = len(axes)
+        for k, (obj, o, s) in enumerate(self._leaves):
+            inside = np.ones(shape, bool)
+            for a in range(d):
+                m = (axes[a] >= o[a]) & (axes[a] < o[a] + s[a])
+                view = [1] * d
+                view[d - 1 - a] = len(axes[a])
+                inside &= m.reshape(view)
+            inside &= idx < 0
+            idx[inside] = k
+        return idx.ravel()
+
 
 class Cartesian2D(Space):
     def __init__(self, root, length=None):

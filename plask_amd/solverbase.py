@@ -267,11 +267,10 @@ class FemSolver:
     def _tables(self):
         msh = self.mesh
         g = self._geometry
-        mids = msh.element_mesh().points()
-        self._mids = mids
-        self._leaf = g.leaf_table(mids)
+        self._leaf = g.leaf_table_grid(msh.element_mesh().axes)
         leaves = g._leaves
         air = _material.Air()
+        self._mat_bg = air
         leaf_mat = [lf[0].material for lf in leaves] + [air]
         none = frozenset()
         leaf_roles = [frozenset(lf[0].roles) for lf in leaves] + [none]
@@ -308,18 +307,22 @@ class FemSolver:
         m = [len(a) - 1 for a in self._axes]
         ncol = int(np.prod(m[:-1]))
         nz = m[-1]
-        lay = np.array([id(x) for x in self._mat]).reshape(nz, ncol)
+        # material of each element as an index (same object = same material)
+        mats = [lf[0].material for lf in self._geometry._leaves] + [None]
+        first = {}
+        leaf_mid = np.array([first.setdefault(id(x), len(first)) for x in mats], dtype=np.int64)
+        lay = leaf_mid[self._leaf].reshape(nz, ncol)
         z = self._axes[-1]
-        h = np.empty((nz, ncol))
-        for c in range(ncol):
-            k = 0
-            while k < nz:
-                k2 = k
-                while k2 + 1 < nz and lay[k2 + 1, c] == lay[k, c]:
-                    k2 += 1
-                h[k:k2 + 1, c] = z[k2 + 1] - z[k]
-                k = k2 + 1
-        return h.ravel()
+        k = np.arange(nz)[:, None]
+        # a run of equal materials starts where the material changes going up and ends
+        # below the next change; every element takes its run's height
+        starts = np.ones((nz, ncol), bool)
+        starts[1:] = lay[1:] != lay[:-1]
+        ends = np.ones((nz, ncol), bool)
+        ends[:-1] = lay[1:] != lay[:-1]
+        s = np.maximum.accumulate(np.where(starts, k, 0), axis=0)
+        e = np.minimum.accumulate(np.where(ends, k, nz)[::-1], axis=0)[::-1]
+        return (z[e + 1] - z[s]).ravel()
 
     def _element_volume(self):
         s = self._esize

@@ -98,9 +98,11 @@ class _StaticBase(_sb.FemSolver):
         c = self._core
         c.inittemp = float(self.inittemp)
         self._h = self._thickness()
-        laws = [m.thermk_law() if m is not None else None for m in self._mat]
+        # one law per leaf (the last entry: the background), indexed per element
+        leaf_mats = [lf[0].material for lf in self._geometry._leaves] + [self._mat_bg]
+        laws = [m.thermk_law() if m is not None else None for m in leaf_mats]
         if all(l is not None for l in laws):
-            L = np.array(laws)
+            L = np.array(laws)[self._leaf]
             L[self._excl, 0], L[self._excl, 1] = _sb.MASKED_K   # masked-out (empty) elements
             c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
         else:

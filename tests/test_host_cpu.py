@@ -167,3 +167,43 @@ def test_plugin_module_names():
         from electrical.shockley import Shockley2D  # noqa: F401
     finally:
         sys.path.remove(plugins.PATH)
+
+
+def test_grid_leaf_table_and_thickness_scan():
+    """The per-axis leaf lookup of a grid equals the point-by-point one, and the
+    vectorized thickness scan equals the column walk of therm2d.cpp:81-113 (an element
+    takes the height of the run of same-material elements in its column)."""
+    from plask_amd import geometry, material, mesh
+    from plask_amd.thermal.static import Static2D
+    a = material.Const(thermk=(2.0, 2.0), name="TA")
+    b = material.Const(thermk=(5.0, 5.0), name="TB")
+    shelf = geometry.Shelf()
+    shelf.append(geometry.Rectangle(3.0, 0.7, b))
+    shelf.append_gap(2.0)
+    shelf.append(geometry.Rectangle(4.0, 0.7, a))
+    stack = geometry.Stack2D()
+    stack.append(geometry.Rectangle(9.0, 1.0, a))
+    stack.append(geometry.Rectangle(9.0, 0.5, a))   # same material: one run with the layer below
+    stack.append(geometry.Rectangle(9.0, 0.4, b))
+    stack.append(shelf)
+    s = Static2D("t")
+    s.geometry = geometry.Cartesian2D(stack)
+    rng = np.random.default_rng(3)
+    x = np.unique(np.r_[0.0, 9.0, np.sort(rng.uniform(0, 9, 14))])
+    z = np.unique(np.r_[0.0, 1.0, 1.5, 1.9, 2.6, np.sort(rng.uniform(0, 2.6, 9)), 3.2])
+    s.mesh = mesh.Rectangular2D(x, z)
+    em = s.mesh.element_mesh()
+    assert np.array_equal(s.geometry.leaf_table_grid(em.axes), s.geometry.leaf_table(em.points()))
+    s._tables()
+    h = s._thickness()
+    lay = np.array([id(m) for m in s._mat]).reshape(len(z) - 1, len(x) - 1)
+    want = np.empty_like(lay, dtype=float)
+    for c in range(lay.shape[1]):
+        k = 0
+        while k < lay.shape[0]:
+            k2 = k
+            while k2 + 1 < lay.shape[0] and lay[k2 + 1, c] == lay[k, c]:
+                k2 += 1
+            want[k:k2 + 1, c] = z[k2 + 1] - z[k]
+            k = k2 + 1
+    assert np.array_equal(h, want.ravel())
