@@ -191,6 +191,8 @@ void ThermalCore::set_boundary_loads(const std::vector<int64_t>& flux_nodes, con
 // current temperature in 2D; in 3D the reference reads the temperature of the mesh node
 // whose index is the wall node's local number (0-7), kept here.
 void ThermalCore::boundary_load() {
+    if ((have_flux_ && (int64_t)flux_.size() != N_) || (have_rad_ && (int64_t)emis_.size() != N_))
+        throw std::runtime_error(id + ": boundary conditions were set for another mesh");
     const double SB = 5.670373e-8;  // plask/phys/constants.hpp:41
     std::vector<double> T;
     if (have_rad_) T = solution();

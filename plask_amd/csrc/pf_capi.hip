@@ -456,6 +456,7 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     HIPCK(hipMemset(ctx->q, 0, sizeof(double) * ctx->E));
     HIPCK(hipMemset(ctx->x, 0, sizeof(double) * N));
     ctx->nd = 0;
+    ctx->nl = 0;  // nodal loads refer to the old grid
     ctx->dvals_sq = 0.;
     ctx->have_coeffs = false;
     ctx->have_source = false;
