@@ -1086,51 +1086,104 @@ __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, c
 // inverse); with v = L^-1 e_F (v_0 = 1, v_k = -la_k w_{k-1} v_{k-1}):
 // (T^-1)_{FF} = sum_k v_k^2 w_k and (T^-1)_{LF} = w_L v_L.
 // ---------------------------------------------------------------------------
-template <typename T, class BC>
-__device__ void part_setup_column(const BC& bc, int n, int m, long long plane, int col, int ncol,
-                                  T* __restrict__ la, T* __restrict__ lw, T* __restrict__ lcp,
-                                  T* __restrict__ schur) {
-    const int P = (n + m - 1) / m;
-    double cprev = 0., cpprev = 0.;
-    for (int k = 0; k < n; ++k) {
-        const long long node = (long long)k * plane + col;
-        double b, c;
-        bc(k, node, b, c);
-        const int s = k / m, u = k - s * m;
-        if (u == m - 1 && s < P - 1) {  // separator
-            la[node] = T(cprev);
-            lw[node] = T(b);
-            lcp[node] = T(c);
-            cprev = 0.;
-            cpprev = 0.;
-        } else {
-            const bool first = u == 0;
-            const bool last = s < P - 1 ? u == m - 2 : k == n - 1;
-            const double a = first ? 0. : cprev;
-            const double w = 1. / (b - a * cpprev);
-            const double cp = last ? 0. : c * w;
-            la[node] = T(a);
-            lw[node] = T(w);
-            lcp[node] = T(cp);
-            cprev = c;
-            cpprev = cp;
-        }
+// The factors are set up in three parallel steps (a serial walk up each column costs
+// ~1 ms per grid whatever its size, as the planes are the critical path):
+// 1. per node: the line's diagonal b and upper coupling c (LineBC3 / LineBC2);
+// 2. per (column, segment): the interior factors of the segment, and the two sums of the segment's inverse the separator step needs
+//    (ff = e_1^T T^-1 e_1 and fl = its first-to-last entry, from the stored factors);
+// 3. per column: the LU recurrence of the separator (Schur complement) system.
+template <typename BC>
+__device__ __forceinline__ void line_bc_node(const BC& bc, int k, long long node, double* __restrict__ bb,
+                                             double* __restrict__ cc) {
+    double b, c;
+    bc(k, node, b, c);
+    bb[node] = b;
+    cc[node] = c;
+}
+
+__global__ void k_line_bc3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                            const uint64_t* __restrict__ dmask, double* __restrict__ bb, double* __restrict__ cc) {
+    const long long plane = (long long)g.n0 * g.n1, N = plane * g.n2;
+    for (long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        const int col = (int)(node % plane);
+        const LineBC3 bc{g, kh, kv, dmask, col % g.n0, col / g.n0, plane};
+        line_bc_node(bc, (int)(node / plane), node, bb, cc);
     }
+}
+
+__global__ void k_line_bc2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
+                            const uint64_t* __restrict__ dmask, double* __restrict__ bb, double* __restrict__ cc) {
+    const long long N = (long long)g.n0 * g.n1;
+    for (long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x; node < N;
+         node += (long long)gridDim.x * blockDim.x) {
+        const LineBC2 bc{g, kh, kv, dmask, (int)(node % g.n0)};
+        line_bc_node(bc, (int)(node / g.n0), node, bb, cc);
+    }
+}
+
+// thread = (column, segment); sff / sfl: [segment][column] (segment 0 unused)
+template <typename T>
+__global__ void k_part_factor(int ncol, int n, int m, long long plane, const double* __restrict__ bb,
+                              const double* __restrict__ cc, T* __restrict__ la, T* __restrict__ lw,
+                              T* __restrict__ lcp, double* __restrict__ sff, double* __restrict__ sfl) {
+    const int P = (n + m - 1) / m;
+    const long long total = (long long)P * ncol;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int col = (int)(t % ncol), s = (int)(t / ncol);
+        const int k0 = s * m, k1 = min(k0 + m, n);
+        double cprev = 0., cpprev = 0.;
+        for (int k = k0; k < k1; ++k) {
+            const long long node = (long long)k * plane + col;
+            const double b = bb[node], c = cc[node];
+            const int u = k - k0;
+            if (u == m - 1 && s < P - 1) {  // separator
+                la[node] = T(cprev);
+                lw[node] = T(b);
+                lcp[node] = T(c);
+            } else {
+                const bool last = s < P - 1 ? u == m - 2 : k == n - 1;
+                const double a = u == 0 ? 0. : cprev;
+                const double w = 1. / (b - a * cpprev);
+                const double cp = last ? 0. : c * w;
+                la[node] = T(a);
+                lw[node] = T(w);
+                lcp[node] = T(cp);
+                cprev = c;
+                cpprev = cp;
+            }
+        }
+        if (s == 0) continue;
+        // interior of this segment (what follows the separator of segment s - 1), from
+        // the stored (precision T) factors as the separator step reads them
+        const int ke = s < P - 1 ? k0 + m - 1 : n;
+        double v = 1., ff = 0., wprev = 0.;
+        for (int k = k0; k < ke; ++k) {
+            const long long node = (long long)k * plane + col;
+            const double w = lw[node];
+            if (k > k0) v = -(double)la[node] * wprev * v;
+            ff += v * v * w;
+            wprev = w;
+        }
+        sff[(size_t)s * ncol + col] = ff;
+        sfl[(size_t)s * ncol + col] = v * wprev;
+    }
+}
+
+template <typename T>
+__global__ void k_part_schur(int ncol, int n, int m, long long plane, const T* __restrict__ la,
+                             const T* __restrict__ lw, const T* __restrict__ lcp, const double* __restrict__ sff,
+                             const double* __restrict__ sfl, T* __restrict__ schur) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int P = (n + m - 1) / m;
     double off_prev = 0., sc_prev = 0.;
     for (int j = 0; j < P - 1; ++j) {
         const long long ns = (long long)((j + 1) * m - 1) * plane + col;
         const double elo = la[ns], bs = lw[ns], ehi = lcp[ns];
-        const double wl = lw[ns - plane];  // (T_j^-1)_{LL}
-        const int kb = (j + 1) * m, ke = j + 1 < P - 1 ? (j + 2) * m - 1 : n;
-        double v = 1., ff = 0., wprev = 0.;
-        for (int k = kb; k < ke; ++k) {
-            const long long node = (long long)k * plane + col;
-            const double w = lw[node];
-            if (k > kb) v = -(double)la[node] * wprev * v;
-            ff += v * v * w;
-            wprev = w;
-        }
-        const double fl = v * wprev;
+        const double wl = lw[ns - plane];
+        const double ff = sff[(size_t)(j + 1) * ncol + col], fl = sfl[(size_t)(j + 1) * ncol + col];
         const double sjj = bs - elo * elo * wl - ehi * ehi * ff;
         const double soff = j + 1 < P - 1 ? -ehi * fl * (double)la[(long long)((j + 2) * m - 1) * plane + col] : 0.;
         const double w = 1. / (sjj - off_prev * sc_prev);
@@ -1141,28 +1194,6 @@ __device__ void part_setup_column(const BC& bc, int n, int m, long long plane, i
         off_prev = soff;
         sc_prev = cp;
     }
-}
-
-template <typename T>
-__global__ void k_part_setup3d(Axes3<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
-                               const uint64_t* __restrict__ dmask, int m, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp, T* __restrict__ schur) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncol = g.n0 * g.n1;
-    if (col >= ncol) return;
-    const long long plane = ncol;
-    LineBC3 bc{g, kh, kv, dmask, col % g.n0, col / g.n0, plane};
-    part_setup_column<T>(bc, g.n2, m, plane, col, ncol, la, lw, lcp, schur);
-}
-
-template <typename T>
-__global__ void k_part_setup2d(Axes2<double> g, const double* __restrict__ kh, const double* __restrict__ kv,
-                               const uint64_t* __restrict__ dmask, int m, T* __restrict__ la, T* __restrict__ lw,
-                               T* __restrict__ lcp, T* __restrict__ schur) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.n0) return;
-    LineBC2 bc{g, kh, kv, dmask, i};
-    part_setup_column<T>(bc, g.n1, m, g.n0, i, g.n0, la, lw, lcp, schur);
 }
 
 // Column kernel: one thread per line segment (blockIdx.y selects the segment; with

@@ -80,6 +80,8 @@ struct pf_ctx {
     double* dvals = nullptr;
     int64_t nd = 0;
     double dvals_sq = 0.;
+    // line-factor set-up scratch: per-node line coefficients, per-segment sums
+    double *sb = nullptr, *sc = nullptr, *sff = nullptr, *sfl = nullptr;
     // nodal load added to the element load vector (boundary heat flux / radiation)
     long long* lnodes = nullptr;
     double* lvals = nullptr;
@@ -307,6 +309,12 @@ static int line_config(pf_ctx* ctx) {
     const size_t n32 = s32.m ? 3 * (size_t)std::max(s32.P - 1, 1) * ncol : 0;
     if ((rc = dalloc(ctx, &ctx->schur, n64))) return rc;
     if ((rc = dalloc(ctx, &ctx->schur32, n32))) return rc;
+    // set-up scratch (the coarse levels have the same planes and fewer columns)
+    const int pmin = (s64.m && s32.m) ? std::min(s64.m, s32.m) : std::max(s64.m, s32.m);
+    const size_t nseg = pmin ? (size_t)((n + pmin - 1) / pmin) * ncol : 0;
+    if ((rc = dalloc(ctx, &ctx->sb, nseg ? (size_t)ctx->N : 0)) || (rc = dalloc(ctx, &ctx->sc, nseg ? (size_t)ctx->N : 0)) ||
+        (rc = dalloc(ctx, &ctx->sff, nseg)) || (rc = dalloc(ctx, &ctx->sfl, nseg)))
+        return rc;
     ctx->dirty = true;  // the factors depend on the partition
     drop_graphs(ctx);
     return PF_OK;
@@ -670,6 +678,19 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
 
 static int mg_refresh(pf_ctx* ctx);
 
+// Partitioned line factors of one grid from the line coefficients in ctx->sb / ctx->sc
+// (k_line_bc3d / k_line_bc2d): segments in parallel, then the separator recurrence per
+// column.
+template <typename T>
+static void part_factor(pf_ctx* ctx, int ncol, int n, int m, T* la, T* lw, T* lcp, T* schur) {
+    const long long nseg = (long long)((n + m - 1) / m) * ncol;
+    hipLaunchKernelGGL((k_part_factor<T>), dim3(grid1(nseg, 256, 1 << 20)), dim3(256), 0, ctx->stream, ncol, n, m,
+                       (long long)ncol, (const double*)ctx->sb, (const double*)ctx->sc, la, lw, lcp, ctx->sff, ctx->sfl);
+    hipLaunchKernelGGL((k_part_schur<T>), dim3(grid1(ncol, 64, 1 << 20)), dim3(64), 0, ctx->stream, ncol, n, m,
+                       (long long)ncol, (const T*)la, (const T*)lw, (const T*)lcp, (const double*)ctx->sff,
+                       (const double*)ctx->sfl, schur);
+}
+
 // F, Dinv (FP64/FP32) and ||B||: B = M (F - A0 xD) + xD, xD = Dirichlet values
 static int prepare(pf_ctx* ctx) {
     if (!ctx->dirty) return PF_OK;
@@ -683,16 +704,17 @@ static int prepare(pf_ctx* ctx) {
         hipLaunchKernelGGL((k_diag3d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0] * ctx->n[1], 64, 1 << 20);
+        const int ncol = (int)(ctx->n[0] * ctx->n[1]), n2 = (int)ctx->n[2];
+        if (ctx->pm64 || ctx->pm32)
+            hipLaunchKernelGGL(k_line_bc3d, dim3(gb), dim3(256), 0, ctx->stream, axes3<double>(ctx), ctx->kh, ctx->kv,
+                               ctx->dmask, ctx->sb, ctx->sc);
         if (ctx->pm64)
-            hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
-                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
+            part_factor<double>(ctx, ncol, n2, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
         else
             hipLaunchKernelGGL((k_line_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
                                ctx->kh, ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
         if (ctx->pm32)
-            hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
-                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32,
-                               ctx->schur32);
+            part_factor<float>(ctx, ncol, n2, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32, ctx->schur32);
         else
             hipLaunchKernelGGL((k_line_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes3<double>(ctx),
                                ctx->kh, ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
@@ -703,16 +725,17 @@ static int prepare(pf_ctx* ctx) {
         hipLaunchKernelGGL((k_diag2d<float>), dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
                            ctx->dmask, ctx->dinv32);
         const int cb = grid1(ctx->n[0], 64, 1 << 20);
+        const int ncol = (int)ctx->n[0], n1 = (int)ctx->n[1];
+        if (ctx->pm64 || ctx->pm32)
+            hipLaunchKernelGGL(k_line_bc2d, dim3(gb), dim3(256), 0, ctx->stream, axes2<double>(ctx), ctx->kh, ctx->kv,
+                               ctx->dmask, ctx->sb, ctx->sc);
         if (ctx->pm64)
-            hipLaunchKernelGGL((k_part_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
-                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
+            part_factor<double>(ctx, ncol, n1, ctx->pm64, ctx->la, ctx->lw, ctx->lcp, ctx->schur);
         else
             hipLaunchKernelGGL((k_line_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
                                ctx->kh, ctx->kv, ctx->dmask, ctx->la, ctx->lw, ctx->lcp);
         if (ctx->pm32)
-            hipLaunchKernelGGL((k_part_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
-                               ctx->kh, ctx->kv, ctx->dmask, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32,
-                               ctx->schur32);
+            part_factor<float>(ctx, ncol, n1, ctx->pm32, ctx->la32, ctx->lw32, ctx->lcp32, ctx->schur32);
         else
             hipLaunchKernelGGL((k_line_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, axes2<double>(ctx),
                                ctx->kh, ctx->kv, ctx->dmask, ctx->la32, ctx->lw32, ctx->lcp32);
@@ -834,20 +857,18 @@ static int mg_refresh(pf_ctx* ctx) {
             if ((rc = dalloc(ctx, &L.schur32, 3 * (size_t)std::max(s32.P - 1, 1) * ncol))) return rc;
             L.schur_m32 = s32.m;
         }
-        const int cb = grid1(ncol, 64, 1 << 20);
+        const int gl = grid1((long long)ncol * n2, 256, 4096);
         if (ctx->dim == 3) {
             const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz};
-            hipLaunchKernelGGL((k_part_setup3d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                               s64.m, L.la, L.lw, L.lcp, L.schur);
-            hipLaunchKernelGGL((k_part_setup3d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                               s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+            hipLaunchKernelGGL(k_line_bc3d, dim3(gl), dim3(256), 0, ctx->stream, g, L.kh, L.kv, L.dmask, ctx->sb,
+                               ctx->sc);
         } else {
             const Axes2<double> g{L.n0, n2, L.hx, ctx->hy, L.rmid};
-            hipLaunchKernelGGL((k_part_setup2d<double>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                               s64.m, L.la, L.lw, L.lcp, L.schur);
-            hipLaunchKernelGGL((k_part_setup2d<float>), dim3(cb), dim3(64), 0, ctx->stream, g, L.kh, L.kv, L.dmask,
-                               s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
+            hipLaunchKernelGGL(k_line_bc2d, dim3(gl), dim3(256), 0, ctx->stream, g, L.kh, L.kv, L.dmask, ctx->sb,
+                               ctx->sc);
         }
+        part_factor<double>(ctx, ncol, n2, s64.m, L.la, L.lw, L.lcp, L.schur);
+        part_factor<float>(ctx, ncol, n2, s32.m, L.la32, L.lw32, L.lcp32, L.schur32);
     }
     HIPCK(hipGetLastError());
     return PF_OK;
