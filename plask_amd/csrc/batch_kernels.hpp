@@ -215,19 +215,51 @@ __device__ inline void line_factors_l(const LView& V) {
 template <int BS>
 __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* tmp, const double* xin, double* xout,
                                     double om) {
+    // The loads of a chunk of U planes do not depend on the recurrence: they are all
+    // issued before its U steps, so the sweep waits for memory once per chunk, not once
+    // per plane.  A chunk reads all of its nodes before writing any (rhs may be tmp,
+    // xin may be xout).
+    constexpr int U = 8;
     const int tid = threadIdx.x;
+    const int n1 = V.n1;
     for (int i = tid; i < V.n0; i += BS) {
         double d = 0.;
-        for (int k = 0; k < V.n1; ++k) {
-            const long long node = (long long)k * V.n0 + i;
-            d = (rhs[node] - V.la[node] * d) * V.lw[node];
-            tmp[node] = d;
+        for (int k0 = 0; k0 < n1; k0 += U) {
+            double rv[U], av[U], wv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(k0 + u, n1 - 1);
+                const long long node = (long long)k * V.n0 + i;
+                rv[u] = rhs[node];
+                av[u] = V.la[node];
+                wv[u] = V.lw[node];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (k0 + u < n1) {
+                    d = (rv[u] - av[u] * d) * wv[u];
+                    tmp[(long long)(k0 + u) * V.n0 + i] = d;
+                }
+            }
         }
         double zn = 0.;
-        for (int k = V.n1 - 1; k >= 0; --k) {
-            const long long node = (long long)k * V.n0 + i;
-            zn = tmp[node] - V.lc[node] * zn;
-            xout[node] = (xin ? xin[node] : 0.) + om * zn;
+        for (int k0 = n1 - 1; k0 >= 0; k0 -= U) {
+            double tv[U], cv[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = max(k0 - u, 0);
+                const long long node = (long long)k * V.n0 + i;
+                tv[u] = tmp[node];
+                cv[u] = V.lc[node];
+                xv[u] = xin ? xin[node] : 0.;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (k0 - u >= 0) {
+                    zn = tv[u] - cv[u] * zn;
+                    xout[(long long)(k0 - u) * V.n0 + i] = xv[u] + om * zn;
+                }
+            }
         }
     }
     __syncthreads();
