@@ -219,7 +219,7 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
     // issued before its U steps, so the sweep waits for memory once per chunk, not once
     // per plane.  A chunk reads all of its nodes before writing any (rhs may be tmp,
     // xin may be xout).
-    constexpr int U = 8;
+    constexpr int U = 16;
     const int tid = threadIdx.x;
     const int n1 = V.n1;
     for (int i = tid; i < V.n0; i += BS) {
