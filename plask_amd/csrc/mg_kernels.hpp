@@ -36,7 +36,26 @@ __device__ __forceinline__ T tweight(int m, T w, int I) {
     return m == I ? T(1) - w : (m + 1 == I ? w : T(0));
 }
 
-// b_c = R r_f (transpose of the interpolation), zero at coarse Dirichlet nodes
+// the (up to) 3 fine nodes of coarse node I along one axis and their weights: the
+// fine nodes strictly between the neighbouring coarse nodes; absent ones repeat the
+// first index with weight 0, so the 3 x 3 gather below has fixed addresses and all its
+// loads issue together
+template <typename T>
+__device__ __forceinline__ void restrict_axis(int I, int nc, const int* __restrict__ cidx, const int* __restrict__ m,
+                                              const T* __restrict__ w, int (&idx)[3], T (&wt)[3]) {
+    const int lo = I > 0 ? cidx[I - 1] + 1 : 0;
+    const int hi = I < nc - 1 ? cidx[I + 1] - 1 : cidx[I];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i = lo + a;
+        const bool ok = i <= hi;
+        idx[a] = ok ? i : lo;
+        wt[a] = ok ? tweight(m[idx[a]], w[idx[a]], I) : T(0);
+    }
+}
+
+// b_c = R r_f (transpose of the interpolation), zero at coarse Dirichlet nodes; a
+// coarse node's support has at most 3 fine nodes per in-plane axis (coarsening by 2)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __restrict__ wx,
                                                      const T* __restrict__ wy, const T* __restrict__ rf,
@@ -47,24 +66,21 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __r
     const long long Nc = planec * tr.n2;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nc;
          idx += (long long)gridDim.x * blockDim.x) {
-        if (dbit(dmask_c, idx)) {
-            bc[idx] = T(0);
-            continue;
-        }
         const int I = (int)(idx % tr.n0c);
         const int J = (int)((idx / tr.n0c) % tr.n1c);
         const long long k = idx / planec;
-        const int ilo = I > 0 ? tr.cx[I - 1] + 1 : 0, ihi = I < tr.n0c - 1 ? tr.cx[I + 1] - 1 : tr.cx[I];
-        const int jlo = J > 0 ? tr.cy[J - 1] + 1 : 0, jhi = J < tr.n1c - 1 ? tr.cy[J + 1] - 1 : tr.cy[J];
+        int ii[3], jj[3];
+        T wi[3], wj[3];
+        restrict_axis(I, tr.n0c, tr.cx, tr.mx, wx, ii, wi);
+        restrict_axis(J, tr.n1c, tr.cy, tr.my, wy, jj, wj);
         const T* src = rf + k * planef;
         T s = T(0);
-        for (int j = jlo; j <= jhi; ++j) {
-            const T wj = tweight(tr.my[j], wy[j], J);
-            T sx = T(0);
-            for (int i = ilo; i <= ihi; ++i) sx += tweight(tr.mx[i], wx[i], I) * src[(long long)j * tr.n0f + i];
-            s += wj * sx;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const T* row = src + (long long)jj[b] * tr.n0f;
+            s += wj[b] * (wi[0] * row[ii[0]] + wi[1] * row[ii[1]] + wi[2] * row[ii[2]]);
         }
-        bc[idx] = s;
+        bc[idx] = dbit(dmask_c, idx) ? T(0) : s;
     }
 }
 
