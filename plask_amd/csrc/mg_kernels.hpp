@@ -95,17 +95,20 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(MgTransfer tr, const T* 
     const long long Nf = planef * tr.n2;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nf;
          idx += (long long)gridDim.x * blockDim.x) {
-        if (dbit(dmask_f, idx)) continue;
+        // the Dirichlet test selects the value instead of skipping the node, so the
+        // mask, table and vector loads do not wait on a branch
+        const bool fixed = dbit(dmask_f, idx);
         const int i = (int)(idx % tr.n0f);
         const int j = (int)((idx / tr.n0f) % tr.n1f);
         const long long k = idx / planef;
         const int I = tr.mx[i], J = tr.my[j];
         const int I1 = min(I + 1, tr.n0c - 1), J1 = min(J + 1, tr.n1c - 1);
         const T a = wx[i], b = wy[j];
+        const T xo = xf[idx];
         const T* c = xc + k * planec;
         const T lo = (T(1) - a) * c[(long long)J * tr.n0c + I] + a * c[(long long)J * tr.n0c + I1];
         const T hi = (T(1) - a) * c[(long long)J1 * tr.n0c + I] + a * c[(long long)J1 * tr.n0c + I1];
-        xf[idx] += (T(1) - b) * lo + b * hi;
+        xf[idx] = fixed ? xo : xo + ((T(1) - b) * lo + b * hi);
     }
 }
 
