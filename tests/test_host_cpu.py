@@ -207,3 +207,24 @@ def test_grid_leaf_table_and_thickness_scan():
             want[k:k2 + 1, c] = z[k2 + 1] - z[k]
             k = k2 + 1
     assert np.array_equal(h, want.ravel())
+
+
+def test_boundary_condition_values_and_lists():
+    """Convection / Radiation values (therm_python.cpp:25-86: object, dict with ambient
+    defaulting to 300 K, or pair) and the node lists handed to the thermal core, in list
+    order (the core keeps a node's first value, boundary_conditions.hpp:182-186)."""
+    from plask_amd import mesh
+    from plask_amd.thermal.static import Convection, Radiation, _bc_lists
+    r = Radiation.of({"emissivity": 0.4})
+    assert (r.emissivity, r.ambient) == (0.4, 300.0)
+    c = Convection.of((7.0, 310.0))
+    assert c["coeff"] == 7.0 and str(c) == "7.0 (310.0K)"
+    with pytest.raises(TypeError):
+        Radiation.of(0.5)
+    bcs = mesh.BoundaryConditions()
+    msh = mesh.Rectangular2D(np.linspace(0, 3, 4), np.linspace(0, 2, 3))
+    bcs.append(msh.Top(), (0.9, 800.0))
+    bcs.append(msh.Right(), Radiation(0.2, 400.0))
+    nodes, vals = _bc_lists(bcs, msh, None, Radiation.of)
+    assert list(nodes) == [8, 9, 10, 11, 3, 7, 11]
+    assert [(v.emissivity, v.ambient) for v in vals] == [(0.9, 800.0)] * 4 + [(0.2, 400.0)] * 3
