@@ -228,3 +228,28 @@ def test_boundary_condition_values_and_lists():
     nodes, vals = _bc_lists(bcs, msh, None, Radiation.of)
     assert list(nodes) == [8, 9, 10, 11, 3, 7, 11]
     assert [(v.emissivity, v.ambient) for v in vals] == [(0.9, 800.0)] * 4 + [(0.2, 400.0)] * 3
+
+
+def test_masked_mesh_tables():
+    """empty_elements: air elements leave the mesh under 'exclude' and under 'default' with
+    the 2D default algorithm 'cholesky' (fem_solver.hpp:182-188); nodes touching only air
+    elements are held.  Here: air rows above a 2-layer stack."""
+    from plask_amd import geometry, material, mesh
+    from plask_amd.thermal.static import Static2D, Static3D
+    stack = geometry.Stack2D()
+    stack.append(geometry.Rectangle(4.0, 1.0, material.Const(thermk=(1.0, 1.0), name="M0")))
+    stack.append(geometry.Rectangle(4.0, 1.0, material.Const(thermk=(2.0, 2.0), name="M1")))
+    s = Static2D("m")
+    s.geometry = geometry.Cartesian2D(stack)
+    s.mesh = mesh.Rectangular2D(np.linspace(0, 4, 5), np.array([0.0, 1.0, 2.0, 3.0, 4.0]))
+    assert s.algorithm == "cholesky" and s._excludes_empty()
+    s._tables()
+    assert s._excl.reshape(4, 4).tolist() == [[False] * 4, [False] * 4, [True] * 4, [True] * 4]
+    assert list(s._isolated) == list(range(15, 25))   # node rows 3 and 4 touch only air
+    s.empty_elements = "include"
+    s._tables()
+    assert not s._excl.any() and len(s._isolated) == 0
+    s.algorithm = "iterative"
+    s.empty_elements = "default"
+    assert not s._excludes_empty()
+    assert Static3D("n").algorithm == "iterative"
