@@ -145,6 +145,12 @@ int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int6
  * :261-268), evaluated by the caller.  Repeated nodes are summed; n = 0 clears it.  Kept until replaced. */
 int pf_load_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n);
 
+/* Linear interpolation of the current solution (node values) at npts points
+ * (pts: npts x dim, row-major; um), the providers' lazy interpolation from the solver
+ * mesh to a destination mesh (getTemperatures therm2d.cpp:543-552, therm3d.cpp:387-395;
+ * getVoltage electr2d.cpp:507-517); points outside the grid get `fill`. */
+int pf_interp_nodes(pf_ctx* ctx, const double* pts, int64_t npts, double fill, double* out);
+
 /* Solve the eliminated system A X = B.  x_inout: host array of N node values
  * (start vector in, solution out); may be NULL with PF_X0_ZERO / PF_X0_DEVICE to
  * leave the solution on the device (pf_get_solution copies it out). */

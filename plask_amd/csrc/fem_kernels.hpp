@@ -1627,6 +1627,54 @@ __global__ void k_scatter(long long m, const long long* __restrict__ nodes, cons
         x[nodes[i]] = vals[i];
 }
 
+// Linear (bi-/trilinear) interpolation of node values at arbitrary points: per axis the
+// interval by bisection over the node coordinates (ax: axis 0, then 1, then 2), points
+// outside the grid (or NaN) get `fill`.  pts: npts x dim, row-major.
+__global__ void k_interp_nodes(const double* __restrict__ ax, int dim, int n0, int n1, int n2,
+                               const double* __restrict__ x, long long npts, const double* __restrict__ pts,
+                               double fill, double* __restrict__ out) {
+    const int n[3] = {n0, n1, n2};
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
+         p += (long long)gridDim.x * blockDim.x) {
+        int lo[3] = {0, 0, 0};
+        double t[3] = {0., 0., 0.};
+        bool inside = true;
+        const double* A = ax;
+        for (int a = 0; a < dim; ++a) {
+            const double c = pts[p * dim + a];
+            const int na = n[a];
+            if (!(c >= A[0] && c <= A[na - 1])) inside = false;
+            int l = 0, h = na - 1;
+            while (h - l > 1) {
+                const int mid = (l + h) >> 1;
+                if (A[mid] <= c) l = mid;
+                else h = mid;
+            }
+            lo[a] = l;
+            t[a] = (c - A[l]) / (A[l + 1] - A[l]);
+            A += na;
+        }
+        if (!inside) {
+            out[p] = fill;
+            continue;
+        }
+        const long long s1 = n0, s2 = (long long)n0 * n1;
+        const long long base = lo[0] + s1 * lo[1] + (dim == 3 ? s2 * lo[2] : 0);
+        double v = 0.;
+        for (int c = 0; c < (1 << dim); ++c) {
+            double w = 1.;
+            long long off = 0;
+            for (int a = 0; a < dim; ++a) {
+                const int b = (c >> a) & 1;
+                w *= b ? t[a] : 1. - t[a];
+                off += b ? (a == 0 ? 1 : (a == 1 ? s1 : s2)) : 0;
+            }
+            v += w * x[base + off];
+        }
+        out[p] = v;
+    }
+}
+
 // x[node] += val (nodes distinct)
 __global__ void k_scatter_add(long long m, const long long* __restrict__ nodes, const double* __restrict__ vals,
                               double* __restrict__ x) {

@@ -57,6 +57,12 @@ std::vector<double> FemCore::solution() {
     return x;
 }
 
+std::vector<double> FemCore::interp(const double* pts, int64_t n, double fill) {
+    std::vector<double> out(n);
+    check(pf_interp_nodes(ctx_, pts, n, fill, out.data()), "interpolation");
+    return out;
+}
+
 void FemCore::solve(double init_value) {
     pf_solve_opts o{};
     o.precision = iter.precision;

@@ -51,6 +51,8 @@ class FemCore {
     int dim() const { return dim_; }
     bool cylindrical() const { return cyl_; }
     std::vector<double> solution();  // node values (natural order)
+    // the solution linearly interpolated at n points (n x dim, um); outside: fill
+    std::vector<double> interp(const double* pts, int64_t n, double fill);
     void invalidate() { initialized_ = false; }
     bool initialized() const { return initialized_; }
 

@@ -48,6 +48,12 @@ PYBIND11_MODULE(_core, m) {
                  s.set_dirichlet(vec(nodes), vec(values));
              })
         .def("solution", [](FemCore& s) { return to_np(s.solution()); })
+        .def("interp",
+             [](FemCore& s, const arr<double>& pts, double fill) {
+                 if (pts.ndim() != 2 || pts.shape(1) != s.dim())
+                     throw std::invalid_argument("points must be an (n, dim) array");
+                 return to_np(s.interp(pts.data(), (int64_t)pts.shape(0), fill));
+             })
         .def("invalidate", &FemCore::invalidate)
         .def_property_readonly("initialized", &FemCore::initialized)
         .def_property_readonly("num_nodes", &FemCore::num_nodes)

@@ -82,6 +82,9 @@ struct pf_ctx {
     double dvals_sq = 0.;
     // line-factor set-up scratch: per-node line coefficients, per-segment sums
     double *sb = nullptr, *sc = nullptr, *sff = nullptr, *sfl = nullptr;
+    // interpolation workspace (axes, points, results)
+    double *iax = nullptr, *ipts = nullptr, *iout = nullptr;
+    size_t ipts_cap = 0;
     // nodal load added to the element load vector (boundary heat flux / radiation)
     long long* lnodes = nullptr;
     double* lvals = nullptr;
@@ -581,6 +584,36 @@ int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int6
     HIPCK(hipMemcpy(ctx->dmask, mask.data(), sizeof(uint64_t) * mask.size(), hipMemcpyHostToDevice));
     ctx->dirty = true;
     drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_interp_nodes(pf_ctx* ctx, const double* pts, int64_t npts, double fill, double* out) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (npts < 0 || (npts > 0 && (!pts || !out))) return fail(ctx, PF_EINVAL, "bad point list");
+    if (npts == 0) return PF_OK;
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no solution on the device");
+    HIPCK(hipSetDevice(ctx->device));
+    const int dim = ctx->dim;
+    const size_t nax = ctx->ax[0].size() + ctx->ax[1].size() + (dim == 3 ? ctx->ax[2].size() : 0);
+    std::vector<double> axcat;
+    axcat.reserve(nax);
+    for (int a = 0; a < dim; ++a) axcat.insert(axcat.end(), ctx->ax[a].begin(), ctx->ax[a].end());
+    int rc;
+    if ((size_t)npts > ctx->ipts_cap) {
+        if ((rc = dalloc(ctx, &ctx->ipts, (size_t)npts * dim)) || (rc = dalloc(ctx, &ctx->iout, (size_t)npts)))
+            return rc;
+        ctx->ipts_cap = (size_t)npts;
+    }
+    if ((rc = dalloc(ctx, &ctx->iax, nax))) return rc;
+    HIPCK(hipMemcpyAsync(ctx->iax, axcat.data(), sizeof(double) * nax, hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->ipts, pts, sizeof(double) * (size_t)npts * dim, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_interp_nodes, dim3(grid1(npts, 256, 8192)), dim3(256), 0, ctx->stream, (const double*)ctx->iax,
+                       dim, (int)ctx->n[0], (int)ctx->n[1], dim == 3 ? (int)ctx->n[2] : 1, (const double*)ctx->x,
+                       (long long)npts, (const double*)ctx->ipts, fill, ctx->iout);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, ctx->iout, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
     return PF_OK;
 }
 

@@ -251,7 +251,7 @@ class _ShockleyBase(_sb.FemSolver):
     def outVoltage(self, mesh, interpolation="linear"):
         if self._core is None or self._core.loopno == 0:
             raise ValueError("%s: voltage not computed" % self.id)
-        return _sb.interp_nodes(self._axes, self._volt(), mesh, fill=np.nan)
+        return _sb.interp_nodes(self._axes, self._volt, mesh, fill=np.nan, device=self._core)
 
     def outCurrentDensity(self, mesh, interpolation="linear"):
         j = self._core.currents().reshape(-1, self.DIM)

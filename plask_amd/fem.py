@@ -22,7 +22,7 @@ KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KC
 
 EXPORTED = [
     "pf_ctx_create", "pf_ctx_destroy", "pf_last_error", "pf_grid_set", "pf_coeffs_set", "pf_source_set",
-    "pf_dirichlet_set", "pf_load_set", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
+    "pf_dirichlet_set", "pf_load_set", "pf_interp_nodes", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host",
@@ -72,6 +72,8 @@ def load_library(path=LIB_PATH):
     L.pf_source_set.argtypes = [P, P, i64]
     L.pf_dirichlet_set.argtypes = [P, P, P, i64]
     L.pf_load_set.argtypes = [P, P, P, i64]
+    L.pf_interp_nodes.argtypes = [P, P, i64, C.c_double, P]
+    L.pf_solution_set.argtypes = [P, P]
     L.pf_solve.argtypes = [P, C.POINTER(SolveOpts), P, C.POINTER(SolveStats)]
     L.pf_get_solution.argtypes = [P, P]
     L.pf_apply.argtypes = [P, I, P, P]
@@ -179,6 +181,18 @@ class FemContext:
         self._dn = np.ascontiguousarray(nodes, dtype=np.int64)
         self._dv = _f64(np.broadcast_to(values, self._dn.shape))
         self._ck(self.L.pf_dirichlet_set(self.h, _p(self._dn), _p(self._dv), len(self._dn)), "pf_dirichlet_set")
+
+    def set_solution(self, x):
+        """Set the device solution (node values, natural order)."""
+        self._x0 = _f64(x)
+        self._ck(self.L.pf_solution_set(self.h, _p(self._x0)), "pf_solution_set")
+
+    def interp(self, points, fill=np.nan):
+        """The solution linearly interpolated at points (n x dim, um); outside: fill."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros(len(pts))
+        self._ck(self.L.pf_interp_nodes(self.h, _p(pts), len(pts), float(fill), _p(out)), "pf_interp_nodes")
+        return out
 
     def set_load(self, nodes, values):
         """Nodal load added to the element load vector (boundary terms); repeated nodes

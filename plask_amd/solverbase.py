@@ -100,16 +100,23 @@ def _corner_mean(axes, values):
     return (acc / (1 << d)).ravel()
 
 
-def interp_nodes(axes, values, dest, fill):
+def interp_nodes(axes, values, dest, fill, device=None):
     """Linear interpolation of node values (natural order) on a rectilinear mesh.
     Destinations that are the mesh itself or its element-midpoint mesh are
-    evaluated directly (same values, no search)."""
+    evaluated directly (same values, no search).  values may be a callable returning
+    them.  device: a solver core whose device solution holds these values; other
+    destinations are then interpolated on the GPU (pf_interp_nodes)."""
     d = len(axes)
     if isinstance(dest, _mesh.Rectangular) and dest.dim == d:
         if _same_axes(dest.axes, axes):
-            return np.array(values, dtype=np.float64)
+            return np.array(values() if callable(values) else values, dtype=np.float64)
         if _same_axes(dest.axes, [0.5 * (a[1:] + a[:-1]) for a in axes]):
-            return _corner_mean(axes, values)
+            return _corner_mean(axes, values() if callable(values) else values)
+    if device is not None:
+        pts = np.ascontiguousarray(points_of(dest), dtype=np.float64).reshape(-1, d)
+        return device.interp(pts, np.nan if fill is None else float(fill))
+    if callable(values):
+        values = values()
     vals = np.asarray(values).reshape([len(a) for a in axes][::-1]).transpose(*range(d)[::-1])
     f = RegularGridInterpolator(axes, vals, method="linear", bounds_error=False, fill_value=np.nan)
     out = f(points_of(dest))

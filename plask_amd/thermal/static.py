@@ -177,7 +177,7 @@ class _StaticBase(_sb.FemSolver):
         (ThermalFem2DSolver::getTemperatures, therm2d.cpp:543-552)."""
         if not self._initialized or self._core is None or self._core.loopno == 0:
             return np.full(len(_sb.points_of(mesh)), float(self.inittemp))
-        return _sb.interp_nodes(self._axes, self._temps(), mesh, fill=300.0)
+        return _sb.interp_nodes(self._axes, self._temps, mesh, fill=300.0, device=self._core)
 
     def _element_T(self):
         T = self._temps().reshape([len(a) for a in self._axes][::-1])
