@@ -190,13 +190,31 @@ class _StaticBase(_sb.FemSolver):
         tav = self._element_T() if (self._core is not None and self._core.loopno > 0) else \
             np.full(len(self._mat), float(self.inittemp))
         h = self._h
+        leaves = self._geometry._leaves
         if leaf_height:
-            leaves = self._geometry._leaves
-            h = np.array([leaves[k][2][-1] if k >= 0 else np.inf for k in self._leaf])
+            h = np.array([lf[2][-1] for lf in leaves] + [np.inf])[self._leaf]
+        # per leaf (the last entry: the background): a closed-form law k0 (300/T)^alpha
+        # for the whole group at once, else the material's thermk per element
+        mats = [lf[0].material for lf in leaves] + [self._mat_bg]
+        li = np.where(self._leaf < 0, len(mats) - 1, self._leaf)
+        order = np.argsort(li, kind="stable")
+        bounds = np.searchsorted(li[order], np.arange(len(mats) + 1))
         k = np.empty((len(self._mat), 2))
-        for e, m in enumerate(self._mat):
-            v = m.thermk(float(tav[e]), float(h[e])) if m is not None else (np.nan, np.nan)
-            k[e] = (v, v) if np.isscalar(v) else v
+        for g, m in enumerate(mats):
+            ix = order[bounds[g]:bounds[g + 1]]
+            if not len(ix):
+                continue
+            law = m.thermk_law() if m is not None else None
+            if m is None:
+                k[ix] = np.nan
+            elif law is not None:
+                t = 300.0 / tav[ix]
+                k[ix, 0] = law[0] * t ** law[2]
+                k[ix, 1] = law[1] * t ** law[3]
+            else:
+                for e in ix:
+                    v = m.thermk(float(tav[e]), float(h[e]))
+                    k[e] = (v, v) if np.isscalar(v) else v
         return k
 
     def outThermalConductivity(self, mesh, interpolation="nearest"):

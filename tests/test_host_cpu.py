@@ -253,3 +253,34 @@ def test_masked_mesh_tables():
     s.empty_elements = "default"
     assert not s._excludes_empty()
     assert Static3D("n").algorithm == "iterative"
+
+
+def test_element_conductivity_per_leaf_matches_per_element():
+    """outThermalConductivity / outHeatFlux conductivities: the per-leaf evaluation (closed-
+    form law for a whole group, thermk per element otherwise) equals thermk element by
+    element, with the thickness or the leaf height as h (therm2d.cpp:525-531)."""
+    from plask_amd import geometry, material, mesh
+    from plask_amd.thermal.static import Static2D
+
+    class HeightLaw(material.Material):
+        name = "height"
+
+        def thermk(self, T=300.0, h=math.inf):
+            return (h, 2.0 * h)
+
+    stack = geometry.Stack2D()
+    stack.append(geometry.Rectangle(4.0, 1.0, material.PowerLaw((3.0, 2.0), (0.7, 1.2), name="P")))
+    stack.append(geometry.Rectangle(3.0, 0.5, HeightLaw()))
+    stack.append(geometry.Rectangle(4.0, 0.8, material.Const(thermk=(5.0, 4.0), name="C")))
+    s = Static2D("k")
+    s.geometry = geometry.Cartesian2D(stack)
+    s.mesh = mesh.Rectangular2D(np.linspace(0, 4, 9), np.linspace(0, 2.8, 12))
+    s.inittemp = 320.0
+    s._tables()
+    s._h = s._thickness()
+    leaves = s.geometry._leaves
+    for leaf_height in (False, True):
+        k = s._element_k(leaf_height=leaf_height)
+        h = np.array([leaves[q][2][-1] if q >= 0 else np.inf for q in s._leaf]) if leaf_height else s._h
+        want = np.array([m.thermk(320.0, float(h[e])) for e, m in enumerate(s._mat)], dtype=float)
+        assert np.array_equal(k, want)
