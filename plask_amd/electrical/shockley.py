@@ -337,8 +337,20 @@ class _ShockleyBase(_sb.FemSolver):
 
     def get_electrostatic_energy(self):
         """Energy of the electrostatic field (J) (getTotalEnergy, electr2d.cpp:574-615)."""
-        T = self.inTemperature(self._element_mesh())
-        eps = np.array([mm.eps(float(t)) if mm is not None else 1.0 for mm, t in zip(self._mat, T)])
+        T = np.asarray(self.inTemperature(self._element_mesh()), dtype=np.float64)
+        # permittivity per leaf (last entry: the background), at each distinct temperature
+        mats = [lf[0].material for lf in self._geometry._leaves] + [self._mat_bg]
+        li = np.where(self._leaf < 0, len(mats) - 1, self._leaf)
+        eps = np.empty(len(T))
+        for g, mm in enumerate(mats):
+            ix = np.nonzero(li == g)[0]
+            if not len(ix):
+                continue
+            if mm is None:
+                eps[ix] = 1.0
+                continue
+            tu, inv = np.unique(T[ix], return_inverse=True)
+            eps[ix] = np.array([float(mm.eps(float(t))) for t in tu])[inv]
         w = np.where(self._excl, 0.0, eps * self._grad2() * self._element_volume())
         if self.DIM == 2 and self.CYL:
             r = np.tile(0.5 * (self._axes[0][1:] + self._axes[0][:-1]), len(self._axes[1]) - 1)
