@@ -13,9 +13,9 @@ Printed JSON line: value = solves/s of FP64 PCG preconditioned by the multigrid 
 the single-level forms of the same solve (exact line preconditioner, Jacobi), the
 FP32-operator forms (FP64 refinement or reliable updates), the learned-warm-start form
 when trained weights are present, per-kernel-class timings of profiled solves (HIP events
-around each launch on the context stream), the roofline of the kernel class with the
-largest share of the headline solve, and the CPU path timed on this host (serial IC(0)-CG
-on the reference's 14-diagonal storage, bounded sample).
+around each launch on the context stream), the roofline of the kernel function with
+the largest total time in a profiled headline solve (all multigrid levels), and the CPU path
+timed on this host (serial IC(0)-PCG on the reference's 14-diagonal storage).
 """
 import argparse
 import json
@@ -89,8 +89,63 @@ def kernel_bytes(shape, m=8):
             "mg_transfer": 12 * N + 8 * Nc}
 
 
+def per_kernel(kstats):
+    """Records of pf_kernel_stats grouped by kernel function: total time, launches and
+    algorithmic bytes over all levels, the coarse-level share, and the per-level rows."""
+    out = {}
+    total = sum(r["seconds"] for r in kstats)
+    for r in kstats:
+        k = out.setdefault(r["name"], dict(seconds=0.0, launches=0, bytes=0.0, levels=[]))
+        k["seconds"] += r["seconds"]
+        k["launches"] += r["launches"]
+        k["bytes"] += r["bytes"]
+        k["levels"].append(dict(level=r["level"], launches=r["launches"],
+                                avg_us=round(1e6 * r["seconds"] / max(r["launches"], 1), 2),
+                                bytes_per_launch=int(r["bytes"] / max(r["launches"], 1)),
+                                gbs=round(r["bytes"] / max(r["seconds"], 1e-30) / 1e9, 1)))
+    for k in out.values():
+        k["levels"].sort(key=lambda r: r["level"])
+        k["share"] = k["seconds"] / max(total, 1e-30)
+    coarse = sum(r["seconds"] for r in kstats if r["level"] > 0)
+    return dict(kernels=out, total_s=total, coarse_share=coarse / max(total, 1e-30))
+
+
+def roofline_line(by_kernel, size, copy_gbs):
+    """Roofline of the kernel function with the largest total time of the profiled FP64
+    multigrid solve: achieved = its algorithmic bytes summed over all its launches (every
+    level) / its summed duration = bytes per launch / average launch duration.  traffic:
+    the calibrated hardware counters of that kernel, from the rocprofv3 --pmc summary
+    committed in profiles/traffic.json (measured in a separate counter run, not here)."""
+    ks = by_kernel["kernels"]
+    top = max(ks, key=lambda k: ks[k]["seconds"])
+    k = ks[top]
+    avg_us = 1e6 * k["seconds"] / k["launches"]
+    bpl = k["bytes"] / k["launches"]
+    achieved = bpl / (1e-6 * avg_us) / 1e9
+    traffic = None
+    src = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as fh:
+            tr = json.load(fh)
+        if tr.get("workload") == "thermal3d_%dx%dx%d" % size and top in tr.get("kernels", {}):
+            traffic = tr["kernels"][top]
+            src = "profiles/traffic.json (%s)" % tr.get("source", "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE")
+    return {"bound": "hbm", "kernel": top, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
+            "bytes_per_launch": int(bpl), "avg_launch_us": round(avg_us, 2), "launches": k["launches"],
+            "time_share_of_kernel_time": round(k["share"], 3),
+            "coarse_level_share_of_kernel_time": round(by_kernel["coarse_share"], 3),
+            "per_level": k["levels"],
+            "kernel_shares": {n: round(v["share"], 3) for n, v in sorted(ks.items(), key=lambda kv: -kv[1]["seconds"])},
+            "stream_copy_gbs": round(copy_gbs, 1)}
+
+
 def cpu_baseline(p, seconds, full_iters):
-    """Serial oracle: assemble the 14-diagonal system, time IC(0)-CG iterations."""
+    """CPU path on this host, one core: assembly of the 14-diagonal system (the
+    reference's SparseBandMatrix storage), the IC(0) factor, and IC(0)-PCG to the same
+    stopping rule as the GPU (oracle restatement, not NSPCG itself).  The whole solve runs
+    when its estimate fits in `seconds`; otherwise a few iterations are timed and scaled
+    to the iteration count of a complete solve (full_iters)."""
     import oracle as O
     g = O.Grid(p["axes"])
     t0 = time.perf_counter()
@@ -98,17 +153,22 @@ def cpu_baseline(p, seconds, full_iters):
     t_asm = time.perf_counter() - t0
     x0 = np.full(g.nnodes, 300.0)
     t0 = time.perf_counter()
-    O.dia_pcg(A, x0, precond=2, rtol=1e-30, maxit=1)
-    t_one = time.perf_counter() - t0
-    n = max(2, int(seconds / max(t_one, 1e-3)))
+    O.ic0_factor(A)
+    t_fact = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, st, it, rel = O.dia_pcg(A, x0, precond=2, rtol=1e-30, maxit=n)
-    t_run = time.perf_counter() - t0
-    t_iter = t_run / max(it, 1)
-    res = {"assemble_s": t_asm, "iters_sampled": it, "s_per_iter": t_iter}
-    if full_iters:
-        res["solve_s"] = t_asm + full_iters * t_iter
-        res["value"] = 1.0 / res["solve_s"]
+    _, st, it, rel = O.dia_pcg(A, x0, precond=2, rtol=1e-30, maxit=3)
+    t_iter = max(time.perf_counter() - t0 - t_fact, 1e-9) / max(it, 1)
+    est = t_fact + (full_iters or 0) * t_iter
+    res = {"assemble_s": t_asm, "ic0_factor_s": t_fact, "s_per_iter": t_iter}
+    if full_iters and est > seconds:
+        res.update(iters=full_iters, full_solve=False, solve_s=t_asm + t_fact + full_iters * t_iter)
+    else:
+        t0 = time.perf_counter()
+        _, st, it, rel = O.dia_pcg(A, x0, precond=2, rtol=1e-8, maxit=100000)
+        t_run = time.perf_counter() - t0
+        res.update(iters=it, full_solve=True, rel_resid=rel, solve_s=t_asm + t_run,
+                   s_per_iter=(t_run - t_fact) / max(it, 1))
+    res["value"] = 1.0 / res["solve_s"]
     return res
 
 
@@ -173,9 +233,33 @@ def config1(steps, with_cpu):
         g = O.Grid(p["axes"], cyl=True)
         t0 = time.perf_counter()
         A = O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_BAND, p["dir_nodes"], p["dir_vals"])
+        t_asm = time.perf_counter() - t0
+        # every LAPACK this host offers (MKL when present, SciPy's OpenBLAS), each with its
+        # thread pool warmed up, median of 3; the fastest is the baseline
+        lap = {}
+        vendors = [O.lapack_vendor()]
+        if vendors[0][0] != "openblas":
+            vendors.append(("openblas", None))
+        for vendor in vendors:
+            O.lapack_band_cholesky_solve(A, vendor=vendor)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                Tl, vname, nthreads = O.lapack_band_cholesky_solve(A, vendor=vendor)
+                ts.append(time.perf_counter() - t0)
+            lap[vname] = (float(np.median(ts)), nthreads, Tl)
+        vname = min(lap, key=lambda k: lap[k][0])
+        t_lapack, nthreads, Tl = lap[vname]
+        t0 = time.perf_counter()
         Tr = O.band_cholesky_solve(A)
-        tc = time.perf_counter() - t0
-        out.update(cpu_band_cholesky_ms=1e3 * tc, cpu_cores=1, kd=int(A.kd),
+        t_text = time.perf_counter() - t0
+        tc = t_asm + t_lapack
+        out.update(cpu_ms=1e3 * tc, cpu_assemble_ms=1e3 * t_asm, cpu_lapack_ms=1e3 * t_lapack,
+                   cpu_lapack="%s dpbtrf/dpbtrs, %d threads (median of 3; fastest of %s)" % (
+                       vname, nthreads, ", ".join("%s %.1f ms" % (k, 1e3 * v[0]) for k, v in lap.items())),
+                   cpu_assembly_threads=1, kd=int(A.kd),
+                   cpu_textbook_band_cholesky_ms_1core=1e3 * t_text,
+                   max_rel_diff_vs_lapack=float(np.abs(T - Tl).max() / np.abs(Tl).max()),
                    max_rel_diff_vs_band_cholesky=float(np.abs(T - Tr).max() / np.abs(Tr).max()),
                    speedup_vs_cpu=tc / t)
     c.close()
@@ -241,7 +325,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", default="128,128,256")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=45.0,
+                    help="CPU baseline: run the complete CPU solve when its estimate fits in this many seconds")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="plain kernel launches instead of hipGraph replay")
@@ -309,6 +394,21 @@ def main():
     # the single-level forms of the same solve, for comparison
     line64, _ = form("fp64 pcg, exact line preconditioner", PF_FP64, PRECOND_LINE)
     jac, _ = form("fp64 pcg, Jacobi preconditioner", PF_FP64, PRECOND_JACOBI, steps=1, warmup=1)
+    # the pair of the benchmark description: FP64 Jacobi-PCG vs FP32 matrix-free apply + FP64 refinement
+    jac32, _ = form("fp32 operator + fp64 refinement, Jacobi preconditioner", PF_FP32_REFINE, PRECOND_JACOBI,
+                    1e-4, steps=1, warmup=1)
+    # compute(1) equivalent: a coefficient change (device-evaluated conductivity law, the
+    # setMatrix part) followed by the solve -- the set-up after the change (load vector,
+    # diagonal, line factors and coarse operators of all levels in both precisions) is timed
+    ctx.set_powerlaw(p["k_h"], p["k_v"])
+
+    def compute1():
+        ctx.coeffs_update()
+        return ctx.solve_uniform(300.0, rtol=rtol, precond=PRECOND_MG)
+    tc1, stc1 = timed(compute1, args.steps, args.warmup)
+    comp1 = dict(method="coefficient update + set-up + fp64 multigrid pcg solve", ms=1e3 * tc1,
+                 solves_per_s=1.0 / tc1, iters=stc1.iters, setup_ms=1e3 * tc1 - fp64["ms"])
+    print("compute(1) equivalent: %.2f ms" % comp1["ms"], file=sys.stderr, flush=True)
     mixed = {}
     warm = None
     if not args.no_mixed:
@@ -327,29 +427,31 @@ def main():
                   (warm["solves_per_s"], warm["ms"], warm["inner_iters"]), file=sys.stderr, flush=True)
     best = min(mixed.values(), key=lambda f: f["ms"]) if mixed else None
 
-    # roofline: profiled solves -- HIP events around every kernel class on the context
-    # stream, one synchronisation per chunk (the in-solve durations a kernel trace reports;
-    # launches after the stop are not counted)
+    # roofline: a profiled FP64 multigrid solve -- HIP events around every launch on the
+    # context stream (no graph replay), one record per (kernel, multigrid level); launches
+    # after the stop are not counted.  The kernel of the roofline is the kernel function with
+    # the largest total time over all levels (how a kernel trace ranks them).
     def profiled(prec, pc, inner=None):
         ctx.set_profile(True)
         st = ctx.solve_uniform(300.0, precision=prec, rtol=rtol, inner_rtol=inner, precond=pc)
         ctx.set_profile(False)
         t, n = ctx.kernel_times(6)
-        return t, n, st.iters
+        return t, n, st.iters, ctx.kernel_stats()
     kb = kernel_bytes(size)
     kern = {}
-    t, n, its = profiled(PF_FP64, PRECOND_MG)
+    t, n, its, kstats = profiled(PF_FP64, PRECOND_MG)
     names = {KCAT_APPLY: "apply_fused", KCAT_MG_RESIDUAL: "mg_residual", KCAT_MG_SMOOTH: "mg_smooth",
              KCAT_MG_TRANSFER: "mg_transfer"}
     for cat, name in names.items():
         kern[name] = dict(avg_us=1e6 * t[cat], launches=int(n[cat]), bytes=kb[name])
     kern["mg_vcycle"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]))
     kern["mg_coarse_levels"] = dict(avg_us=1e6 * t[KCAT_MG_COARSE], launches=int(n[KCAT_MG_COARSE]))
-    t, n, _ = profiled(PF_FP32_REFINE, PRECOND_MG, INNER[PF_FP32_REFINE])
+    by_kernel = per_kernel(kstats)
+    t, n, _, _ = profiled(PF_FP32_REFINE, PRECOND_MG, INNER[PF_FP32_REFINE])
     kern["apply_fused_fp32"] = dict(avg_us=1e6 * t[KCAT_APPLY], launches=int(n[KCAT_APPLY]),
                                     bytes=kb["apply_fused_fp32"])
     kern["mg_vcycle_fp32"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]))
-    t, n, _ = profiled(PF_FP64, PRECOND_LINE)
+    t, n, _, _ = profiled(PF_FP64, PRECOND_LINE)
     kern["line"] = dict(avg_us=1e6 * t[KCAT_PRECOND], launches=int(n[KCAT_PRECOND]), bytes=kb["line"])
     kern["apply_plain"] = dict(avg_us=1e6 * ctx.time_apply(PF_FP64, reps=50), bytes=kb["apply_plain"])
     for k in kern.values():
@@ -358,31 +460,22 @@ def main():
         k["avg_us"] = round(k["avg_us"], 2)
     t_copy = ctx.stream_copy(1 << 30, reps=5)
     copy_gbs = 2 * (1 << 30) / t_copy / 1e9
-    # dominant kernel class of the headline solve: largest total time per solve
-    dom = max(names.values(), key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
-    k = kern[dom]
-    traffic = None
-    if os.path.exists(TRAFFIC):
-        with open(TRAFFIC) as fh:
-            tr = json.load(fh)
-        if tr.get("workload") == "thermal3d_%dx%dx%d" % size:
-            traffic = tr.get("kernels", {}).get(dom)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": k["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": k["bytes"], "avg_launch_us": k["avg_us"],
-                "time_share_of_solve": round(1e-3 * k["avg_us"] * k["launches"] / fp64["ms"], 3),
-                "stream_copy_gbs": round(copy_gbs, 1)}
+    roofline = roofline_line(by_kernel, size, copy_gbs)
 
     cpu = None
     if not args.no_cpu:
         full = CPU_FULL_ITERS.get(size)
         c = cpu_baseline(p, args.cpu_seconds, full)
-        cpu = {"value": c.get("value"), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": "serial IC(0)-CG (the reference's default preconditioner, NSPCG ic2 semantics) on the "
-                         "14-diagonal SparseBandMatrix storage of the same system: assembly timed whole, "
-                         "%d iterations timed (%.3f s/iter) and scaled to the %s iterations of the full "
-                         "solve to the same stopping rule" % (c["iters_sampled"], c["s_per_iter"], full),
-               "assemble_s": round(c["assemble_s"], 3), "s_per_iter": c["s_per_iter"]}
+        how = ("the complete solve timed (%d iterations)" % c["iters"] if c["full_solve"] else
+               "3 iterations timed (%.3f s/iter) and scaled to the %d iterations of a complete solve" %
+               (c["s_per_iter"], c["iters"]))
+        cpu = {"value": c["value"], "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": "serial IC(0)-PCG (this build's restatement of the reference's default incomplete-Cholesky "
+                         "preconditioner family, not NSPCG) on the 14-diagonal SparseBandMatrix storage of the same "
+                         "system, same stopping rule: assembly %.3f s + IC(0) factor %.3f s + iterations; %s"
+                         % (c["assemble_s"], c["ic0_factor_s"], how),
+               "assemble_s": round(c["assemble_s"], 3), "ic0_factor_s": round(c["ic0_factor_s"], 3),
+               "s_per_iter": c["s_per_iter"], "iters": c["iters"], "full_solve": c["full_solve"]}
 
     c1 = config1(args.steps, not args.no_cpu)
     print("config 1 (2D cyl): %.2f ms GPU" % c1["gpu_ms"], file=sys.stderr, flush=True)
@@ -406,6 +499,9 @@ def main():
                    "iters": fp64["iters"], "rel_resid": fp64["rel_resid"]},
         "fp64_line_pcg": line64,
         "fp64_jacobi_pcg": jac,
+        "config2_pair": {"fp64_jacobi_pcg": jac, "fp32_apply_fp64_refinement_jacobi": jac32,
+                         "speedup_fp32_refine_vs_fp64": jac["ms"] / jac32["ms"]},
+        "compute1_equivalent": comp1,
         "mixed_precision": best,
         "mixed_precision_forms": mixed,
         "learned_warm_start": warm,

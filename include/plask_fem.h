@@ -86,6 +86,11 @@ typedef struct pf_ctx pf_ctx;
                             zero potential, electr2d.cpp:176), i.e. ||b|| of the correction system */
 #define PF_NORM_B 1      /* ||B|| over the free rows */
 #define PF_NORM_B_FULL 2 /* ||B|| including the Dirichlet rows (B exactly as assembled by the reference) */
+#define PF_NORM_START 3  /* max(||B - A X_start||, 1e-4 ||B - A X_init||), X_start the start vector of this
+                            solve: the start residual reduced by rtol, the way a Picard loop that warm-starts
+                            each linear solve from the previous loop's solution matches a direct solve per
+                            loop (the 1e-4 floor keeps a converged loop from asking for more than FP64 CG
+                            attains, ~3e-14 ||B - A X_init|| on the 3D junction systems) */
 
 typedef struct {
     int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE | PF_FP64_PREC32 */
@@ -112,6 +117,9 @@ typedef struct {
     double seconds;     /* wall time of the solve on the device */
     int status;         /* PF_OK or an error code */
     int start;          /* PF_X0_DEVICE_CHECKED: 0 = the device vector was used, 1 = the uniform start */
+    int fallback;       /* FP32 forms: 1 when a refinement step failed to halve the FP64 residual (the FP32
+                           correction no longer contracts: kappa(A) eps_fp32 ~ 1) and the solve finished with
+                           FP64 CG; `outer` counts the FP32 steps and the FP64 runs */
 } pf_solve_stats;
 
 /* context ------------------------------------------------------------------ */
@@ -190,6 +198,41 @@ int64_t pf_num_elements(const pf_ctx* ctx);
 #define PF_KCAT_MG_COARSE 5   /* V-cycle: all work on the coarse levels (one record per V-cycle) */
 #define PF_KCAT_COUNT 6
 
+/* Kernels of one PCG iteration, for pf_kernel_stats (each id is one kernel function of
+ * this library; the multigrid ones run on every level):
+ *   PF_K_APPLY     fused operator apply (k_apply3d / k_apply3r / k_apply2d, MODE_OP, p and dot fused)
+ *   PF_K_RESIDUAL  smoother residual r = b - A x (k_apply3d / k_apply2d, MODE_RESG)
+ *   PF_K_SMOOTH    damped line sweep (k_mg_smooth)
+ *   PF_K_RESTRICT  restriction (k_mg_restrict), level = the finer level
+ *   PF_K_PROLONG   prolongation (k_mg_prolong_add), level = the finer level
+ *   PF_K_LINE      line preconditioner update (k_cg_part, or k_cg_line when sequential)
+ *   PF_K_JACOBI    Jacobi update (k_cg_update)
+ *   PF_K_TO_F32, PF_K_FROM_F32  precision conversions around an FP32 preconditioner */
+#define PF_K_APPLY 0
+#define PF_K_RESIDUAL 1
+#define PF_K_SMOOTH 2
+#define PF_K_RESTRICT 3
+#define PF_K_PROLONG 4
+#define PF_K_LINE 5
+#define PF_K_JACOBI 6
+#define PF_K_TO_F32 7
+#define PF_K_FROM_F32 8
+
+typedef struct {
+    int kernel;       /* PF_K_* */
+    int level;        /* multigrid level (0 = the solve grid) */
+    int value_bytes;  /* 8 (FP64) or 4 (FP32) */
+    int launches;     /* timed launches */
+    double seconds;   /* summed duration of those launches (HIP events around each on the context stream) */
+    double bytes;     /* summed algorithmic HBM bytes: every value read or written once, two conductivities
+                         per element, the Dirichlet mask as one bit per node */
+} pf_kernel_stat;
+
+/* Per-kernel statistics of the iterations of the last pf_solve run with profiling on
+ * (pf_set_profile): one entry per (kernel, level, precision) that ran.  Copies up to nmax
+ * entries; returns the number of entries (or < 0 on error).  For benchmarks. */
+int pf_kernel_stats(const pf_ctx* ctx, pf_kernel_stat* out, int nmax);
+
 /* Average duration (s) of each kernel class (PF_KCAT_*, n <= 8 entries) over the
  * iterations of the last pf_solve run with profiling on (pf_set_profile), measured with
  * HIP events around each launch on the context stream; counts[] receives the number of
@@ -237,10 +280,13 @@ int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes);
  * that XCD's L2; 0 = plain block order.  A tuning hook, no reference counterpart. */
 int pf_set_block_order(pf_ctx* ctx, int xcd);
 
-/* on = 1 (default): CG iterations are replayed in chunks from hipGraphs; 0: plain
- * launches (e.g. under tools that do not follow graph launches).  No reference
- * counterpart. */
+/* on = 1: CG iterations are replayed in chunks from hipGraphs; 0: plain launches.
+ * Default on, except in a process run under a kernel-tracing profiler (rocprofv3: the
+ * environment holds ROCP_TOOL_LIBRARIES or ROCPROF_OUTPUT_PATH), where graph replay
+ * is off and a one-line notice goes to stderr.  pf_graphs_enabled reports the setting.
+ * No reference counterpart. */
 int pf_set_graphs(pf_ctx* ctx, int on);
+int pf_graphs_enabled(const pf_ctx* ctx);
 
 /* Planes of node and element data the 3D operator kernel keeps in flight ahead
  * of the plane it computes (1 or 2).  A tuning hook with no reference counterpart. */
@@ -332,20 +378,36 @@ typedef struct {
     int maxloops;   /* Picard loops (compute(loops)), default 30 */
     int maxit;      /* CG iterations per loop */
     double maxerr;  /* loop error limit in % (ElectricalFem2DSolver::maxerr, default 0.05) */
-    double rtol;    /* CG: ||B - A V|| <= rtol ||B - A V_init|| (V_init = Dirichlet values, 0 elsewhere) */
+    double rtol;    /* CG of each Picard loop: ||B - A V|| <= rtol max(||B - A V_start||, 1e-4 ||B - A V_init||),
+                       V_start the previous loop's potential (the loop's own start residual reduced by rtol, as
+                       a direct solve of every loop would), V_init = Dirichlet values, 0 elsewhere.  The true
+                       residual is recomputed with compensated (double-double) sums and CG restarted once from
+                       it when it has drifted above that level (a loop that still misses it -- junction systems
+                       with conductivities spanning ~1e14 -- has the potential FP64 CG reaches; `rel` and `bwd`
+                       report it) */
     int stable;     /* CONVERGENCE_STABLE */
     int block;      /* threads per workgroup 256/512/1024; 0 = autotuned or 1024 */
     int nwg;        /* persistent workgroups; 0 = autotuned or one per CU */
     int precond;    /* 3 (or 0, the default) = multigrid V-cycle (axis 0 coarsened per instance,
                        damped line smoother, omega 0.65, 2 coarsest sweeps); 2 = line preconditioner */
+    int loop_budget;/* Picard loops one kernel launch runs per instance (0: all of them in one launch).
+                       The solve relaunches the kernel until every instance has finished, so a
+                       launch returns after a bounded amount of work whatever the batch */
+    /* called between launches (may be NULL): launches so far, instances finished, total,
+       seconds since the first launch, progress_user */
+    void (*progress)(int launches, int finished, int total, double seconds, void* user);
+    void* progress_user;
 } pf_batch_opts;
 
 typedef struct {
-    int loops, iters, status, pad; /* status 0 ok, 1 CG maxit, 2 Picard maxloops, <0 breakdown */
+    int loops, iters, status; /* status 0 ok, 1 CG maxit, 2 Picard maxloops, <0 breakdown */
+    int restarts;    /* CG restarts from a recomputed (compensated) true residual, summed over loops */
     double err;      /* last loop error (%) */
     double maxj;     /* max |j| at the junction (kA/cm^2) */
-    double rel;      /* last CG relative residual */
+    double rel;      /* last loop: final true residual over its reference residual (see rtol) */
     double current;  /* total current through the middle junction row (mA; 2D Cartesian per um of length) */
+    double bwd;      /* last loop: componentwise backward error max_i |r_i| / (|A| |V|)_i of its potential (a
+                        backward-stable direct solve reaches a small multiple of eps = 2.2e-16) */
 } pf_batch_stats;
 
 int pf_batch_create(pf_batch** out, int device);
@@ -361,8 +423,16 @@ int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, con
 /* Times each (block, workgroups) schedule on the batch and keeps the fastest. */
 int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int* best_nwg, double* best_seconds);
 /* Solve every instance from V = Dirichlet values / 0; V_out (concatenated, may be NULL),
- * stats[ninst] (may be NULL), seconds = device time of the graph launch. */
+ * stats[ninst] (may be NULL), seconds = device time of all launches (the start launch and
+ * every launch of the persistent kernel, with the host's progress reads between them).
+ * The launches are replayed from hipGraphs (plain launches under a kernel-tracing
+ * profiler, as for pf_solve). */
 int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_stats* stats, double* seconds);
+/* Element conductivities (in-plane, vertical; concatenated, natural order per instance) the
+ * last loop of the last pf_batch_solve used: the material values, and at the junction the
+ * values of the last junction update (saveConductivities, electr2d.cpp:355-361).  Either
+ * pointer may be NULL. */
+int pf_batch_conductivities(pf_batch* b, double* sh, double* sv);
 
 #ifdef __cplusplus
 }

@@ -202,6 +202,65 @@ def dia_pcg(A, x0, precond=1, rtol=1e-8, maxit=100000, bnorm=None):
     return u, int(st), int(it.value), float(rel.value)
 
 
+def ic0_factor(A):
+    """IC(0) factor of the symmetric diagonal storage (orc_ic0_factor): pivots, lower
+    factor values, and the number of replaced pivots."""
+    piv = np.zeros(A.rank)
+    lfac = np.zeros_like(A.data)
+    bad = lib().orc_ic0_factor(A.rank, len(A.offsets), _p(A.offsets), _p(A.data), _p(piv), _p(lfac))
+    return piv, lfac, int(bad)
+
+
+_MKL_PATHS = ["/opt/conda/lib/libmkl_rt.so", "/opt/intel/oneapi/mkl/latest/lib/libmkl_rt.so"]
+
+
+def lapack_vendor():
+    """The LAPACK this host offers for the band Cholesky baseline: ('mkl', handle) when
+    libmkl_rt is present (the library the reference's build instructions suggest),
+    else ('openblas', None) for SciPy's bundled OpenBLAS."""
+    import ctypes as _C
+    for path in _MKL_PATHS:
+        if os.path.exists(path):
+            try:
+                return "mkl", _C.CDLL(path)
+            except OSError:
+                pass
+    return "openblas", None
+
+
+def lapack_band_cholesky_solve(A, B=None, threads=None, vendor=None):
+    """The same band system through LAPACK dpbtrf/dpbtrs (the routines DpbMatrix calls,
+    cholesky_matrix.hpp:95,107): lower band storage AB[i-j, j] = A[i, j], i.e. the first
+    kd + 1 entries of each column of the restated layout (index ld*c + r).  Returns
+    (x, vendor name, threads)."""
+    name, mkl = vendor if vendor is not None else lapack_vendor()
+    ab = np.array(A.data.reshape(A.rank, A.ld + 1)[:, :A.kd + 1].T, order="F", copy=True)  # factorised in place
+    x = np.array(A.B if B is None else B, dtype=np.float64)
+    nt = int(threads or os.cpu_count() or 1)
+    if name == "mkl":
+        n, kd, ldab, nrhs, info = C.c_int(A.rank), C.c_int(A.kd), C.c_int(A.kd + 1), C.c_int(1), C.c_int(0)
+        uplo = C.c_char(b"L")
+        mkl.MKL_Set_Num_Threads(nt)
+        mkl.dpbtrf_(C.byref(uplo), C.byref(n), C.byref(kd), _p(ab), C.byref(ldab), C.byref(info))
+        if info.value != 0:
+            raise ArithmeticError("dpbtrf info %d" % info.value)
+        mkl.dpbtrs_(C.byref(uplo), C.byref(n), C.byref(kd), C.byref(nrhs), _p(ab), C.byref(ldab), _p(x), C.byref(n),
+                    C.byref(info))
+        if info.value != 0:
+            raise ArithmeticError("dpbtrs info %d" % info.value)
+        return x, name, nt
+    from scipy.linalg import lapack
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(nt):
+        c, info = lapack.dpbtrf(ab, lower=1)
+        if info != 0:
+            raise ArithmeticError("dpbtrf info %d" % info)
+        x, info = lapack.dpbtrs(c, x, lower=1)
+    if info != 0:
+        raise ArithmeticError("dpbtrs info %d" % info)
+    return x, name, nt
+
+
 def apply(grid, mode, kh, kv, dmask, x):
     """Matrix-free y = M A0 (M x) + (I - M) x (orc_apply)."""
     kh = np.ascontiguousarray(kh, dtype=np.float64)

@@ -56,11 +56,12 @@ struct BatchStats {
     int loops;      // Picard loops done
     int iters;      // CG iterations summed over loops
     int status;     // 0 ok, 1 CG maxit, 2 Picard maxloops, <0 breakdown
-    int pad;
+    int restarts;   // CG restarts from a recomputed true residual, summed over loops
     double err;     // last loop error (%)
     double maxj;    // max |j| at the junction (kA/cm^2)
-    double rel;     // last CG relative residual
+    double rel;     // last loop: final true residual over its reference residual
     double current; // total current through the junction middle row (mA), cylindrical pi sum j (r2^2 - r1^2) 0.01
+    double bwd;     // last loop: componentwise backward error max_i |r_i| / (|A0| |V|)_i of its solution
 };
 
 struct BatchParams {
@@ -70,6 +71,16 @@ struct BatchParams {
     int precond;  // 2 line, 3 multigrid V-cycle
     double omega; // multigrid smoother damping
     int coarse_sweeps;
+    int loop_budget;  // Picard loops one launch runs per instance (0: the whole loop)
+};
+
+// What an instance carries from one launch to the next: the denominator of its CG
+// stopping test (the residual norm of the start field) and whether its Picard loop
+// has ended.  The loop count, status and iterations live in BatchStats; potentials,
+// conductivities and currents stay in their device arrays.
+struct BatchResume {
+    double bb;
+    int done;
     int pad;
 };
 
@@ -131,6 +142,54 @@ __device__ inline double row2d(const BatchInst& I, const double* __restrict__ h0
         }
     }
     return RES ? -s : s;
+}
+
+// Error-free transformations (Knuth's TwoSum; TwoProduct with a fused multiply-add,
+// Ogita, Rump & Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bv = s - a;
+    e = (a - (s - bv)) + (b - bv);
+}
+
+// Residual row r = M (0 - A0 x) of row2d<true>, with the products and their sum carried in
+// double-double (Dot2): the residual of a nearly converged potential is then accurate to
+// about the rounding of the result, not to eps * |A0| |x|, so CG restarted from it can
+// reduce the true residual below the level its recursive residual drifts to.  mag
+// receives sum_j |a_ij x_j|, the row of |A0| |x| (no FP64 potential has a residual much
+// below eps times it).
+__device__ inline double row2d_res_comp(const BatchInst& I, const double* __restrict__ h0,
+                                        const double* __restrict__ h1, const double* __restrict__ rm,
+                                        const double* __restrict__ kh, const double* __restrict__ kv,
+                                        const uint64_t* __restrict__ dmask, const double* __restrict__ x, int i,
+                                        int k, double& mag) {
+    const long long nb = I.node_off;
+    const long long node = nb + (long long)k * I.n0 + i;
+    mag = 0.;
+    if (dbit(dmask, node)) return 0.;
+    double hi = 0., lo = 0.;
+    for (int c = 0; c < 4; ++c) {
+        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
+        if (ei < 0 || ek < 0 || ei >= I.n0 - 1 || ek >= I.n1 - 1) continue;
+        const long long e = (long long)ek * (I.n0 - 1) + ei;
+        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
+        const double r = I.cyl ? rm[I.h0_off + ei] : 1.;
+        const double kx = r * kh[I.elem_off + e] * h / w, ky = r * kv[I.elem_off + e] * w / h;
+        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
+        for (int b = 0; b < 4; ++b) {
+            const int bx = b & 1, by = b >> 1;
+            const long long nn = nb + (long long)(ek + by) * I.n0 + (ei + bx);
+            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
+            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
+            const double a = kx * Sx * My + ky * Mx * Sy, xv = x[nn - nb];
+            const double pr = a * xv, pe = fma(a, xv, -pr);
+            double se;
+            two_sum(hi, pr, hi, se);
+            lo += pe + se;
+            mag += fabs(pr);
+        }
+    }
+    return -(hi + lo);
 }
 
 // One level of an instance as the workgroup sees it (level 0: the instance grid).
@@ -336,7 +395,7 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                                                          double* __restrict__ Z, double* __restrict__ LA,
                                                          double* __restrict__ LW, double* __restrict__ LC,
                                                          double* __restrict__ CUR, BatchStats* __restrict__ stats,
-                                                         BatchMg mg) {
+                                                         BatchResume* __restrict__ resume, BatchMg mg) {
     __shared__ double red[16];
     __shared__ int s_q;
     const int tid = threadIdx.x;
@@ -347,6 +406,8 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
         __syncthreads();
         if (qi >= ninst) break;
         const int id = order[qi];
+        BatchResume RS = resume[id];
+        if (RS.done) continue;  // uniform across the workgroup: every thread read the same entry
         const BatchInst I = insts[id];
         const int n0 = I.n0, n1 = I.n1;
         const long long N = (long long)n0 * n1, E = (long long)(n0 - 1) * (n1 - 1);
@@ -362,7 +423,7 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
         double* sh = kh + I.elem_off;
         double* sv = kv + I.elem_off;
         const bool has_j = I.jhi > I.jlo;
-        BatchStats S{};
+        BatchStats S = stats[id];  // zero before the first launch, then the state the last launch left
         // multigrid: level views (0 = this grid) and their b / x / scratch vectors; on the
         // instance grid b is the CG residual, x the preconditioned residual z, the
         // scratch q (free between the update and the next operator apply)
@@ -407,15 +468,21 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                 line_sweep_l<BS>(V, tl, tl, xl, xl, om);
             }
         };
-        // denominator: residual of the start field (Dirichlet values, zero elsewhere)
-        for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
-        __syncthreads();
-        double bb = 0.;
-        for (long long n = tid; n < N; n += BS) bb += r[n] * r[n];
-        bb = wg_sum(bb, red);
-        const double stop2 = prm.rtol * prm.rtol * (bb > 0. ? bb : 1.);
-        double err = 0., mcur = 0.;
-        for (int loop = 0; loop < prm.maxloops; ++loop) {
+        // denominator: residual of the start field (Dirichlet values, zero elsewhere),
+        // computed by the launch that starts the instance
+        double bb = RS.bb;
+        if (S.loops == 0) {
+            for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
+            __syncthreads();
+            bb = 0.;
+            for (long long n = tid; n < N; n += BS) bb += r[n] * r[n];
+            bb = wg_sum(bb, red);
+        }
+        const double bbs = bb > 0. ? bb : 1.;
+        double err = S.err, mcur = S.maxj;
+        bool finished = false;
+        const int loop_end = prm.loop_budget > 0 ? min(prm.maxloops, S.loops + prm.loop_budget) : prm.maxloops;
+        for (int loop = S.loops; loop < loop_end; ++loop) {
             // junction update from the current potential (not before the first solve)
             if (loop > 0 && has_j) {
                 const long long nje = (long long)(I.jhi - I.jlo) * (n0 - 1);
@@ -443,9 +510,21 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                     line_factors_l<BS>(Cv);
                 }
             // CG from the current potential
-            for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
-            __syncthreads();
-            double rz = 0., rr = 0.;
+            // true residual (compensated) and the componentwise backward error of x
+            double bwd = 0.;
+            auto true_residual = [&]() {
+                double acc = 0., bw = 0.;
+                for (long long n = tid; n < N; n += BS) {
+                    double mg;
+                    const double v = row2d_res_comp(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0, mg);
+                    r[n] = v;
+                    acc += v * v;
+                    if (mg > 0.) bw = fmax(bw, fabs(v) / mg);
+                }
+                bwd = wg_max(bw, red);
+                return wg_sum(acc, red);
+            };
+            double rz = 0., rr = true_residual();
             auto precond = [&]() {
                 double acc = 0.;
                 if (mgon) {
@@ -471,10 +550,24 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                 __syncthreads();
                 return wg_sum(acc, red);
             };
-            for (long long n = tid; n < N; n += BS) rr += r[n] * r[n];
-            rr = wg_sum(rr, red);
-            int it = 0;
-            if (rr > stop2) {
+            // each loop solves its own system: its start residual (the previous loop's
+            // potential against the updated junction) reduced by rtol, as a direct solve
+            // would per loop, but never below rtol * 1e-4 of the start field's residual
+            const double stop2 = prm.rtol * prm.rtol * fmax(rr, 1e-8 * bbs);
+            int it = 0, cst = 0;  // this loop's CG status (0, 1 maxit, -6 / -7 breakdown)
+            // CG runs from the true residual; when the recursive residual meets the
+            // tolerance, the true one is recomputed (compensated) and CG restarts once from
+            // it if it has drifted above the tolerance.  On junction systems whose
+            // conductivities span ~1e14 no FP64 Krylov run reaches a late loop's tolerance;
+            // the potential is then as accurate as FP64 CG makes it (measured: within ~1e-8
+            // relative of a band-Cholesky solve of the same system), which `bwd` reports.
+            double rr_run = rr;
+            for (int run = 0; run < 2 && rr > stop2 && cst == 0; ++run) {
+                if (run > 0) {
+                    if (rr > 0.25 * rr_run) break;
+                    ++S.restarts;  // summed over loops
+                    rr_run = rr;
+                }
                 rz = precond();
                 for (long long n = tid; n < N; n += BS) p[n] = z[n];
                 __syncthreads();
@@ -487,7 +580,7 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                     }
                     pq = wg_sum(pq, red);
                     if (!(pq > 0.)) {
-                        S.status = -6;
+                        cst = -6;
                         break;
                     }
                     const double alpha = rz / pq;
@@ -502,12 +595,12 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                     ++it;
                     if (rr <= stop2) break;
                     if (it >= prm.maxit) {
-                        S.status = 1;
+                        cst = 1;
                         break;
                     }
                     const double rz_new = precond();
                     if (!(rz_new > 0.)) {
-                        S.status = -7;
+                        cst = -7;
                         break;
                     }
                     const double beta = rz_new / rz;
@@ -515,9 +608,12 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                     for (long long n = tid; n < N; n += BS) p[n] = z[n] + beta * p[n];
                     __syncthreads();
                 }
+                if (cst == 0) rr = true_residual();
             }
+            if (cst != 0) S.status = cst;
             S.iters += it;
-            S.rel = sqrt(rr / (bb > 0. ? bb : 1.));
+            S.rel = sqrt(rr / stop2) * prm.rtol;  // final residual over the loop's reference residual
+            S.bwd = bwd;
             // currents and the loop error
             double dmax = 0., jmax = 0.;
             for (long long e = tid; e < E; e += BS) {
@@ -538,9 +634,14 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
             mcur = sqrt(jmax);
             err = 100. * sqrt(dmax) / fmax(mcur, 100e-7);
             S.loops = loop + 1;
-            if (S.status < 0) break;
-            if (S.status == 0 && err <= prm.maxerr) break;
-            if (loop + 1 == prm.maxloops) S.status = 2;
+            if (S.status < 0 || (S.status == 0 && err <= prm.maxerr)) {
+                finished = true;
+                break;
+            }
+            if (loop + 1 == prm.maxloops) {
+                S.status = 2;
+                finished = true;
+            }
         }
         // total current through the middle junction row (integrateCurrent, electr2d.cpp:466-496)
         double tot = 0.;
@@ -567,6 +668,10 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
             S.maxj = mcur;
             S.current = I.cyl ? tot * 3.14159265358979323846 * 0.01 : tot * 0.01;
             stats[id] = S;
+            RS.bb = bb;
+            RS.done = finished ? 1 : 0;
+            resume[id] = RS;
+            if (finished) atomicAdd(queue + 1, 1u);  // instances finished, read by the host between launches
         }
         __syncthreads();
     }
@@ -574,6 +679,19 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
 
 __global__ void k_reset_counter(unsigned int* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// Start of a batch solve: every instance not started, no instance finished.
+__global__ void k_batch_reset(int ninst, BatchStats* __restrict__ stats, BatchResume* __restrict__ resume,
+                              unsigned int* __restrict__ queue) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+        stats[i] = BatchStats{};
+        resume[i] = BatchResume{};
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[0] = 0u;
+        queue[1] = 0u;
+    }
 }
 
 }  // namespace pf

@@ -63,7 +63,7 @@ std::vector<double> FemCore::interp(const double* pts, int64_t n, double fill) {
     return out;
 }
 
-void FemCore::solve(double init_value) {
+void FemCore::solve(double init_value, int norm) {
     pf_solve_opts o{};
     o.precision = iter.precision;
     o.precond = iter.precond ? iter.precond : 3;
@@ -73,7 +73,7 @@ void FemCore::solve(double init_value) {
     o.inner_rtol = iter.inner_rtol;
     o.max_outer = 50;
     o.check_every = 32;
-    o.norm = PF_NORM_INIT;
+    o.norm = norm;
     o.init_value = init_value;
     pf_solve_stats st{};
     log(LOG_DETAIL, id + ": Iterating linear system");
@@ -81,6 +81,8 @@ void FemCore::solve(double init_value) {
     iter.iters = st.iters;
     iter.err = st.rel_resid;
     iter.seconds = st.seconds;
+    log(LOG_DEBUG, fmt("%s: %d CG iterations in %d runs, %.3f ms, residual %g of %g", id.c_str(), st.iters, st.outer,
+                       1e3 * st.seconds, st.rel_resid, st.bnorm));
     if (rc == PF_ENOCONV) {
         iter.converged = false;
         // iterative_matrix.hpp:299-314
@@ -377,7 +379,9 @@ double ElectricalCore::compute(int loops) {
     toterr = 0.;
     do {
         if (loopno != 0) check(pf_junction_update(ctx_, stable ? 1 : 0), "junction update");
-        solve(0.);
+        // every loop solves the system of its junction conductivities to the tolerance, from
+        // the previous loop's potential (the reference solves each loop directly in 2D)
+        solve(0., PF_NORM_START);
         double dj2 = 0., j2 = 0.;
         check(pf_currents(ctx_, &dj2, &j2), "currents");
         double mcur = std::sqrt(j2);

@@ -65,8 +65,10 @@ class FemCore {
     void log(int level, const std::string& msg) const {
         if (logger) logger(level, msg);
     }
-    // one linear solve from the device solution; fills iter.*
-    void solve(double init_value);
+    // one linear solve from the device solution; fills iter.*.  norm: PF_NORM_INIT (relative
+    // to the residual of the uniform init_value field) or PF_NORM_START (relative to the start
+    // residual of this solve, for Picard loops that warm-start every solve)
+    void solve(double init_value, int norm = PF_NORM_INIT);
 
     pf_ctx* ctx_ = nullptr;
     int dim_;

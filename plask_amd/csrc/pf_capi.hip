@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -136,14 +137,19 @@ struct pf_ctx {
     // category (PF_KCAT_*) over the iterations that ran
     bool profile = false;
     int prof_iter = -1;  // iteration index inside a profiled chunk, -1 outside
+    // cat: a kernel class (PF_KCAT_*, -1 none); kern / level / tbytes / bytes: one launch
+    // of kernel PF_K_* on multigrid level `level` in precision tbytes (8 or 4) moving
+    // `bytes` algorithmic HBM bytes (kern -1: a class record only)
     struct ProfRec {
-        int a, b, cat, iter;
+        int a, b, cat, iter, kern, level, tbytes;
+        double bytes;
     };
     std::vector<hipEvent_t> prof_ev;
     std::vector<ProfRec> prof_recs;
     size_t prof_used = 0;
     double kt[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
     int64_t kc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<pf_kernel_stat> kstat;  // per (kernel, level, precision) of the profiled solve
 
     // Picard-loop state
     double *k0h = nullptr, *k0v = nullptr, *ah = nullptr, *av = nullptr;
@@ -199,18 +205,41 @@ static void drop_graphs(pf_ctx* ctx) {
 // Profiled chunks: prof_open records a start event for one kernel category (returns a
 // handle, -1 when not profiling), prof_close its end event; prof_collect (after a stream
 // synchronisation) adds the durations of the records of iterations < ran.
-static int prof_open(pf_ctx* ctx, int cat) {
+static int prof_rec(pf_ctx* ctx, int cat, int kern, int level, int tbytes, double bytes) {
     if (!ctx->profile || ctx->prof_iter < 0) return -1;
     while (ctx->prof_used + 2 > ctx->prof_ev.size()) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return -1;
         ctx->prof_ev.push_back(e);
     }
-    pf_ctx::ProfRec r{(int)ctx->prof_used, (int)ctx->prof_used + 1, cat, ctx->prof_iter};
+    pf_ctx::ProfRec r{(int)ctx->prof_used, (int)ctx->prof_used + 1, cat, ctx->prof_iter, kern, level, tbytes, bytes};
     ctx->prof_used += 2;
     hipEventRecord(ctx->prof_ev[r.a], ctx->stream);
     ctx->prof_recs.push_back(r);
     return (int)ctx->prof_recs.size() - 1;
+}
+
+static int prof_open(pf_ctx* ctx, int cat) { return prof_rec(ctx, cat, -1, 0, 0, 0.); }
+
+// one launch of kernel `kern` (PF_K_*) on level `level` in precision T moving `bytes`
+template <typename T> static int prof_kern(pf_ctx* ctx, int kern, int level, double bytes) {
+    return prof_rec(ctx, -1, kern, level, (int)sizeof(T), bytes);
+}
+
+// Algorithmic HBM bytes of the launches (each value read or written once; element data
+// two conductivities; the Dirichlet mask one bit per node, whole 64-bit words).
+static double mask_bytes(long long n) { return 8.0 * (double)((n + 63) / 64); }
+// fused PCG apply: z, p_old, x read; p_new, q, x written
+static double bytes_apply_fused(long long n, long long e, int t) { return t * (6.0 * n + 2.0 * e) + mask_bytes(n); }
+// residual r = b - A x: x, b read; r written
+static double bytes_residual(long long n, long long e, int t) { return t * (3.0 * n + 2.0 * e) + mask_bytes(n); }
+// damped line sweep: right-hand side and the three line factors read, the sweep written;
+// with the PCG update also q read and the right-hand side written back; a previous
+// iterate read when there is one; the PCG residual read for the final dot products;
+// three factors per separator and column
+static double bytes_smooth(long long n, int ncol, int P, int t, bool upd, bool xin, bool fin) {
+    const double v = 5.0 + (upd ? 2.0 : 0.0) + (xin ? 1.0 : 0.0) + (fin && !upd ? 1.0 : 0.0);
+    return t * (v * n + 3.0 * std::max(P - 1, 0) * (double)ncol);
 }
 
 static void prof_close(pf_ctx* ctx, int h) {
@@ -222,8 +251,22 @@ static void prof_collect(pf_ctx* ctx, int ran) {
         if (r.iter >= ran) continue;
         float t = 0.f;
         hipEventElapsedTime(&t, ctx->prof_ev[r.a], ctx->prof_ev[r.b]);
-        ctx->kt[r.cat] += 1e-3 * t;
-        ctx->kc[r.cat] += 1;
+        if (r.cat >= 0) {
+            ctx->kt[r.cat] += 1e-3 * t;
+            ctx->kc[r.cat] += 1;
+        }
+        if (r.kern >= 0) {
+            pf_kernel_stat* k = nullptr;
+            for (auto& q : ctx->kstat)
+                if (q.kernel == r.kern && q.level == r.level && q.value_bytes == r.tbytes) k = &q;
+            if (!k) {
+                ctx->kstat.push_back(pf_kernel_stat{r.kern, r.level, r.tbytes, 0, 0., 0.});
+                k = &ctx->kstat.back();
+            }
+            k->launches += 1;
+            k->seconds += 1e-3 * t;
+            k->bytes += r.bytes;
+        }
     }
     ctx->prof_recs.clear();
     ctx->prof_used = 0;
@@ -328,6 +371,32 @@ static inline int grid1(long long n, int bs = 256, int maxb = 2048) {
     return (int)std::max(1LL, std::min<long long>(b, maxb));
 }
 
+// A kernel-tracing profiler attached to the process (rocprofv3 exports its tool
+// library list and output path into the profiled program's environment).  Under
+// rocprofv3 --kernel-trace, replaying hipGraphs ended the process in a host-side copy
+// inside the profiler library reading a device address (the graph's kernel arguments);
+// the same solves with plain launches trace cleanly.  Contexts created under such a
+// profiler therefore start with graphs off (pf_set_graphs can turn them back on).
+static bool profiler_attached() {
+    const char* vars[] = {"ROCP_TOOL_LIBRARIES", "ROCPROF_OUTPUT_PATH"};
+    for (const char* v : vars) {
+        const char* s = std::getenv(v);
+        if (s && *s) return true;
+    }
+    return false;
+}
+
+static bool graphs_default(const char* who) {
+    static bool warned = false;
+    if (!profiler_attached()) return true;
+    if (!warned) {
+        std::fprintf(stderr, "%s: profiler attached (ROCP_TOOL_LIBRARIES / ROCPROF_OUTPUT_PATH set): "
+                             "hipGraph replay off, kernels launched one by one\n", who);
+        warned = true;
+    }
+    return false;
+}
+
 extern "C" {
 
 int pf_ctx_create(pf_ctx** out, int device, int flags) {
@@ -339,6 +408,7 @@ int pf_ctx_create(pf_ctx** out, int device, int flags) {
     pf_ctx* ctx = new pf_ctx();
     ctx->device = device;
     ctx->profile = (flags & 1) != 0;
+    ctx->graphs = graphs_default("plask_amd");
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return PF_EDEVICE;
@@ -1001,6 +1071,11 @@ struct MgView {
     const uint64_t* dmask;
     T *b, *x, *r;
     int pm;
+    int level;
+    long long nodes() const { return (long long)ncol * planes; }
+    long long elems(int dim) const {
+        return dim == 3 ? (long long)(g.n0 - 1) * (g.n1 - 1) * (planes - 1) : (long long)(g2.n0 - 1) * (planes - 1);
+    }
 };
 
 // FP64 / FP32 members of the context and of a level
@@ -1068,6 +1143,7 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
     MgView<T> v{};
     v.pm = S::pm(ctx);
     v.planes = mg_planes(ctx);
+    v.level = l;
     if (l == 0) {
         if (ctx->dim == 3) v.g = axes3<T>(ctx);
         else v.g2 = axes2<T>(ctx);
@@ -1107,7 +1183,7 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
 
 // r = b - A x on level l (skipped after the stop)
 template <typename T>
-static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
+static void mg_residual_launch(pf_ctx* ctx, int l, const MgView<T>& v) {
     if (l == 0) {
         launch_apply<T, MODE_RESG, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv);
         return;
@@ -1137,6 +1213,13 @@ static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
     hipLaunchKernelGGL((k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream, v.g,
                        BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr, (const T*)nullptr,
                        (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv, ctx->st, ctx->partials);
+}
+
+template <typename T>
+static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
+    const int hk = prof_kern<T>(ctx, PF_K_RESIDUAL, l, bytes_residual(v.nodes(), v.elems(ctx->dim), (int)sizeof(T)));
+    mg_residual_launch<T>(ctx, l, v);
+    prof_close(ctx, hk);
 }
 
 // one smoothing sweep on a level: xout = xin + omega M_line^-1 r (see k_mg_smooth)
@@ -1172,12 +1255,15 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
     const dim3 grid((ncol + ps.CB - 1) / ps.CB);
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
     const T om = omega > T(0) ? omega : T(ctx->mg_omega);
+    const int hk = prof_kern<T>(ctx, PF_K_SMOOTH, v.level,
+                                bytes_smooth(v.nodes(), ncol, ps.P, (int)sizeof(T), upd, xin != nullptr, fin != 0));
     switch (ps.m) {
         case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
         case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
         case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
         case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
     }
+    prof_close(ctx, hk);
 }
 
 // z = V(r): one V-cycle.  upd: apply r -= alpha q first (inside the first sweep);
@@ -1215,8 +1301,11 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
             const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
             const MgTransfer tr = mg_transfer(ctx, (size_t)l);
             h = open(cat(l, PF_KCAT_MG_TRANSFER));
+            const int hk = prof_kern<T>(ctx, PF_K_RESTRICT, l,
+                                        sizeof(T) * (double)(v.nodes() + c.nodes()) + mask_bytes(c.nodes()));
             hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0, ctx->stream,
                                tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
+            prof_close(ctx, hk);
             prof_close(ctx, h);
         }
     }
@@ -1227,8 +1316,10 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
         const MgTransfer tr = mg_transfer(ctx, (size_t)l);
         const long long nf = (long long)v.ncol * v.planes;
         int h = open(cat(l, PF_KCAT_MG_TRANSFER));
+        const int hk = prof_kern<T>(ctx, PF_K_PROLONG, l, sizeof(T) * (double)(2 * nf + c.nodes()) + mask_bytes(nf));
         hipLaunchKernelGGL((k_mg_prolong_add<T>), dim3(grid1(nf, 256, 4096)), dim3(256), 0, ctx->stream, tr, c.wx,
                            c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
+        prof_close(ctx, hk);
         prof_close(ctx, h);
         h = open(cat(l, PF_KCAT_MG_RESIDUAL));
         mg_residual<T>(ctx, l, v);
@@ -1260,16 +1351,20 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     const bool p32 = sizeof(T) == 8 && ctx->prec32;
     auto precond32 = [&](bool start) {
         if constexpr (sizeof(T) == 8) {
+            int hk = prof_kern<T>(ctx, PF_K_TO_F32, 0, (start ? 12.0 : 28.0) * N);
             hipLaunchKernelGGL(k_cg_r_to_f32, dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, !start,
                                (double*)r, (const double*)q, ctx->r32, ctx->st);
+            prof_close(ctx, hk);
             if (mg) {
                 mg_vcycle<float>(ctx, ctx->r32, nullptr, ctx->z32, false, 0);
             } else {  // exact line solve in FP32 (the smoother kernel with omega = 1)
                 const MgView<float> v0 = mg_view<float>(ctx, 0, ctx->r32, ctx->z32);
                 mg_smooth<float>(ctx, v0, false, 0, ctx->r32, nullptr, nullptr, ctx->z32, nullptr, 1.f);
             }
+            hk = prof_kern<T>(ctx, PF_K_FROM_F32, 0, 20.0 * N);
             hipLaunchKernelGGL(k_cg_z_from_f32, dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, start ? 2 : 1,
                                (const double*)r, (const float*)ctx->z32, (double*)z, ctx->st, ctx->partials);
+            prof_close(ctx, hk);
         }
     };
     const int pm = sizeof(T) == 8 ? ctx->pm64 : ctx->pm32;
@@ -1291,20 +1386,27 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     else
         hipLaunchKernelGGL((k_cg_start<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, dinv, z, ctx->st,
                            ctx->partials);
+    const double tb = (double)sizeof(T);
     auto update = [&](T* pout) {
         (void)pout;
-        if (p32)
+        int hk = -1;
+        if (p32) {
             precond32(false);
-        else if (mg)
+        } else if (mg) {
             mg_vcycle<T>(ctx, r, q, z, true, 1);
-        else if (line && pm)
+        } else if (line && pm) {
+            hk = prof_kern<T>(ctx, PF_K_LINE, 0, tb * (7.0 * N + 3.0 * std::max(ps.P - 1, 0) * (double)ncol));
             launch_part<T, false>(ctx, ps, ncol, nplanes, plane, r, q, la, lw, lcp, schur, z);
-        else if (line)
+        } else if (line) {
+            hk = prof_kern<T>(ctx, PF_K_LINE, 0, tb * 9.0 * N);
             hipLaunchKernelGGL((k_cg_line<T, false>), lgrid, dim3(64), 0, ctx->stream, ncol, nplanes, plane, nplanes,
                                r, (const T*)q, la, lw, lcp, z, ctx->st, ctx->partials);
-        else
+        } else {
+            hk = prof_kern<T>(ctx, PF_K_JACOBI, 0, tb * 5.0 * N);
             hipLaunchKernelGGL((k_cg_update<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, r, q, dinv, z,
                                ctx->st, ctx->partials);
+        }
+        prof_close(ctx, hk);
     };
     // a fresh run starts both direction buffers at zero (beta = 0 on the first iteration);
     // a resumed run keeps the previous direction in p
@@ -1333,7 +1435,9 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
                 T* pout = pb[(c + 1) & 1];
                 ctx->prof_iter = c;
                 int h = prof_open(ctx, PF_KCAT_APPLY);
+                const int hk = prof_kern<T>(ctx, PF_K_APPLY, 0, bytes_apply_fused(N, ctx->E, (int)sizeof(T)));
                 launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
+                prof_close(ctx, hk);
                 prof_close(ctx, h);
                 h = prof_open(ctx, PF_KCAT_PRECOND);
                 update(pout);
@@ -1422,6 +1526,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
         ctx->kt[i] = 0.;
         ctx->kc[i] = 0;
     }
+    ctx->kstat.clear();
     // start vector with the Dirichlet values
     if (opts->x0 == PF_X0_GIVEN) {
         if (!x_inout) return fail(ctx, PF_EINVAL, "x_inout required with PF_X0_GIVEN");
@@ -1437,7 +1542,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     double bnorm = ctx->bnorm;
     if (opts->norm == PF_NORM_B_FULL) {
         bnorm = std::sqrt(ctx->bnorm * ctx->bnorm + ctx->dvals_sq);
-    } else if (opts->norm == PF_NORM_INIT) {
+    } else if (opts->norm == PF_NORM_INIT || opts->norm == PF_NORM_START) {
         hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, ctx->stream, N, ctx->tmp, opts->init_value);
         if (ctx->nd)
             hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd,
@@ -1460,6 +1565,11 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     int total_iters = 0, outer = 0, status = 0;
     double rn = 0.;
     if ((rc = true_residual(ctx, &rn))) return rc;
+    if (opts->norm == PF_NORM_START) {
+        if (opts->x0 == PF_X0_DEVICE_CHECKED) return fail(ctx, PF_EINVAL, "PF_X0_DEVICE_CHECKED needs PF_NORM_INIT");
+        bnorm = std::max(rn, 1e-4 * bnorm);
+        if (!(bnorm > 0.)) bnorm = 1.;
+    }
     int start = 0;
     if (opts->x0 == PF_X0_DEVICE_CHECKED) {
         if (opts->norm != PF_NORM_INIT) return fail(ctx, PF_EINVAL, "PF_X0_DEVICE_CHECKED needs PF_NORM_INIT");
@@ -1472,29 +1582,40 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
             start = 1;
         }
     }
-    if (opts->precision == PF_FP64 || opts->precision == PF_FP64_PREC32) {
-        if (opts->precision == PF_FP64_PREC32 && (precond == 1 || !ctx->pm32))
-            return fail(ctx, PF_EINVAL, "PF_FP64_PREC32 needs the partitioned line or the multigrid preconditioner");
-        // restarts only if the recursive residual drifted from the true one
+    // FP64 CG from the current x (r = the true residual); restarts only if the recursive
+    // residual drifted from the true one
+    auto fp64_cg = [&](bool prec32) -> int {
         for (int attempt = 0; attempt < 4 && rn > rtol * bnorm && total_iters < maxit; ++attempt) {
             int it = 0, stt = 0;
             double stop2 = (rtol * bnorm) * (rtol * bnorm);
-            ctx->prec32 = opts->precision == PF_FP64_PREC32;
-            rc = run_cg<double>(ctx, precond, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->la,
-                                ctx->lw, ctx->lcp, ctx->kh, ctx->kv, stop2, maxit - total_iters, opts->check_every,
-                                &ctx->graph64, &it, &stt);
+            ctx->prec32 = prec32;
+            int rc2 = run_cg<double>(ctx, precond, ctx->x, ctx->r, ctx->p, ctx->p2, ctx->qv, ctx->z, ctx->dinv, ctx->la,
+                                     ctx->lw, ctx->lcp, ctx->kh, ctx->kv, stop2, maxit - total_iters, opts->check_every,
+                                     &ctx->graph64, &it, &stt);
             ctx->prec32 = false;
-            if (rc) return rc;
+            if (rc2) return rc2;
             total_iters += it;
             status = stt;
-            if ((rc = true_residual(ctx, &rn))) return rc;
+            if ((rc2 = true_residual(ctx, &rn))) return rc2;
             if (stt != 0) break;
             outer++;
         }
+        return PF_OK;
+    };
+    // FP32 forms: iterative refinement converges while the FP32 correction contracts the
+    // FP64 residual (roughly kappa(A) * eps_fp32 < 1); a step that fails to halve the true
+    // residual means it no longer does, and the solve finishes with FP64 CG
+    int fallback = 0;
+    auto stalled = [&](double before) { return rn > rtol * bnorm && rn > 0.5 * before; };
+    if (opts->precision == PF_FP64 || opts->precision == PF_FP64_PREC32) {
+        if (opts->precision == PF_FP64_PREC32 && (precond == 1 || !ctx->pm32))
+            return fail(ctx, PF_EINVAL, "PF_FP64_PREC32 needs the partitioned line or the multigrid preconditioner");
+        if ((rc = fp64_cg(opts->precision == PF_FP64_PREC32))) return rc;
     } else if (opts->precision == PF_FP32_REFINE) {
         const double inner = opts->inner_rtol > 0. ? opts->inner_rtol : 1e-4;
         const int max_outer = opts->max_outer > 0 ? opts->max_outer : 50;
         while (rn > rtol * bnorm && outer < max_outer && total_iters < maxit) {
+            const double before = rn;
             // r holds M(F - A0 x) in FP64; scale to unit norm and solve A d = r in FP32
             double s = 1. / rn;
             hipLaunchKernelGGL(k_scale_to_f32, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->r, s, ctx->r32);
@@ -1513,6 +1634,10 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
             if ((rc = true_residual(ctx, &rn))) return rc;
             if (stt < 0) {
                 status = stt;
+                break;
+            }
+            if (stalled(before)) {
+                fallback = 1;
                 break;
             }
         }
@@ -1559,6 +1684,10 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
                 status = stt;
                 break;
             }
+            if (stalled(rn_prev)) {
+                fallback = 1;
+                break;
+            }
             // a direction is kept only if the replaced residual is close to the recursive
             // one; otherwise it is no longer conjugate to the new residual's error and the
             // next run restarts from the preconditioned residual
@@ -1567,6 +1696,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
     } else {
         return fail(ctx, PF_EINVAL, "unknown precision");
     }
+    if (fallback && status == 0 && (rc = fp64_cg(false))) return rc;
     HIPCK(hipStreamSynchronize(ctx->stream));
     auto t1 = std::chrono::steady_clock::now();
     ctx->have_solution = true;
@@ -1583,6 +1713,7 @@ int pf_solve(pf_ctx* ctx, const pf_solve_opts* opts, double* x_inout, pf_solve_s
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
         stats->status = ret;
         stats->start = start;
+        stats->fallback = fallback;
     }
     if (ret == PF_EBREAK) ctx->err = "CG breakdown: stiffness matrix is not positive definite";
     if (ret == PF_ENONFINITE) ctx->err = "non-finite value in CG iteration";
@@ -1689,6 +1820,13 @@ int pf_rhs_host(pf_ctx* ctx, double* b_out) {
         for (int64_t i = 0; i < ctx->nd; ++i) b_out[nodes[i]] = vals[i];
     }
     return PF_OK;
+}
+
+int pf_kernel_stats(const pf_ctx* ctx, pf_kernel_stat* out, int nmax) {
+    if (!ctx || nmax < 0 || (nmax > 0 && !out)) return PF_EINVAL;
+    const int n = (int)std::min<size_t>((size_t)nmax, ctx->kstat.size());
+    for (int i = 0; i < n; ++i) out[i] = ctx->kstat[i];
+    return (int)ctx->kstat.size();
 }
 
 int pf_kernel_times(pf_ctx* ctx, double* seconds, int64_t* counts, int n) {
@@ -2060,6 +2198,8 @@ int pf_set_graphs(pf_ctx* ctx, int on) {
     return PF_OK;
 }
 
+int pf_graphs_enabled(const pf_ctx* ctx) { return ctx && ctx->graphs ? 1 : 0; }
+
 int pf_set_prefetch(pf_ctx* ctx, int depth) {
     if (!ctx) return PF_EINVAL;
     if (depth != 1 && depth != 2) return fail(ctx, PF_EINVAL, "prefetch depth 1 or 2");
@@ -2240,12 +2380,15 @@ struct pf_batch {
     double *x = nullptr, *x0 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *la = nullptr,
            *lw = nullptr, *lc = nullptr, *cur = nullptr;
     BatchStats* dstats = nullptr;
+    BatchResume* dres = nullptr;
+    unsigned int* hfin = nullptr;  // pinned: instances finished, read between launches
     // multigrid levels of all instances (pools described by BatchLevel)
     BatchMg mg{};
     std::vector<long long> node_off;
     int tuned_block = 0, tuned_nwg = 0;
     std::vector<void*> allocs;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr, graph_init = nullptr;
+    bool graphs = true;
     int graph_block = 0, graph_nwg = 0;
     BatchParams graph_prm{};
 };
@@ -2280,7 +2423,13 @@ int pf_batch_create(pf_batch** out, int device) {
     if (device < 0 || device >= ndev) return PF_EINVAL;
     pf_batch* b = new pf_batch();
     b->device = device;
+    b->graphs = graphs_default("plask_amd batch");
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return PF_EDEVICE;
+    }
+    if (hipHostMalloc((void**)&b->hfin, sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
+        hipStreamDestroy(b->stream);
         delete b;
         return PF_EDEVICE;
     }
@@ -2293,7 +2442,9 @@ void pf_batch_destroy(pf_batch* b) {
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
     if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph_init) hipGraphExecDestroy(b->graph_init);
     for (void* p : b->allocs) hipFree(p);
+    if (b->hfin) hipHostFree(b->hfin);
     hipStreamDestroy(b->stream);
     delete b;
 }
@@ -2442,7 +2593,8 @@ int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, con
     b->N = N;
     b->E = E;
     int rc;
-    if ((rc = balloc(b, &b->dinst, ninst)) || (rc = balloc(b, &b->dorder, ninst)) || (rc = balloc(b, &b->dqueue, 1)) ||
+    if ((rc = balloc(b, &b->dinst, ninst)) || (rc = balloc(b, &b->dorder, ninst)) || (rc = balloc(b, &b->dqueue, 2)) ||
+        (rc = balloc(b, &b->dres, ninst)) ||
         (rc = balloc(b, &b->h0, h0.size())) || (rc = balloc(b, &b->h1, h1.size())) || (rc = balloc(b, &b->rm, rm.size())) ||
         (rc = balloc(b, &b->kh, E)) || (rc = balloc(b, &b->kv, E)) || (rc = balloc(b, &b->kh0, E)) ||
         (rc = balloc(b, &b->kv0, E)) || (rc = balloc(b, &b->dmask, mask.size())) || (rc = balloc(b, &b->x, N)) ||
@@ -2493,17 +2645,26 @@ int pf_batch_set(pf_batch* b, int ninst, int cylindrical, const int32_t* n0, con
 
 }  // extern "C"
 
-static void batch_launch(pf_batch* b, int block, int nwg, const BatchParams& prm) {
+// Start of a batch solve: potentials from the Dirichlet values, material
+// conductivities, zero currents, every instance not started.
+static void batch_launch_init(pf_batch* b) {
     const int gb = 2048;
     hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->N, b->x0, b->x);
     hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->E, b->kh0, b->kh);
     hipLaunchKernelGGL(k_copy_d, dim3(gb), dim3(256), 0, b->stream, b->E, b->kv0, b->kv);
     hipLaunchKernelGGL((k_fill<double>), dim3(gb), dim3(256), 0, b->stream, 2 * b->E, b->cur, 0.);
+    hipLaunchKernelGGL(k_batch_reset, dim3((b->ninst + 255) / 256), dim3(256), 0, b->stream, b->ninst, b->dstats,
+                       b->dres, b->dqueue);
+}
+
+// One launch of the persistent kernel: every unfinished instance advances by at most
+// prm.loop_budget Picard loops (all of them when the budget is 0).
+static void batch_launch_round(pf_batch* b, int block, int nwg, const BatchParams& prm) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->stream, b->dqueue);
 #define PF_BATCH_LAUNCH(BS)                                                                                         \
     hipLaunchKernelGGL((k_batch_shockley2d<BS>), dim3(nwg), dim3(BS), 0, b->stream, b->dinst, b->dorder, b->ninst, \
                        b->dqueue, prm, b->h0, b->h1, b->rm, b->kh, b->kv, b->dmask, b->x, b->r, b->p, b->q, b->z,   \
-                       b->la, b->lw, b->lc, b->cur, b->dstats, b->mg)
+                       b->la, b->lw, b->lc, b->cur, b->dstats, b->dres, b->mg)
     if (block == 256) PF_BATCH_LAUNCH(256);
     else if (block == 512) PF_BATCH_LAUNCH(512);
     else PF_BATCH_LAUNCH(1024);
@@ -2512,28 +2673,67 @@ static void batch_launch(pf_batch* b, int block, int nwg, const BatchParams& prm
 
 static bool same_prm(const BatchParams& a, const BatchParams& c) {
     return a.maxloops == c.maxloops && a.maxit == c.maxit && a.maxerr == c.maxerr && a.rtol == c.rtol &&
-           a.stable == c.stable && a.precond == c.precond && a.omega == c.omega && a.coarse_sweeps == c.coarse_sweeps;
+           a.stable == c.stable && a.precond == c.precond && a.omega == c.omega && a.coarse_sweeps == c.coarse_sweeps &&
+           a.loop_budget == c.loop_budget;
 }
 
-static int batch_run(pf_batch* b, int block, int nwg, const BatchParams& prm, double* seconds) {
-    if (!b->graph || b->graph_block != block || b->graph_nwg != nwg || !same_prm(b->graph_prm, prm)) {
-        if (b->graph) hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        hipGraph_t g;
-        BHIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        batch_launch(b, block, nwg, prm);
-        BHIPCK(hipStreamEndCapture(b->stream, &g));
-        BHIPCK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
-        hipGraphDestroy(g);
-        b->graph_block = block;
-        b->graph_nwg = nwg;
-        b->graph_prm = prm;
+// Kernel parameters from the caller's options (0 = the documented default).
+static BatchParams batch_params(const pf_batch_opts* o) {
+    return BatchParams{o->maxloops > 0 ? o->maxloops : 30, o->maxit > 0 ? o->maxit : 100000,
+                       o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable,
+                       o->precond == 2 ? 2 : 3, 0.65, 2, o->loop_budget};
+}
+
+template <typename F> static int batch_capture(pf_batch* b, hipGraphExec_t* exec, F&& launches) {
+    hipGraph_t g;
+    BHIPCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launches();
+    BHIPCK(hipStreamEndCapture(b->stream, &g));
+    BHIPCK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    return PF_OK;
+}
+
+// The whole batch: the start launch, then launches of the persistent kernel until every
+// instance has finished its Picard loop.  Each launch runs at most prm.loop_budget loops
+// per instance, so it returns after a bounded amount of work; between launches the host
+// reads the number of finished instances and reports it through `progress`.  Every
+// launch advances each unfinished instance by at least one loop, so at most
+// ceil(maxloops / loop_budget) launches run.
+static int batch_run(pf_batch* b, int block, int nwg, const BatchParams& prm, double* seconds,
+                     void (*progress)(int, int, int, double, void*), void* user) {
+    int rc;
+    if (b->graphs) {
+        if (!b->graph_init && (rc = batch_capture(b, &b->graph_init, [&] { batch_launch_init(b); }))) return rc;
+        if (!b->graph || b->graph_block != block || b->graph_nwg != nwg || !same_prm(b->graph_prm, prm)) {
+            if (b->graph) hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            if ((rc = batch_capture(b, &b->graph, [&] { batch_launch_round(b, block, nwg, prm); }))) return rc;
+            b->graph_block = block;
+            b->graph_nwg = nwg;
+            b->graph_prm = prm;
+        }
     }
+    const int rounds_max = prm.loop_budget > 0 ? (prm.maxloops + prm.loop_budget - 1) / prm.loop_budget : 1;
     hipEvent_t e0, e1;
     BHIPCK(hipEventCreate(&e0));
     BHIPCK(hipEventCreate(&e1));
     BHIPCK(hipEventRecord(e0, b->stream));
-    BHIPCK(hipGraphLaunch(b->graph, b->stream));
+    if (b->graphs) BHIPCK(hipGraphLaunch(b->graph_init, b->stream));
+    else batch_launch_init(b);
+    auto t0 = std::chrono::steady_clock::now();
+    int finished = 0, round = 0;
+    while (finished < b->ninst && round < rounds_max) {
+        if (b->graphs) BHIPCK(hipGraphLaunch(b->graph, b->stream));
+        else batch_launch_round(b, block, nwg, prm);
+        BHIPCK(hipMemcpyAsync(b->hfin, b->dqueue + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, b->stream));
+        BHIPCK(hipStreamSynchronize(b->stream));
+        finished = (int)*b->hfin;
+        ++round;
+        if (progress)
+            progress(round, finished, b->ninst, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                     user);
+    }
     BHIPCK(hipEventRecord(e1, b->stream));
     BHIPCK(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -2541,6 +2741,11 @@ static int batch_run(pf_batch* b, int block, int nwg, const BatchParams& prm, do
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (seconds) *seconds = 1e-3 * ms;
+    if (finished < b->ninst) {
+        b->err = "batch: " + std::to_string(b->ninst - finished) + " instances unfinished after " +
+                 std::to_string(round) + " launches";
+        return PF_ESTATE;
+    }
     return PF_OK;
 }
 
@@ -2553,7 +2758,11 @@ int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int*
         return PF_ESTATE;
     }
     BHIPCK(hipSetDevice(b->device));
-    BatchParams prm{o->maxloops, o->maxit, o->maxerr, o->rtol, o->stable, o->precond == 2 ? 2 : 3, 0.65, 2, 0};
+    if (o->maxloops < 0 || o->loop_budget < 0) {
+        b->err = "maxloops and loop_budget must be >= 0";
+        return PF_EINVAL;
+    }
+    const BatchParams prm = batch_params(o);
     int props_cu = 256;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) props_cu = pr.multiProcessorCount;
@@ -2565,7 +2774,7 @@ int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int*
         for (int mi = 0; mi < 2; ++mi) {
             int nwg = std::min(b->ninst, props_cu * mult[mi]);
             double t = 0.;
-            int rc = batch_run(b, blocks[bi], nwg, prm, &t);
+            int rc = batch_run(b, blocks[bi], nwg, prm, &t, o->progress, o->progress_user);
             if (rc) return rc;
             if (t < best) {
                 best = t;
@@ -2578,6 +2787,18 @@ int pf_batch_autotune(pf_batch* b, const pf_batch_opts* o, int* best_block, int*
     if (best_block) *best_block = bb;
     if (best_nwg) *best_nwg = bn;
     if (best_seconds) *best_seconds = best;
+    return PF_OK;
+}
+
+int pf_batch_conductivities(pf_batch* b, double* sh, double* sv) {
+    if (!b) return PF_EINVAL;
+    if (!b->ninst) {
+        b->err = "pf_batch_set first";
+        return PF_ESTATE;
+    }
+    BHIPCK(hipSetDevice(b->device));
+    if (sh) BHIPCK(hipMemcpy(sh, b->kh, sizeof(double) * b->E, hipMemcpyDeviceToHost));
+    if (sv) BHIPCK(hipMemcpy(sv, b->kv, sizeof(double) * b->E, hipMemcpyDeviceToHost));
     return PF_OK;
 }
 
@@ -2597,10 +2818,12 @@ int pf_batch_solve(pf_batch* b, const pf_batch_opts* o, double* V_out, pf_batch_
         return PF_EINVAL;
     }
     int nwg = o->nwg ? o->nwg : (b->tuned_nwg ? b->tuned_nwg : std::min(b->ninst, props_cu));
-    BatchParams prm{o->maxloops > 0 ? o->maxloops : 30, o->maxit > 0 ? o->maxit : 100000,
-                    o->maxerr > 0. ? o->maxerr : 0.05, o->rtol > 0. ? o->rtol : 1e-8, o->stable,
-                    o->precond == 2 ? 2 : 3, 0.65, 2, 0};
-    int rc = batch_run(b, block, nwg, prm, seconds);
+    if (o->maxloops < 0 || o->loop_budget < 0) {
+        b->err = "maxloops and loop_budget must be >= 0";
+        return PF_EINVAL;
+    }
+    const BatchParams prm = batch_params(o);
+    int rc = batch_run(b, block, nwg, prm, seconds, o->progress, o->progress_user);
     if (rc) return rc;
     if (V_out) BHIPCK(hipMemcpy(V_out, b->x, sizeof(double) * b->N, hipMemcpyDeviceToHost));
     if (stats) {

@@ -16,7 +16,7 @@ PF_GEOM_CARTESIAN, PF_GEOM_CYLINDRICAL = 0, 1
 PF_THERMAL, PF_ELECTRICAL = 0, 1
 PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64_PREC32 = 0, 1, 2, 3
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
-PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL = 0, 1, 2
+PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL, PF_NORM_START = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3
 KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KCAT_MG_COARSE = range(6)
 
@@ -25,8 +25,16 @@ EXPORTED = [
     "pf_dirichlet_set", "pf_load_set", "pf_interp_nodes", "pf_solve", "pf_get_solution", "pf_apply", "pf_apply_host", "pf_rhs_host",
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
-    "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host",
+    "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
+    "pf_coeffs_powerlaw", "pf_coeffs_update",
 ]
+
+# kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
+KERNEL_NAMES = {0: "apply (k_apply3d/k_apply2d MODE_OP, fused PCG step)",
+                1: "residual (k_apply3d/k_apply2d MODE_RESG)",
+                2: "line sweep (k_mg_smooth)", 3: "restriction (k_mg_restrict)",
+                4: "prolongation (k_mg_prolong_add)", 5: "line preconditioner (k_cg_part)",
+                6: "jacobi update (k_cg_update)", 7: "k_cg_r_to_f32", 8: "k_cg_z_from_f32"}
 
 
 class SolveOpts(C.Structure):
@@ -35,9 +43,14 @@ class SolveOpts(C.Structure):
                 ("norm", C.c_int), ("init_value", C.c_double)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("level", C.c_int), ("value_bytes", C.c_int), ("launches", C.c_int),
+                ("seconds", C.c_double), ("bytes", C.c_double)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("rel_resid", C.c_double), ("bnorm", C.c_double),
-                ("seconds", C.c_double), ("status", C.c_int), ("start", C.c_int)]
+                ("seconds", C.c_double), ("status", C.c_int), ("start", C.c_int), ("fallback", C.c_int)]
 
 
 class FemError(RuntimeError):
@@ -86,6 +99,9 @@ def load_library(path=LIB_PATH):
     L.pf_num_elements.argtypes = [P]
     L.pf_num_elements.restype = i64
     L.pf_kernel_times.argtypes = [P, P, P, I]
+    L.pf_kernel_stats.argtypes = [P, P, I]
+    L.pf_coeffs_powerlaw.argtypes = [P, I, P, P, P, P, i64]
+    L.pf_coeffs_update.argtypes = [P]
     L.pf_time_apply.argtypes = [P, I, I, C.POINTER(C.c_double)]
     L.pf_set_profile.argtypes = [P, I]
     L.pf_set_launch.argtypes = [P, I, I]
@@ -93,6 +109,7 @@ def load_library(path=LIB_PATH):
     L.pf_set_prefetch.argtypes = [P, I]
     L.pf_set_apply.argtypes = [P, I, I, I]
     L.pf_set_graphs.argtypes = [P, I]
+    L.pf_graphs_enabled.argtypes = [P]
     L.pf_set_block_order.argtypes = [P, I]
     L.pf_time_kernels.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pf_stream_copy.argtypes = [P, i64, I, C.POINTER(C.c_double)]
@@ -265,6 +282,30 @@ class FemContext:
         self._ck(self.L.pf_kernel_times(self.h, _p(s), _p(c), int(n)), "pf_kernel_times")
         return s, c
 
+    def kernel_stats(self):
+        """Per (kernel, level, precision) records of the last profiled solve: dicts with
+        kernel (PF_K_* id), name, level, value_bytes, launches, seconds (summed), bytes
+        (summed algorithmic HBM bytes)."""
+        n = self.L.pf_kernel_stats(self.h, None, 0)
+        if n < 0:
+            self._ck(n, "pf_kernel_stats")
+        arr = (KernelStat * max(n, 1))()
+        self.L.pf_kernel_stats(self.h, C.cast(arr, C.c_void_p), n)
+        return [dict(kernel=a.kernel, name=KERNEL_NAMES.get(a.kernel, str(a.kernel)), level=a.level,
+                     value_bytes=a.value_bytes, launches=a.launches, seconds=a.seconds, bytes=a.bytes)
+                for a in arr[:n]]
+
+    def set_powerlaw(self, k0_h, k0_v, a_h=None, a_v=None, equation=PF_THERMAL):
+        """Per-element conductivity law k = k0 (300 / T_avg)^a, evaluated on the device by
+        coeffs_update() from the current solution."""
+        E = len(np.ravel(k0_h))
+        self._law = [_f64(np.ravel(v)) if v is not None else np.zeros(E) for v in (k0_h, k0_v, a_h, a_v)]
+        self._ck(self.L.pf_coeffs_powerlaw(self.h, equation, *[_p(v) for v in self._law], E), "pf_coeffs_powerlaw")
+
+    def coeffs_update(self):
+        """Element conductivities from the law of set_powerlaw at the current element temperatures."""
+        self._ck(self.L.pf_coeffs_update(self.h), "pf_coeffs_update")
+
     def set_launch(self, rows, planes):
         self._ck(self.L.pf_set_launch(self.h, int(rows), int(planes)), "pf_set_launch")
 
@@ -298,6 +339,10 @@ class FemContext:
 
     def set_graphs(self, on):
         self._ck(self.L.pf_set_graphs(self.h, 1 if on else 0), "pf_set_graphs")
+
+    def graphs_enabled(self):
+        """Whether CG chunks are replayed from hipGraphs (off under a kernel-tracing profiler)."""
+        return bool(self.L.pf_graphs_enabled(self.h))
 
     def set_prefetch(self, depth):
         self._ck(self.L.pf_set_prefetch(self.h, int(depth)), "pf_set_prefetch")
