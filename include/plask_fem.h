@@ -312,6 +312,20 @@ int pf_time_apply(pf_ctx* ctx, int precision, int reps, double* seconds);
 
 /* Picard-loop support ------------------------------------------------------ */
 
+/* pf_solution_prolong for a coarse field that is already in device memory (e.g. the output of
+ * pf_net_predict with out = NULL, pf_net_output_device): x = offset + scale * coarse_dev,
+ * trilinear, then the Dirichlet values -- the same arithmetic, so the same result bit for
+ * bit -- and this entry point allocates nothing: no hipMalloc, hipFree or allocating
+ * library call.  No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (tight serving loops, code that must not allocate between solves).  Its
+ * per-axis tables (4 + 4 bytes per node of each fine axis) live in a fixed 256 KiB device
+ * workspace, one per GPU for the life of the process: grids whose three axes have more
+ * than 32768 nodes in all are refused (PF_ESHAPE).  Calls on one GPU share that workspace:
+ * calls that overlap in time on one GPU (from several threads) race; each call has finished
+ * with it when it returns.  coarse_dev must stay valid until the call returns. */
+int pf_solution_prolong_dev(pf_ctx* ctx, const float* coarse_dev, const double* cx, int ncx, const double* cy,
+                            int ncy, const double* cz, int ncz, double offset, double scale);
+
 /* Set the device solution (N values), e.g. the initial field inittemp (therm2d.cpp:79). */
 int pf_solution_set(pf_ctx* ctx, const double* x);
 

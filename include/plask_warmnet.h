@@ -57,9 +57,14 @@ int pf_net_sample_weights(pf_net* net, int idx, const float* wz);
 /* Training sample idx: inputs [v][8] and target [v] (host fp32). */
 int pf_net_sample_set(pf_net* net, int idx, const float* feat, const float* target);
 
-/* Forward pass: inputs from the host, or a stored sample; out = host [v]. */
+/* Forward pass: inputs from the host, or a stored sample; out = host [v], or NULL to
+ * leave the prediction only on the device (pf_net_output_device). */
 int pf_net_predict(pf_net* net, const float* feat, float* out);
 int pf_net_predict_sample(pf_net* net, int idx, float* out);
+
+/* Device address of the last prediction ([v] fp32, owned by the network, valid until
+ * the next forward pass or pf_net_destroy); complete when pf_net_predict returns. */
+const float* pf_net_output_device(const pf_net* net);
 
 /* Loss (as pf_net_set_loss) and its gradient with respect to every
  * parameter for stored sample idx (for gradient checks). */

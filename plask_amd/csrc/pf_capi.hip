@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -371,6 +372,26 @@ static inline int grid1(long long n, int bs = 256, int maxb = 2048) {
     return (int)std::max(1LL, std::min<long long>(b, maxb));
 }
 
+// Workspace of pf_solution_prolong_dev, the entry point that allocates nothing: the
+// per-axis interpolation tables (an int index and a float weight per fine node of each
+// axis) live in this file-scope device array.  Each GPU holds its own copy until the
+// process ends; host code uses the address hipGetSymbolAddress returns for the device
+// (scratch_ptr[device], set when a context is created on it), never the array's name.
+#define PF_PROLONG_SCRATCH_BYTES (256 * 1024)
+static __device__ __align__(256) unsigned char scratch[PF_PROLONG_SCRATCH_BYTES];
+static void* scratch_ptr[64] = {nullptr};
+static std::mutex scratch_mu;
+
+static hipError_t scratch_init(int device) {
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e == hipSuccess) scratch_ptr[device] = ptr;
+    return e;
+}
+
 // A kernel-tracing profiler attached to the process (rocprofv3 exports its tool
 // library list and output path into the profiled program's environment).  Under
 // rocprofv3 --kernel-trace, replaying hipGraphs ended the process in a host-side copy
@@ -419,6 +440,10 @@ int pf_ctx_create(pf_ctx** out, int device, int flags) {
         return PF_EDEVICE;
     }
     hipMemset(ctx->st, 0, sizeof(PcgState));
+    if (scratch_init(device) != hipSuccess) {  // device current (hipSetDevice above)
+        pf_ctx_destroy(ctx);
+        return PF_EDEVICE;
+    }
     *out = ctx;
     return PF_OK;
 }
@@ -1842,40 +1867,53 @@ static ElemGrid egrid(pf_ctx* ctx) {
     return ElemGrid{ctx->dim, (int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2]};
 }
 
+}  // extern "C"
+
 // Trilinear interpolation of node values of a coarser rectilinear grid onto the
 // solution vector: per fine index along each axis, the coarse interval (lo) and
-// the weight of its upper end.
-__global__ void k_prolong3d(int nx, int ny, int nz, int cx, int cy, const float* __restrict__ src,
-                            const int* __restrict__ ix, const int* __restrict__ iy, const int* __restrict__ iz,
-                            const float* __restrict__ wx, const float* __restrict__ wy, const float* __restrict__ wz,
-                            double* __restrict__ x) {
+// the weight of its upper end.  RAW: the coarse values are raw network outputs c and
+// the node values offset + scale c are formed here, rounded to float as the host does
+// for the RAW = false form (separate multiply and add, no fused contraction), so both
+// forms give identical results.
+template <bool RAW>
+__global__ void k_prolong3d(int nx, int ny, int nz, int cx, int cy, const float* __restrict__ src, double offset,
+                            double scale, const int* __restrict__ ix, const int* __restrict__ iy,
+                            const int* __restrict__ iz, const float* __restrict__ wx, const float* __restrict__ wy,
+                            const float* __restrict__ wz, double* __restrict__ x) {
     const long long n = (long long)nx * ny * nz;
+    auto at = [&](long long i) -> float {
+        if (RAW) return (float)__dadd_rn(offset, __dmul_rn(scale, (double)src[i]));
+        return src[i];
+    };
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
         const int a = ix[i], b = iy[j], c = iz[k];
         const float u = wx[i], w = wy[j], t = wz[k];
         const long long base = ((long long)c * cy + b) * cx + a;
         const long long sy = cx, sz = (long long)cx * cy;
-        const float v00 = src[base] + u * (src[base + 1] - src[base]);
-        const float v10 = src[base + sy] + u * (src[base + sy + 1] - src[base + sy]);
-        const float v01 = src[base + sz] + u * (src[base + sz + 1] - src[base + sz]);
-        const float v11 = src[base + sz + sy] + u * (src[base + sz + sy + 1] - src[base + sz + sy]);
+        const float s000 = at(base), s100 = at(base + 1), s010 = at(base + sy), s110 = at(base + sy + 1);
+        const float s001 = at(base + sz), s101 = at(base + sz + 1), s011 = at(base + sz + sy),
+                    s111 = at(base + sz + sy + 1);
+        const float v00 = s000 + u * (s100 - s000);
+        const float v10 = s010 + u * (s110 - s010);
+        const float v01 = s001 + u * (s101 - s001);
+        const float v11 = s011 + u * (s111 - s011);
         const float lo = v00 + w * (v10 - v00), hi = v01 + w * (v11 - v01);
         x[v] = (double)(lo + t * (hi - lo));
     }
 }
 
-int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int ncx, const double* cy, int ncy,
-                        const double* cz, int ncz, double offset, double scale) {
-    if (!ctx || !coarse || !cx || !cy || !cz) return PF_EINVAL;
-    if (ctx->dim != 3) return fail(ctx, PF_ESHAPE, "pf_solution_prolong: 3D grids only");
+// Per fine node of each axis: the coarse interval and the weight of its upper end
+// (trilinear in physical coordinates; fine nodes outside the coarse axis clamp).
+static int prolong_tables(pf_ctx* ctx, const double* cx, int ncx, const double* cy, int ncy, const double* cz,
+                          int ncz, std::vector<int> idx[3], std::vector<float> wt[3]) {
+    if (ctx->dim != 3) return fail(ctx, PF_ESHAPE, "solution prolongation: 3D grids only");
     if (ncx < 2 || ncy < 2 || ncz < 2) return fail(ctx, PF_ESHAPE, "coarse grid needs >= 2 nodes per axis");
-    HIPCK(hipSetDevice(ctx->device));
     const double* ca[3] = {cx, cy, cz};
     const int nc[3] = {ncx, ncy, ncz};
-    std::vector<int> idx[3];
-    std::vector<float> wt[3];
     for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i + 1 < nc[a]; ++i)
+            if (!(ca[a][i + 1] > ca[a][i])) return fail(ctx, PF_EINVAL, "coarse axis coordinates must increase");
         const std::vector<double>& f = ctx->ax[a];
         idx[a].resize(f.size());
         wt[a].resize(f.size());
@@ -1888,6 +1926,19 @@ int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int 
             wt[a][i] = (float)std::max(0., std::min(1., w));
         }
     }
+    return PF_OK;
+}
+
+extern "C" {
+
+int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int ncx, const double* cy, int ncy,
+                        const double* cz, int ncz, double offset, double scale) {
+    if (!ctx || !coarse || !cx || !cy || !cz) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    std::vector<int> idx[3];
+    std::vector<float> wt[3];
+    int rc;
+    if ((rc = prolong_tables(ctx, cx, ncx, cy, ncy, cz, ncz, idx, wt))) return rc;
     const size_t ncoarse = (size_t)ncx * ncy * ncz;
     std::vector<float> src(ncoarse);
     for (size_t i = 0; i < ncoarse; ++i) src[i] = (float)(offset + scale * (double)coarse[i]);
@@ -1912,9 +1963,9 @@ int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int 
         o += idx[a].size();
     }
     hipMemcpyAsync(dsrc, src.data(), sizeof(float) * ncoarse, hipMemcpyHostToDevice, ctx->stream);
-    hipLaunchKernelGGL(k_prolong3d, dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream, (int)ctx->n[0],
-                       (int)ctx->n[1], (int)ctx->n[2], ncx, ncy, dsrc, dix[0], dix[1], dix[2], dwx[0], dwx[1], dwx[2],
-                       ctx->x);
+    hipLaunchKernelGGL((k_prolong3d<false>), dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream,
+                       (int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ncx, ncy, dsrc, 0., 1., dix[0], dix[1], dix[2],
+                       dwx[0], dwx[1], dwx[2], ctx->x);
     if (ctx->nd)
         hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
                            ctx->dvals, ctx->x);
@@ -1922,6 +1973,46 @@ int pf_solution_prolong(pf_ctx* ctx, const float* coarse, const double* cx, int 
     hipFree(dsrc);
     hipFree(dtab);
     if (err != hipSuccess) return fail(ctx, PF_EDEVICE, std::string("pf_solution_prolong: ") + hipGetErrorString(err));
+    ctx->have_solution = true;
+    return PF_OK;
+}
+
+int pf_solution_prolong_dev(pf_ctx* ctx, const float* coarse_dev, const double* cx, int ncx, const double* cy, int ncy,
+                            const double* cz, int ncz, double offset, double scale) {
+    if (!ctx || !coarse_dev || !cx || !cy || !cz) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    void* ws = scratch_ptr[ctx->device];
+    if (!ws) return fail(ctx, PF_ESTATE, "pf_solution_prolong_dev: no workspace address for this device");
+    const size_t nidx = ctx->ax[0].size() + ctx->ax[1].size() + ctx->ax[2].size();
+    if ((sizeof(int) + sizeof(float)) * nidx > PF_PROLONG_SCRATCH_BYTES)
+        return fail(ctx, PF_ESHAPE, "pf_solution_prolong_dev: the grid's axes have " + std::to_string(nidx) +
+                                        " nodes in all; at most " +
+                                        std::to_string(PF_PROLONG_SCRATCH_BYTES / (sizeof(int) + sizeof(float))) +
+                                        " fit the fixed workspace");
+    std::vector<int> idx[3];
+    std::vector<float> wt[3];
+    int rc;
+    if ((rc = prolong_tables(ctx, cx, ncx, cy, ncy, cz, ncz, idx, wt))) return rc;
+    int* di = (int*)ws;
+    float* dw = (float*)((char*)ws + sizeof(int) * nidx);
+    size_t o = 0;
+    int* dix[3];
+    float* dwx[3];
+    for (int a = 0; a < 3; ++a) {
+        dix[a] = di + o;
+        dwx[a] = dw + o;
+        HIPCK(hipMemcpyAsync(dix[a], idx[a].data(), sizeof(int) * idx[a].size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCK(hipMemcpyAsync(dwx[a], wt[a].data(), sizeof(float) * wt[a].size(), hipMemcpyHostToDevice, ctx->stream));
+        o += idx[a].size();
+    }
+    hipLaunchKernelGGL((k_prolong3d<true>), dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream,
+                       (int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ncx, ncy, coarse_dev, offset, scale, dix[0],
+                       dix[1], dix[2], dwx[0], dwx[1], dwx[2], ctx->x);
+    if (ctx->nd)
+        hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
+                           ctx->dvals, ctx->x);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));  // the host tables must outlive the copies
     ctx->have_solution = true;
     return PF_OK;
 }

@@ -370,16 +370,18 @@ int pf_net_sample_set(pf_net* net, int idx, const float* feat, const float* targ
 }
 
 int pf_net_predict(pf_net* net, const float* feat, float* out) {
-    if (!net || !feat || !out) return PF_NET_EINVAL;
+    if (!net || !feat) return PF_NET_EINVAL;
     NCK(hipSetDevice(net->device));
     int rc;
     if ((rc = upload_feat(net, feat))) return rc;
     forward(net);
     NCK(hipGetLastError());
-    NCK(hipMemcpyAsync(out, net->y, sizeof(float) * net->L[0].n(), hipMemcpyDeviceToHost, net->stream));
+    if (out) NCK(hipMemcpyAsync(out, net->y, sizeof(float) * net->L[0].n(), hipMemcpyDeviceToHost, net->stream));
     NCK(hipStreamSynchronize(net->stream));
     return PF_NET_OK;
 }
+
+const float* pf_net_output_device(const pf_net* net) { return net ? net->y : nullptr; }
 
 int pf_net_predict_sample(pf_net* net, int idx, float* out) {
     if (!net || !out) return PF_NET_EINVAL;

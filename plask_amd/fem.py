@@ -26,7 +26,7 @@ EXPORTED = [
     "pf_solution_device", "pf_num_nodes", "pf_num_elements", "pf_kernel_times", "pf_time_apply", "pf_set_profile",
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
-    "pf_coeffs_powerlaw", "pf_coeffs_update",
+    "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
     L.pf_time_kernels.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pf_stream_copy.argtypes = [P, i64, I, C.POINTER(C.c_double)]
     L.pf_solution_prolong.argtypes = [P, P, P, I, P, I, P, I, C.c_double, C.c_double]
+    L.pf_solution_prolong_dev.argtypes = [P, P, P, I, P, I, P, I, C.c_double, C.c_double]
     L.pf_set_multigrid.argtypes = [P, I, I, C.c_double]
     L.pf_multigrid_levels.argtypes = [P]
     L.pf_precond_apply_host.argtypes = [P, I, I, P, P]
@@ -355,6 +356,14 @@ class FemContext:
         assert c.size == np.prod([len(a) for a in ax])
         self._ck(self.L.pf_solution_prolong(self.h, _p(c), _p(ax[0]), len(ax[0]), _p(ax[1]), len(ax[1]), _p(ax[2]),
                                             len(ax[2]), float(offset), float(scale)), "pf_solution_prolong")
+
+    def prolong_device(self, coarse_dev, axes, offset=0.0, scale=1.0):
+        """As prolong, for a coarse field already in device memory (address coarse_dev,
+        float32 node values on the 3D grid `axes`); allocates no device memory."""
+        ax = [_f64(a) for a in axes]
+        self._ck(self.L.pf_solution_prolong_dev(self.h, C.c_void_p(int(coarse_dev)), _p(ax[0]), len(ax[0]), _p(ax[1]),
+                                                len(ax[1]), _p(ax[2]), len(ax[2]), float(offset), float(scale)),
+                 "pf_solution_prolong_dev")
 
     def stream_copy(self, nbytes, reps=20):
         """Seconds per device-to-device copy of nbytes (read nbytes, write nbytes)."""

@@ -43,6 +43,8 @@ def _lib():
     L.pf_net_sample_weights.argtypes = [P, I, P]
     L.pf_net_predict.argtypes = [P, P, P]
     L.pf_net_predict_sample.argtypes = [P, I, P]
+    L.pf_net_output_device.argtypes = [P]
+    L.pf_net_output_device.restype = P
     L.pf_net_gradient.argtypes = [P, I, P, C.POINTER(C.c_double)]
     L.pf_net_train_batch.argtypes = [P, P, I, C.c_float, C.POINTER(C.c_double)]
     L._net_declared = True
@@ -140,6 +142,14 @@ class WarmNet:
         out = np.zeros(self.nv, dtype=np.float32)
         self._ck(self.L.pf_net_predict(self.h, _ptr(feat), _ptr(out)), "pf_net_predict")
         return out
+
+    def predict_device(self, feat):
+        """Forward pass leaving the prediction on the device; returns its device address
+        (valid until the next forward pass)."""
+        feat = _f32(feat)
+        assert feat.size == self.nv * NIN
+        self._ck(self.L.pf_net_predict(self.h, _ptr(feat), None), "pf_net_predict")
+        return int(self.L.pf_net_output_device(self.h))
 
     def predict_sample(self, idx):
         out = np.zeros(self.nv, dtype=np.float32)
