@@ -15,7 +15,8 @@ FP32-operator forms (FP64 refinement or reliable updates), the learned-warm-star
 when trained weights are present, per-kernel-class timings of profiled solves (HIP events
 around each launch on the context stream), the roofline of the kernel function with
 the largest total time in a profiled headline solve (all multigrid levels), and the CPU path
-timed on this host (serial IC(0)-PCG on the reference's 14-diagonal storage).
+timed on this host: the reference's NSPCG (CG + IC(0)) compiled from its sources, and this
+build's serial IC(0)-PCG restatement, both on the reference's 14-diagonal storage.
 """
 import argparse
 import json
@@ -40,6 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # complete CPU solve; used to scale the bounded per-iteration sample to a whole
 # solve.  Iteration counts depend on the algorithm and the inputs, not the host.
 CPU_FULL_ITERS = {(128, 128, 256): 161}
+# The same for the reference's own NSPCG (CG + IC(0), zeta = maxerr = 1e-8; a complete run
+# on 128x128x256 took 161 iterations, 77.6 s on one core, and reached ||B-AT||/||B-AT_init||
+# = 9.5e-9).
+CPU_REF_ITERS = {(128, 128, 256): 161}
 WEIGHTS = os.path.join(ROOT, "plask_amd", "data", "warmstart_64x64x128.npz")
 TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 # relative reduction asked of each FP32 run: restarted refinement needs deep inner solves,
@@ -140,6 +145,38 @@ def roofline_line(by_kernel, size, copy_gbs):
             "stream_copy_gbs": round(copy_gbs, 1)}
 
 
+def cpu_reference(A, seconds, full_iters):
+    """The reference's NSPCG (CG + IC(0), its default; compiled from the reference sources
+    into oracle/_ref) on the assembled 14-diagonal system, one core.  NSPCG factorises on
+    every call, so two calls stopped after k1 and k2 iterations give the set-up time and
+    the time per iteration; the complete solve is run instead when its estimate fits in
+    `seconds`.  None when the library was not built."""
+    import oracle as O
+    if O.ref_lib() is None:
+        return None
+    x0 = np.full(A.rank, 300.0)
+    k1, k2 = 2, 12
+    t0 = time.perf_counter()
+    O.nspcg_solve(A, x0, "ic", 1e-30, k1)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.nspcg_solve(A, x0, "ic", 1e-30, k2)
+    t2 = time.perf_counter() - t0
+    t_it = max(t2 - t1, 1e-9) / (k2 - k1)
+    setup = max(t1 - k1 * t_it, 0.0)
+    res = {"setup_s": setup, "s_per_iter": t_it}
+    if full_iters and setup + full_iters * t_it > seconds:
+        res.update(iters=full_iters, full_solve=False, solve_s=setup + full_iters * t_it)
+    else:
+        t0 = time.perf_counter()
+        u, it, zo, ier, _, _ = O.nspcg_solve(A, x0, "ic", 1e-8, 5000)
+        t = time.perf_counter() - t0
+        r0 = np.linalg.norm(A.B - A.mult(x0))
+        res.update(iters=it, full_solve=True, solve_s=t, ier=ier,
+                   rel_resid=float(np.linalg.norm(A.B - A.mult(u)) / r0))
+    return res
+
+
 def cpu_baseline(p, seconds, full_iters):
     """CPU path on this host, one core: assembly of the 14-diagonal system (the
     reference's SparseBandMatrix storage), the IC(0) factor, and IC(0)-PCG to the same
@@ -169,7 +206,7 @@ def cpu_baseline(p, seconds, full_iters):
         res.update(iters=it, full_solve=True, rel_resid=rel, solve_s=t_asm + t_run,
                    s_per_iter=(t_run - t_fact) / max(it, 1))
     res["value"] = 1.0 / res["solve_s"]
-    return res
+    return res, A
 
 
 def timed(fn, steps, warmup):
@@ -462,20 +499,35 @@ def main():
     copy_gbs = 2 * (1 << 30) / t_copy / 1e9
     roofline = roofline_line(by_kernel, size, copy_gbs)
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu:
         full = CPU_FULL_ITERS.get(size)
-        c = cpu_baseline(p, args.cpu_seconds, full)
+        c, A_dia = cpu_baseline(p, args.cpu_seconds, full)
         how = ("the complete solve timed (%d iterations)" % c["iters"] if c["full_solve"] else
                "3 iterations timed (%.3f s/iter) and scaled to the %d iterations of a complete solve" %
                (c["s_per_iter"], c["iters"]))
-        cpu = {"value": c["value"], "unit": "solves/s", "cores": 1, "kind": "port",
+        cpu_port = {"value": c["value"], "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": "serial IC(0)-PCG (this build's restatement of the reference's default incomplete-Cholesky "
                          "preconditioner family, not NSPCG) on the 14-diagonal SparseBandMatrix storage of the same "
                          "system, same stopping rule: assembly %.3f s + IC(0) factor %.3f s + iterations; %s"
                          % (c["assemble_s"], c["ic0_factor_s"], how),
                "assemble_s": round(c["assemble_s"], 3), "ic0_factor_s": round(c["ic0_factor_s"], 3),
                "s_per_iter": c["s_per_iter"], "iters": c["iters"], "full_solve": c["full_solve"]}
+        cpu = cpu_port
+        r = cpu_reference(A_dia, args.cpu_seconds, CPU_REF_ITERS.get(size))
+        del A_dia
+        if r is not None:
+            solve = c["assemble_s"] + r["solve_s"]
+            how = ("the complete solve timed (%d iterations, rel. residual %.2e)" % (r["iters"], r["rel_resid"])
+                   if r["full_solve"] else
+                   "runs stopped after 2 and 12 iterations timed (set-up %.3f s, %.3f s/iter) and scaled to the "
+                   "%d iterations of a complete run at zeta = 1e-8" % (r["setup_s"], r["s_per_iter"], r["iters"]))
+            cpu = {"value": 1.0 / solve, "unit": "solves/s", "cores": 1, "kind": "reference",
+                   "sample": "the reference's NSPCG (CG + IC(0), its default solver; nspcg.c compiled from the "
+                             "reference sources) on the 14-diagonal SparseBandMatrix system of the same workload: "
+                             "assembly %.3f s (this build's restatement of setMatrix) + %s" % (c["assemble_s"], how),
+                   "assemble_s": round(c["assemble_s"], 3), "setup_s": round(r["setup_s"], 3),
+                   "s_per_iter": r["s_per_iter"], "iters": r["iters"], "full_solve": r["full_solve"]}
 
     c1 = config1(args.steps, not args.no_cpu)
     print("config 1 (2D cyl): %.2f ms GPU" % c1["gpu_ms"], file=sys.stderr, flush=True)
@@ -509,6 +561,7 @@ def main():
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_port": cpu_port if cpu is not cpu_port else None,
     }
     if cpu and cpu.get("value"):
         line["speedup_vs_cpu"] = round(fp64["solves_per_s"] / cpu["value"], 1)

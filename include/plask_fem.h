@@ -153,6 +153,16 @@ int pf_dirichlet_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int6
  * :261-268), evaluated by the caller.  Repeated nodes are summed; n = 0 clears it.  Kept until replaced. */
 int pf_load_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n);
 
+/* Boundary terms of the stiffness matrix: the convection entries of setMatrix
+ * (setBoundaries with the convection K functions, therm2d.cpp:242-254, cylindrical
+ * :370-392, therm3d.cpp:250-259), evaluated by the caller as the reference adds them,
+ * A(row, col) += val for col <= row (an off-diagonal entry stands for both triangles).
+ * Repeated entries are summed; nnz = 0 clears them.  They are part of the operator of
+ * pf_solve, pf_apply and every residual, and take part in the Dirichlet elimination like
+ * the element entries; the preconditioners see their diagonal.  Not combined with a
+ * transient operator (pf_capacity_set). */
+int pf_boundary_matrix_set(pf_ctx* ctx, const int64_t* rows, const int64_t* cols, const double* vals, int64_t nnz);
+
 /* Linear interpolation of the current solution (node values) at npts points
  * (pts: npts x dim, row-major; um), the providers' lazy interpolation from the solver
  * mesh to a destination mesh (getTemperatures therm2d.cpp:543-552, therm3d.cpp:387-395;
@@ -351,6 +361,39 @@ int pf_coeffs_powerlaw(pf_ctx* ctx, int equation, const double* k0_h, const doub
                        const double* a_v, int64_t nelem);
 int pf_coeffs_update(pf_ctx* ctx);
 
+/* Transient thermal solve (DynamicThermalFem2DSolver / DynamicThermalFem3DSolver,
+ * solvers/thermal/dynamic/femT2d.cpp, femT3d.cpp: the theta method of setMatrix
+ * :140-279 and of the time loop of compute :282-331): with the stiffness K of
+ * pf_coeffs_set / pf_coeffs_update and the load of pf_source_set / pf_load_set, each step
+ * solves A T^{n+1} = B T^n + F with A = theta K + C, B = -(1 - theta) K + C, C the lumped
+ * heat-capacity matrix (the reference's default, lumping = true): each node of an element
+ * gets c_elem[e], the element's capacity over the time step in the units of K as the
+ * reference forms it (3D: cp rho 0.125e-9 dx dy dz / timestep; 2D: cp rho 0.25e-12 w h /
+ * timestep / 1e-9, times the mid radius in cylindrical geometry).  theta = methodparam
+ * (0 explicit, 0.5 Crank-Nicolson, 1 implicit).  c_elem = NULL returns to the steady
+ * operator.  While a capacity is set, pf_solve solves with A and pf_get_coeffs returns K. */
+int pf_capacity_set(pf_ctx* ctx, const double* c_elem, int64_t nelem, double theta);
+
+/* Dirichlet nodes in a time step (pf_dynamic_step's held argument).
+ * PF_HELD_FIXED: a Dirichlet node keeps its boundary value (the theta method on the
+ *   free nodes).
+ * PF_HELD_REFERENCE: the reference's update.  Its setMatrix eliminates the conditions from A
+ *   and F only (A.applyBC(btemperature, F), femT2d.cpp:243, femT3d.cpp:234) and B is left
+ *   whole, so a Dirichlet node's new value is its boundary value plus the (B T^n) of its row
+ *   (the free rows use the boundary value).  With realistic 2D capacities that term grows
+ *   geometrically; it is offered for comparisons with the reference. */
+#define PF_HELD_FIXED 0
+#define PF_HELD_REFERENCE 1
+
+/* One time step from the device temperature (the previous step or pf_solution_set): the
+ * right-hand side B T^n + F on the device, then pf_solve with opts (x0 forced to
+ * PF_X0_DEVICE: the step starts from T^n; PF_NORM_START matches a direct solve per step).
+ * The reference factorises A once per compute() and back-substitutes every step
+ * (femT2d.cpp:413-432; femT3d.cpp:304-317, whose extra swap of temperatures and X at :306
+ * leaves the right-hand side, not the solution, in temperatures: this entry point follows
+ * the 2D loop in both dimensions). */
+int pf_dynamic_step(pf_ctx* ctx, const pf_solve_opts* opts, int held, pf_solve_stats* stats);
+
 /* Current element conductivities (E values each) to the host. */
 int pf_get_coeffs(pf_ctx* ctx, double* k_inplane, double* k_vert);
 
@@ -367,6 +410,17 @@ int pf_junction_params(pf_ctx* ctx, const double* beta, const double* js, int64_
  * ElectricalFem2DSolver::setMatrix, electr2d.cpp:243-264; law beta.hpp:43-46).
  * stable = 1: CONVERGENCE_STABLE averaging. */
 int pf_junction_update(pf_ctx* ctx, int stable);
+/* The inputs of a junction law given by the caller (PythonCondSolver::activeCond of the
+ * ActiveCond solvers, electr_python.cpp:171-178, called from electr2d.cpp:247-256): per
+ * junction element the junction voltage U (V; mean over the element's columns of the
+ * potential at the active region's top minus its bottom) and the vertical current density
+ * jy = 0.1 sigma_v U / height (kA/cm^2, signed), from the device potential and the current
+ * conductivities.  Either pointer may be NULL. */
+int pf_junction_state(pf_ctx* ctx, double* U, double* jy, int64_t nje);
+/* Set the junction elements' conductivities (S/m, in-plane and vertical) from such a law:
+ * stable = 1 averages with the current values (CONVERGENCE_STABLE), then a vertical value
+ * that is NaN or below 1e-16 in magnitude becomes 1e-16 (electr2d.cpp:257-262). */
+int pf_junction_cond_set(pf_ctx* ctx, const double* cond_h, const double* cond_v, int64_t nje, int stable);
 
 /* Per-element flags: select = elements whose |j| enters the loop error (junction
  * elements, electr2d.cpp:410; NULL = all), noheat = elements with zero Joule heat

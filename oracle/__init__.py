@@ -11,6 +11,9 @@ beside this file) plus the reference's solver loops restated in Python:
   ``ThermalFem{2D,3D}Solver::compute`` (solvers/thermal/static/therm2d.cpp:473-496,
   therm3d.cpp:311-334) with band Cholesky (cholesky_matrix.hpp:90-111) or
   PCG on the symmetric diagonal storage (iterative_matrix.hpp:141-339).
+* ``thermal_dynamic`` -- the time loop of ``DynamicThermalFem{2D,3D}Solver::compute``
+  (solvers/thermal/dynamic/femT2d.cpp:413-440) on the lumped theta-method matrices of
+  ``setMatrix`` (femT2d.cpp:186-243, femT3d.cpp:208-234), dense for small grids.
 * ``electrical_compute`` -- the Picard loop of ``ElectricalFem{2D,3D}Solver::compute``
   (solvers/electrical/shockley/electr2d.cpp:363-440, electr3d.cpp:356-442) with the
   junction law of ``BetaSolver::activeCond`` (beta.hpp:43-46).
@@ -259,6 +262,98 @@ def lapack_band_cholesky_solve(A, B=None, threads=None, vendor=None):
     if info != 0:
         raise ArithmeticError("dpbtrs info %d" % info)
     return x, name, nt
+
+
+_REF = None
+
+
+def ref_lib():
+    """The reference's own NSPCG (oracle/_ref/libnspcg.so, built by the Makefile beside this
+    file from the reference sources); None when it was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libnspcg.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_ref_nspcg.argtypes = [C.c_long, C.c_int, P, P, P, P, C.c_int, C.c_long, C.c_double, C.c_long, C.c_long,
+                                    C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.orc_ref_nspcg.restype = C.c_long
+        _REF = L
+    return _REF
+
+
+def nspcg_solve(A, x0, precond="ic", zeta=1e-6, maxit=1000):
+    """Solve the 14- (3D) / 5-diagonal (2D) system A (ST_DIA) with the reference's NSPCG as
+    SparseMatrix::solverhs calls it (iterative_matrix.hpp:141-268): CG with the IC(0)
+    ('ic', PLaSK's default) or Jacobi ('jac') preconditioner, NSPCG's own stopping test on
+    zeta (PLaSK's maxerr).  Returns (u, iterations, final stopping value, ier, NSPCG's
+    factorisation seconds, iteration seconds)."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libnspcg.so not built (make -C oracle with the reference present)")
+    g = A.grid
+    minor = g.s[g.order[1]] if g.dim == 3 else g.s[g.order[0]]
+    major = g.s[g.order[0]] if g.dim == 3 else 0
+    u = np.array(x0, dtype=np.float64)
+    it, zo, tf, ti = C.c_long(0), C.c_double(0.), C.c_double(0.), C.c_double(0.)
+    ier = L.orc_ref_nspcg(A.rank, len(A.offsets), _p(A.offsets), _p(A.data), _p(A.B), _p(u),
+                          1 if precond == "ic" else 0, int(maxit), float(zeta), int(minor), int(major),
+                          C.byref(it), C.byref(zo), C.byref(tf), C.byref(ti))
+    return u, int(it.value), float(zo.value), int(ier), float(tf.value), float(ti.value)
+
+
+def node_capacity(grid, cap):
+    """Lumped capacity per node (oracle numbering): every element adds its c to each of its
+    nodes (femT2d.cpp:188-191 ``A(i,i) += ... + c``, femT3d.cpp:211-216)."""
+    cap = np.asarray(cap, dtype=np.float64)
+    m = [k - 1 for k in grid.n][::-1]               # element counts, major axis first
+    ce = cap.reshape(m)
+    acc = np.zeros([k + 1 for k in m])
+    for c in range(1 << grid.dim):
+        off = [(c >> a) & 1 for a in range(grid.dim)]
+        acc[tuple(slice(o, o + k) for o, k in zip(off, m))] += ce
+    out = np.zeros(grid.nnodes)
+    out[grid.natural_perm()] = acc.ravel()
+    return out
+
+
+def thermal_dynamic(grid, kh, kv, heat, cap, theta, T0, steps, dnodes=(), dvals=(), held="reference"):
+    """``steps`` passes of the transient loop (femT2d.cpp:419-440): X = B T + F, A T' = X.
+
+    A = theta K + C and B = -(1 - theta) K + C (lumped C, femT2d.cpp:186-205); the Dirichlet
+    conditions go into A and F only (``A.applyBC(btemperature, F)``, femT2d.cpp:243; band
+    elimination of matrix.hpp:132-150: F[c] -= A(c, r) val, row and column r cleared,
+    A(r, r) = 1, F[r] = val), so with held="reference" a held node takes val + (B T)_r each
+    step; held="fixed" keeps it at val.  Element arrays in natural order, ``cap`` per
+    element in stiffness units; returns every step's T."""
+    Ka = assemble(grid, 0, kh, kv, heat, ST_BAND)
+    K = Ka.dense()
+    F = Ka.B.copy()
+    Cn = node_capacity(grid, cap)
+    A = theta * K + np.diag(Cn)
+    B = -(1.0 - theta) * K + np.diag(Cn)
+    for r, v in zip(np.asarray(dnodes, dtype=np.int64), np.asarray(dvals, dtype=np.float64)):
+        F -= A[:, r] * v
+        A[r, :] = 0.0
+        A[:, r] = 0.0
+        A[r, r] = 1.0
+    for r, v in zip(np.asarray(dnodes, dtype=np.int64), np.asarray(dvals, dtype=np.float64)):
+        F[r] = v
+    import scipy.linalg as sl
+    fac = sl.cho_factor(A, lower=True)
+    T = np.array(T0, dtype=np.float64)
+    out = []
+    if held not in ("reference", "fixed"):
+        raise ValueError("held must be 'reference' or 'fixed'")
+    for _ in range(int(steps)):
+        T = sl.cho_solve(fac, B @ T + F)
+        if held == "fixed":
+            T[np.asarray(dnodes, dtype=np.int64)] = np.asarray(dvals, dtype=np.float64)
+        out.append(T.copy())
+    return out
 
 
 def apply(grid, mode, kh, kv, dmask, x):

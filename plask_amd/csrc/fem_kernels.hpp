@@ -50,11 +50,15 @@ struct PcgState {
     unsigned int counter[4];  // block tickets for the last-block reductions
 };
 
+// dsh: a diagonal added to the operator, per node (the lumped heat-capacity matrix of
+// the transient solver, C = diag(sum of the element capacities c_e over the elements
+// touching the node), femT3d.cpp:233-242); nullptr for the steady operator.
 template <typename T> struct Axes3 {
     int n0, n1, n2;
     const T* hx;  // element sizes along each axis (um)
     const T* hy;
     const T* hz;
+    const T* dsh = nullptr;
 };
 
 template <typename T> struct Axes2 {
@@ -62,6 +66,7 @@ template <typename T> struct Axes2 {
     const T* hx;    // element sizes (um)
     const T* hy;
     const T* rmid;  // cylindrical: element mid radius, else nullptr
+    const T* dsh = nullptr;
 };
 
 __device__ __forceinline__ bool dbit(const uint64_t* __restrict__ m, long long node) {
@@ -327,6 +332,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         T s = o[(0 * BY + ty) * PX + tx + 1] + o[(1 * BY + ty) * PX + tx] + o[(2 * BY + ty - 1) * PX + tx + 1] +
               o[(3 * BY + ty - 1) * PX + tx];
         long long node = (long long)Pg * plane + (long long)j * n0 + tx;
+        if (g.dsh) s += g.dsh[node] * own_g;
         T out;
         if (MODE == MODE_OP) {
             out = dir_g ? own_g : s;
@@ -634,8 +640,8 @@ __global__ __launch_bounds__(512) void k_apply3r(Axes3<T> g, int KR, int nb, con
             }
             __syncthreads();
             if (P >= kb && ival) {
-                const T s = own_c + oc[i];
                 const long long node = (long long)P * plane + (long long)j * n0 + i;
+                const T s = own_c + oc[i] + (g.dsh ? g.dsh[node] * own_cur : T(0));
                 T out;
                 if (MODE == MODE_OP) out = dir_cur ? own_cur : s;
                 else out = dir_cur ? T(0) : (F[node] - s);
@@ -783,8 +789,9 @@ __global__ __launch_bounds__(1024) void k_apply2d(Axes2<T> g, int KC, const T* _
             }
             __syncthreads();
             if (nvalid) {
-                T s = oc[0 * PX + tx + 1] + oc[1 * PX + tx];
                 long long node = (long long)P * n0 + tx;
+                T s = oc[0 * PX + tx + 1] + oc[1 * PX + tx];
+                if (g.dsh) s += g.dsh[node] * own_cur;
                 T out;
                 if (MODE == MODE_OP) out = dir_cur ? own_cur : s;
                 else out = dir_cur ? T(0) : (F[node] - s);
@@ -876,6 +883,83 @@ __global__ __launch_bounds__(256) void k_cg_update(long long n, T* __restrict__ 
     }
     double v[2] = {rz, rr}, tot[2];
     if (grid_reduce<2>(v, partials, &st->counter[1], tot)) finish_update(st, tot);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary matrix: the off-diagonal part of a sparse symmetric addition to the operator
+// (convection faces), rows in CSR over the nodes it touches; its diagonal travels with the
+// operator's diagonal shift.  MODE_OP: y += M Kb M x; MODE_RES: y -= M Kb x.  Skipped
+// after the stop when st is given (inside a CG run).
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_bnd_apply(long long nr, const long long* __restrict__ rows,
+                                                   const int* __restrict__ ptr, const long long* __restrict__ cols,
+                                                   const T* __restrict__ vals, const uint64_t* __restrict__ dmask,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   const PcgState* __restrict__ st) {
+    if (st && st->done) return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nr; i += (long long)gridDim.x * blockDim.x) {
+        const long long n = rows[i];
+        if (dbit(dmask, n)) continue;
+        T s = T(0);
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const long long c = cols[k];
+            if (MODE == MODE_OP && dbit(dmask, c)) continue;
+            s += vals[k] * x[c];
+        }
+        if (MODE == MODE_OP)
+            y[n] += s;
+        else
+            y[n] -= s;
+    }
+}
+
+// Unfused CG direction step (used with a boundary matrix): p = z + beta p_old and the
+// previous iteration's x += alpha_prev p_old, the work the fused apply does on the fly
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_pstep(long long n, const T* __restrict__ z, const T* __restrict__ pin,
+                                                  T* __restrict__ pout, T* __restrict__ x,
+                                                  const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const T beta = T((st->iters == 0) ? st->beta0 : st->rz / st->rz_prev);
+    const T ap = T(st->alpha);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const T po = pin[i];
+        pout[i] = z[i] + beta * po;
+        x[i] += ap * po;
+    }
+}
+
+// (p, q) over the free nodes -> pap, alpha = rz / pap (the fused apply's DOT step)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_dot(long long n, const T* __restrict__ p, const T* __restrict__ q,
+                                                const uint64_t* __restrict__ dmask, PcgState* __restrict__ st,
+                                                double* __restrict__ partials) {
+    if (st->done) return;
+    double d = 0.;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (!dbit(dmask, i)) d += (double)p[i] * (double)q[i];
+    double v[1] = {d}, tot[1];
+    if (grid_reduce<1>(v, partials, &st->counter[0], tot)) {
+        st->pap = tot[0];
+        if (!(tot[0] > 0.) || !isfinite(tot[0])) {
+            st->status = isfinite(tot[0]) ? -6 : -8;
+            st->done = 1;
+            st->alpha = 0.;
+        } else {
+            st->alpha = st->rz / tot[0];
+        }
+    }
+}
+
+// node diagonal shift = capacity + boundary diagonal (either may be absent)
+__global__ void k_sum_dsh(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                          double* __restrict__ d, float* __restrict__ d32) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double v = (a ? a[i] : 0.) + (b ? b[i] : 0.);
+        d[i] = v;
+        d32[i] = (float)v;
+    }
 }
 
 // last solution update after the iteration stopped: x += alpha p_last, p_last in
@@ -993,6 +1077,7 @@ struct LineBC3 {
             b += (kx1 + ky1 + kz1) / 9.;
             if (ek == k) c += (kx1 + ky1 - 2. * kz1) / 18.;  // z edge (k, k+1) of this element
         }
+        if (g.dsh) b += g.dsh[node];
         const bool dk = dbit(dmask, node);
         if (dk) b = 1.;
         if (dk || k == g.n2 - 1 || dbit(dmask, node + plane)) c = 0.;
@@ -1020,6 +1105,7 @@ struct LineBC2 {
             b += (kx1 + ky1) / 3.;
             if (ek == k) c += (kx1 - 2. * ky1) / 6.;
         }
+        if (g.dsh) b += g.dsh[node];
         const bool dk = dbit(dmask, node);
         if (dk) b = 1.;
         if (dk || k == g.n1 - 1 || dbit(dmask, node + g.n0)) c = 0.;
@@ -1548,6 +1634,7 @@ __global__ void k_diag3d(Axes3<double> g, const double* __restrict__ kh, const d
             double kxy = kh[e] * 1e-6, kz = kv[e] * 1e-6;
             d += (kxy * dy * dz / dx + kxy * dx * dz / dy + kz * dx * dy / dz) / 9.;
         }
+        if (g.dsh) d += g.dsh[node];
         dinv[node] = dbit(dmask, node) ? T(1) : T(1. / d);
     }
 }
@@ -1569,6 +1656,7 @@ __global__ void k_diag2d(Axes2<double> g, const double* __restrict__ kh, const d
             double r = g.rmid ? g.rmid[ei] : 1.;
             d += r * (kh[e] * h / w + kv[e] * w / h) / 3.;
         }
+        if (g.dsh) d += g.dsh[node];
         dinv[node] = dbit(dmask, node) ? T(1) : T(1. / d);
     }
 }
@@ -1845,10 +1933,46 @@ __global__ void k_junction_update(long long nje, int ncorner, const long long* _
         double cond = 10. * jy * d * beta[q] / log(1e7 * jy / js[q] + 1.);
         if (stable) cond = 0.5 * (kv[e] + cond);
         if (isnan(cond) || fabs(cond) < 1e-16) cond = 1e-16;
+        // the law's in-plane component is 0 (averaged with the previous one when stable)
+        const double ch = stable ? 0.5 * kh[e] : 0.;
         kv[e] = cond;
-        kh[e] = 0.;
+        kh[e] = ch;
         kv32[e] = (float)cond;
-        kh32[e] = 0.f;
+        kh32[e] = (float)ch;
+    }
+}
+
+// U and jy of every junction element for a law evaluated by the caller
+__global__ void k_junction_state(long long nje, int ncorner, const long long* __restrict__ elems,
+                                 const long long* __restrict__ bot, const long long* __restrict__ top,
+                                 const int* __restrict__ jid, const double* __restrict__ height,
+                                 const double* __restrict__ V, const double* __restrict__ kv, double* __restrict__ Uo,
+                                 double* __restrict__ jo) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nje; q += (long long)gridDim.x * blockDim.x) {
+        double U = 0.;
+        for (int c = 0; c < ncorner; ++c) U += V[top[q * ncorner + c]] - V[bot[q * ncorner + c]];
+        U /= ncorner;
+        Uo[q] = U;
+        jo[q] = 0.1 * kv[elems[q]] * U / height[jid[q]];
+    }
+}
+
+// junction conductivities from the caller's law
+__global__ void k_junction_cond_set(long long nje, const long long* __restrict__ elems, const double* __restrict__ ch_in,
+                                    const double* __restrict__ cv_in, int stable, double* __restrict__ kh,
+                                    double* __restrict__ kv, float* __restrict__ kh32, float* __restrict__ kv32) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nje; q += (long long)gridDim.x * blockDim.x) {
+        const long long e = elems[q];
+        double ch = ch_in[q], cv = cv_in[q];
+        if (stable) {
+            ch = 0.5 * (kh[e] + ch);
+            cv = 0.5 * (kv[e] + cv);
+        }
+        if (isnan(cv) || fabs(cv) < 1e-16) cv = 1e-16;
+        kh[e] = ch;
+        kv[e] = cv;
+        kh32[e] = (float)ch;
+        kv32[e] = (float)cv;
     }
 }
 

@@ -279,9 +279,138 @@ void ThermalCore::boundary_load() {
             }
         }
     }
+    if (have_conv_)
+        for (size_t q = 0; q < conv_nodes_.size(); ++q) add(conv_nodes_[q], conv_load_[q]);
     std::vector<double> vals(nodes.size());
     for (size_t q = 0; q < nodes.size(); ++q) vals[q] = F[nodes[q]];
     check(pf_load_set(ctx_, nodes.data(), vals.data(), (int64_t)nodes.size()), "boundary load");
+}
+
+// The convection part of setMatrix (setBoundaries with the convection functions):
+// an element side whose two nodes (2D) or wall whose four nodes (3D) all carry a
+// convection condition adds, as the reference does,
+//   2D (therm2d.cpp:242-254): load 0.5e-6 len coeff_n ambient_n per node, matrix
+//     (coeff_a + coeff_b) len / 6 on each diagonal and len / 12 off it (without the
+//     1e-6 um -> m factor of the load: the reference's 2D convection matrix is 1e6 times
+//     its load's scale, reproduced);
+//   cylindrical (therm2d.cpp:370-392): load 0.125e-6 len (coeff_a + coeff_b)
+//     (ambient_a + ambient_b) and the same matrix terms, each times r_edge on r = const
+//     sides and (r_mid -/+ len/6) (diagonal and load) or r_mid (off-diagonal) along r;
+//   3D (therm3d.cpp:250-259): load 0.25e-12 area coeff ambient per wall node and matrix
+//     0.125e-12 area (coeff_i + coeff_j) / (9 | 18 | 36) for the same node, an edge or a
+//     diagonal of the wall; as with the flux (therm3d.cpp:160-163) the wall's terms go to
+//     the element's local nodes 0-3 (its lower face), reproduced.
+void ThermalCore::set_convection(const std::vector<int64_t>& nodes, const std::vector<double>& coeff,
+                                 const std::vector<double>& ambient) {
+    if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
+    if (nodes.size() != coeff.size() || nodes.size() != ambient.size())
+        throw std::invalid_argument(id + ": convection node and value lists differ in length");
+    conv_nodes_.clear();
+    conv_load_.clear();
+    have_conv_ = !nodes.empty();
+    if (!have_conv_) {
+        if (conv_matrix_) check(pf_boundary_matrix_set(ctx_, nullptr, nullptr, nullptr, 0), "convection matrix");
+        conv_matrix_ = false;
+        return;
+    }
+    const double nan = std::nan("");
+    std::vector<double> cc(N_, nan), ca(N_, nan);
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        const int64_t n = nodes[q];
+        if (n < 0 || n >= N_) throw std::invalid_argument(id + ": boundary node out of range");
+        if (std::isnan(cc[n])) {
+            cc[n] = coeff[q];
+            ca[n] = ambient[q];
+        }
+    }
+    auto has = [&](int64_t n) { return !std::isnan(cc[n]); };
+    const std::vector<double>& ax0 = axes_[0];
+    const std::vector<double>& ax1 = axes_[1];
+    const int64_t n0 = (int64_t)ax0.size(), n1 = (int64_t)ax1.size();
+    const int64_t m0 = n0 - 1, m1 = n1 - 1, m2 = dim_ == 3 ? (int64_t)axes_[2].size() - 1 : 1;
+    std::vector<double> F(N_, 0.);
+    std::vector<char> seen(N_, 0);
+    std::vector<int64_t> rows, cols;
+    std::vector<double> vals;
+    auto load = [&](int64_t n, double v) {
+        if (!seen[n]) {
+            seen[n] = 1;
+            conv_nodes_.push_back(n);
+        }
+        F[n] += v;
+    };
+    auto mat = [&](int64_t a, int64_t b, double v) {
+        rows.push_back(std::max(a, b));
+        cols.push_back(std::min(a, b));
+        vals.push_back(v);
+    };
+    for (int64_t e = 0; e < E_; ++e) {
+        if (dim_ == 2) {
+            const int64_t i = e % m0, j = e / m0;
+            const int64_t lolo = i + n0 * j, uplo = lolo + 1, upup = uplo + n0, loup = lolo + n0;
+            if (!has(lolo) && !has(upup)) continue;  // every side has one of these two corners
+            const double w = ax0[i + 1] - ax0[i], h = ax1[j + 1] - ax1[j];
+            const double r = 0.5 * (ax0[i] + ax0[i + 1]);
+            // sides in the reference's order: bottom, right, top, left
+            const int64_t sa[4] = {lolo, uplo, upup, loup}, sb[4] = {uplo, upup, loup, lolo};
+            const double len[4] = {w, h, w, h};
+            for (int side = 0; side < 4; ++side) {
+                const int64_t a = sa[side], b = sb[side];
+                if (!has(a) || !has(b)) continue;
+                const double L = len[side], csum = cc[a] + cc[b];
+                if (!cyl_) {
+                    load(a, 0.5e-6 * L * cc[a] * ca[a]);
+                    load(b, 0.5e-6 * L * cc[b] * ca[b]);
+                    mat(a, a, csum * L / 6.);
+                    mat(b, b, csum * L / 6.);
+                    mat(a, b, csum * L / 12.);
+                } else {
+                    auto wdiag = [&](int64_t p, int64_t o) {
+                        if (side == 3) return ax0[i];      // left: r of the side
+                        if (side == 1) return ax0[i + 1];  // right
+                        return r + (p < o ? -L / 6. : L / 6.);
+                    };
+                    const double woff = side == 3 ? ax0[i] : side == 1 ? ax0[i + 1] : r;
+                    const double fa = 0.125e-6 * L * csum * (ca[a] + ca[b]);
+                    load(a, fa * wdiag(a, b));
+                    load(b, fa * wdiag(b, a));
+                    mat(a, a, csum * L / 6. * wdiag(a, b));
+                    mat(b, b, csum * L / 6. * wdiag(b, a));
+                    mat(a, b, csum * L / 12. * woff);
+                }
+            }
+        } else {
+            const int64_t i = e % m0, j = (e / m0) % m1, k = e / (m0 * m1);
+            if (k >= m2) continue;
+            int64_t idx[8];
+            for (int l = 0; l < 8; ++l) idx[l] = (i + (l & 1)) + n0 * ((j + ((l >> 1) & 1)) + n1 * (k + (l >> 2)));
+            if (!has(idx[0]) && !has(idx[7])) continue;  // every wall has one of these two corners
+            const std::vector<double>& ax2 = axes_[2];
+            const double dx = ax0[i + 1] - ax0[i], dy = ax1[j + 1] - ax1[j], dz = ax2[k + 1] - ax2[k];
+            static const int walls[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 2, 4, 6},
+                                            {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}};
+            const double areas[3] = {dx * dy, dy * dz, dz * dx};
+            for (int side = 0; side < 6; ++side) {
+                const int* wl = walls[side];
+                if (!has(idx[wl[0]]) || !has(idx[wl[1]]) || !has(idx[wl[2]]) || !has(idx[wl[3]])) continue;
+                const double area = areas[side / 2];
+                for (int p = 0; p < 4; ++p) {
+                    const int64_t np_ = idx[wl[p]];
+                    load(idx[p], 0.25e-12 * area * cc[np_] * ca[np_]);
+                    for (int q = 0; q <= p; ++q) {
+                        const int ij = p ^ q;
+                        const double d = p == q ? 9. : (ij == 1 || ij == 2) ? 18. : 36.;
+                        mat(idx[p], idx[q], 0.125e-12 * area * (cc[np_] + cc[idx[wl[q]]]) / d);
+                    }
+                }
+            }
+        }
+    }
+    conv_load_.resize(conv_nodes_.size());
+    for (size_t q = 0; q < conv_nodes_.size(); ++q) conv_load_[q] = F[conv_nodes_[q]];
+    check(pf_boundary_matrix_set(ctx_, rows.data(), cols.data(), vals.data(), (int64_t)vals.size()),
+          "convection matrix");
+    conv_matrix_ = !vals.empty();
 }
 
 // onInitialize: temperatures = inittemp, loopno = 0 (therm3d.cpp:71-80)
@@ -293,6 +422,17 @@ void ThermalCore::initialize() {
     initialized_ = true;
 }
 
+void ThermalCore::update_conductivity() {
+    if (kcb_) {
+        std::vector<double> tavg(E_), kh(E_), kv(E_);
+        check(pf_element_means(ctx_, tavg.data()), "element temperatures");
+        kcb_(tavg.data(), kh.data(), kv.data(), E_);
+        check(pf_coeffs_set(ctx_, PF_THERMAL, kh.data(), kv.data(), E_), "conductivity");
+    } else if (tdep_) {
+        check(pf_coeffs_update(ctx_), "conductivity");
+    }
+}
+
 double ThermalCore::compute(int loops) {
     if (!initialized_) initialize();
     log(LOG_INFO, id + ": Running thermal calculations");
@@ -302,16 +442,9 @@ double ThermalCore::compute(int loops) {
     check(pf_solution_change(ctx_, nullptr, nullptr), "loop start");  // T_prev = T
     do {
         // setMatrix: element conductivities at the current element temperatures
-        if (kcb_) {
-            std::vector<double> tavg(E_), kh(E_), kv(E_);
-            check(pf_element_means(ctx_, tavg.data()), "element temperatures");
-            kcb_(tavg.data(), kh.data(), kv.data(), E_);
-            check(pf_coeffs_set(ctx_, PF_THERMAL, kh.data(), kv.data(), E_), "conductivity");
-        } else if (tdep_) {
-            check(pf_coeffs_update(ctx_), "conductivity");
-        }
+        update_conductivity();
         // boundary loads at the loop's start temperatures (radiation changes every loop)
-        if (have_rad_ || (have_flux_ && loop == 0)) boundary_load();
+        if (have_rad_ || ((have_flux_ || have_conv_) && loop == 0)) boundary_load();
         solve(inittemp);
         check(pf_solution_change(ctx_, &err, &maxT), "loop error");
         if (err > toterr) toterr = err;
@@ -320,6 +453,82 @@ double ThermalCore::compute(int loops) {
         log(LOG_RESULT, fmt("%s: Loop %d(%d): max(T) = %.3f K, error = %g K", id.c_str(), loop, loopno, maxT, err));
     } while ((!iter.converged || err > maxerr) && (loops == 0 || loop < loops));
     return toterr;
+}
+
+// ---------------------------------------------------------------------------
+// onInitialize: temperatures = inittemp, elapstime = 0 (femT3d.cpp:71-80)
+void DynamicCore::initialize() {
+    ThermalCore::initialize();
+    elapstime = 0.;
+}
+
+// setMatrix (femT3d.cpp:140-279): conductivities and capacities at the element
+// temperatures, A = theta K + C on the device
+void DynamicCore::set_matrix() {
+    log(LOG_DETAIL, id + ": Setting up matrix system");
+    update_conductivity();
+    if (!ccb_) throw std::runtime_error(id + ": no heat capacity");
+    std::vector<double> tavg(E_), c(E_);
+    check(pf_element_means(ctx_, tavg.data()), "element temperatures");
+    ccb_(tavg.data(), c.data(), E_);
+    check(pf_capacity_set(ctx_, c.data(), E_, methodparam), "heat capacity");
+}
+
+// compute(time) (femT3d.cpp:282-331): steps of timestep from t = 0 while t < time + timestep/2,
+// rebuilding the matrices every rebuildfreq steps and logging max(T) every logfreq steps
+double DynamicCore::compute(double time) {
+    if (!initialized_) initialize();
+    if (!(timestep > 0.)) throw std::invalid_argument(id + ": timestep must be positive");
+    log(LOG_INFO, id + ": Running thermal calculations");
+    set_matrix();
+    pf_solve_opts o{};
+    o.precision = iter.precision;
+    o.precond = iter.precond ? iter.precond : 3;
+    o.maxit = iter.maxit;
+    o.rtol = iter.maxerr;
+    o.inner_rtol = iter.inner_rtol;
+    o.max_outer = 50;
+    o.check_every = 32;
+    // each step solved as the reference's back-substitution would: the step's start residual
+    // reduced by maxerr, floored at 1e-4 ||B|| (X_init = 0) once the steps become small
+    o.norm = PF_NORM_START;
+    o.init_value = 0.;
+    unsigned long r = (unsigned long)rebuildfreq, l = (unsigned long)logfreq;
+    time += timestep / 2.;
+    for (double t = 0.; t < time; t += timestep) {
+        if (rebuildfreq && r == 0) {
+            set_matrix();
+            r = (unsigned long)rebuildfreq;
+        }
+        pf_solve_stats st{};
+        int rc = pf_dynamic_step(ctx_, &o, held, &st);
+        iter.iters = st.iters;
+        iter.err = st.rel_resid;
+        iter.seconds = st.seconds;
+        if (rc == PF_ENOCONV) {
+            iter.converged = false;
+            if (iter.noconv == 0)
+                throw ComputationError(fmt("%s: failed to converge in %d iterations (error %g)", id.c_str(), iter.maxit,
+                                           st.rel_resid));
+            if (iter.noconv == 1)
+                log(LOG_WARNING, fmt("%s: Failed to converge in %d iterations (error %g)", id.c_str(), iter.maxit,
+                                     st.rel_resid));
+        } else {
+            check(rc, "time step");
+            iter.converged = true;
+        }
+        if (logfreq && l == 0) {
+            check(pf_solution_change(ctx_, nullptr, &maxT), "max temperature");
+            log(LOG_RESULT, fmt("%s: Time %.2f ns: max(T) = %.3f K", id.c_str(), elapstime, maxT));
+            l = (unsigned long)logfreq;
+        }
+        --r;
+        --l;
+        elapstime += timestep;
+        ++loopno;
+    }
+    elapstime -= timestep;
+    return 0.;
 }
 
 // ---------------------------------------------------------------------------
@@ -338,6 +547,7 @@ void ElectricalCore::set_junctions(const std::vector<int64_t>& elems, int ncorne
           "junctions");
     jelems_ = elems;
     jcond_.assign(elems.size(), start_cond_);
+    jcond_h_.assign(elems.size(), start_cond_h_);
 }
 
 void ElectricalCore::set_junction_params(const std::vector<double>& beta, const std::vector<double>& js) {
@@ -358,6 +568,7 @@ void ElectricalCore::initialize() {
     std::vector<double> z(E_ * dim_, 0.);
     loopno = 0;
     jcond_.assign(jelems_.size(), start_cond_);
+    jcond_h_.assign(jelems_.size(), start_cond_h_);
     initialized_ = true;
 }
 
@@ -368,7 +579,7 @@ double ElectricalCore::compute(int loops) {
     // loadConductivities (electr2d.cpp:331-353): materials, then the saved junction values
     std::vector<double> kh = sh_, kv = sv_;
     for (size_t q = 0; q < jelems_.size(); ++q) {
-        kh[jelems_[q]] = 0.;
+        kh[jelems_[q]] = jcond_h_[q];
         kv[jelems_[q]] = (std::isnan(jcond_[q]) || std::abs(jcond_[q]) < 1e-16) ? 1e-16 : jcond_[q];
     }
     check(pf_coeffs_set(ctx_, PF_ELECTRICAL, kh.data(), kv.data(), E_), "conductivity");
@@ -378,7 +589,17 @@ double ElectricalCore::compute(int loops) {
     double err = 0.;
     toterr = 0.;
     do {
-        if (loopno != 0) check(pf_junction_update(ctx_, stable ? 1 : 0), "junction update");
+        if (loopno != 0) {
+            if (law_) {
+                const int64_t nq = (int64_t)jelems_.size();
+                std::vector<double> U(nq), jy(nq), ch(nq, 0.), cv(nq, 0.);
+                check(pf_junction_state(ctx_, U.data(), jy.data(), nq), "junction state");
+                law_(U.data(), jy.data(), ch.data(), cv.data(), nq);
+                check(pf_junction_cond_set(ctx_, ch.data(), cv.data(), nq, stable ? 1 : 0), "junction conductivity");
+            } else {
+                check(pf_junction_update(ctx_, stable ? 1 : 0), "junction update");
+            }
+        }
         // every loop solves the system of its junction conductivities to the tolerance, from
         // the previous loop's potential (the reference solves each loop directly in 2D)
         solve(0., PF_NORM_START);
@@ -395,9 +616,12 @@ double ElectricalCore::compute(int loops) {
     } while ((!iter.converged || err > maxerr) && (loops == 0 || loop < (unsigned)loops));
     // saveConductivities (electr2d.cpp:355-361)
     if (!jelems_.empty()) {
-        std::vector<double> cv(E_);
-        check(pf_get_coeffs(ctx_, nullptr, cv.data()), "conductivity");
-        for (size_t q = 0; q < jelems_.size(); ++q) jcond_[q] = cv[jelems_[q]];
+        std::vector<double> ch(E_), cv(E_);
+        check(pf_get_coeffs(ctx_, ch.data(), cv.data()), "conductivity");
+        for (size_t q = 0; q < jelems_.size(); ++q) {
+            jcond_[q] = cv[jelems_[q]];
+            jcond_h_[q] = ch[jelems_[q]];
+        }
     }
     return toterr;
 }

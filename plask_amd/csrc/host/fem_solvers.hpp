@@ -97,6 +97,10 @@ class ThermalCore : public FemCore {
     void set_boundary_loads(const std::vector<int64_t>& flux_nodes, const std::vector<double>& flux,
                             const std::vector<int64_t>& rad_nodes, const std::vector<double>& emissivity,
                             const std::vector<double>& ambient);
+    // Convection (coefficient W/m^2/K, ambient K) per node, first value kept; sets the
+    // boundary matrix of the operator and the convection load.  Empty lists clear them.
+    void set_convection(const std::vector<int64_t>& nodes, const std::vector<double>& coeff,
+                        const std::vector<double>& ambient);
     double compute(int loops);
     void initialize();
 
@@ -105,14 +109,50 @@ class ThermalCore : public FemCore {
     int loopno = 0;
     double toterr = 0., maxT = 0.;
 
-  private:
-    void boundary_load();  // pf_load_set from the boundary values and the current temperatures
+  protected:
+    // element conductivities at the current element temperatures (the law on the device,
+    // or the host callback); false when they are constant
+    void update_conductivity();
     std::vector<double> k0h_, k0v_, ah_, av_;
     bool tdep_ = false;
+
+  private:
+    void boundary_load();  // pf_load_set from the boundary values and the current temperatures
     std::vector<double> flux_, emis_, ramb_;  // per node, NaN: no condition
     std::vector<int64_t> bc_elems_;           // elements with a node under a flux / radiation condition
     bool have_flux_ = false, have_rad_ = false;
+    std::vector<int64_t> conv_nodes_;  // convection load (constant), summed per node
+    std::vector<double> conv_load_;
+    bool have_conv_ = false, conv_matrix_ = false;
+
+  protected:
     std::function<void(const double*, double*, double*, int64_t)> kcb_;
+};
+
+// DynamicThermalFem2DSolver / DynamicThermalFem3DSolver time loop
+// (solvers/thermal/dynamic/femT2d.cpp:395-449, femT3d.cpp:282-331): theta method with the
+// lumped heat-capacity matrix, the system set up at the start of compute and every
+// rebuildfreq steps.  Element capacities c_e (the units of the stiffness, over the time
+// step) come from a host callback capacity(T_avg[E], c[E]) at each set-up.
+class DynamicCore : public ThermalCore {
+  public:
+    DynamicCore(int dim, bool cylindrical, int device, std::string id) : ThermalCore(dim, cylindrical, device, id) {}
+
+    void set_capacity_callback(std::function<void(const double*, double*, int64_t)> cb) { ccb_ = std::move(cb); }
+    double compute(double time);
+    void initialize();
+
+    double methodparam = 0.5;  // theta
+    double timestep = 0.1;     // ns
+    long rebuildfreq = 0;
+    long logfreq = 500;
+    double elapstime = 0.;
+    // Dirichlet nodes: PF_HELD_FIXED (keep their values) or PF_HELD_REFERENCE
+    int held = PF_HELD_FIXED;
+
+  private:
+    void set_matrix();
+    std::function<void(const double*, double*, int64_t)> ccb_;
 };
 
 // ElectricalFem2DSolver / ElectricalFem3DSolver + BetaSolver compute loop
@@ -129,7 +169,15 @@ class ElectricalCore : public FemCore {
                        const std::vector<double>& heights);
     void set_junction_params(const std::vector<double>& beta, const std::vector<double>& js);
     void set_flags(const std::vector<unsigned char>& select, const std::vector<unsigned char>& noheat);
-    void set_start_cond(double cond) { start_cond_ = cond; }
+    // default_junction_conductivity (c00 in-plane, c11 vertical)
+    void set_start_cond(double cond_v, double cond_h = 0.) {
+        start_cond_ = cond_v;
+        start_cond_h_ = cond_h;
+    }
+    // a junction law given by the caller (PythonCondSolver::activeCond): from U and jy of every
+    // junction element it writes the in-plane and vertical conductivities; empty: Shockley's law
+    using JunctionLaw = std::function<void(const double* U, const double* jy, double* cond_h, double* cond_v, int64_t n)>;
+    void set_junction_law(JunctionLaw law) { law_ = std::move(law); }
     double compute(int loops);
     void initialize();
     std::vector<double> currents();  // E * dim, kA/cm^2
@@ -145,8 +193,10 @@ class ElectricalCore : public FemCore {
   private:
     std::vector<double> sh_, sv_;
     std::vector<int64_t> jelems_;
-    std::vector<double> jcond_;  // saved junction conductivity per junction element
-    double start_cond_ = 5.;     // default_junction_conductivity c11 (electr2d.cpp:24)
+    std::vector<double> jcond_, jcond_h_;  // saved junction conductivity (c11, c00) per junction element
+    double start_cond_ = 5.;               // default_junction_conductivity c11 (electr2d.cpp:24)
+    double start_cond_h_ = 0.;             // and its c00
+    JunctionLaw law_;
     bool have_flags_ = false;
 };
 

@@ -82,6 +82,10 @@ PYBIND11_MODULE(_core, m) {
                  });
              })
         .def("set_heat", [](ThermalCore& s, const arr<double>& q) { s.set_heat(vec(q)); })
+        .def("set_convection",
+             [](ThermalCore& s, const arr<int64_t>& nodes, const arr<double>& coeff, const arr<double>& ambient) {
+                 s.set_convection(vec(nodes), vec(coeff), vec(ambient));
+             })
         .def("set_boundary_loads",
              [](ThermalCore& s, const arr<int64_t>& flux_nodes, const arr<double>& flux, const arr<int64_t>& rad_nodes,
                 const arr<double>& emissivity, const arr<double>& ambient) {
@@ -94,6 +98,28 @@ PYBIND11_MODULE(_core, m) {
         .def_readwrite("loopno", &ThermalCore::loopno)
         .def_readonly("toterr", &ThermalCore::toterr)
         .def_readonly("maxT", &ThermalCore::maxT);
+
+    py::class_<DynamicCore, ThermalCore>(m, "DynamicCore")
+        .def(py::init<int, bool, int, std::string>(), py::arg("dim"), py::arg("cylindrical") = false,
+             py::arg("device") = 0, py::arg("id") = "dynamic")
+        .def("set_capacity_callback",
+             [](DynamicCore& s, py::function f) {
+                 s.set_capacity_callback([f](const double* t, double* c, int64_t n) {
+                     py::gil_scoped_acquire g;
+                     py::array_t<double> ta(n, t);
+                     auto a = f(ta).cast<py::array_t<double, py::array::c_style | py::array::forcecast>>();
+                     if (a.size() != n) throw std::runtime_error("capacity callback size mismatch");
+                     std::copy(a.data(), a.data() + n, c);
+                 });
+             })
+        .def("initialize", &DynamicCore::initialize)
+        .def("compute", &DynamicCore::compute, py::arg("time"))
+        .def_readwrite("methodparam", &DynamicCore::methodparam)
+        .def_readwrite("timestep", &DynamicCore::timestep)
+        .def_readwrite("rebuildfreq", &DynamicCore::rebuildfreq)
+        .def_readwrite("logfreq", &DynamicCore::logfreq)
+        .def_readwrite("elapstime", &DynamicCore::elapstime)
+        .def_readwrite("held", &DynamicCore::held);
 
     py::class_<ElectricalCore, FemCore>(m, "ElectricalCore")
         .def(py::init<int, bool, int, std::string>(), py::arg("dim"), py::arg("cylindrical") = false,
@@ -113,7 +139,25 @@ PYBIND11_MODULE(_core, m) {
              [](ElectricalCore& s, const arr<unsigned char>& select, const arr<unsigned char>& noheat) {
                  s.set_flags(vec(select), vec(noheat));
              })
-        .def("set_start_cond", &ElectricalCore::set_start_cond)
+        .def("set_start_cond", &ElectricalCore::set_start_cond, py::arg("cond_v"), py::arg("cond_h") = 0.)
+        .def("set_junction_law",
+             [](ElectricalCore& s, py::object f) {
+                 if (f.is_none()) {
+                     s.set_junction_law(nullptr);
+                     return;
+                 }
+                 s.set_junction_law([f](const double* U, const double* jy, double* ch, double* cv, int64_t n) {
+                     py::gil_scoped_acquire g;
+                     py::array_t<double> ua(n, U), ja(n, jy);
+                     py::tuple r = f(ua, ja).cast<py::tuple>();
+                     if (r.size() != 2) throw std::runtime_error("junction law must return (cond_h, cond_v)");
+                     auto a = r[0].cast<py::array_t<double, py::array::c_style | py::array::forcecast>>();
+                     auto b = r[1].cast<py::array_t<double, py::array::c_style | py::array::forcecast>>();
+                     if (a.size() != n || b.size() != n) throw std::runtime_error("junction law size mismatch");
+                     std::copy(a.data(), a.data() + n, ch);
+                     std::copy(b.data(), b.data() + n, cv);
+                 });
+             })
         .def("initialize", &ElectricalCore::initialize)
         .def("compute", &ElectricalCore::compute, py::arg("loops") = 0)
         .def("currents", [](ElectricalCore& s) { return to_np(s.currents()); })

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -57,6 +58,9 @@ struct MgLevel {
     // right-hand side, iterate and residual of the level
     double *b = nullptr, *x = nullptr, *r = nullptr;
     float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+    // transient operator: the level's capacity diagonal (P^T of the finer level's)
+    double* dsh = nullptr;
+    float* dsh32 = nullptr;
 };
 
 struct pf_ctx {
@@ -151,6 +155,28 @@ struct pf_ctx {
     double kt[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
     int64_t kc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<pf_kernel_stat> kstat;  // per (kernel, level, precision) of the profiled solve
+
+    // transient (theta-method) operator A = theta K + C: the lumped capacity diagonal C
+    // (FP64 / FP32), the stiffness K as set (khB, kvB; kh, kv then hold theta K), the
+    // static load F0, the B T^n terms of the Dirichlet nodes, an all-free mask
+    bool dyn = false;
+    double theta = 1.;
+    size_t ydir_cap = 0;
+    double *dcap = nullptr, *khB = nullptr, *kvB = nullptr, *F0 = nullptr, *ydir = nullptr;
+    uint64_t* zmask = nullptr;
+    // diagonal shift of the operator (dsh = dcap + dconv, FP64 / FP32), on while either is set
+    bool dsh_on = false;
+    double* dsh = nullptr;
+    float* dsh32 = nullptr;
+    // boundary matrix (pf_boundary_matrix_set): its diagonal dconv, its off-diagonal part in
+    // CSR over the nbr rows it touches
+    bool have_dconv = false, have_bnd = false;
+    double* dconv = nullptr;
+    long long nbr = 0;
+    long long *bnd_rows = nullptr, *bnd_cols = nullptr;
+    int* bnd_ptr = nullptr;
+    double* bnd_vals = nullptr;
+    float* bnd_vals32 = nullptr;
 
     // Picard-loop state
     double *k0h = nullptr, *k0v = nullptr, *ah = nullptr, *av = nullptr;
@@ -281,6 +307,8 @@ static void dfree(pf_ctx* ctx, void** ptr) {
     *ptr = nullptr;
 }
 
+static void dyn_capture(pf_ctx* ctx);
+
 // release the multigrid levels (grid change)
 static void mg_free(pf_ctx* ctx) {
     for (MgLevel& L : ctx->mg) {
@@ -290,7 +318,8 @@ static void mg_free(pf_ctx* ctx) {
                          (void**)&L.wx32,  (void**)&L.wy32,  (void**)&L.la,      (void**)&L.lw,    (void**)&L.lcp,
                          (void**)&L.schur, (void**)&L.la32,  (void**)&L.lw32,    (void**)&L.lcp32, (void**)&L.schur32,
                          (void**)&L.b,     (void**)&L.x,     (void**)&L.r,       (void**)&L.b32,   (void**)&L.x32,
-                         (void**)&L.r32,   (void**)&L.rmid,  (void**)&L.rmid32};
+                         (void**)&L.r32,   (void**)&L.rmid,  (void**)&L.rmid32, (void**)&L.dsh,
+                         (void**)&L.dsh32};
         for (void** q : ptrs) dfree(ctx, q);
     }
     ctx->mg.clear();
@@ -558,6 +587,18 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     HIPCK(hipMemset(ctx->xprev, 0, sizeof(double) * N));
     ctx->have_law = ctx->have_sel = ctx->have_noheat = false;
     ctx->nje = 0;
+    ctx->dyn = false;  // a transient operator and a boundary matrix refer to the old grid
+    ctx->dsh_on = ctx->have_dconv = ctx->have_bnd = false;
+    ctx->nbr = 0;
+    for (double** pp : {&ctx->dsh, &ctx->dcap, &ctx->F0, &ctx->khB, &ctx->kvB, &ctx->ydir, &ctx->dconv, &ctx->bnd_vals})
+        dalloc(ctx, pp, 0);
+    dalloc(ctx, &ctx->dsh32, 0);
+    dalloc(ctx, &ctx->bnd_vals32, 0);
+    dalloc(ctx, &ctx->bnd_rows, 0);
+    dalloc(ctx, &ctx->bnd_cols, 0);
+    dalloc(ctx, &ctx->bnd_ptr, 0);
+    dalloc(ctx, &ctx->zmask, 0);
+    ctx->ydir_cap = 0;
     HIPCK(hipMemset(ctx->dmask, 0, sizeof(uint64_t) * ((N + 63) / 64)));
     HIPCK(hipMemset(ctx->q, 0, sizeof(double) * ctx->E));
     HIPCK(hipMemset(ctx->x, 0, sizeof(double) * N));
@@ -615,6 +656,7 @@ int pf_coeffs_set(pf_ctx* ctx, int equation, const double* k_inplane, const doub
     int b = grid1(nelem);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(b), dim3(256), 0, ctx->stream, (long long)nelem, ctx->kh, ctx->kh32);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(b), dim3(256), 0, ctx->stream, (long long)nelem, ctx->kv, ctx->kv32);
+    dyn_capture(ctx);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(ctx->stream));
     ctx->have_coeffs = true;
@@ -753,24 +795,45 @@ int pf_load_set(pf_ctx* ctx, const int64_t* nodes, const double* vals, int64_t n
 // kernel launch helpers
 // ---------------------------------------------------------------------------
 template <typename T> static Axes3<T> axes3(pf_ctx* ctx);
+// (with the diagonal shift -- capacity and boundary-matrix diagonal -- when one is set)
 template <> Axes3<double> axes3<double>(pf_ctx* ctx) {
-    return Axes3<double>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx, ctx->hy, ctx->hz};
+    return Axes3<double>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx, ctx->hy, ctx->hz,
+                         ctx->dsh_on ? ctx->dsh : nullptr};
 }
 template <> Axes3<float> axes3<float>(pf_ctx* ctx) {
-    return Axes3<float>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx32, ctx->hy32, ctx->hz32};
+    return Axes3<float>{(int)ctx->n[0], (int)ctx->n[1], (int)ctx->n[2], ctx->hx32, ctx->hy32, ctx->hz32,
+                        ctx->dsh_on ? ctx->dsh32 : nullptr};
 }
 template <typename T> static Axes2<T> axes2(pf_ctx* ctx);
 template <> Axes2<double> axes2<double>(pf_ctx* ctx) {
-    return Axes2<double>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx, ctx->hy, ctx->rmid};
+    return Axes2<double>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx, ctx->hy, ctx->rmid, ctx->dsh_on ? ctx->dsh : nullptr};
 }
 template <> Axes2<float> axes2<float>(pf_ctx* ctx) {
-    return Axes2<float>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx32, ctx->hy32, ctx->rmid32};
+    return Axes2<float>{(int)ctx->n[0], (int)ctx->n[1], ctx->hx32, ctx->hy32, ctx->rmid32,
+                        ctx->dsh_on ? ctx->dsh32 : nullptr};
 }
 
+// off-diagonal boundary-matrix part: MODE_OP y += M Kb M x, MODE_RES y -= M Kb x
+template <typename T, int MODE> static void bnd_apply(pf_ctx* ctx, const T* x, T* y, const PcgState* st) {
+    const T* vals = (const T*)(sizeof(T) == 8 ? (const void*)ctx->bnd_vals : (const void*)ctx->bnd_vals32);
+    hipLaunchKernelGGL((k_bnd_apply<T, MODE>), dim3(grid1(ctx->nbr, 256, 4096)), dim3(256), 0, ctx->stream, ctx->nbr,
+                       (const long long*)ctx->bnd_rows, (const int*)ctx->bnd_ptr, (const long long*)ctx->bnd_cols, vals,
+                       (const uint64_t*)ctx->dmask, x, y, st);
+}
+
+// raw: the operator without the Dirichlet elimination and without the capacity diagonal
+// (all nodes free), for the B T^n product of the transient step.  The plain operator and
+// residual forms include the boundary matrix; the fused CG forms do not (run_cg switches
+// to the unfused step when one is set) and neither does MODE_RESG (multigrid residuals:
+// the preconditioner sees the boundary matrix's diagonal only).
 template <typename T, int MODE, bool FUSEP, bool DOT>
 static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, T* pout, T* y, const T* F, const T* kh,
-                         const T* kv, T* xsol = nullptr) {
+                         const T* kv, T* xsol = nullptr, bool raw = false) {
     const int pk = sizeof(T) == 8 ? 0 : 1;
+    const uint64_t* dmask = raw ? ctx->zmask : ctx->dmask;
+    Axes3<T> g3 = axes3<T>(ctx);
+    Axes2<T> g2 = axes2<T>(ctx);
+    if (raw) g3.dsh = g2.dsh = nullptr;
     if (ctx->dim == 3 && ctx->apply_kind[pk] == 1) {
         const int n0 = (int)ctx->n[0], n1 = (int)ctx->n[1], n2 = (int)ctx->n[2];
         const int KR = ctx->KR[pk];
@@ -779,7 +842,7 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
         const int threads = (n0 + 63) / 64 * 64;
         const size_t shm = sizeof(T) * (size_t)(4 * (n0 + 1));
         hipLaunchKernelGGL((k_apply3r<T, MODE, FUSEP, DOT>), dim3(grid), dim3(threads), shm, ctx->stream,
-                           axes3<T>(ctx), KR, nb, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                           g3, KR, nb, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
                            ctx->partials);
     } else if (ctx->dim == 3) {
         const int BX = (int)ctx->n[0], BY = ctx->BY[pk], KC = ctx->KC[pk];
@@ -789,22 +852,109 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
         const size_t shm = tile_lds(BY, BX) / sizeof(double) * sizeof(T);
         if (ctx->pfd == 2)
             hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
-                               axes3<T>(ctx), BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                               g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
                                ctx->partials);
         else
             hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
-                               axes3<T>(ctx), BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st,
+                               g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
                                ctx->partials);
     } else {
         const int BX = (int)ctx->n[0];
         int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
         size_t shm = sizeof(T) * (size_t)(4 * (BX + 1));
-        hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, axes2<T>(ctx),
-                           ctx->KC2, xin, zin, pin, pout, xsol, y, F, ctx->dmask, kh, kv, ctx->st, ctx->partials);
+        hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, g2,
+                           ctx->KC2, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st, ctx->partials);
+    }
+    if constexpr (!FUSEP && !DOT && (MODE == MODE_OP || MODE == MODE_RES)) {
+        if (ctx->have_bnd && !raw) bnd_apply<T, MODE>(ctx, xin, y, nullptr);
     }
 }
 
 static int mg_refresh(pf_ctx* ctx);
+
+__global__ void k_copy_dev(long long n, const double* __restrict__ a, double* __restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// ---------------------------------------------------------------------------
+// Transient (theta-method) operator of DynamicThermalFem{2D,3D}Solver
+// (solvers/thermal/dynamic/femT3d.cpp:140-279, femT2d.cpp:140-330):
+//   A = theta K + C,  B = -(1 - theta) K + C,  A T^{n+1} = B T^n + F,
+// C the lumped heat-capacity matrix (each node of an element gets the element's c,
+// the capacity over the time step in the units of K).  The Dirichlet elimination is
+// applied to A and F only (femT3d.cpp:250), so a Dirichlet node's new value is its
+// boundary value plus (B T^n) of its row.
+// ---------------------------------------------------------------------------
+
+// node capacity: sum of the capacities of the elements touching the node
+__global__ void k_node_capacity(ElemGrid eg, const double* __restrict__ ce, double* __restrict__ dcap) {
+    const long long N = (long long)eg.n0 * eg.n1 * (eg.dim == 3 ? eg.n2 : 1);
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < N; n += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(n % eg.n0), j = (int)((n / eg.n0) % eg.n1);
+        const int k = eg.dim == 3 ? (int)(n / ((long long)eg.n0 * eg.n1)) : 0;
+        double s = 0.;
+        for (int c = 0; c < (eg.dim == 3 ? 8 : 4); ++c) {
+            const int ei = i - 1 + (c & 1), ej = j - 1 + ((c >> 1) & 1), ek = eg.dim == 3 ? k - 1 + (c >> 2) : 0;
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= eg.n0 - 1 || ej >= eg.n1 - 1 || (eg.dim == 3 && ek >= eg.n2 - 1))
+                continue;
+            s += ce[((long long)ek * (eg.n1 - 1) + ej) * (eg.n0 - 1) + ei];
+        }
+        dcap[n] = s;
+    }
+}
+
+// kh = theta khB, kv = theta kvB (and the FP32 copies)
+__global__ void k_theta_scale(long long E, double theta, const double* __restrict__ khB, const double* __restrict__ kvB,
+                              double* __restrict__ kh, double* __restrict__ kv, float* __restrict__ kh32,
+                              float* __restrict__ kv32) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        const double a = theta * khB[e], b = theta * kvB[e];
+        kh[e] = a;
+        kv[e] = b;
+        kh32[e] = (float)a;
+        kv32[e] = (float)b;
+    }
+}
+
+// F = F0 + C T - (1 - theta) K T  (= B T^n + F with KT = K T over all nodes); the
+// Dirichlet rows' B T^n saved in ydir
+__global__ void k_dyn_rhs(long long N, double omt, const double* __restrict__ F0, const double* __restrict__ dsh,
+                          const double* __restrict__ T, const double* __restrict__ KT, double* __restrict__ F) {
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < N; n += (long long)gridDim.x * blockDim.x)
+        F[n] = F0[n] + (dsh[n] * T[n] - omt * KT[n]);
+}
+
+__global__ void k_dyn_dir(long long nd, double omt, const long long* __restrict__ dn, const double* __restrict__ dsh,
+                          const double* __restrict__ T, const double* __restrict__ KT, double* __restrict__ ydir) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nd; i += (long long)gridDim.x * blockDim.x) {
+        const long long n = dn[i];
+        ydir[i] = dsh[n] * T[n] - omt * KT[n];
+    }
+}
+
+__global__ void k_dyn_dir_add(long long nd, const long long* __restrict__ dn, const double* __restrict__ ydir,
+                              double* __restrict__ T) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nd; i += (long long)gridDim.x * blockDim.x)
+        T[dn[i]] += ydir[i];
+}
+
+// theta K into kh/kv from the stiffness as set (khB/kvB)
+static void dyn_apply(pf_ctx* ctx) {
+    hipLaunchKernelGGL(k_theta_scale, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, (long long)ctx->E,
+                       ctx->theta, (const double*)ctx->khB, (const double*)ctx->kvB, ctx->kh, ctx->kv, ctx->kh32,
+                       ctx->kv32);
+}
+
+// after kh/kv were (re)set to the stiffness: keep it in khB/kvB, scale kh/kv by theta
+static void dyn_capture(pf_ctx* ctx) {
+    if (!ctx->dyn) return;
+    hipLaunchKernelGGL(k_copy_dev, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, (long long)ctx->E,
+                       (const double*)ctx->kh, ctx->khB);
+    hipLaunchKernelGGL(k_copy_dev, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, (long long)ctx->E,
+                       (const double*)ctx->kv, ctx->kvB);
+    dyn_apply(ctx);
+}
 
 // Partitioned line factors of one grid from the line coefficients in ctx->sb / ctx->sc
 // (k_line_bc3d / k_line_bc2d): segments in parallel, then the separator recurrence per
@@ -880,6 +1030,8 @@ static int prepare(pf_ctx* ctx) {
     if (ctx->nd)
         hipLaunchKernelGGL(k_scatter, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, ctx->dnodes,
                            ctx->dvals, ctx->tmp);
+    if (ctx->dyn)  // the transient step adds B T^n to this static load every step
+        hipLaunchKernelGGL(k_copy_dev, dim3(gb), dim3(256), 0, ctx->stream, N, (const double*)ctx->F, ctx->F0);
     launch_apply<double, MODE_RES, false, false>(ctx, ctx->tmp, nullptr, nullptr, nullptr, ctx->r, ctx->F, ctx->kh, ctx->kv);
     hipLaunchKernelGGL((k_norm2<double>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, ctx->r, ctx->st,
                        ctx->partials, 0);
@@ -926,6 +1078,31 @@ static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long 
 
 // planes along the layer (line) axis: axis 2 in 3D, axis 1 in 2D
 static int mg_planes(const pf_ctx* ctx) { return (int)(ctx->dim == 3 ? ctx->n[2] : ctx->n[1]); }
+
+// Capacity diagonal of a coarse level: c_c = P^T c_f, the row sums of P^T C P (P
+// reproduces constants), i.e. the lumped form of the Galerkin coarse capacity matrix.
+__global__ void k_restrict_diag(MgTransfer tr, const double* __restrict__ wx, const double* __restrict__ wy,
+                                const double* __restrict__ cf, double* __restrict__ cc, float* __restrict__ cc32) {
+    const long long planec = (long long)tr.n0c * tr.n1c, planef = (long long)tr.n0f * tr.n1f;
+    const long long Nc = planec * tr.n2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nc;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int I = (int)(idx % tr.n0c), J = (int)((idx / tr.n0c) % tr.n1c);
+        const long long k = idx / planec;
+        int ii[3], jj[3];
+        double wi[3], wj[3];
+        restrict_axis(I, tr.n0c, tr.cx, tr.mx, wx, ii, wi);
+        restrict_axis(J, tr.n1c, tr.cy, tr.my, wy, jj, wj);
+        const double* src = cf + k * planef;
+        double s = 0.;
+        for (int b = 0; b < 3; ++b) {
+            const double* row = src + (long long)jj[b] * tr.n0f;
+            s += wj[b] * (wi[0] * row[ii[0]] + wi[1] * row[ii[1]] + wi[2] * row[ii[2]]);
+        }
+        cc[idx] = s;
+        cc32[idx] = (float)s;
+    }
+}
 
 static MgTransfer mg_transfer(pf_ctx* ctx, size_t l) {
     const MgLevel& L = ctx->mg[l];
@@ -986,12 +1163,17 @@ static int mg_refresh(pf_ctx* ctx) {
             L.schur_m32 = s32.m;
         }
         const int gl = grid1((long long)ncol * n2, 256, 4096);
+        if (ctx->dsh_on) {
+            if (!L.dsh && ((rc = dalloc(ctx, &L.dsh, L.N)) || (rc = dalloc(ctx, &L.dsh32, L.N)))) return rc;
+            hipLaunchKernelGGL(k_restrict_diag, dim3(grid1(L.N, 256, 4096)), dim3(256), 0, ctx->stream, tr, L.wx,
+                               L.wy, (const double*)(F ? F->dsh : ctx->dsh), L.dsh, L.dsh32);
+        }
         if (ctx->dim == 3) {
-            const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz};
+            const Axes3<double> g{L.n0, L.n1, n2, L.hx, L.hy, ctx->hz, ctx->dsh_on ? L.dsh : nullptr};
             hipLaunchKernelGGL(k_line_bc3d, dim3(gl), dim3(256), 0, ctx->stream, g, L.kh, L.kv, L.dmask, ctx->sb,
                                ctx->sc);
         } else {
-            const Axes2<double> g{L.n0, n2, L.hx, ctx->hy, L.rmid};
+            const Axes2<double> g{L.n0, n2, L.hx, ctx->hy, L.rmid, ctx->dsh_on ? L.dsh : nullptr};
             hipLaunchKernelGGL(k_line_bc2d, dim3(gl), dim3(256), 0, ctx->stream, g, L.kh, L.kv, L.dmask, ctx->sb,
                                ctx->sc);
         }
@@ -1132,6 +1314,7 @@ template <> struct MgPick<double> {
     static double* b(const MgLevel& L) { return L.b; }
     static double* x(const MgLevel& L) { return L.x; }
     static double* r(const MgLevel& L) { return L.r; }
+    static const double* dsh(const MgLevel& L) { return L.dsh; }
 };
 template <> struct MgPick<float> {
     static int pm(const pf_ctx* c) { return c->pm32; }
@@ -1160,6 +1343,7 @@ template <> struct MgPick<float> {
     static float* b(const MgLevel& L) { return L.b32; }
     static float* x(const MgLevel& L) { return L.x32; }
     static float* r(const MgLevel& L) { return L.r32; }
+    static const float* dsh(const MgLevel& L) { return L.dsh32; }
 };
 
 template <typename T>
@@ -1188,8 +1372,9 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
     }
     const MgLevel& L = S::lv(ctx, l);
     v.pm = S::pm(L);
-    if (ctx->dim == 3) v.g = Axes3<T>{L.n0, L.n1, v.planes, S::hx(L), S::hyl(L), S::hz(ctx)};
-    else v.g2 = Axes2<T>{L.n0, v.planes, S::hx(L), S::hy(ctx), S::rmid(L)};
+    const T* dl = ctx->dsh_on ? S::dsh(L) : nullptr;
+    if (ctx->dim == 3) v.g = Axes3<T>{L.n0, L.n1, v.planes, S::hx(L), S::hyl(L), S::hz(ctx), dl};
+    else v.g2 = Axes2<T>{L.n0, v.planes, S::hx(L), S::hy(ctx), S::rmid(L), dl};
     v.ncol = L.n0 * L.n1;
     v.kh = S::kh(L);
     v.kv = S::kv(L);
@@ -1440,10 +1625,24 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
     HIPCK(hipGetLastError());
     // iteration t reads p_old from pb[t % 2] and writes p_new to pb[(t + 1) % 2]
     T* pb[2] = {p, p2};
+    // p = z + beta p_old, x += alpha_prev p_old, q = A p, (p, q) and alpha: one fused apply,
+    // or with a boundary matrix the same steps unfused (its rows are added between the
+    // operator and the dot product)
+    auto apply_step = [&](T* pin, T* pout) {
+        if (!ctx->have_bnd) {
+            launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
+            return;
+        }
+        hipLaunchKernelGGL((k_cg_pstep<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, (const T*)z,
+                           (const T*)pin, pout, x, (const PcgState*)ctx->st);
+        launch_apply<T, MODE_OP, false, false>(ctx, pout, nullptr, nullptr, nullptr, q, nullptr, kh, kv);
+        hipLaunchKernelGGL((k_cg_dot<T>), dim3(ctx->upd_blocks), dim3(256), 0, ctx->stream, N, (const T*)pout,
+                           (const T*)q, (const uint64_t*)ctx->dmask, ctx->st, ctx->partials);
+    };
     auto one_iter = [&](int t) {
         T* pin = pb[t & 1];
         T* pout = pb[(t + 1) & 1];
-        launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
+        apply_step(pin, pout);
         update(pout);
     };
     int chunk = check_every > 0 ? check_every : 32;
@@ -1461,7 +1660,7 @@ static int run_cg(pf_ctx* ctx, int precond, T* x, T* r, T* p, T* p2, T* q, T* z,
                 ctx->prof_iter = c;
                 int h = prof_open(ctx, PF_KCAT_APPLY);
                 const int hk = prof_kern<T>(ctx, PF_K_APPLY, 0, bytes_apply_fused(N, ctx->E, (int)sizeof(T)));
-                launch_apply<T, MODE_OP, true, true>(ctx, nullptr, z, pin, pout, q, nullptr, kh, kv, x);
+                apply_step(pin, pout);
                 prof_close(ctx, hk);
                 prof_close(ctx, h);
                 h = prof_open(ctx, PF_KCAT_PRECOND);
@@ -2083,6 +2282,7 @@ int pf_coeffs_update(pf_ctx* ctx) {
     hipLaunchKernelGGL(k_powerlaw, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, egrid(ctx),
                        (long long)ctx->E, ctx->x, ctx->k0h, ctx->k0v, ctx->ah, ctx->av, ctx->kh, ctx->kv, ctx->kh32,
                        ctx->kv32);
+    dyn_capture(ctx);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(ctx->stream));
     ctx->have_coeffs = true;
@@ -2093,9 +2293,180 @@ int pf_coeffs_update(pf_ctx* ctx) {
 int pf_get_coeffs(pf_ctx* ctx, double* k_inplane, double* k_vert) {
     if (!ctx) return PF_EINVAL;
     HIPCK(hipSetDevice(ctx->device));
-    if (k_inplane) HIPCK(hipMemcpy(k_inplane, ctx->kh, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
-    if (k_vert) HIPCK(hipMemcpy(k_vert, ctx->kv, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
+    // the conductivities as set (with a transient operator, kh / kv hold theta times them)
+    const double* kh = ctx->dyn ? ctx->khB : ctx->kh;
+    const double* kv = ctx->dyn ? ctx->kvB : ctx->kv;
+    if (k_inplane) HIPCK(hipMemcpy(k_inplane, kh, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
+    if (k_vert) HIPCK(hipMemcpy(k_vert, kv, sizeof(double) * ctx->E, hipMemcpyDeviceToHost));
     return PF_OK;
+}
+
+// dsh = capacity (transient) + boundary-matrix diagonal; off when neither is set
+static int rebuild_dsh(pf_ctx* ctx) {
+    ctx->dsh_on = ctx->dyn || ctx->have_dconv;
+    if (!ctx->dsh_on) return PF_OK;
+    int rc;
+    if (!ctx->dsh && ((rc = dalloc(ctx, &ctx->dsh, ctx->N)) || (rc = dalloc(ctx, &ctx->dsh32, ctx->N)))) return rc;
+    hipLaunchKernelGGL(k_sum_dsh, dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream, (long long)ctx->N,
+                       (const double*)(ctx->dyn ? ctx->dcap : nullptr),
+                       (const double*)(ctx->have_dconv ? ctx->dconv : nullptr), ctx->dsh, ctx->dsh32);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+}
+
+int pf_boundary_matrix_set(pf_ctx* ctx, const int64_t* rows, const int64_t* cols, const double* vals, int64_t nnz) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (nnz < 0 || (nnz > 0 && (!rows || !cols || !vals))) return fail(ctx, PF_EINVAL, "bad boundary matrix");
+    if (nnz > 0 && ctx->dyn) return fail(ctx, PF_ESTATE, "a boundary matrix with a transient operator is not supported");
+    const int64_t N = ctx->N;
+    std::vector<double> diag;
+    std::map<std::pair<int64_t, int64_t>, double> off;  // (row, col), both triangles
+    for (int64_t k = 0; k < nnz; ++k) {
+        const int64_t r = rows[k], c = cols[k];
+        if (r < 0 || r >= N || c < 0 || c >= N) return fail(ctx, PF_EINVAL, "boundary matrix index out of range");
+        if (c > r) return fail(ctx, PF_EINVAL, "boundary matrix entries must be in the lower triangle (col <= row)");
+        if (!std::isfinite(vals[k])) return fail(ctx, PF_EINVAL, "boundary matrix value not finite");
+        if (r == c) {
+            if (diag.empty()) diag.assign(N, 0.);
+            diag[r] += vals[k];
+        } else {
+            off[{r, c}] += vals[k];
+            off[{c, r}] += vals[k];
+        }
+    }
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    int rc;
+    ctx->have_dconv = !diag.empty();
+    if (ctx->have_dconv) {
+        if ((rc = dalloc(ctx, &ctx->dconv, N))) return rc;
+        HIPCK(hipMemcpy(ctx->dconv, diag.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    }
+    std::vector<long long> rr, cc;
+    std::vector<int> ptr{0};
+    std::vector<double> vv;
+    for (const auto& kv : off) {
+        if (rr.empty() || rr.back() != kv.first.first) {
+            if (!rr.empty()) ptr.push_back((int)cc.size());
+            rr.push_back(kv.first.first);
+        }
+        cc.push_back(kv.first.second);
+        vv.push_back(kv.second);
+    }
+    if (!rr.empty()) ptr.push_back((int)cc.size());
+    ctx->nbr = (long long)rr.size();
+    ctx->have_bnd = ctx->nbr > 0;
+    if (ctx->have_bnd) {
+        std::vector<float> v32(vv.begin(), vv.end());
+        if ((rc = upload(ctx, &ctx->bnd_rows, rr)) || (rc = upload(ctx, &ctx->bnd_ptr, ptr)) ||
+            (rc = upload(ctx, &ctx->bnd_cols, cc)) || (rc = upload(ctx, &ctx->bnd_vals, vv)) ||
+            (rc = upload(ctx, &ctx->bnd_vals32, v32)))
+            return rc;
+    }
+    if ((rc = rebuild_dsh(ctx))) return rc;
+    ctx->dirty = true;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_capacity_set(pf_ctx* ctx, const double* c_elem, int64_t nelem, double theta) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (!c_elem) {  // back to the steady operator
+        if (ctx->dyn && ctx->have_coeffs) {
+            hipLaunchKernelGGL(k_theta_scale, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream,
+                               (long long)ctx->E, 1., (const double*)ctx->khB, (const double*)ctx->kvB, ctx->kh,
+                               ctx->kv, ctx->kh32, ctx->kv32);
+            HIPCK(hipStreamSynchronize(ctx->stream));
+        }
+        ctx->dyn = false;
+        int rc = rebuild_dsh(ctx);
+        ctx->dirty = true;
+        drop_graphs(ctx);
+        return rc;
+    }
+    if (ctx->have_dconv || ctx->have_bnd)
+        return fail(ctx, PF_ESTATE, "a transient operator with a boundary matrix is not supported (the reference's "
+                                    "transient solvers take temperature conditions only)");
+    if (nelem != ctx->E) return fail(ctx, PF_EINVAL, "capacity array must have one value per element");
+    if (!(theta >= 0. && theta <= 1.)) return fail(ctx, PF_EINVAL, "theta (methodparam) must be in [0, 1]");
+    for (int64_t e = 0; e < nelem; ++e)
+        if (!std::isfinite(c_elem[e]) || c_elem[e] < 0.)
+            return fail(ctx, PF_EINVAL, "capacities must be finite and >= 0 (element " + std::to_string(e) + ")");
+    int rc;
+    const size_t N = ctx->N;
+    if (!ctx->dcap) {
+        if ((rc = dalloc(ctx, &ctx->dcap, N)) || (rc = dalloc(ctx, &ctx->F0, N)) ||
+            (rc = dalloc(ctx, &ctx->khB, ctx->E)) || (rc = dalloc(ctx, &ctx->kvB, ctx->E)) ||
+            (rc = dalloc(ctx, &ctx->zmask, (N + 63) / 64)))
+            return rc;
+        HIPCK(hipMemset(ctx->zmask, 0, sizeof(uint64_t) * ((N + 63) / 64)));
+    }
+    double* ce = nullptr;
+    HIPCK(hipMalloc((void**)&ce, sizeof(double) * ctx->E));
+    hipError_t e1 = hipMemcpyAsync(ce, c_elem, sizeof(double) * ctx->E, hipMemcpyHostToDevice, ctx->stream);
+    if (e1 == hipSuccess)
+        hipLaunchKernelGGL(k_node_capacity, dim3(grid1(ctx->N, 256, 8192)), dim3(256), 0, ctx->stream, egrid(ctx),
+                           (const double*)ce, ctx->dcap);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(ce);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(ctx, PF_EDEVICE, "pf_capacity_set: device copy failed");
+    const bool entering = !ctx->dyn;
+    ctx->theta = theta;
+    ctx->dyn = true;
+    if ((rc = rebuild_dsh(ctx))) return rc;
+    if (ctx->have_coeffs) {
+        if (entering) dyn_capture(ctx);  // kh / kv hold the stiffness as set
+        else dyn_apply(ctx);             // khB / kvB do; theta may have changed
+    }
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->dirty = true;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_dynamic_step(pf_ctx* ctx, const pf_solve_opts* opts, int held, pf_solve_stats* stats) {
+    if (!ctx || !opts) return PF_EINVAL;
+    if (held != PF_HELD_FIXED && held != PF_HELD_REFERENCE) return fail(ctx, PF_EINVAL, "held must be PF_HELD_*");
+    if (!ctx->dyn) return fail(ctx, PF_ESTATE, "pf_capacity_set first");
+    if (ctx->have_dconv || ctx->have_bnd) return fail(ctx, PF_ESTATE, "a boundary matrix is set");
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no temperature on the device (pf_solution_set)");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = prepare(ctx))) return rc;
+    const long long N = ctx->N;
+    const double omt = 1. - ctx->theta;
+    // K T^n over all nodes (no elimination), then F = F0 + C T^n - (1 - theta) K T^n
+    launch_apply<double, MODE_OP, false, false>(ctx, ctx->x, nullptr, nullptr, nullptr, ctx->tmp, nullptr, ctx->khB,
+                                                ctx->kvB, nullptr, true);
+    const bool quirk = held == PF_HELD_REFERENCE && ctx->nd;
+    if (quirk) {
+        if ((size_t)ctx->nd > ctx->ydir_cap) {
+            if ((rc = dalloc(ctx, &ctx->ydir, (size_t)ctx->nd))) return rc;
+            ctx->ydir_cap = (size_t)ctx->nd;
+        }
+        hipLaunchKernelGGL(k_dyn_dir, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd, omt,
+                           (const long long*)ctx->dnodes, (const double*)ctx->dcap, (const double*)ctx->x,
+                           (const double*)ctx->tmp, ctx->ydir);
+    }
+    hipLaunchKernelGGL(k_dyn_rhs, dim3(grid1(N, 256, 4096)), dim3(256), 0, ctx->stream, N, omt, (const double*)ctx->F0,
+                       (const double*)ctx->dcap, (const double*)ctx->x, (const double*)ctx->tmp, ctx->F);
+    HIPCK(hipGetLastError());
+    pf_solve_opts o = *opts;
+    o.x0 = PF_X0_DEVICE;  // from T^n
+    rc = pf_solve(ctx, &o, nullptr, stats);
+    if (rc != PF_OK && rc != PF_ENOCONV) return rc;
+    if (quirk)
+        hipLaunchKernelGGL(k_dyn_dir_add, dim3(grid1(ctx->nd)), dim3(256), 0, ctx->stream, (long long)ctx->nd,
+                           (const long long*)ctx->dnodes, (const double*)ctx->ydir, ctx->x);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return rc;
 }
 
 int pf_junctions_set(pf_ctx* ctx, int64_t nje, int ncorner, const int64_t* elems, const int64_t* bottom_nodes,
@@ -2161,6 +2532,53 @@ int pf_junction_update(pf_ctx* ctx, int stable) {
         HIPCK(hipStreamSynchronize(ctx->stream));
         ctx->dirty = true;
     }
+    return PF_OK;
+}
+
+int pf_junction_state(pf_ctx* ctx, double* U, double* jy, int64_t nje) {
+    if (!ctx) return PF_EINVAL;
+    if (nje != ctx->nje) return fail(ctx, PF_EINVAL, "one value per junction element");
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no potential on the device");
+    if (nje == 0) return PF_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    double* d = nullptr;
+    HIPCK(hipMalloc((void**)&d, sizeof(double) * 2 * nje));
+    hipLaunchKernelGGL(k_junction_state, dim3(grid1(nje, 256, 4096)), dim3(256), 0, ctx->stream, (long long)nje,
+                       ctx->jcorner, (const long long*)ctx->je, (const long long*)ctx->jbot, (const long long*)ctx->jtop,
+                       (const int*)ctx->jid, (const double*)ctx->jheight, (const double*)ctx->x, (const double*)ctx->kv,
+                       d, d + nje);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && U) e = hipMemcpyAsync(U, d, sizeof(double) * nje, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && jy) e = hipMemcpyAsync(jy, d + nje, sizeof(double) * nje, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (e != hipSuccess || e2 != hipSuccess) return fail(ctx, PF_EDEVICE, "pf_junction_state: device error");
+    return PF_OK;
+}
+
+int pf_junction_cond_set(pf_ctx* ctx, const double* cond_h, const double* cond_v, int64_t nje, int stable) {
+    if (!ctx || !cond_h || !cond_v) return PF_EINVAL;
+    if (nje != ctx->nje) return fail(ctx, PF_EINVAL, "one value per junction element");
+    if (!ctx->have_coeffs) return fail(ctx, PF_ESTATE, "pf_coeffs_set first");
+    for (int64_t q = 0; q < nje; ++q)
+        if (std::isinf(cond_h[q]) || std::isnan(cond_h[q]) || std::isinf(cond_v[q]))
+            return fail(ctx, PF_EINVAL, "junction conductivity not finite (element " + std::to_string(q) + ")");
+    if (nje == 0) return PF_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    double* d = nullptr;
+    HIPCK(hipMalloc((void**)&d, sizeof(double) * 2 * nje));
+    hipError_t e = hipMemcpyAsync(d, cond_h, sizeof(double) * nje, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + nje, cond_v, sizeof(double) * nje, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_junction_cond_set, dim3(grid1(nje, 256, 4096)), dim3(256), 0, ctx->stream,
+                           (long long)nje, (const long long*)ctx->je, (const double*)d, (const double*)(d + nje),
+                           stable, ctx->kh, ctx->kv, ctx->kh32, ctx->kv32);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (e != hipSuccess || e2 != hipSuccess) return fail(ctx, PF_EDEVICE, "pf_junction_cond_set: device error");
+    ctx->dirty = true;
     return PF_OK;
 }
 

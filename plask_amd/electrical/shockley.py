@@ -6,6 +6,10 @@ inTemperature, outVoltage, outCurrentDensity, outHeat, outConductivity,
 voltage_boundary, maxerr, convergence, pcond, ncond, start_cond (pnjcond), beta/beta#,
 js/js# (float or callable of T), get_electrostatic_energy, get_capacitance,
 get_total_heat.  outPotential raises TypeError as in the reference.
+
+ActiveCond2D, ActiveCondCyl, ActiveCond3D (PythonCondSolver, electr_python.cpp:113-190)
+take the junction conductivity from a Python callable cond/cond# (U in V, j in kA/cm^2,
+T in K -> S/m, or an (in-plane, vertical) pair) instead of the Shockley law.
 """
 import math
 import re
@@ -36,10 +40,12 @@ def _active_number(roles):
 class _ShockleyBase(_sb.FemSolver):
     _core_class = ElectricalCore
     inTemperature = _sb.ReceiverAttr(300.0)
+    _PARAMS = re.compile(r"(beta|js)(\d*)")
 
     def __init__(self, name=""):
         object.__setattr__(self, "_beta", {})
         object.__setattr__(self, "_js", {})
+        object.__setattr__(self, "_cond", {})
         super().__init__(name)
         self.voltage_boundary = _mesh.BoundaryConditions()
         self.maxerr = 0.05
@@ -53,7 +59,10 @@ class _ShockleyBase(_sb.FemSolver):
 
     # beta#, js# (BetaSolver, beta.hpp:49-77; Python callables electr_python.cpp:96-111)
     def __getattr__(self, attr):
-        m = re.fullmatch(r"(beta|js)(\d*)", attr)
+        m = type(self)._PARAMS.fullmatch(attr)
+        if m and m.group(1) == "cond":
+            n = int(m.group(2)) if m.group(2) else 0
+            return object.__getattribute__(self, "_cond").get(n, None)
         if m:
             d = object.__getattribute__(self, "_" + m.group(1))
             n = int(m.group(2)) if m.group(2) else 0
@@ -63,7 +72,14 @@ class _ShockleyBase(_sb.FemSolver):
         raise AttributeError("'%s' object has no attribute '%s'" % (type(self).__name__, attr))
 
     def __setattr__(self, attr, value):
-        m = re.fullmatch(r"(beta|js)(\d*)", attr)
+        m = type(self)._PARAMS.fullmatch(attr)
+        if m and m.group(1) == "cond":
+            n = int(m.group(2)) if m.group(2) else 0
+            if not callable(value):
+                raise TypeError("%s: cond%d must be a a callable" % (self.id, n))
+            self._cond[n] = value
+            self.invalidate()
+            return
         if m:
             if not (callable(value) or isinstance(value, (int, float))):
                 raise TypeError("%s: %s must be a float or a callable" % (self.id, attr))
@@ -130,7 +146,7 @@ class _ShockleyBase(_sb.FemSolver):
                 jid.append(n - 1)
         self._jelems = np.array(elems, dtype=np.int64)
         self._jid = np.array(jid, dtype=np.int32)
-        c.set_start_cond(float(self._start_cond[1]))
+        c.set_start_cond(float(self._start_cond[1]), float(self._start_cond[0]))
         c.set_junctions(self._jelems, 2 if self.DIM == 2 else 4, np.array(bot, dtype=np.int64),
                         np.array(top, dtype=np.int64), list(self._jid), heights)
         leaves = self._geometry._leaves
@@ -211,24 +227,7 @@ class _ShockleyBase(_sb.FemSolver):
         sh, sv = self._load_conductivities(T)
         sh[self._excl], sv[self._excl] = _sb.MASKED_K   # masked-out (empty) elements
         c.set_conductivity(sh, sv)
-        # beta, js per junction element at the junction mid-row temperature (electr2d.cpp:255-256)
-        beta = np.empty(len(self._jelems))
-        js = np.empty(len(self._jelems))
-        m = [len(a) - 1 for a in self._axes]
-        for q, (e, n) in enumerate(zip(self._jelems, self._jid)):
-            kb, kt, _ = self._junctions[n]
-            kmid = (kb + kt) // 2
-            if self.DIM == 2:
-                ti = e % m[0] + m[0] * kmid
-            else:
-                ti = e % (m[0] * m[1]) + m[0] * m[1] * kmid
-            b = self._beta.get(int(n), None)
-            j = self._js.get(int(n), 1.0)
-            if b is None:
-                raise ValueError("%s: no beta given for junction %d" % (self.id, n))
-            beta[q] = b(float(T[ti])) if callable(b) else float(b)
-            js[q] = j(float(T[ti])) if callable(j) else float(j)
-        c.set_junction_params(beta, js)
+        self._junction_setup(np.asarray(T, dtype=np.float64))
         nodes, vals = self.voltage_boundary.resolve(self.mesh, self.geometry)
         if len(self._isolated):  # nodes of the masked-out region: held at 0 V, coupled to nothing
             nodes = np.concatenate([nodes, self._isolated])
@@ -237,6 +236,35 @@ class _ShockleyBase(_sb.FemSolver):
         err = c.compute(int(loops))
         self._V = None
         return err
+
+    def _junction_temps(self, T):
+        """Per junction element the temperature of its column's element in the junction's
+        middle row (electr2d.cpp:255)."""
+        m = [len(a) - 1 for a in self._axes]
+        ti = np.empty(len(self._jelems), dtype=np.int64)
+        for q, (e, n) in enumerate(zip(self._jelems, self._jid)):
+            kb, kt, _ = self._junctions[n]
+            kmid = (kb + kt) // 2
+            if self.DIM == 2:
+                ti[q] = e % m[0] + m[0] * kmid
+            else:
+                ti[q] = e % (m[0] * m[1]) + m[0] * m[1] * kmid
+        return T[ti]
+
+    def _junction_setup(self, T):
+        """beta, js per junction element at its temperature (BetaSolver getBeta/getJs and the
+        Python callables of electr_python.cpp:102-108)."""
+        Tj = self._junction_temps(T)
+        beta = np.empty(len(self._jelems))
+        js = np.empty(len(self._jelems))
+        for q, n in enumerate(self._jid):
+            b = self._beta.get(int(n), None)
+            j = self._js.get(int(n), 1.0)
+            if b is None:
+                raise ValueError("%s: no beta given for junction %d" % (self.id, n))
+            beta[q] = b(float(Tj[q])) if callable(b) else float(b)
+            js[q] = j(float(Tj[q])) if callable(j) else float(j)
+        self._core.set_junction_params(beta, js)
 
     @property
     def err(self):
@@ -368,6 +396,57 @@ class _ShockleyBase(_sb.FemSolver):
         return 2e12 * self.get_electrostatic_energy() / (U * U)
 
 
+def _cond_values(f, U, j, T):
+    """cond(U, j, T) of a junction for arrays of its elements: one vectorised call when the
+    callable takes arrays, else one call per element as the reference makes them.
+    Returns (in-plane, vertical) arrays; a plain number is (0, value) (electr_python.cpp:174-177)."""
+    n = len(U)
+
+    def split(r):
+        if isinstance(r, (tuple, list)) and len(r) == 2:
+            return np.broadcast_to(np.asarray(r[0], float), (n,)), np.broadcast_to(np.asarray(r[1], float), (n,))
+        a = np.asarray(r, dtype=float)
+        if a.ndim == 2 and a.shape == (n, 2):
+            return a[:, 0], a[:, 1]
+        return np.zeros(n), np.broadcast_to(a, (n,))
+    try:
+        with np.errstate(all="ignore"):
+            r = f(U, j, T)
+        h, v = split(r)
+        return np.array(h, dtype=float), np.array(v, dtype=float)
+    except (TypeError, ValueError):
+        pass
+    h, v = np.zeros(n), np.zeros(n)
+    for q in range(n):
+        r = f(float(U[q]), float(j[q]), float(T[q]))
+        if isinstance(r, (tuple, list)) or np.ndim(r) == 1:
+            h[q], v[q] = float(r[0]), float(r[1])
+        else:
+            v[q] = float(r)
+    return h, v
+
+
+class _ActiveCondBase(_ShockleyBase):
+    """Junction conductivity from cond# callables (PythonCondSolver::activeCond,
+    electr_python.cpp:171-178), evaluated in each loop's junction update (electr2d.cpp:247-256)."""
+    _PARAMS = re.compile(r"(cond)(\d*)")
+
+    def _junction_setup(self, T):
+        Tj = self._junction_temps(T)
+        jid = np.asarray(self._jid)
+        groups = [(n, np.nonzero(jid == n)[0]) for n in np.unique(jid)]
+        for n, _ in groups:
+            if self._cond.get(int(n)) is None:
+                raise IndexError("no conductivity for active region %d" % n)
+
+        def law(U, jy):
+            ch, cv = np.zeros(len(U)), np.zeros(len(U))
+            for n, ix in groups:
+                ch[ix], cv[ix] = _cond_values(self._cond[int(n)], U[ix], jy[ix], Tj[ix])
+            return ch, cv
+        self._core.set_junction_law(law)
+
+
 class Shockley2D(_ShockleyBase):
     """Finite element electrical solver for 2D Cartesian geometry."""
     DIM, CYL = 2, False
@@ -380,4 +459,22 @@ class ShockleyCyl(_ShockleyBase):
 
 class Shockley3D(_ShockleyBase):
     """Finite element electrical solver for 3D Cartesian geometry."""
+    DIM, CYL = 3, False
+
+
+class ActiveCond2D(_ActiveCondBase):
+    """Finite element electrical solver for 2D Cartesian geometry with a junction
+    conductivity function."""
+    DIM, CYL = 2, False
+
+
+class ActiveCondCyl(_ActiveCondBase):
+    """Finite element electrical solver for 2D cylindrical geometry with a junction
+    conductivity function."""
+    DIM, CYL = 2, True
+
+
+class ActiveCond3D(_ActiveCondBase):
+    """Finite element electrical solver for 3D Cartesian geometry with a junction
+    conductivity function."""
     DIM, CYL = 3, False

@@ -18,6 +18,7 @@ PF_FP64, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64_PREC32 = 0, 1, 2, 3
 PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 1, 2, 3, 4
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL, PF_NORM_START = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3
+PF_HELD_FIXED, PF_HELD_REFERENCE = 0, 1
 KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KCAT_MG_COARSE = range(6)
 
 EXPORTED = [
@@ -27,6 +28,7 @@ EXPORTED = [
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
     "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
+    "pf_capacity_set", "pf_dynamic_step",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
@@ -118,6 +120,8 @@ def load_library(path=LIB_PATH):
     L.pf_set_multigrid.argtypes = [P, I, I, C.c_double]
     L.pf_multigrid_levels.argtypes = [P]
     L.pf_precond_apply_host.argtypes = [P, I, I, P, P]
+    L.pf_capacity_set.argtypes = [P, P, i64, C.c_double]
+    L.pf_dynamic_step.argtypes = [P, C.POINTER(SolveOpts), I, C.POINTER(SolveStats)]
     _lib = L
     return L
 
@@ -258,6 +262,27 @@ class FemContext:
         st = SolveStats()
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
         self._ck(rc, "pf_solve")
+        return st
+
+    def set_capacity(self, c_elem, theta=0.5):
+        """Transient operator: per-element lumped capacities c (stiffness units, each of the
+        element's nodes takes c) and the theta of the theta-method; None: steady operator."""
+        if c_elem is None:
+            self._ck(self.L.pf_capacity_set(self.h, None, 0, float(theta)), "pf_capacity_set")
+            return
+        self._ce = _f64(np.ravel(c_elem))
+        self._ck(self.L.pf_capacity_set(self.h, _p(self._ce), len(self._ce), float(theta)), "pf_capacity_set")
+
+    def dynamic_step(self, precision=PF_FP64, rtol=1e-8, maxit=200000, inner_rtol=None, check_every=32,
+                     norm=PF_NORM_START, init_value=0.0, precond=PRECOND_MG, raise_noconv=True, held=PF_HELD_FIXED):
+        """One theta-method step from the device temperature: (theta K + C) T' = (C - (1 - theta) K) T + F;
+        held = PF_HELD_REFERENCE gives Dirichlet nodes the reference's value + (B T)_r."""
+        o = SolveOpts(precision, precond, PF_X0_DEVICE, int(maxit), float(rtol), float(inner_rtol or 0.0), 50,
+                      int(check_every), int(norm), float(init_value))
+        st = SolveStats()
+        rc = self.L.pf_dynamic_step(self.h, C.byref(o), int(held), C.byref(st))
+        if rc != PF_OK and not (rc == PF_ENOCONV and not raise_noconv):
+            self._ck(rc, "pf_dynamic_step")
         return st
 
     def solution(self):

@@ -1,8 +1,9 @@
 """Materials: the interface of PLaSK's Material the thermal/electrical path calls.
 
 ``thermk(T, h)`` -> conductivity (W/m/K, scalar or (in-plane, vertical)),
-``cond(T)`` -> electrical conductivity (S/m, scalar or pair), ``eps(T)``.
-(plask/material/material.hpp:649,691.)
+``cond(T)`` -> electrical conductivity (S/m, scalar or pair), ``eps(T)``,
+``cp(T)`` -> specific heat (J/kg/K) and ``dens(T)`` -> density (kg/m^3), which the
+transient solver reads.  (plask/material/material.hpp:649,691,698,705.)
 
 Laws the GPU evaluates itself (no host round trip inside a Picard loop):
 ``Const`` and power laws k = k0 (300/T)^alpha (``PowerLaw``).  Any other Python
@@ -24,8 +25,18 @@ class Material:
     def eps(self, T=300.0):
         return 1.0
 
+    def cp(self, T=300.0):
+        raise NotImplementedError("%s: cp not defined" % self.name)
+
+    def dens(self, T=300.0):
+        raise NotImplementedError("%s: dens not defined" % self.name)
+
     # device law: (k0_inplane, k0_vertical, alpha_inplane, alpha_vertical) or None
     def thermk_law(self):
+        return None
+
+    # cp * dens when it does not depend on T (evaluated once per material), else None
+    def heat_capacity_const(self):
         return None
 
     def cond_law(self):
@@ -56,11 +67,22 @@ def _pair(v):
 class Const(Material):
     """Constant (possibly anisotropic) conductivities."""
 
-    def __init__(self, thermk=None, cond=None, eps=1.0, name="const"):
+    def __init__(self, thermk=None, cond=None, eps=1.0, name="const", cp=None, dens=None):
         self._k = _pair(thermk) if thermk is not None else None
         self._c = _pair(cond) if cond is not None else None
         self._eps = eps
+        self._cp, self._dens = cp, dens
         self.name = name
+
+    def cp(self, T=300.0):
+        if self._cp is None:
+            raise NotImplementedError("%s: cp not defined" % self.name)
+        return float(self._cp)
+
+    def dens(self, T=300.0):
+        if self._dens is None:
+            raise NotImplementedError("%s: dens not defined" % self.name)
+        return float(self._dens)
 
     def thermk(self, T=300.0, h=math.inf):
         if self._k is None:
@@ -78,6 +100,9 @@ class Const(Material):
     def thermk_law(self):
         return None if self._k is None else (self._k[0], self._k[1], 0.0, 0.0)
 
+    def heat_capacity_const(self):
+        return None if self._cp is None or self._dens is None else float(self._cp) * float(self._dens)
+
     def cond_law(self):
         return None if self._c is None else (self._c[0], self._c[1], 0.0, 0.0)
 
@@ -85,12 +110,23 @@ class Const(Material):
 class PowerLaw(Material):
     """k(T) = k0 (300/T)^alpha for thermal conductivity (and optional constant cond)."""
 
-    def __init__(self, k0, alpha, cond=None, eps=1.0, name="powerlaw"):
+    def __init__(self, k0, alpha, cond=None, eps=1.0, name="powerlaw", cp=None, dens=None):
         self.k0 = _pair(k0)
         self.alpha = _pair(alpha)
         self._c = _pair(cond) if cond is not None else None
         self._eps = eps
+        self._cp, self._dens = cp, dens
         self.name = name
+
+    def cp(self, T=300.0):
+        if self._cp is None:
+            raise NotImplementedError("%s: cp not defined" % self.name)
+        return float(self._cp)
+
+    def dens(self, T=300.0):
+        if self._dens is None:
+            raise NotImplementedError("%s: dens not defined" % self.name)
+        return float(self._dens)
 
     def thermk(self, T=300.0, h=math.inf):
         t = 300.0 / T
@@ -107,21 +143,27 @@ class PowerLaw(Material):
     def thermk_law(self):
         return (self.k0[0], self.k0[1], self.alpha[0], self.alpha[1])
 
+    def heat_capacity_const(self):
+        return None if self._cp is None or self._dens is None else float(self._cp) * float(self._dens)
+
     def cond_law(self):
         return None if self._c is None else (self._c[0], self._c[1], 0.0, 0.0)
 
 
 class _GaAs(PowerLaw):
-    """GaAs with the commonly used k = 45 (300/T)^1.28 W/m/K and eps = 12.9 (a synthetic
-    stand-in for the reference material database, which is out of scope)."""
+    """GaAs with the commonly used k = 45 (300/T)^1.28 W/m/K and eps = 12.9, cp = 327 J/kg/K
+    and density 5317.49 kg/m^3 (a stand-in for the reference material database, which is
+    out of scope)."""
 
     def __init__(self):
-        super().__init__((45.0, 45.0), (1.28, 1.28), cond=(1e-3, 1e-3), eps=12.9, name="GaAs")
+        super().__init__((45.0, 45.0), (1.28, 1.28), cond=(1e-3, 1e-3), eps=12.9, name="GaAs", cp=0.327e3,
+                         dens=5.31749e3)
 
 
 class _Air(Material):
     """The background material of every space (plask/material/air.cpp): kind EMPTY,
-    cond 0.55e-14 S/m, thermk 0.0258 (T/300)^0.84 W/m/K."""
+    cond 0.55e-14 S/m, thermk 0.0258 (T/300)^0.84 W/m/K, cp 1007 J/kg/K, density
+    1.161 kg/m^3 (air.cpp:52-94)."""
     name = "air"
     kind = "empty"
 
@@ -135,8 +177,17 @@ class _Air(Material):
     def eps(self, T=300.0):
         return 1.0
 
+    def cp(self, T=300.0):
+        return 1.007e3
+
+    def dens(self, T=300.0):
+        return 1.161
+
     def thermk_law(self):
         return (0.0258, 0.0258, -0.84, -0.84)
+
+    def heat_capacity_const(self):
+        return 1.007e3 * 1.161
 
     def cond_law(self):
         return (0.55e-14, 0.55e-14, 0.0, 0.0)

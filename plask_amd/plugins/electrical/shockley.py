@@ -1,1 +1,2 @@
-from plask_amd.electrical.shockley import Shockley2D, ShockleyCyl, Shockley3D  # noqa: F401
+from plask_amd.electrical.shockley import (ActiveCond2D, ActiveCond3D, ActiveCondCyl, Shockley2D,  # noqa: F401
+                                           Shockley3D, ShockleyCyl)

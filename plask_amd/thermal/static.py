@@ -72,6 +72,46 @@ def _bc_lists(bcs, mesh, geometry, convert):
     return (np.concatenate(nodes) if nodes else np.zeros(0, np.int64)), vals
 
 
+def _setup_conductivity(solver):
+    """Element conductivities of a thermal solver's core: one device law k0 (300/T)^alpha
+    per leaf when every material has one, else a host callback evaluating thermk(T, h)
+    per element at the element temperatures (setMatrix, therm3d.cpp:209-216)."""
+    c = solver._core
+    leaf_mats = [lf[0].material for lf in solver._geometry._leaves] + [solver._mat_bg]
+    laws = [m.thermk_law() if m is not None else None for m in leaf_mats]
+    if all(l is not None for l in laws):
+        L = np.array(laws)[solver._leaf]
+        L[solver._excl, 0], L[solver._excl, 1] = _sb.MASKED_K   # masked-out (empty) elements
+        c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
+        return
+    mats = solver._mat
+    h = solver._h
+    groups = {}
+    for e, m in enumerate(mats):
+        groups.setdefault(id(m), (m, []))[1].append(e)
+    groups = [(m, np.array(ix)) for m, ix in groups.values()]
+    excl = solver._excl
+
+    def kfun(tavg):
+        kh = np.full(len(tavg), _sb.MASKED_K[0])
+        kv = np.full(len(tavg), _sb.MASKED_K[1])
+        for m, ix in groups:
+            ix = ix[~excl[ix]]
+            if len(ix) and m is None:
+                raise ValueError("%s: element without material" % solver.id)
+            for e in ix:
+                k = m.thermk(float(tavg[e]), float(h[e]))
+                k = (k, k) if np.isscalar(k) else k
+                kh[e], kv[e] = float(k[0]), float(k[1])
+        return kh, kv
+
+    c.set_conductivity_callback(kfun)
+    # a placeholder law so that the device holds valid coefficients before the first loop
+    E = len(mats)
+    c.set_conductivity(np.ones(E), np.ones(E), np.zeros(E), np.zeros(E))
+    c.set_conductivity_callback(kfun)
+
+
 class _StaticBase(_sb.FemSolver):
     _core_class = ThermalCore
     inHeat = _sb.ReceiverAttr(0.0)
@@ -98,50 +138,13 @@ class _StaticBase(_sb.FemSolver):
         c = self._core
         c.inittemp = float(self.inittemp)
         self._h = self._thickness()
-        # one law per leaf (the last entry: the background), indexed per element
-        leaf_mats = [lf[0].material for lf in self._geometry._leaves] + [self._mat_bg]
-        laws = [m.thermk_law() if m is not None else None for m in leaf_mats]
-        if all(l is not None for l in laws):
-            L = np.array(laws)[self._leaf]
-            L[self._excl, 0], L[self._excl, 1] = _sb.MASKED_K   # masked-out (empty) elements
-            c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
-        else:
-            mats = self._mat
-            h = self._h
-            groups = {}
-            for e, m in enumerate(mats):
-                groups.setdefault(id(m), (m, []))[1].append(e)
-            groups = [(m, np.array(ix)) for m, ix in groups.values()]
-
-            excl = self._excl
-
-            def kfun(tavg):
-                kh = np.full(len(tavg), _sb.MASKED_K[0])
-                kv = np.full(len(tavg), _sb.MASKED_K[1])
-                for m, ix in groups:
-                    ix = ix[~excl[ix]]
-                    if len(ix) and m is None:
-                        raise ValueError("%s: element without material" % self.id)
-                    for e in ix:
-                        k = m.thermk(float(tavg[e]), float(h[e]))
-                        k = (k, k) if np.isscalar(k) else k
-                        kh[e], kv[e] = float(k[0]), float(k[1])
-                return kh, kv
-
-            c.set_conductivity_callback(kfun)
-            # a placeholder law so that the device holds valid coefficients before the first loop
-            E = len(mats)
-            c.set_conductivity(np.ones(E), np.ones(E), np.zeros(E), np.zeros(E))
-            c.set_conductivity_callback(kfun)
+        _setup_conductivity(self)
         c.initialize()
         self._T = None
 
     def compute(self, loops=0):
         """Run thermal calculations; returns the maximum temperature update (K)."""
         self.initialize()
-        if len(self.convection_boundary):
-            raise NotImplementedError("%s: convection boundary conditions change the operator and are not "
-                                      "implemented on the GPU path (heat flux and radiation are)" % self.id)
         c = self._core
         c.inittemp = float(self.inittemp)
         c.maxerr = float(self.maxerr)
@@ -158,6 +161,9 @@ class _StaticBase(_sb.FemSolver):
         c.set_boundary_loads(fn, np.asarray(fv, dtype=np.float64), rn,
                              np.array([v.emissivity for v in rv], dtype=np.float64),
                              np.array([v.ambient for v in rv], dtype=np.float64))
+        cn, cv = _bc_lists(self.convection_boundary, self.mesh, self.geometry, Convection.of)
+        c.set_convection(cn, np.array([v.coeff for v in cv], dtype=np.float64),
+                         np.array([v.ambient for v in cv], dtype=np.float64))
         err = c.compute(int(loops))
         self._T = None
         return err

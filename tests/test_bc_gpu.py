@@ -1,5 +1,6 @@
-"""Heat-flux and radiation boundary conditions through thermal.static on the GPU, against
-the CPU statement of the reference's boundary loads (oracle/bc.py) and band Cholesky."""
+"""Heat-flux, radiation and convection boundary conditions through thermal.static on the
+GPU, against the CPU statement of the reference's boundary terms (oracle/bc.py) and band
+Cholesky / dense Cholesky."""
 import numpy as np
 import pytest
 
@@ -147,11 +148,131 @@ def test_boundary_loads_clear():
     assert np.abs(s._core.solution() - T0).max() <= 1e-9 * 300.
 
 
-def test_convection_is_refused():
-    s = _stack_2d(Static2D, False)
+def _oracle_parts(s, cyl=False):
+    """Unconstrained stiffness and load of a solver's mesh (element conductivities at 300 K)."""
+    msh = s.mesh
+    g = O.Grid(msh.axes, cyl=cyl)
+    k = np.array([m.thermk(300.) for m in s._mat])
+    k = k if k.ndim == 2 else np.column_stack([k, k])
+    A0 = O.assemble(g, 0, k[:, 0], k[:, 1], np.zeros(g.nelems), O.ST_BAND)
+    dn, dv = s.temperature_boundary.resolve(msh, s.geometry)
+    return A0, dn, dv
+
+
+@pytest.mark.parametrize("cls,cyl", [(Static2D, False), (StaticCyl, True)])
+def test_convection_2d_vs_oracle(cls, cyl):
+    """Convection on the top and the right side (first place wins at the corner), a flux
+    on the left: the operator's boundary matrix and the loads against the oracle's
+    statement of therm2d.cpp:242-254 / :370-392 (dense, eliminated)."""
+    s = _stack_2d(cls, cyl)
+    print("convection 2d", cls.__name__, flush=True)
+    top, right, left = s.mesh.Top(), s.mesh.Right(), s.mesh.Left()
+    s.convection_boundary.append(top, Convection(40., 350.))
+    s.convection_boundary.append(right, {"coeff": 5., "ambient": 280.})
+    s.heatflux_boundary.append(left, 2.0e5)
+    s.initialize()
+    s.compute(1)
+    T = s._core.solution()
+    A0, dn, dv = _oracle_parts(s, cyl)
+    conv = BC.first_values([(_nodes(s, top), (40., 350.)), (_nodes(s, right), (5., 280.))])
+    Fc, Kc = BC.convection_2d(s.mesh.axes, conv, cyl=cyl)
+    Ff = BC.load_2d(s.mesh.axes, flux=BC.first_values([(_nodes(s, left), 2.0e5)]), cyl=cyl)
+    T_ref = BC.solve_with_matrix(A0.dense(), A0.B, Kc, Fc + Ff, dn, dv)
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 1.0
+
+
+def test_convection_3d_vs_oracle():
+    """Convection on the top (whose terms the reference places on the plane below) and on
+    the y-low side wall, against the oracle (therm3d.cpp:250-259)."""
+    s = _stack_3d()
+    print("convection 3d", flush=True)
+    top, front = s.mesh.Top(), s.mesh.Front()
+    s.convection_boundary.append(top, Convection(3e5, 400.))
+    s.convection_boundary.append(front, Convection(1e5, 250.))
+    s.initialize()
+    s.compute(1)
+    T = s._core.solution()
+    A0, dn, dv = _oracle_parts(s)
+    conv = BC.first_values([(_nodes(s, top), (3e5, 400.)), (_nodes(s, front), (1e5, 250.))])
+    Fc, Kc = BC.convection_3d(s.mesh.axes, conv)
+    T_ref = BC.solve_with_matrix(A0.dense(), A0.B, Kc, Fc, dn, dv)
+    assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    assert np.abs(T_ref - 300.).max() > 1.0
+
+
+def test_convection_2d_closed_form_through_the_solver():
+    """The 2D slab of the reference's equations (matrix without the 1e-6 of the load):
+    T = T0 + g z, g = (1e-6 h T_amb - h T0) / (k + h L); 2.97 K on top for k = 1, L = 10,
+    h = 10, T0 = T_amb = 300."""
+    stack = geometry.Stack2D()
+    stack.append(geometry.Rectangle(4., 10., material.Const(thermk=(1.0, 1.0), name="K1")))
+    s = Static2D("slab")
+    s.geometry = geometry.Cartesian2D(stack)
+    s.mesh = mesh.Rectangular2D(np.array([0.0, 0.7, 2.0, 4.0]), np.array([0.0, 1.5, 4.0, 7.0, 10.0]))
+    s.temperature_boundary.append(s.mesh.Bottom(), 300.)
     s.convection_boundary.append(s.mesh.Top(), Convection(10., 300.))
-    with pytest.raises(NotImplementedError):
-        s.compute(1)
+    s.iterative.maxerr = 1e-13
+    s.compute(1)
+    T = s._core.solution()
+    g = (1e-6 * 10. * 300. - 10. * 300.) / (1. + 10. * 10.)
+    z = np.repeat(s.mesh.axes[1], 4)
+    assert np.allclose(T, 300. + g * z, rtol=0, atol=1e-8 * 300.)
+
+
+def test_apply_with_boundary_matrix_vs_oracle():
+    """pf_boundary_matrix_set through the C ABI: pf_apply equals the dense eliminated
+    operator K + Kc, and a solve matches the oracle, with both preconditioners."""
+    from plask_amd.fem import PRECOND_LINE, PRECOND_MG, FemContext
+    axes = _grid_axes_3d()
+    g = O.Grid(axes)
+    E = g.nelems
+    rng = np.random.default_rng(4)
+    kh, kv = rng.uniform(1, 50, E), rng.uniform(1, 50, E)
+    A0 = O.assemble(g, 0, kh, kv, np.zeros(E), O.ST_BAND)
+    n0, n1, n2 = (len(a) for a in axes)
+    plane = n0 * n1
+    dn = np.arange(plane)
+    dv = np.full(plane, 300.)
+    side = np.array([i + n0 * (j + n1 * k) for k in range(n2) for j in range(n1) for i in (n0 - 1,)])
+    conv = BC.first_values([(np.arange((n2 - 1) * plane, n2 * plane), (2e5, 350.)), (side, (5e4, 280.))])
+    Fc, Kc = BC.convection_3d(axes, conv)
+    c = FemContext()
+    c.set_grid(axes)
+    c.set_coeffs(kh, kv)
+    c.set_source(None)
+    c.set_dirichlet(dn, dv)
+    rows, cols, vals = (np.array([e[i] for e in Kc]) for i in range(3))
+    assert c.L.pf_boundary_matrix_set(c.h, rows.astype(np.int64).ctypes.data, cols.astype(np.int64).ctypes.data,
+                                      vals.astype(np.float64).ctypes.data, len(vals)) == 0
+    nz = np.nonzero(Fc)[0]
+    c.set_load(nz, Fc[nz])
+    K = A0.dense()
+    for r, cc, v in Kc:
+        K[r, cc] += v
+        if r != cc:
+            K[cc, r] += v
+    free = np.ones(g.nnodes, bool)
+    free[dn] = False
+    x = rng.uniform(-1, 1, g.nnodes)
+    y = c.apply(x)
+    xm = np.where(free, x, 0.0)
+    want = np.where(free, K @ xm, x)
+    assert np.abs(y - want).max() <= 1e-12 * np.abs(want).max()
+    T_ref = BC.solve_with_matrix(A0.dense(), A0.B, Kc, Fc, dn, dv)
+    for pc in (PRECOND_LINE, PRECOND_MG):
+        T, st = c.solve(x0=np.full(g.nnodes, 300.), rtol=1e-12, precond=pc)
+        assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max(), (pc, st.iters)
+    # clearing the matrix restores the plain operator
+    c.L.pf_boundary_matrix_set(c.h, None, None, None, 0)
+    y0 = c.apply(x)
+    assert np.abs(y0 - np.where(free, A0.dense() @ xm, x)).max() <= 1e-12 * np.abs(y0).max()
+    c.close()
+
+
+def _grid_axes_3d():
+    rng = np.random.default_rng(11)
+    return [np.cumsum(np.r_[0.0, rng.uniform(0.3, 1.5, n - 1)]) for n in (9, 7, 8)]
 
 
 def test_condition_values():

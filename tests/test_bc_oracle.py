@@ -128,3 +128,93 @@ def test_3d_radiation_reads_the_wall_node_number():
             for l in range(4):
                 counts[(i + (l & 1)) + 3 * (j + (l >> 1)) + 9 * 1] += 1
     assert np.allclose(F, per * counts, rtol=1e-14, atol=0)
+
+
+# --- convection (the matrix terms take part in the elimination) ---------------------------
+
+def _conv_solve(axes, k, dn, dv, conv_entries, Fc, cyl=False):
+    g = O.Grid(axes, cyl=cyl)
+    E = g.nelems
+    A0 = O.assemble(g, 0, np.full(E, k), np.full(E, k), np.zeros(E), O.ST_BAND)
+    return BC.solve_with_matrix(A0.dense(), A0.B, conv_entries, Fc, dn, dv)
+
+
+def test_convection_2d_slab_closed_form():
+    """2D slab held at T0 below, convection (h, T_amb) on top.  The reference's equations
+    at a top node (per unit of the node's width): k (T_top - T_below) / dz + h T_top =
+    1e-6 h T_amb -- its matrix lacks the 1e-6 um -> m of its load (therm2d.cpp:246-252).
+    The profile stays linear, T = T0 + g z with g = (1e-6 h T_amb - h T0) / (k + h L);
+    for k = 1, L = 10, h = 10, T0 = T_amb = 300 the top sits at 2.97 K."""
+    axes = [np.array([0.0, 0.7, 2.0, 2.2, 4.0]), np.array([0.0, 1.5, 4.0, 7.0, 10.0])]
+    n0, n1 = 5, 5
+    top = np.arange((n1 - 1) * n0, n1 * n0)
+    k, h, T0, Tamb = 1.0, 10.0, 300.0, 300.0
+    Fc, Kc = BC.convection_2d(axes, BC.first_values([(top, (h, Tamb))]))
+    dn = np.arange(n0)
+    T = _conv_solve(axes, k, dn, np.full(n0, T0), Kc, Fc)
+    L = axes[1][-1]
+    g = (1e-6 * h * Tamb - h * T0) / (k + h * L)
+    z = np.repeat(axes[1], n0)
+    assert np.allclose(T, T0 + g * z, rtol=0, atol=1e-9 * T0)
+    assert abs(T[-1] - 2.9703) < 1e-3
+
+
+def test_convection_3d_bottom_closed_form():
+    """3D slab with convection on its bottom wall (whose nodes are the element's local
+    nodes 0-3, so the reference's placement is the geometric one) and T_top held.  The
+    reference's wall matrix 0.125e-12 area (c_i + c_j) / (9 | 18 | 36) sums to a quarter
+    of its load's 0.25e-12 area c per node (therm3d.cpp:252-257), so per unit area
+    k 1e-6 (T_b - T_a) / dz + 0.25e-12 h T_b = 1e-12 h T_amb and the profile is linear:
+    T_b = (1e-12 h T_amb + 1e-6 k T_top / L) / (1e-6 k / L + 0.25e-12 h)."""
+    axes = [np.array([0.0, 1.0, 2.5, 3.0]), np.array([0.0, 0.5, 2.0]), np.array([0.0, 0.4, 1.6, 3.0, 5.0])]
+    n0, n1, n2 = 4, 3, 5
+    plane = n0 * n1
+    bottom = np.arange(plane)
+    top = np.arange((n2 - 1) * plane, n2 * plane)
+    k, h, Ttop, Tamb = 2.0, 3e5, 310.0, 290.0
+    Fc, Kc = BC.convection_3d(axes, BC.first_values([(bottom, (h, Tamb))]))
+    T = _conv_solve(axes, k, top, np.full(plane, Ttop), Kc, Fc)
+    L = axes[2][-1]
+    Tb = (1e-12 * h * Tamb + 1e-6 * k * Ttop / L) / (1e-6 * k / L + 0.25e-12 * h)
+    z = np.repeat(axes[2], plane)
+    assert np.allclose(T, Tb + (Ttop - Tb) * z / L, rtol=0, atol=1e-9 * Ttop)
+    assert abs(Tb - Ttop) > 1.0
+
+
+def test_convection_3d_top_wall_goes_to_the_lower_face():
+    """A top convective wall puts its terms on the element's local nodes 0-3: the top
+    plane then carries no equation but conduction and stays at the plane below's value."""
+    axes = [np.array([0.0, 1.0, 2.0]), np.array([0.0, 1.0]), np.array([0.0, 1.0, 2.0, 3.0])]
+    n0, n1, n2 = 3, 2, 4
+    plane = n0 * n1
+    top = np.arange((n2 - 1) * plane, n2 * plane)
+    Fc, Kc = BC.convection_3d(axes, BC.first_values([(top, (1e5, 350.0))]))
+    assert not Fc[top].any() and Fc[(n2 - 2) * plane:(n2 - 1) * plane].all()
+    T = _conv_solve(axes, 1.0, np.arange(plane), np.full(plane, 300.0), Kc, Fc)
+    assert np.allclose(T[top], T[(n2 - 2) * plane:(n2 - 1) * plane], rtol=0, atol=1e-9)
+
+
+def test_convection_cylindrical_far_from_the_axis():
+    """An annulus far from the axis (r in [1000, 1004]) is the 2D slab times r: the
+    reference's cylindrical terms reduce to the Cartesian closed form to O(len / r)."""
+    axes = [np.array([1000.0, 1001.0, 1002.5, 1004.0]), np.array([0.0, 2.0, 5.0, 10.0])]
+    n0, n1 = 4, 4
+    top = np.arange((n1 - 1) * n0, n1 * n0)
+    k, h, T0, Tamb = 1.0, 10.0, 300.0, 300.0
+    Fc, Kc = BC.convection_2d(axes, BC.first_values([(top, (h, Tamb))]), cyl=True)
+    T = _conv_solve(axes, k, np.arange(n0), np.full(n0, T0), Kc, Fc, cyl=True)
+    g = (1e-6 * h * Tamb - h * T0) / (k + h * axes[1][-1])
+    z = np.repeat(axes[1], n0)
+    assert np.allclose(T, T0 + g * z, rtol=0, atol=2e-3 * T0)
+
+
+def test_convection_matrix_is_positive_semidefinite():
+    axes = [np.array([0.0, 1.0, 2.0, 3.0]), np.array([0.0, 1.0, 3.0]), np.array([0.0, 1.0, 2.0])]
+    conv = BC.first_values([(np.arange(36), (50.0, 300.0))])   # every node
+    _, Kc = BC.convection_3d(axes, conv)
+    M = np.zeros((36, 36))
+    for r, c, v in Kc:
+        M[r, c] += v
+        if r != c:
+            M[c, r] += v
+    assert np.linalg.eigvalsh(M).min() > -1e-12 * np.abs(M).max()

@@ -289,3 +289,29 @@ def test_element_conductivity_per_leaf_matches_per_element():
         h = np.array([leaves[q][2][-1] if q >= 0 else np.inf for q in s._leaf]) if leaf_height else s._h
         want = np.array([m.thermk(320.0, float(h[e])) for e, m in enumerate(s._mat)], dtype=float)
         assert np.array_equal(k, want)
+
+
+def test_cond_callable_evaluation():
+    """ActiveCond's cond(U, j, T): one vectorised call when the callable takes arrays, one
+    call per element otherwise; a number is (0, value), a pair (in-plane, vertical)."""
+    import math
+    from plask_amd.electrical.shockley import _cond_values
+    U = np.array([0.9, 1.0, 1.1])
+    j = np.array([-0.05, 0.02, 0.3])
+    T = np.array([300.0, 310.0, 320.0])
+    calls = []
+
+    def vec(u, jj, t):
+        calls.append(np.ndim(u))
+        return 0.05 + np.abs(jj) + 1e-3 * t
+    h, v = _cond_values(vec, U, j, T)
+    assert calls == [1]
+    assert np.allclose(v, 0.05 + np.abs(j) + 1e-3 * T) and not h.any()
+    h2, v2 = _cond_values(lambda u, jj, t: max(0.05, math.fabs(jj)) + 1e-3 * t, U, j, T)
+    assert np.allclose(v2, [max(0.05, abs(x)) + 1e-3 * t for x, t in zip(j, T)])
+    h3, v3 = _cond_values(lambda u, jj, t: (2.0, 0.1), U, j, T)
+    assert np.allclose(h3, 2.0) and np.allclose(v3, 0.1)
+    h4, v4 = _cond_values(lambda u, jj, t: (u, 1.0) if jj > 0 else (0.0, 2.0), U, j, T)
+    assert np.allclose(h4, [0.0, 1.0, 1.1]) and np.allclose(v4, [2.0, 1.0, 1.0])
+    with pytest.raises(ZeroDivisionError):
+        _cond_values(lambda u, jj, t: 1 / 0, U, j, T)

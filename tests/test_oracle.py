@@ -14,6 +14,7 @@ import pytest
 import scipy.linalg.lapack as lapack
 
 import oracle as O
+from plask_amd import workloads
 
 
 def gauss_K(d, sizes, kdiag):
@@ -200,3 +201,40 @@ def test_matrix_free_apply_matches_assembled():
         m[dn] = 1
         x = np.random.default_rng(0).normal(size=g.nnodes)
         assert np.allclose(O.apply(g, 0, kh, kv, m, x), A.mult(x), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref/libnspcg.so not built (reference sources absent)")
+def test_cg_restatement_pinned_by_reference_nspcg():
+    """The oracle's PCG (NSPCG itcg order, oracle/fem_oracle.c) against the reference's own
+    NSPCG built from its sources (oracle/_ref): Jacobi-CG takes the same number of
+    iterations to the same accuracy, IC(0)-CG within one iteration (the two stop on
+    different tests: NSPCG's error estimate vs the true residual); both agree with band
+    Cholesky."""
+    p = workloads.thermal_3d(seed=2, n=(12, 10, 24), nlayers=6, active=3, width=20.0)
+    g = O.Grid(p["axes"])
+    T_ref = O.band_cholesky_solve(O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_BAND, p["dir_nodes"],
+                                             p["dir_vals"]))
+    A = O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_DIA, p["dir_nodes"], p["dir_vals"])
+    x0 = np.full(g.nnodes, 300.0)
+    for pc, ours in (("jac", 1), ("ic", 2)):
+        u, it, _, ier, _, _ = O.nspcg_solve(A, x0, pc, 1e-10, 5000)
+        v, st, it2, rel = O.dia_pcg(A, x0, precond=ours, rtol=1e-10)
+        assert ier == 0 and st >= 0
+        assert np.abs(u - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+        assert np.abs(v - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+        assert abs(it - it2) <= (0 if pc == "jac" else 1), (pc, it, it2)
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref/libnspcg.so not built (reference sources absent)")
+def test_reference_nspcg_default_tolerance_accuracy():
+    """PLaSK's default iterative settings (maxerr = zeta = 1e-6, IC(0)) give a solution
+    about 1e-6 relative from the direct one -- the accuracy the reference's 3D path
+    delivers by default (the GPU solves are run to a 1e-8 residual, well inside it)."""
+    p = workloads.thermal_3d(seed=2, n=(12, 10, 24), nlayers=6, active=3, width=20.0)
+    g = O.Grid(p["axes"])
+    T_ref = O.band_cholesky_solve(O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_BAND, p["dir_nodes"],
+                                             p["dir_vals"]))
+    A = O.assemble(g, 0, p["k_h"], p["k_v"], p["q"], O.ST_DIA, p["dir_nodes"], p["dir_vals"])
+    u, it, _, ier, _, _ = O.nspcg_solve(A, np.full(g.nnodes, 300.0), "ic", 1e-6, 1000)
+    err = np.abs(u - T_ref).max() / np.abs(T_ref).max()
+    assert ier == 0 and 1e-9 < err < 1e-5, err
