@@ -394,6 +394,25 @@ int pf_capacity_set(pf_ctx* ctx, const double* c_elem, int64_t nelem, double the
  * the 2D loop in both dimensions). */
 int pf_dynamic_step(pf_ctx* ctx, const pf_solve_opts* opts, int held, pf_solve_stats* stats);
 
+/* Element fields after a solve (the providers' post-processing, on the device):
+ * PF_POST_FLUX    heat flux -k grad T (W/m^2; dim values per element; saveHeatFluxes,
+ *                 therm2d.cpp:504-540, therm3d.cpp:342-384), with k_h / k_v the element
+ *                 conductivities to use (the reference evaluates thermk at the final element
+ *                 temperatures with the leaf's height), or NULL: the conductivity law of
+ *                 pf_coeffs_powerlaw at the current temperatures, else the conductivities as set;
+ * PF_POST_CURRENT current density (kA/cm^2; dim values) and
+ * PF_POST_JOULE   Joule heat density (W/m^3) of the last pf_currents (electr2d.cpp:395-464).
+ * The field stays on the device for pf_interp_elements; out (E or E * dim values) may be NULL. */
+#define PF_POST_FLUX 0
+#define PF_POST_CURRENT 1
+#define PF_POST_JOULE 2
+int pf_postprocess(pf_ctx* ctx, int what, const double* k_h, const double* k_v, double* out);
+/* An element field of pf_postprocess at points (npts x dim, um) into out (npts x components):
+ * linear between element midpoints, the outermost midpoint's value up to the mesh boundary,
+ * fill outside the mesh (interpolation on the element mesh of getCurrentDensities /
+ * getHeatDensities / getHeatFluxes, electr2d.cpp:518-562, therm2d.cpp:556-569). */
+int pf_interp_elements(pf_ctx* ctx, int what, const double* pts, int64_t npts, double fill, double* out);
+
 /* Current element conductivities (E values each) to the host. */
 int pf_get_coeffs(pf_ctx* ctx, double* k_inplane, double* k_vert);
 

@@ -1809,6 +1809,109 @@ __device__ __forceinline__ void elem_coords(const ElemGrid& g, long long e, int&
     }
 }
 
+// Heat flux -k grad T per element in W/m^2 (saveHeatFluxes, therm2d.cpp:504-540 and
+// therm3d.cpp:342-384): the element's corner-difference gradient (x 0.5e6 in 2D, 0.25e6 in
+// 3D: 1e6 um -> m over the corners averaged), in-plane k along x (and y), vertical k along
+// the last axis.  k from the law k0 (300 / T_avg)^alpha at the element's mean temperature
+// when ah is given, else from kh / kv.
+__global__ void k_heat_flux(ElemGrid g, long long E, const double* __restrict__ hx, const double* __restrict__ hy,
+                            const double* __restrict__ hz, const double* __restrict__ T, const double* __restrict__ kh,
+                            const double* __restrict__ kv, const double* __restrict__ ah, const double* __restrict__ av,
+                            double* __restrict__ flux) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        elem_coords(g, e, i, j, k);
+        if (g.dim == 2) {
+            const long long ll = i + (long long)g.n0 * j, ul = ll + 1, lu = ll + g.n0, uu = lu + 1;
+            const double tll = T[ll], tul = T[ul], tlu = T[lu], tuu = T[uu];
+            double a = kh[e], b = kv[e];
+            if (ah) {
+                const double r = 300. / (0.25 * (tll + tul + tlu + tuu));
+                a *= pow(r, ah[e]);
+                b *= pow(r, av[e]);
+            }
+            flux[2 * e] = -0.5e6 * a * (-tll + tul - tlu + tuu) / hx[i];
+            flux[2 * e + 1] = -0.5e6 * b * (-tll - tul + tlu + tuu) / hy[j];
+        } else {
+            double t[8], s = 0.;
+            for (int c = 0; c < 8; ++c) {
+                t[c] = T[(i + (c & 1)) + (long long)g.n0 * ((j + ((c >> 1) & 1)) + (long long)g.n1 * (k + (c >> 2)))];
+                s += t[c];
+            }
+            double a = kh[e], b = kv[e];
+            if (ah) {
+                const double r = 300. / (0.125 * s);
+                a *= pow(r, ah[e]);
+                b *= pow(r, av[e]);
+            }
+            double dx = 0., dy = 0., dz = 0.;
+            for (int c = 0; c < 8; ++c) {
+                dx += (c & 1) ? t[c] : -t[c];
+                dy += (c & 2) ? t[c] : -t[c];
+                dz += (c & 4) ? t[c] : -t[c];
+            }
+            flux[3 * e] = -0.25e6 * a * dx / hx[i];
+            flux[3 * e + 1] = -0.25e6 * a * dy / hy[j];
+            flux[3 * e + 2] = -0.25e6 * b * dz / hz[k];
+        }
+    }
+}
+
+// Element (midpoint) field at points: linear between midpoints, the edge value between
+// the outermost midpoint and the mesh boundary, fill outside the mesh.  ax holds the node
+// axes, mid the midpoint axes (n - 1 each), both concatenated; nc components per element.
+__global__ void k_interp_elems(const double* __restrict__ ax, const double* __restrict__ mid, int dim, int n0, int n1,
+                               int n2, const double* __restrict__ f, int nc, long long npts,
+                               const double* __restrict__ pts, double fill, double* __restrict__ out) {
+    const int n[3] = {n0, n1, n2};
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
+         p += (long long)gridDim.x * blockDim.x) {
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        double t[3] = {0., 0., 0.};
+        bool inside = true;
+        const double* A = ax;
+        const double* M = mid;
+        for (int a = 0; a < dim; ++a) {
+            const double c = pts[p * dim + a];
+            const int na = n[a], m = na - 1;
+            if (!(c >= A[0] && c <= A[na - 1])) inside = false;
+            const double cc = fmin(fmax(c, M[0]), M[m - 1]);
+            int l = 0, h = m - 1;
+            while (h - l > 1) {
+                const int md = (l + h) >> 1;
+                if (M[md] <= cc) l = md;
+                else h = md;
+            }
+            lo[a] = l;
+            hi[a] = m > 1 ? l + 1 : l;
+            t[a] = m > 1 ? (cc - M[l]) / (M[l + 1] - M[l]) : 0.;
+            A += na;
+            M += m;
+        }
+        if (!inside) {
+            for (int q = 0; q < nc; ++q) out[p * nc + q] = fill;
+            continue;
+        }
+        const long long m0 = n0 - 1, m1 = n1 - 1;
+        for (int q = 0; q < nc; ++q) {
+            double v = 0.;
+            for (int c = 0; c < (1 << dim); ++c) {
+                double w = 1.;
+                int id[3] = {0, 0, 0};
+                for (int a = 0; a < dim; ++a) {
+                    const int b = (c >> a) & 1;
+                    w *= b ? t[a] : 1. - t[a];
+                    id[a] = b ? hi[a] : lo[a];
+                }
+                const long long e = id[0] + m0 * (id[1] + (dim == 3 ? m1 * id[2] : 0));
+                v += w * f[e * nc + q];
+            }
+            out[p * nc + q] = v;
+        }
+    }
+}
+
+
 __device__ __forceinline__ long long node_of(const ElemGrid& g, int i, int j, int k) {
     return i + (long long)g.n0 * (j + (long long)g.n1 * k);
 }

@@ -63,6 +63,26 @@ std::vector<double> FemCore::interp(const double* pts, int64_t n, double fill) {
     return out;
 }
 
+std::vector<double> FemCore::postprocess(int what, const std::vector<double>& kh, const std::vector<double>& kv,
+                                         bool fetch) {
+    const bool given = !kh.empty();
+    if (given && ((int64_t)kh.size() != E_ || (int64_t)kv.size() != E_))
+        throw std::invalid_argument(id + ": conductivity arrays need one value per element");
+    const size_t nc = what == PF_POST_JOULE ? 1 : (size_t)dim_;
+    std::vector<double> out(fetch ? (size_t)E_ * nc : 0);
+    check(pf_postprocess(ctx_, what, given ? kh.data() : nullptr, given ? kv.data() : nullptr,
+                         fetch ? out.data() : nullptr),
+          "post-processing");
+    return out;
+}
+
+std::vector<double> FemCore::interp_elements(int what, const double* pts, int64_t n, double fill) {
+    const size_t nc = what == PF_POST_JOULE ? 1 : (size_t)dim_;
+    std::vector<double> out((size_t)n * nc);
+    check(pf_interp_elements(ctx_, what, pts, n, fill, out.data()), "element interpolation");
+    return out;
+}
+
 void FemCore::solve(double init_value, int norm) {
     pf_solve_opts o{};
     o.precision = iter.precision;

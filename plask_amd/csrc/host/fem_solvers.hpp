@@ -53,6 +53,11 @@ class FemCore {
     std::vector<double> solution();  // node values (natural order)
     // the solution linearly interpolated at n points (n x dim, um); outside: fill
     std::vector<double> interp(const double* pts, int64_t n, double fill);
+    // element fields on the device (pf_postprocess / pf_interp_elements, PF_POST_*): the
+    // field's values per element, or interpolated at points (n x components)
+    std::vector<double> postprocess(int what, const std::vector<double>& kh, const std::vector<double>& kv,
+                                    bool fetch);
+    std::vector<double> interp_elements(int what, const double* pts, int64_t n, double fill);
     void invalidate() { initialized_ = false; }
     bool initialized() const { return initialized_; }
 

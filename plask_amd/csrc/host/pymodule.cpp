@@ -54,6 +54,22 @@ PYBIND11_MODULE(_core, m) {
                      throw std::invalid_argument("points must be an (n, dim) array");
                  return to_np(s.interp(pts.data(), (int64_t)pts.shape(0), fill));
              })
+        .def("postprocess",
+             [](FemCore& s, int what, py::object kh, py::object kv, bool fetch) {
+                 std::vector<double> a, b;
+                 if (!kh.is_none()) {
+                     a = vec(kh.cast<arr<double>>());
+                     b = vec(kv.cast<arr<double>>());
+                 }
+                 return to_np(s.postprocess(what, a, b, fetch));
+             },
+             py::arg("what"), py::arg("kh") = py::none(), py::arg("kv") = py::none(), py::arg("fetch") = false)
+        .def("interp_elements",
+             [](FemCore& s, int what, const arr<double>& pts, double fill) {
+                 if (pts.ndim() != 2 || pts.shape(1) != s.dim())
+                     throw std::invalid_argument("points must be an (n, dim) array");
+                 return to_np(s.interp_elements(what, pts.data(), (int64_t)pts.shape(0), fill));
+             })
         .def("invalidate", &FemCore::invalidate)
         .def_property_readonly("initialized", &FemCore::initialized)
         .def_property_readonly("num_nodes", &FemCore::num_nodes)

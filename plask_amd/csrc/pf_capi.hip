@@ -89,8 +89,8 @@ struct pf_ctx {
     // line-factor set-up scratch: per-node line coefficients, per-segment sums
     double *sb = nullptr, *sc = nullptr, *sff = nullptr, *sfl = nullptr;
     // interpolation workspace (axes, points, results)
-    double *iax = nullptr, *ipts = nullptr, *iout = nullptr;
-    size_t ipts_cap = 0;
+    double *iax = nullptr, *ipts = nullptr, *iout = nullptr, *imid = nullptr;
+    size_t ipts_cap = 0, iout_cap = 0;
     // nodal load added to the element load vector (boundary heat flux / radiation)
     long long* lnodes = nullptr;
     double* lvals = nullptr;
@@ -183,6 +183,8 @@ struct pf_ctx {
     bool have_law = false;
     double* xprev = nullptr;
     double *cur = nullptr, *heat = nullptr, *emean = nullptr;
+    double *flux = nullptr, *kpost = nullptr;  // element heat flux, conductivities given for it
+    bool have_cur = false, have_flux = false;
     unsigned char *sel = nullptr, *noheat = nullptr;
     bool have_sel = false, have_noheat = false;
     double* dout = nullptr;  // small device results
@@ -586,6 +588,9 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     HIPCK(hipMemset(ctx->cur, 0, sizeof(double) * ctx->E * dim));
     HIPCK(hipMemset(ctx->xprev, 0, sizeof(double) * N));
     ctx->have_law = ctx->have_sel = ctx->have_noheat = false;
+    ctx->have_cur = ctx->have_flux = false;
+    dalloc(ctx, &ctx->flux, 0);
+    dalloc(ctx, &ctx->kpost, 0);
     ctx->nje = 0;
     ctx->dyn = false;  // a transient operator and a boundary matrix refer to the old grid
     ctx->dsh_on = ctx->have_dconv = ctx->have_bnd = false;
@@ -738,9 +743,12 @@ int pf_interp_nodes(pf_ctx* ctx, const double* pts, int64_t npts, double fill, d
     for (int a = 0; a < dim; ++a) axcat.insert(axcat.end(), ctx->ax[a].begin(), ctx->ax[a].end());
     int rc;
     if ((size_t)npts > ctx->ipts_cap) {
-        if ((rc = dalloc(ctx, &ctx->ipts, (size_t)npts * dim)) || (rc = dalloc(ctx, &ctx->iout, (size_t)npts)))
-            return rc;
+        if ((rc = dalloc(ctx, &ctx->ipts, (size_t)npts * dim))) return rc;
         ctx->ipts_cap = (size_t)npts;
+    }
+    if ((size_t)npts > ctx->iout_cap) {
+        if ((rc = dalloc(ctx, &ctx->iout, (size_t)npts))) return rc;
+        ctx->iout_cap = (size_t)npts;
     }
     if ((rc = dalloc(ctx, &ctx->iax, nax))) return rc;
     HIPCK(hipMemcpyAsync(ctx->iax, axcat.data(), sizeof(double) * nax, hipMemcpyHostToDevice, ctx->stream));
@@ -2613,6 +2621,102 @@ int pf_currents(pf_ctx* ctx, double* max_dj2, double* max_j2) {
     HIPCK(hipStreamSynchronize(ctx->stream));
     if (max_dj2) *max_dj2 = h[0];
     if (max_j2) *max_j2 = h[1];
+    ctx->have_cur = true;
+    return PF_OK;
+}
+
+int pf_postprocess(pf_ctx* ctx, int what, const double* k_h, const double* k_v, double* out) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    HIPCK(hipSetDevice(ctx->device));
+    const size_t E = (size_t)ctx->E, dim = (size_t)ctx->dim;
+    if (what == PF_POST_CURRENT || what == PF_POST_JOULE) {
+        if (!ctx->have_cur) return fail(ctx, PF_ESTATE, "no currents: run pf_currents after a solve");
+        if (out) {
+            if (what == PF_POST_CURRENT)
+                HIPCK(hipMemcpy(out, ctx->cur, sizeof(double) * E * dim, hipMemcpyDeviceToHost));
+            else
+                HIPCK(hipMemcpy(out, ctx->heat, sizeof(double) * E, hipMemcpyDeviceToHost));
+        }
+        return PF_OK;
+    }
+    if (what != PF_POST_FLUX) return fail(ctx, PF_EINVAL, "what must be PF_POST_FLUX, PF_POST_CURRENT or PF_POST_JOULE");
+    if (!ctx->have_solution) return fail(ctx, PF_ESTATE, "no temperature on the device");
+    if ((k_h == nullptr) != (k_v == nullptr)) return fail(ctx, PF_EINVAL, "give both conductivity arrays or neither");
+    if (!k_h && !ctx->have_law && !ctx->have_coeffs) return fail(ctx, PF_ESTATE, "no conductivities");
+    int rc;
+    if (!ctx->flux && (rc = dalloc(ctx, &ctx->flux, E * dim))) return rc;
+    const double *kh = nullptr, *kv = nullptr, *ah = nullptr, *av = nullptr;
+    if (k_h) {
+        if (!ctx->kpost && (rc = dalloc(ctx, &ctx->kpost, 2 * E))) return rc;
+        HIPCK(hipMemcpyAsync(ctx->kpost, k_h, sizeof(double) * E, hipMemcpyHostToDevice, ctx->stream));
+        HIPCK(hipMemcpyAsync(ctx->kpost + E, k_v, sizeof(double) * E, hipMemcpyHostToDevice, ctx->stream));
+        kh = ctx->kpost;
+        kv = ctx->kpost + E;
+    } else if (ctx->have_law) {  // the law at the current temperatures
+        kh = ctx->k0h;
+        kv = ctx->k0v;
+        ah = ctx->ah;
+        av = ctx->av;
+    } else {  // the conductivities as set
+        kh = ctx->dyn ? ctx->khB : ctx->kh;
+        kv = ctx->dyn ? ctx->kvB : ctx->kv;
+    }
+    hipLaunchKernelGGL(k_heat_flux, dim3(grid1(ctx->E, 256, 8192)), dim3(256), 0, ctx->stream, egrid(ctx),
+                       (long long)ctx->E, (const double*)ctx->hx, (const double*)ctx->hy, (const double*)ctx->hz,
+                       (const double*)ctx->x, kh, kv, ah, av, ctx->flux);
+    HIPCK(hipGetLastError());
+    if (out) HIPCK(hipMemcpyAsync(out, ctx->flux, sizeof(double) * E * dim, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->have_flux = true;
+    return PF_OK;
+}
+
+int pf_interp_elements(pf_ctx* ctx, int what, const double* pts, int64_t npts, double fill, double* out) {
+    if (!ctx) return PF_EINVAL;
+    if (!ctx->N) return fail(ctx, PF_ESTATE, "pf_grid_set first");
+    if (npts < 0 || (npts > 0 && (!pts || !out))) return fail(ctx, PF_EINVAL, "bad point list");
+    const double* f = nullptr;
+    int nc = 1;
+    if (what == PF_POST_FLUX) {
+        if (!ctx->have_flux) return fail(ctx, PF_ESTATE, "no heat flux: pf_postprocess(PF_POST_FLUX) first");
+        f = ctx->flux;
+        nc = ctx->dim;
+    } else if (what == PF_POST_CURRENT || what == PF_POST_JOULE) {
+        if (!ctx->have_cur) return fail(ctx, PF_ESTATE, "no currents: run pf_currents after a solve");
+        f = what == PF_POST_CURRENT ? ctx->cur : ctx->heat;
+        nc = what == PF_POST_CURRENT ? ctx->dim : 1;
+    } else {
+        return fail(ctx, PF_EINVAL, "what must be PF_POST_FLUX, PF_POST_CURRENT or PF_POST_JOULE");
+    }
+    if (npts == 0) return PF_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    const int dim = ctx->dim;
+    std::vector<double> axcat, midcat;
+    for (int a = 0; a < dim; ++a) {
+        const std::vector<double>& A = ctx->ax[a];
+        axcat.insert(axcat.end(), A.begin(), A.end());
+        for (size_t i = 0; i + 1 < A.size(); ++i) midcat.push_back(0.5 * (A[i] + A[i + 1]));
+    }
+    int rc;
+    if ((size_t)npts > ctx->ipts_cap) {
+        if ((rc = dalloc(ctx, &ctx->ipts, (size_t)npts * dim))) return rc;
+        ctx->ipts_cap = (size_t)npts;
+    }
+    if ((size_t)npts * nc > ctx->iout_cap) {
+        if ((rc = dalloc(ctx, &ctx->iout, (size_t)npts * nc))) return rc;
+        ctx->iout_cap = (size_t)npts * nc;
+    }
+    if ((rc = dalloc(ctx, &ctx->iax, axcat.size())) || (rc = dalloc(ctx, &ctx->imid, midcat.size()))) return rc;
+    HIPCK(hipMemcpyAsync(ctx->iax, axcat.data(), sizeof(double) * axcat.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->imid, midcat.data(), sizeof(double) * midcat.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->ipts, pts, sizeof(double) * (size_t)npts * dim, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_interp_elems, dim3(grid1(npts, 256, 8192)), dim3(256), 0, ctx->stream, (const double*)ctx->iax,
+                       (const double*)ctx->imid, dim, (int)ctx->n[0], (int)ctx->n[1], dim == 3 ? (int)ctx->n[2] : 1, f,
+                       nc, (long long)npts, (const double*)ctx->ipts, fill, ctx->iout);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, ctx->iout, sizeof(double) * (size_t)npts * nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
     return PF_OK;
 }
 

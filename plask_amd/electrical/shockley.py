@@ -21,6 +21,7 @@ from .. import solverbase as _sb
 from .._core import ElectricalCore
 
 EPSILON0 = 8.854187817e-12  # F/m (plask/phys/constants.hpp)
+_POST_CURRENT, _POST_JOULE = 1, 2   # PF_POST_CURRENT, PF_POST_JOULE
 
 
 def _active_number(roles):
@@ -233,7 +234,7 @@ class _ShockleyBase(_sb.FemSolver):
             nodes = np.concatenate([nodes, self._isolated])
             vals = np.concatenate([vals, np.zeros(len(self._isolated))])
         c.set_dirichlet(nodes.astype(np.int64), vals)
-        err = c.compute(int(loops))
+        err = self._run_core(c.compute, int(loops))
         self._V = None
         return err
 
@@ -281,15 +282,29 @@ class _ShockleyBase(_sb.FemSolver):
             raise ValueError("%s: voltage not computed" % self.id)
         return _sb.interp_nodes(self._axes, self._volt, mesh, fill=np.nan, device=self._core)
 
+    def _element_field(self, what, mesh):
+        """A device element field at the points of mesh: the values themselves on the
+        element mesh, else linear interpolation on the element mesh on the device
+        (getCurrentDensities / getHeatDensities, electr2d.cpp:518-562)."""
+        c = self._core
+        mids = [0.5 * (a[1:] + a[:-1]) for a in self._axes]
+        if isinstance(mesh, _mesh.Rectangular) and mesh.dim == self.DIM and _sb._same_axes(mesh.axes, mids):
+            return c.currents().reshape(-1, self.DIM) if what == _POST_CURRENT else c.heats()
+        pts = np.ascontiguousarray(_sb.points_of(mesh), dtype=np.float64).reshape(-1, self.DIM)
+        out = c.interp_elements(what, pts, 0.0)
+        return out.reshape(-1, self.DIM) if what == _POST_CURRENT else out
+
     def outCurrentDensity(self, mesh, interpolation="linear"):
-        j = self._core.currents().reshape(-1, self.DIM)
-        return _sb.interp_elements(self._axes, j, mesh, fill=0.0)
+        """Current density (kA/cm^2)."""
+        if self._core is None or self._core.loopno == 0:
+            raise ValueError("%s: current density not computed" % self.id)
+        return self._element_field(_POST_CURRENT, mesh)
 
     def outHeat(self, mesh, interpolation="linear"):
         """Joule heat density (W/m^3) (getHeatDensities, electr2d.cpp:541-562)."""
         if self._core is None or self._core.loopno == 0:
             raise ValueError("%s: heat density not computed" % self.id)
-        return _sb.interp_elements(self._axes, self._core.heats(), mesh, fill=0.0)
+        return self._element_field(_POST_JOULE, mesh)
 
     def outConductivity(self, mesh, interpolation="nearest"):
         self.initialize()

@@ -19,6 +19,7 @@ PF_X0_GIVEN, PF_X0_ZERO, PF_X0_DEVICE, PF_X0_UNIFORM, PF_X0_DEVICE_CHECKED = 0, 
 PF_NORM_INIT, PF_NORM_B, PF_NORM_B_FULL, PF_NORM_START = 0, 1, 2, 3
 PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG = 1, 2, 3
 PF_HELD_FIXED, PF_HELD_REFERENCE = 0, 1
+PF_POST_FLUX, PF_POST_CURRENT, PF_POST_JOULE = 0, 1, 2
 KCAT_APPLY, KCAT_PRECOND, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH, KCAT_MG_TRANSFER, KCAT_MG_COARSE = range(6)
 
 EXPORTED = [
@@ -28,7 +29,8 @@ EXPORTED = [
     "pf_set_launch", "pf_set_line_segment", "pf_set_prefetch", "pf_stream_copy", "pf_set_apply", "pf_set_graphs", "pf_time_kernels", "pf_set_block_order", "pf_solution_prolong",
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
     "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
-    "pf_capacity_set", "pf_dynamic_step",
+    "pf_capacity_set", "pf_dynamic_step", "pf_boundary_matrix_set", "pf_postprocess", "pf_interp_elements",
+    "pf_junction_state", "pf_junction_cond_set",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
@@ -121,6 +123,9 @@ def load_library(path=LIB_PATH):
     L.pf_multigrid_levels.argtypes = [P]
     L.pf_precond_apply_host.argtypes = [P, I, I, P, P]
     L.pf_capacity_set.argtypes = [P, P, i64, C.c_double]
+    L.pf_boundary_matrix_set.argtypes = [P, P, P, P, i64]
+    L.pf_postprocess.argtypes = [P, I, P, P, P]
+    L.pf_interp_elements.argtypes = [P, I, P, i64, C.c_double, P]
     L.pf_dynamic_step.argtypes = [P, C.POINTER(SolveOpts), I, C.POINTER(SolveStats)]
     _lib = L
     return L
@@ -263,6 +268,35 @@ class FemContext:
         rc = self.L.pf_solve(self.h, C.byref(o), None, C.byref(st))
         self._ck(rc, "pf_solve")
         return st
+
+    def heat_flux(self, k_h=None, k_v=None):
+        """Element heat flux -k grad T (W/m^2), E x dim (pf_postprocess(PF_POST_FLUX))."""
+        out = np.zeros((self.nelems, self.dim))
+        if k_h is None:
+            self._ck(self.L.pf_postprocess(self.h, PF_POST_FLUX, None, None, _p(out)), "pf_postprocess")
+        else:
+            a, b = _f64(k_h), _f64(k_v)
+            self._ck(self.L.pf_postprocess(self.h, PF_POST_FLUX, _p(a), _p(b), _p(out)), "pf_postprocess")
+        return out
+
+    def interp_elements(self, what, points, fill=0.0):
+        """An element field of pf_postprocess at points (n x dim)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        nc = 1 if what == PF_POST_JOULE else self.dim
+        out = np.zeros((len(pts), nc))
+        self._ck(self.L.pf_interp_elements(self.h, int(what), _p(pts), len(pts), float(fill), _p(out)),
+                 "pf_interp_elements")
+        return out if nc > 1 else out[:, 0]
+
+    def set_boundary_matrix(self, rows, cols, vals):
+        """Boundary terms of the operator, A(row, col) += val for col <= row (both triangles
+        for off-diagonal entries); empty lists clear them."""
+        self._br = np.ascontiguousarray(rows, dtype=np.int64)
+        self._bc = np.ascontiguousarray(cols, dtype=np.int64)
+        self._bv = _f64(vals)
+        n = len(self._bv)
+        self._ck(self.L.pf_boundary_matrix_set(self.h, _p(self._br) if n else None, _p(self._bc) if n else None,
+                                               _p(self._bv) if n else None, n), "pf_boundary_matrix_set")
 
     def set_capacity(self, c_elem, theta=0.5):
         """Transient operator: per-element lumped capacities c (stiffness units, each of the

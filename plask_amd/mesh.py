@@ -107,6 +107,33 @@ class Rectangular:
         return Boundary("bottomof", (obj, path))
 
     @staticmethod
+    def LeftOf(obj, path=None):
+        return Boundary("leftof", (obj, path))
+
+    @staticmethod
+    def RightOf(obj, path=None):
+        return Boundary("rightof", (obj, path))
+
+    @staticmethod
+    def FrontOf(obj, path=None):
+        return Boundary("frontof", (obj, path))
+
+    @staticmethod
+    def BackOf(obj, path=None):
+        return Boundary("backof", (obj, path))
+
+    @staticmethod
+    def VerticalLineNear(at, start, stop):
+        """2D: the mesh line of axis 0 nearest `at`, nodes from `start` to `stop` along axis 1
+        (createVerticalBoundaryNear, plask/mesh/rectangular2d.cpp:140-145)."""
+        return Boundary("vline", (float(at), float(start), float(stop)))
+
+    @staticmethod
+    def HorizontalLineNear(at, start, stop):
+        """2D: the mesh line of axis 1 nearest `at`, nodes from `start` to `stop` along axis 0."""
+        return Boundary("hline", (float(at), float(start), float(stop)))
+
+    @staticmethod
     def Predicate(fn, vectorized=False):
         """Nodes where fn(point) is true; vectorized=True: fn takes the (N, dim)
         array of all points and returns N booleans."""
@@ -162,6 +189,34 @@ class Boundary:
                     raise ValueError("vectorized predicate must return one boolean per point")
                 return np.nonzero(ok)[0].astype(np.int64)
             return np.nonzero([bool(fn(p)) for p in pts])[0].astype(np.int64)
+        if self.kind in ("leftof", "rightof", "frontof", "backof"):
+            if geometry is None:
+                raise ValueError("%s needs a geometry" % self.kind)
+            if d == 3:
+                ax = {"backof": 0, "frontof": 0, "leftof": 1, "rightof": 1}[self.kind]
+            else:
+                if self.kind in ("frontof", "backof"):
+                    raise ValueError("%s is a 3D place" % self.kind)
+                ax = 0
+            hi = self.kind in ("rightof", "frontof")
+            out = []
+            for o, s in geometry.leaves_of(*self.arg):
+                got = _box_face(mesh.axes, ax, hi, o, o + s)
+                if got is not None:
+                    out.append(got)
+            return np.unique(np.concatenate(out)).astype(np.int64) if out else np.zeros(0, np.int64)
+        if self.kind in ("vline", "hline"):
+            if d != 2:
+                raise ValueError("line places are 2D")
+            at, start, stop = self.arg
+            ax = 0 if self.kind == "vline" else 1   # the axis the line is at
+            other = 1 - ax
+            line = int(np.argmin(np.abs(mesh.axes[ax] - at)))
+            lo, hi = _bounds_ext(mesh.axes[other], start, stop)
+            if lo >= hi:
+                return np.zeros(0, np.int64)
+            idx = np.arange(lo, hi)
+            return (line + n[0] * idx if ax == 0 else idx + n[0] * line).astype(np.int64)
         if self.kind in ("topof", "bottomof"):
             if geometry is None:
                 raise ValueError("TopOf/BottomOf need a geometry")
@@ -175,6 +230,67 @@ class Boundary:
                 out.append(np.nonzero(ok)[0])
             return np.unique(np.concatenate(out)).astype(np.int64) if out else np.zeros(0, np.int64)
         raise ValueError("unknown boundary %s" % self.kind)
+
+
+def _find_index(axis, x):
+    """Index of the first coordinate >= x (MeshAxis::findIndex)."""
+    return int(np.searchsorted(axis, x, side="left"))
+
+
+def _bounds(axis, lo, hi):
+    """[begin, end) of the coordinates in [lo, hi] (getIndexesInBounds,
+    plask/mesh/rectangular_common.hpp:72-78)."""
+    if lo > hi:
+        return 0, 0
+    b, e = _find_index(axis, lo), _find_index(axis, hi)
+    if e != len(axis) and axis[e] == hi:
+        e += 1
+    return b, e
+
+
+def _bounds_ext(axis, lo, hi):
+    """_bounds widened by one coordinate at either end that lies within 1/100 of its
+    interval of the bound (getIndexesInBoundsExt with tryMakeLower / tryMakeHigher,
+    rectangular_common.hpp:86-115)."""
+    b, e = _bounds(axis, lo, hi)
+    if 0 < b < len(axis) and (lo - axis[b - 1]) * 100.0 < (axis[b] - axis[b - 1]):
+        b -= 1
+    if 0 < e < len(axis) and (axis[e] - hi) * 100.0 < (axis[e] - axis[e - 1]):
+        e += 1
+    return b, e
+
+
+def _box_face(axes, ax, hi, lo_pt, hi_pt):
+    """Nodes of the mesh line along `ax` at a box's low (getLineLo) or high (getLineHi)
+    face, over the box's extent on the other axes; None when the box holds no line."""
+    a = axes[ax]
+    if hi:
+        line = _find_index(a, hi_pt[ax])
+        if not (line != len(a) and a[line] == hi_pt[ax]):
+            if line == 0:
+                return None
+            line -= 1
+            if a[line] < lo_pt[ax]:
+                return None
+    else:
+        line = _find_index(a, lo_pt[ax])
+        if line == len(a) or a[line] > hi_pt[ax]:
+            return None
+    ranges = []
+    for o in range(len(axes)):
+        if o == ax:
+            ranges.append(np.array([line]))
+        else:
+            b, e = _bounds(axes[o], lo_pt[o], hi_pt[o])
+            if b >= e:
+                return None
+            ranges.append(np.arange(b, e))
+    n = [len(x) for x in axes]
+    strides = np.cumprod([1] + n[:-1])
+    out = np.zeros(1, np.int64)
+    for o in range(len(axes) - 1, -1, -1):
+        out = (out[:, None] + strides[o] * ranges[o][None, :]).ravel()
+    return out
 
 
 class BoundaryConditions:

@@ -161,6 +161,74 @@ def nearest_elements(axes, values, dest):
     return np.asarray(values)[flat]
 
 
+class IterativeParams(_core.IterParams):
+    """``solver.iterative`` (IterativeMatrixParams, plask/common/fem/iterative_matrix.hpp:25-94).
+
+    maxit, maxerr and noconv act as in the reference; the GPU always runs CG, so
+    ``accelerator`` is recorded (any NSPCG name is accepted), and ``preconditioner`` picks
+    the GPU preconditioner: 'jac' Jacobi, 'ljac'/'ljacx' the exact line solve along the
+    layer axis, any other (the default 'ic' included) the multigrid V-cycle.  nfact, omega,
+    ndeg, lvfill, ltrunc, nsave and nrestart are NSPCG settings, recorded only.  After a
+    solve, converged / iters / err hold its outcome."""
+    ACCELERATORS = ("cg", "si", "sor", "srcg", "srsi", "basic", "me", "cgnr", "lsqr", "odir", "omin", "ores",
+                    "iom", "gmres", "usymlq", "usymqr", "landir", "lanmin", "lanres", "cgcr", "bcgs")
+    PRECONDITIONERS = ("rich", "jac", "ljac", "ljacx", "sor", "ssor", "ic", "mic", "lsp", "neu", "lsor", "lssor",
+                       "llsp", "lneu", "bic", "bicx", "mbic", "mbicx")
+    NOCONV = ("error", "warning", "continue")
+
+    def __init__(self):
+        super().__init__()
+        self._accel = "cg"
+        self._prec = "ic"
+        self.nfact, self.omega, self.ndeg, self.lvfill, self.ltrunc = 10, 1.0, 1, 0, 0
+        self.nsave, self.nrestart = 5, 100000
+        self._results = (True, 0, 0.0, 0.0)
+
+    @property
+    def accelerator(self):
+        return self._accel
+
+    @accelerator.setter
+    def accelerator(self, v):
+        v = str(v).lower()
+        if v not in self.ACCELERATORS:
+            raise ValueError("unknown accelerator '%s'" % v)
+        self._accel = v
+
+    @property
+    def preconditioner(self):
+        return self._prec
+
+    @preconditioner.setter
+    def preconditioner(self, v):
+        v = str(v).lower()
+        if v not in self.PRECONDITIONERS:
+            raise ValueError("unknown preconditioner '%s'" % v)
+        self._prec = v
+        self.precond = 1 if v == "jac" else 2 if v in ("ljac", "ljacx") else 0
+
+    @property
+    def noconv(self):
+        return self.NOCONV[_core.IterParams.noconv.__get__(self)]
+
+    @noconv.setter
+    def noconv(self, v):
+        if isinstance(v, str):
+            v = v.lower()
+            if v not in self.NOCONV:
+                raise ValueError("noconv must be one of %s" % ", ".join(self.NOCONV))
+            v = self.NOCONV.index(v)
+        _core.IterParams.noconv.__set__(self, int(v))
+
+    def _pull(self, raw):
+        self._results = (bool(raw.converged), int(raw.iters), float(raw.err), float(raw.seconds))
+
+    converged = property(lambda self: self._results[0])
+    iters = property(lambda self: self._results[1])
+    err = property(lambda self: self._results[2])
+    seconds = property(lambda self: self._results[3])
+
+
 class FemSolver:
     """Shared geometry/mesh/boundary handling (FemSolverWithMaskedMesh, fem_solver.hpp:145-241)."""
 
@@ -179,7 +247,7 @@ class FemSolver:
         # what empty_elements='default' means (fem_solver.hpp:182-188)
         self._algorithm = "cholesky" if self.DIM == 2 else "iterative"
         self._empty = "default"
-        self._iter = _core.IterParams()
+        self._iter = IterativeParams()
         self._initialized = False
 
     @property
@@ -249,6 +317,15 @@ class FemSolver:
         self._initialized = False
         if self._core is not None:
             self._core.invalidate()
+
+    def _run_core(self, fn, *args):
+        """Call a core compute with the current solver.iterative settings and copy the
+        solve's outcome back into solver.iterative."""
+        self._core.iter = self._iter
+        try:
+            return fn(*args)
+        finally:
+            self._iter._pull(self._core.iter)
 
     def initialize(self):
         if self._initialized:

@@ -111,7 +111,7 @@ class _DynamicBase(_sb.FemSolver):
             nodes = np.concatenate([nodes, self._isolated])
             vals = np.concatenate([vals, np.full(len(self._isolated), float(self.inittemp))])
         c.set_dirichlet(nodes.astype(np.int64), vals)
-        r = c.compute(float(time))
+        r = self._run_core(c.compute, float(time))
         self._T = None
         return r
 

@@ -12,6 +12,8 @@ from .. import mesh as _mesh
 from .. import solverbase as _sb
 from .._core import ThermalCore
 
+_POST_FLUX = 0   # PF_POST_FLUX
+
 
 class _BcValue:
     """Value of a convection / radiation boundary condition (therm_python.cpp:25-86):
@@ -79,7 +81,8 @@ def _setup_conductivity(solver):
     c = solver._core
     leaf_mats = [lf[0].material for lf in solver._geometry._leaves] + [solver._mat_bg]
     laws = [m.thermk_law() if m is not None else None for m in leaf_mats]
-    if all(l is not None for l in laws):
+    solver._laws_on_device = all(l is not None for l in laws)
+    if solver._laws_on_device:
         L = np.array(laws)[solver._leaf]
         L[solver._excl, 0], L[solver._excl, 1] = _sb.MASKED_K   # masked-out (empty) elements
         c.set_conductivity(L[:, 0].copy(), L[:, 1].copy(), L[:, 2].copy(), L[:, 3].copy())
@@ -164,7 +167,7 @@ class _StaticBase(_sb.FemSolver):
         cn, cv = _bc_lists(self.convection_boundary, self.mesh, self.geometry, Convection.of)
         c.set_convection(cn, np.array([v.coeff for v in cv], dtype=np.float64),
                          np.array([v.ambient for v in cv], dtype=np.float64))
-        err = c.compute(int(loops))
+        err = self._run_core(c.compute, int(loops))
         self._T = None
         return err
 
@@ -231,33 +234,25 @@ class _StaticBase(_sb.FemSolver):
         return _sb.nearest_elements(self._axes, k, mesh)
 
     def outHeatFlux(self, mesh, interpolation="linear"):
-        """Heat flux -k grad T (W/m^2); uses the leaf height for thermk as the reference
-        does (therm2d.cpp:525-531)."""
+        """Heat flux -k grad T (W/m^2) on the device (saveHeatFluxes, therm2d.cpp:504-540,
+        therm3d.cpp:342-384).  k is thermk at the final element temperatures with the
+        leaf's height (therm2d.cpp:525-531): the device law when every material has one
+        (no height dependence), else evaluated here and handed over.  Interpolated on the
+        element mesh on the device (getHeatFluxes, therm2d.cpp:556-569)."""
         if self._core is None or self._core.loopno == 0:
             return np.zeros((len(_sb.points_of(mesh)), self.DIM))
-        T = self._temps().reshape([len(a) for a in self._axes][::-1])
-        k = self._element_k(leaf_height=True)
-        s = self._esize
-        if self.DIM == 2:
-            gx = (-T[:-1, :-1] + T[:-1, 1:] - T[1:, :-1] + T[1:, 1:]) / s[0][None, :]
-            gy = (-T[:-1, :-1] - T[:-1, 1:] + T[1:, :-1] + T[1:, 1:]) / s[1][:, None]
-            fl = np.column_stack([-0.5e6 * k[:, 0] * gx.ravel(), -0.5e6 * k[:, 1] * gy.ravel()])
+        c = self._core
+        if getattr(self, "_laws_on_device", False):
+            fetch_k = (None, None)
         else:
-            def dif(ax):
-                sl0 = [slice(None, -1)] * 3
-                sl1 = [slice(None, -1)] * 3
-                tot = 0
-                for c in range(8):
-                    sl = tuple(slice(1, None) if (c >> (2 - a)) & 1 else slice(None, -1) for a in range(3))
-                    sign = 1.0 if (c >> (2 - ax)) & 1 else -1.0
-                    tot = tot + sign * T[sl]
-                return tot
-            gz = dif(0) / s[2][:, None, None]
-            gy = dif(1) / s[1][None, :, None]
-            gx = dif(2) / s[0][None, None, :]
-            fl = np.column_stack([-0.25e6 * k[:, 0] * gx.ravel(), -0.25e6 * k[:, 0] * gy.ravel(),
-                                  -0.25e6 * k[:, 1] * gz.ravel()])
-        return _sb.interp_elements(self._axes, fl, mesh, fill=0.0)
+            k = self._element_k(leaf_height=True)
+            fetch_k = (k[:, 0].copy(), k[:, 1].copy())
+        mids = [0.5 * (a[1:] + a[:-1]) for a in self._axes]
+        if isinstance(mesh, _mesh.Rectangular) and mesh.dim == self.DIM and _sb._same_axes(mesh.axes, mids):
+            return c.postprocess(_POST_FLUX, fetch_k[0], fetch_k[1], True).reshape(-1, self.DIM)
+        c.postprocess(_POST_FLUX, fetch_k[0], fetch_k[1], False)
+        pts = np.ascontiguousarray(_sb.points_of(mesh), dtype=np.float64).reshape(-1, self.DIM)
+        return c.interp_elements(_POST_FLUX, pts, 0.0).reshape(-1, self.DIM)
 
 
 class Static2D(_StaticBase):

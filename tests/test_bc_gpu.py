@@ -243,8 +243,7 @@ def test_apply_with_boundary_matrix_vs_oracle():
     c.set_source(None)
     c.set_dirichlet(dn, dv)
     rows, cols, vals = (np.array([e[i] for e in Kc]) for i in range(3))
-    assert c.L.pf_boundary_matrix_set(c.h, rows.astype(np.int64).ctypes.data, cols.astype(np.int64).ctypes.data,
-                                      vals.astype(np.float64).ctypes.data, len(vals)) == 0
+    c.set_boundary_matrix(rows, cols, vals)
     nz = np.nonzero(Fc)[0]
     c.set_load(nz, Fc[nz])
     K = A0.dense()
@@ -264,7 +263,7 @@ def test_apply_with_boundary_matrix_vs_oracle():
         T, st = c.solve(x0=np.full(g.nnodes, 300.), rtol=1e-12, precond=pc)
         assert np.abs(T - T_ref).max() <= 1e-9 * np.abs(T_ref).max(), (pc, st.iters)
     # clearing the matrix restores the plain operator
-    c.L.pf_boundary_matrix_set(c.h, None, None, None, 0)
+    c.set_boundary_matrix([], [], [])
     y0 = c.apply(x)
     assert np.abs(y0 - np.where(free, A0.dense() @ xm, x)).max() <= 1e-12 * np.abs(y0).max()
     c.close()
