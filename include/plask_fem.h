@@ -275,6 +275,12 @@ int pf_set_line_segment(pf_ctx* ctx, int seg);
  * no multigrid); a solver option. */
 int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omega);
 
+/* 3D multigrid: run the grid transfers inside the neighbouring kernels (restriction in a
+ * coarse level's first sweep, prolongation in the residual and sweep that follow it; on = 1)
+ * or as kernels of their own (0, the default: measured faster).  Same preconditioner
+ * either way.  No reference counterpart (tuning hook). */
+int pf_set_multigrid_fusion(pf_ctx* ctx, int on);
+
 /* Number of multigrid levels built (the solve grid included; 0 before the first
  * multigrid solve). */
 int pf_multigrid_levels(const pf_ctx* ctx);

@@ -30,7 +30,32 @@
 
 namespace pf {
 
-enum { MODE_OP = 0, MODE_RES = 1, MODE_RESG = 2 };  // MODE_RESG: MODE_RES, skipped after the stop
+// MODE_RESG: MODE_RES, skipped after the stop; MODE_RESP: MODE_RESG of x + P x_c, the
+// next coarser level's correction interpolated where x is read (MgProl)
+enum { MODE_OP = 0, MODE_RES = 1, MODE_RESG = 2, MODE_RESP = 3 };
+
+// Coarse-grid correction of a multigrid level read together with x: x + P x_c at the
+// free nodes, P the Q1 interpolation in the plane of the next coarser level (node i of
+// an in-plane axis lies in coarse interval [m[i], m[i] + 1] at relative position w[i]).
+template <typename T> struct MgProl {
+    int n0c = 0, n1c = 0;
+    const int* mx = nullptr;
+    const int* my = nullptr;
+    const T* wx = nullptr;
+    const T* wy = nullptr;
+    const T* xc = nullptr;
+};
+
+template <typename T>
+__device__ __forceinline__ T mg_prol_at(const MgProl<T>& pr, long long k, int i, int j) {
+    const int I = pr.mx[i], J = pr.my[j];
+    const int I1 = min(I + 1, pr.n0c - 1), J1 = min(J + 1, pr.n1c - 1);
+    const T a = pr.wx[i], b = pr.wy[j];
+    const T* c = pr.xc + k * pr.n0c * pr.n1c;
+    const T lo = (T(1) - a) * c[(long long)J * pr.n0c + I] + a * c[(long long)J * pr.n0c + I1];
+    const T hi = (T(1) - a) * c[(long long)J1 * pr.n0c + I] + a * c[(long long)J1 * pr.n0c + I1];
+    return (T(1) - b) * lo + b * hi;
+}
 
 // Scalars of one PCG solve, resident on the device.  Every kernel of the
 // iteration reads them; the last block of a reduction kernel writes them.
@@ -168,13 +193,14 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
                                                   T* __restrict__ pout, T* __restrict__ xsol, T* __restrict__ y,
                                                   const T* __restrict__ F, const uint64_t* __restrict__ dmask,
                                                   const T* __restrict__ kh, const T* __restrict__ kv,
-                                                  PcgState* __restrict__ st, double* __restrict__ partials) {
+                                                  PcgState* __restrict__ st, double* __restrict__ partials,
+                                                  MgProl<T> pr) {
     extern __shared__ unsigned char smem_raw[];
     const int BX = g.n0;
     const int PX = BX + 1;  // padded row: column BX holds zeros
     T* xs = reinterpret_cast<T*>(smem_raw);       // [2][BY+1][PX]
     T* oc = xs + 2 * (BY + 1) * PX;               // [2][4][BY][PX], corner outputs at column tx+1
-    if (FUSEP || DOT || MODE == MODE_RESG) {
+    if (FUSEP || DOT || MODE == MODE_RESG || MODE == MODE_RESP) {
         if (st->done) return;
     }
     // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a contiguous
@@ -226,7 +252,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     const int jc = min(max(j, 0), n1 - 1), jxc = min(max(j + 1, 0), n1 - 1);
     const int ejc = min(max(j, 0), n1 - 2), etx = min(tx, n0 - 2);
     struct Raw {
-        T z, p, zx, px, kxy, kz, x, f, hz;
+        T z, p, zx, px, kxy, kz, x, f, hz, pc, pcx;
         uint32_t dw, dwx;
     };
     auto gload = [&](int k, Raw& R) {
@@ -241,6 +267,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         }
         if (MODE != MODE_OP) R.f = F[node];
         R.dw = dmask32[node >> 5];
+        if constexpr (MODE == MODE_RESP) R.pc = mg_prol_at(pr, kc, tx, jc);
         if (ty == BY - 1) {
             const long long nx = (long long)kc * plane + (long long)jxc * n0 + tx;
             if (FUSEP) {
@@ -250,6 +277,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
                 R.zx = xin[nx];
             }
             R.dwx = dmask32[nx >> 5];
+            if constexpr (MODE == MODE_RESP) R.pcx = mg_prol_at(pr, kc, tx, jxc);
         }
         const int ek = max(kc - 1, 0);
         const long long e = (long long)ek * (n0 - 1) * (n1 - 1) + (long long)ejc * (n0 - 1) + etx;
@@ -268,6 +296,9 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
         if (nvalid) {
             v = FUSEP ? R.z + T(beta) * R.p : R.z;
             d = bit(R.dw, k, j);
+            if constexpr (MODE == MODE_RESP) {
+                if (!d) v += R.pc;
+            }
             if (FUSEP && store_p) {
                 const long long node = (long long)k * plane + (long long)j * n0 + tx;
                 pout[node] = v;
@@ -283,6 +314,9 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
             if (xrow) {
                 w = FUSEP ? R.zx + T(beta) * R.px : R.zx;
                 if (MODE == MODE_OP && bit(R.dwx, k, j + 1)) w = T(0);
+                if constexpr (MODE == MODE_RESP) {
+                    if (!bit(R.dwx, k, j + 1)) w += R.pcx;
+                }
             }
             xs[(buf * (BY + 1) + BY) * PX + tx] = w;
         }
@@ -305,7 +339,7 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     bool dir_cur, dir_next = false;
     // PFD planes in flight: R1 = plane P + 1, R2 = plane P + 2 (PFD == 2)
     Raw R1, R2;
-    R1.z = R1.p = R1.zx = R1.px = R1.kxy = R1.kz = R1.x = R1.f = R1.hz = T(0);
+    R1.z = R1.p = R1.zx = R1.px = R1.kxy = R1.kz = R1.x = R1.f = R1.hz = R1.pc = R1.pcx = T(0);
     R1.dw = R1.dwx = 0u;
     R2 = R1;
     __syncthreads();
@@ -1555,28 +1589,85 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
     }
 }
 
+// Grid transfer fused into a multigrid sweep (template parameter XF of k_mg_smooth):
+// XF 1, restriction: the sweep's right-hand side is R r_f of the finer level's residual
+// src (computed per node and stored into r); XF 2, prolongation: the sweep starts from
+// xin + P src, src the coarser level's correction.  dmask is the Dirichlet mask of the
+// level the sweep works on; the tables are those between the two levels (mg_kernels.hpp).
+template <typename T> struct MgXfer {
+    int n0f = 0, n1f = 0, n0c = 0, n1c = 0;
+    const int* cx = nullptr;
+    const int* cy = nullptr;
+    const int* mx = nullptr;
+    const int* my = nullptr;
+    const T* wx = nullptr;
+    const T* wy = nullptr;
+    const T* src = nullptr;
+    const uint64_t* dmask = nullptr;
+};
+
+// the (up to) 3 fine nodes of coarse node I along one axis and their restriction
+// weights (absent ones repeat the first index with weight 0)
+template <typename T>
+__device__ __forceinline__ void mg_restrict_axis(int I, int nc, const int* __restrict__ cidx,
+                                                 const int* __restrict__ m, const T* __restrict__ w, int (&idx)[3],
+                                                 T (&wt)[3]) {
+    const int lo = I > 0 ? cidx[I - 1] + 1 : 0;
+    const int hi = I < nc - 1 ? cidx[I + 1] - 1 : cidx[I];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i = lo + a;
+        const bool ok = i <= hi;
+        idx[a] = ok ? i : lo;
+        const int mm = m[idx[a]];
+        const T ww = w[idx[a]];
+        wt[a] = ok ? (mm == I ? T(1) - ww : (mm + 1 == I ? ww : T(0))) : T(0);
+    }
+}
+
 // Damped line-Jacobi sweep of the multigrid smoother, same partitioned line solve:
 // xout = xin + omega T^-1 r (xin null: start from zero; xin may be xout).
 // UPD: the PCG residual update r -= alpha q is applied (and stored) first.
 // FIN 1 / 2: this sweep yields the preconditioned PCG residual z = xout, and
 // (rp, z), (rp, rp) of the PCG residual rp end the iteration (1) or start a run (2).
-// Exits at once after the stop (st->done).
-template <typename T, int M, bool UPD, int FIN>
+// XF: a fused grid transfer (MgXfer).  Exits at once after the stop (st->done).
+template <typename T, int M, bool UPD, int FIN, int XF = 0>
 __global__ __launch_bounds__((PartLB<T, M>::value)) void k_mg_smooth(
     int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q, const T* xin,
     T* xout, const T* __restrict__ rp, T omega, const T* __restrict__ la, const T* __restrict__ lw,
     const T* __restrict__ lcp, const T* __restrict__ schur, PcgState* __restrict__ st,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, MgXfer<T> xf) {
     if (st->done) return;
     extern __shared__ __align__(16) unsigned char part_smem[];
     const PartPos pp = part_pos(ncol, n, P, CB, plane, M);
     T rv[M], av[M], wv[M], cv[M];
     const T alpha = UPD ? T(st->alpha) : T(0);
+    // transfer tables of this thread's column (XF 1: its fine support; XF 2: its coarse cell)
+    int ii[3] = {0, 0, 0}, jj[3] = {0, 0, 0};
+    T wi[3] = {T(0), T(0), T(0)}, wj[3] = {T(0), T(0), T(0)};
+    const int ccol = pp.act ? pp.col : 0;
+    if constexpr (XF == 1) {
+        mg_restrict_axis(ccol % xf.n0c, xf.n0c, xf.cx, xf.mx, xf.wx, ii, wi);
+        mg_restrict_axis(ccol / xf.n0c, xf.n1c, xf.cy, xf.my, xf.wy, jj, wj);
+    }
 #pragma unroll
     for (int u = 0; u < M; ++u) {
         if (u < pp.len) {
             const long long node = pp.base + (long long)u * plane;
-            rv[u] = r[node];
+            if constexpr (XF == 1) {
+                const long long k = (long long)pp.seg * M + u;
+                const T* src = xf.src + k * (long long)xf.n0f * xf.n1f;
+                T s = T(0);
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const T* row = src + (long long)jj[b] * xf.n0f;
+                    s += wj[b] * (wi[0] * row[ii[0]] + wi[1] * row[ii[1]] + wi[2] * row[ii[2]]);
+                }
+                rv[u] = dbit(xf.dmask, node) ? T(0) : s;
+                r[node] = rv[u];
+            } else {
+                rv[u] = r[node];
+            }
             av[u] = la[node];
             wv[u] = lw[node];
             cv[u] = lcp[node];
@@ -1589,11 +1680,26 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_mg_smooth(
     T zv[M];
     part_solve<T, M>(pp, P, CB, ncol, schur, reinterpret_cast<T*>(part_smem), rv, av, wv, cv, zv);
     double rz = 0., rr = 0.;
+    MgProl<T> pr;
+    if constexpr (XF == 2) {
+        pr.n0c = xf.n0c;
+        pr.n1c = xf.n1c;
+        pr.mx = xf.mx;
+        pr.my = xf.my;
+        pr.wx = xf.wx;
+        pr.wy = xf.wy;
+        pr.xc = xf.src;
+    }
 #pragma unroll
     for (int u = 0; u < M; ++u) {
         if (u < pp.len) {
             const long long node = pp.base + (long long)u * plane;
-            const T xv = (xin ? xin[node] : T(0)) + omega * zv[u];
+            T x0 = xin ? xin[node] : T(0);
+            if constexpr (XF == 2) {
+                if (!dbit(xf.dmask, node))
+                    x0 += mg_prol_at(pr, (long long)pp.seg * M + u, ccol % xf.n0f, ccol / xf.n0f);
+            }
+            const T xv = x0 + omega * zv[u];
             xout[node] = xv;
             if (FIN) {
                 const double g = (double)(UPD ? rv[u] : rp[node]);

@@ -132,6 +132,7 @@ struct pf_ctx {
     float *mg_r0_32 = nullptr;
     int mg_max_levels = 0;      // 0: coarsen until both in-plane axes have <= 3 nodes
     int mg_coarse_sweeps = 1;   // smoothing sweeps on the coarsest level
+    int mg_fuse = 0;            // 3D: grid transfers fused into the smoother / residual kernels (measured slower)
     double mg_omega = 0.65;     // damping of the line-Jacobi smoother
     // cached graphs of one chunk of iterations
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
@@ -836,23 +837,26 @@ template <typename T, int MODE> static void bnd_apply(pf_ctx* ctx, const T* x, T
 // the preconditioner sees the boundary matrix's diagonal only).
 template <typename T, int MODE, bool FUSEP, bool DOT>
 static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, T* pout, T* y, const T* F, const T* kh,
-                         const T* kv, T* xsol = nullptr, bool raw = false) {
+                         const T* kv, T* xsol = nullptr, bool raw = false, MgProl<T> pr = MgProl<T>()) {
     const int pk = sizeof(T) == 8 ? 0 : 1;
     const uint64_t* dmask = raw ? ctx->zmask : ctx->dmask;
     Axes3<T> g3 = axes3<T>(ctx);
     Axes2<T> g2 = axes2<T>(ctx);
     if (raw) g3.dsh = g2.dsh = nullptr;
-    if (ctx->dim == 3 && ctx->apply_kind[pk] == 1) {
+    // MODE_RESP (the multigrid residual with the coarse correction read in) exists in the
+    // 3D tile kernel only; mg_vcycle fuses the transfers only where that kernel runs
+    if (MODE != MODE_RESP && ctx->dim == 3 && ctx->apply_kind[pk] == 1) {
         const int n0 = (int)ctx->n[0], n1 = (int)ctx->n[1], n2 = (int)ctx->n[2];
         const int KR = ctx->KR[pk];
         const int nb = n1 * ((n2 + KR - 1) / KR);
         const int grid = (nb + 7) / 8 * 8;
         const int threads = (n0 + 63) / 64 * 64;
         const size_t shm = sizeof(T) * (size_t)(4 * (n0 + 1));
-        hipLaunchKernelGGL((k_apply3r<T, MODE, FUSEP, DOT>), dim3(grid), dim3(threads), shm, ctx->stream,
-                           g3, KR, nb, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
-                           ctx->partials);
-    } else if (ctx->dim == 3) {
+        if constexpr (MODE != MODE_RESP)
+            hipLaunchKernelGGL((k_apply3r<T, MODE, FUSEP, DOT>), dim3(grid), dim3(threads), shm, ctx->stream,
+                               g3, KR, nb, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
+                               ctx->partials);
+    } else if (MODE == MODE_RESP || ctx->dim == 3) {
         const int BX = (int)ctx->n[0], BY = ctx->BY[pk], KC = ctx->KC[pk];
         const int ytiles = (int)((ctx->n[1] + BY - 2) / (BY - 1));
         const int zt = (int)((ctx->n[2] + KC - 1) / KC);
@@ -861,17 +865,18 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
         if (ctx->pfd == 2)
             hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
                                g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
-                               ctx->partials);
+                               ctx->partials, pr);
         else
             hipLaunchKernelGGL((k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
                                g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
-                               ctx->partials);
+                               ctx->partials, pr);
     } else {
         const int BX = (int)ctx->n[0];
         int zt = (int)((ctx->n[1] + ctx->KC2 - 1) / ctx->KC2);
         size_t shm = sizeof(T) * (size_t)(4 * (BX + 1));
-        hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, g2,
-                           ctx->KC2, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st, ctx->partials);
+        if constexpr (MODE != MODE_RESP)
+            hipLaunchKernelGGL((k_apply2d<T, MODE, FUSEP, DOT>), dim3(zt), dim3(BX), shm, ctx->stream, g2,
+                               ctx->KC2, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st, ctx->partials);
     }
     if constexpr (!FUSEP && !DOT && (MODE == MODE_OP || MODE == MODE_RES)) {
         if (ctx->have_bnd && !raw) bnd_apply<T, MODE>(ctx, xin, y, nullptr);
@@ -1401,9 +1406,13 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
 
 // r = b - A x on level l (skipped after the stop)
 template <typename T>
-static void mg_residual_launch(pf_ctx* ctx, int l, const MgView<T>& v) {
+static void mg_residual_launch(pf_ctx* ctx, int l, const MgView<T>& v, MgProl<T> pr = MgProl<T>()) {
     if (l == 0) {
-        launch_apply<T, MODE_RESG, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv);
+        if (pr.xc)
+            launch_apply<T, MODE_RESP, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv,
+                                                     nullptr, false, pr);
+        else
+            launch_apply<T, MODE_RESG, false, false>(ctx, v.x, nullptr, nullptr, nullptr, v.r, v.b, v.kh, v.kv);
         return;
     }
     if (ctx->dim == 2) {
@@ -1428,15 +1437,24 @@ static void mg_residual_launch(pf_ctx* ctx, int l, const MgView<T>& v) {
     while (KC > 1 && (long long)ytiles * ((n2 + KC - 1) / KC) < 1024) KC >>= 1;
     const int nb = ytiles * ((n2 + KC - 1) / KC), grid = (nb + 7) / 8 * 8;
     const size_t shm = tile_lds(BY, n0) / sizeof(double) * sizeof(T);
-    hipLaunchKernelGGL((k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream, v.g,
-                       BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr, (const T*)nullptr,
-                       (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv, ctx->st, ctx->partials);
+    if (pr.xc)
+        hipLaunchKernelGGL((k_apply3d<T, MODE_RESP, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream,
+                           v.g, BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr,
+                           (const T*)nullptr, (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv,
+                           ctx->st, ctx->partials, pr);
+    else
+        hipLaunchKernelGGL((k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream,
+                           v.g, BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr,
+                           (const T*)nullptr, (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv,
+                           ctx->st, ctx->partials, pr);
 }
 
 template <typename T>
-static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
-    const int hk = prof_kern<T>(ctx, PF_K_RESIDUAL, l, bytes_residual(v.nodes(), v.elems(ctx->dim), (int)sizeof(T)));
-    mg_residual_launch<T>(ctx, l, v);
+static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v, MgProl<T> pr = MgProl<T>(), long long ncoarse = 0) {
+    const double extra = pr.xc ? (double)sizeof(T) * (double)ncoarse : 0.;  // the coarse correction read
+    const int hk =
+        prof_kern<T>(ctx, PF_K_RESIDUAL, l, bytes_residual(v.nodes(), v.elems(ctx->dim), (int)sizeof(T)) + extra);
+    mg_residual_launch<T>(ctx, l, v, pr);
     prof_close(ctx, hk);
 }
 
@@ -1444,24 +1462,31 @@ static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v) {
 template <typename T, int M>
 static void mg_smooth_m(pf_ctx* ctx, dim3 grid, int threads, size_t lds, const PartShape& ps, int ncol, int n,
                         long long plane, bool upd, int fin, T* r, const T* q, const T* xin, T* xout, const T* rp,
-                        T omega, const MgView<T>& v) {
-#define PF_MGS(U, F)                                                                                              \
-    hipLaunchKernelGGL((k_mg_smooth<T, M, U, F>), grid, dim3(threads), lds, ctx->stream, ncol, n, ps.P, ps.CB,    \
-                       plane, r, q, xin, xout, rp, omega, v.la, v.lw, v.lcp, v.schur, ctx->st, ctx->partials)
-    if (upd) {
-        if (fin == 0) PF_MGS(true, 0);
-        else PF_MGS(true, 1);
+                        T omega, const MgView<T>& v, int xfk, const MgXfer<T>& xf) {
+#define PF_MGS(U, F, X)                                                                                          \
+    hipLaunchKernelGGL((k_mg_smooth<T, M, U, F, X>), grid, dim3(threads), lds, ctx->stream, ncol, n, ps.P, ps.CB, \
+                       plane, r, q, xin, xout, rp, omega, v.la, v.lw, v.lcp, v.schur, ctx->st, ctx->partials, xf)
+    if (xfk == 1) {  // coarse levels: first sweep from zero, restriction fused
+        PF_MGS(false, 0, 1);
+    } else if (xfk == 2) {  // post-smoothing with the prolongation fused
+        if (fin == 0) PF_MGS(false, 0, 2);
+        else if (fin == 1) PF_MGS(false, 1, 2);
+        else PF_MGS(false, 2, 2);
+    } else if (upd) {
+        if (fin == 0) PF_MGS(true, 0, 0);
+        else PF_MGS(true, 1, 0);
     } else {
-        if (fin == 0) PF_MGS(false, 0);
-        else if (fin == 1) PF_MGS(false, 1);
-        else PF_MGS(false, 2);
+        if (fin == 0) PF_MGS(false, 0, 0);
+        else if (fin == 1) PF_MGS(false, 1, 0);
+        else PF_MGS(false, 2, 0);
     }
 #undef PF_MGS
 }
 
 template <typename T>
 static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, const T* q, const T* xin, T* xout,
-                      const T* rp, T omega = T(-1)) {
+                      const T* rp, T omega = T(-1), int xfk = 0, const MgXfer<T>& xf = MgXfer<T>(),
+                      double xbytes = 0.) {
     const int ncol = v.ncol, n = v.planes;
     PartShape ps = part_shape<T>(v.pm, n, ncol);
     // small coarse levels: fewer columns per block, so that the few columns spread over
@@ -1474,21 +1499,58 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
     const T om = omega > T(0) ? omega : T(ctx->mg_omega);
     const int hk = prof_kern<T>(ctx, PF_K_SMOOTH, v.level,
-                                bytes_smooth(v.nodes(), ncol, ps.P, (int)sizeof(T), upd, xin != nullptr, fin != 0));
+                                bytes_smooth(v.nodes(), ncol, ps.P, (int)sizeof(T), upd, xin != nullptr, fin != 0) +
+                                    xbytes);
     switch (ps.m) {
-        case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
-        case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
-        case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
-        case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v); break;
+        case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
+        case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
+        case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
+        case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
     }
     prof_close(ctx, hk);
 }
 
+// tables of the transfer between level t and t + 1 with a source vector, for the fused forms
+template <typename T>
+static MgXfer<T> mg_xfer(pf_ctx* ctx, int t, const MgView<T>& coarse, const T* src, const uint64_t* dmask) {
+    const MgTransfer tr = mg_transfer(ctx, (size_t)t);
+    MgXfer<T> x;
+    x.n0f = tr.n0f;
+    x.n1f = tr.n1f;
+    x.n0c = tr.n0c;
+    x.n1c = tr.n1c;
+    x.cx = tr.cx;
+    x.cy = tr.cy;
+    x.mx = tr.mx;
+    x.my = tr.my;
+    x.wx = coarse.wx;
+    x.wy = coarse.wy;
+    x.src = src;
+    x.dmask = dmask;
+    return x;
+}
+
+template <typename T> static MgProl<T> mg_prol(const MgXfer<T>& x) {
+    MgProl<T> p;
+    p.n0c = x.n0c;
+    p.n1c = x.n1c;
+    p.mx = x.mx;
+    p.my = x.my;
+    p.wx = x.wx;
+    p.wy = x.wy;
+    p.xc = x.src;
+    return p;
+}
+
 // z = V(r): one V-cycle.  upd: apply r -= alpha q first (inside the first sweep);
 // fin: 1 the last sweep ends the PCG iteration, 2 it starts a PCG run.
+// 3D with the tile operator kernel: the grid transfers run inside the neighbouring
+// kernels (restriction in a coarse level's first sweep, prolongation in the residual and
+// the sweep that follow it), which saves two launches and a vector pass per level.
 template <typename T>
 static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
     const int L = (int)ctx->mg.size() + 1;
+    const bool fuse = ctx->mg_fuse && ctx->dim == 3 && L > 1 && ctx->apply_kind[sizeof(T) == 8 ? 0 : 1] == 0;
     // profiled chunks: level-0 kernels by kind, all coarse-level work as one record
     auto cat = [](int l, int kind) { return l == 0 ? kind : -1; };
     auto open = [&](int c) { return c >= 0 ? prof_open(ctx, c) : -1; };
@@ -1506,31 +1568,52 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
                 prof_close(ctx, h);
             }
             const int h = open(cat(l, PF_KCAT_MG_SMOOTH));
-            if (s == 0)
+            if (s == 0 && fuse && l > 0) {
+                // b = R r of the finer level, read and stored by the sweep itself
+                const MgView<T> fv = mg_view<T>(ctx, l - 1, r, z);
+                const MgXfer<T> xf = mg_xfer<T>(ctx, l - 1, v, fv.r, v.dmask);
+                mg_smooth<T>(ctx, v, false, f, v.b, nullptr, nullptr, v.x, r, T(-1), 1, xf,
+                             (double)sizeof(T) * (double)fv.nodes());
+            } else if (s == 0) {
                 mg_smooth<T>(ctx, v, l == 0 && upd, f, v.b, q, nullptr, v.x, r);
-            else
+            } else {
                 mg_smooth<T>(ctx, v, false, f, v.r, nullptr, v.x, v.x, r);
+            }
             prof_close(ctx, h);
         }
         if (!last) {
             int h = open(cat(l, PF_KCAT_MG_RESIDUAL));
             mg_residual<T>(ctx, l, v);
             prof_close(ctx, h);
-            const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
-            const MgTransfer tr = mg_transfer(ctx, (size_t)l);
-            h = open(cat(l, PF_KCAT_MG_TRANSFER));
-            const int hk = prof_kern<T>(ctx, PF_K_RESTRICT, l,
-                                        sizeof(T) * (double)(v.nodes() + c.nodes()) + mask_bytes(c.nodes()));
-            hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0, ctx->stream,
-                               tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
-            prof_close(ctx, hk);
-            prof_close(ctx, h);
+            if (!fuse) {
+                const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
+                const MgTransfer tr = mg_transfer(ctx, (size_t)l);
+                h = open(cat(l, PF_KCAT_MG_TRANSFER));
+                const int hk = prof_kern<T>(ctx, PF_K_RESTRICT, l,
+                                            sizeof(T) * (double)(v.nodes() + c.nodes()) + mask_bytes(c.nodes()));
+                hipLaunchKernelGGL((k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0,
+                                   ctx->stream, tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
+                prof_close(ctx, hk);
+                prof_close(ctx, h);
+            }
         }
     }
     for (int l = L - 2; l >= 0; --l) {
         if (l == 0) prof_close(ctx, hc);
         const MgView<T> v = mg_view<T>(ctx, l, r, z);
         const MgView<T> c = mg_view<T>(ctx, l + 1, r, z);
+        if (fuse) {
+            // r = b - A (x + P x_c), then x = (x + P x_c) + omega M^-1 r
+            const MgXfer<T> xf = mg_xfer<T>(ctx, l, c, c.x, v.dmask);
+            int h = open(cat(l, PF_KCAT_MG_RESIDUAL));
+            mg_residual<T>(ctx, l, v, mg_prol(xf), c.nodes());
+            prof_close(ctx, h);
+            h = open(cat(l, PF_KCAT_MG_SMOOTH));
+            mg_smooth<T>(ctx, v, false, l == 0 ? fin : 0, v.r, nullptr, v.x, v.x, r, T(-1), 2, xf,
+                         (double)sizeof(T) * (double)c.nodes());
+            prof_close(ctx, h);
+            continue;
+        }
         const MgTransfer tr = mg_transfer(ctx, (size_t)l);
         const long long nf = (long long)v.ncol * v.planes;
         int h = open(cat(l, PF_KCAT_MG_TRANSFER));
@@ -2775,6 +2858,15 @@ int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omeg
     return PF_OK;
 }
 
+int pf_set_multigrid_fusion(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    if (ctx->stream) HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->mg_fuse = on ? 1 : 0;
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
 int pf_multigrid_levels(const pf_ctx* ctx) { return ctx && ctx->mg_built ? (int)ctx->mg.size() + 1 : 0; }
 
 int pf_set_apply(pf_ctx* ctx, int precision, int kind, int planes) {
@@ -2827,26 +2919,39 @@ int pf_set_profile(pf_ctx* ctx, int on) {
     return PF_OK;
 }
 
-// Plain streaming copy b = a of `bytes` bytes (16 bytes per thread and step):
-// the achievable HBM bandwidth of the device, and the calibration kernel for
-// hardware traffic counters.
-__global__ void k_stream_copy(long long n16, const double2* __restrict__ a, double2* __restrict__ b) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
-        b[i] = a[i];
+// Device copy b = a, the achievable-bandwidth probe and the calibration kernel for the
+// hardware traffic counters: four independent 16-byte loads per thread in flight before
+// their stores (grid-stride, coalesced), non-temporal so the copy keeps no lines in L2.
+typedef double pf_d2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_stream_copy(long long n16, const pf_d2* __restrict__ a,
+                                                     pf_d2* __restrict__ b) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const pf_d2 v0 = __builtin_nontemporal_load(a + i);
+        const pf_d2 v1 = __builtin_nontemporal_load(a + i + stride);
+        const pf_d2 v2 = __builtin_nontemporal_load(a + i + 2 * stride);
+        const pf_d2 v3 = __builtin_nontemporal_load(a + i + 3 * stride);
+        __builtin_nontemporal_store(v0, b + i);
+        __builtin_nontemporal_store(v1, b + i + stride);
+        __builtin_nontemporal_store(v2, b + i + 2 * stride);
+        __builtin_nontemporal_store(v3, b + i + 3 * stride);
+    }
+    for (; i < n16; i += stride) b[i] = a[i];
 }
 
 int pf_stream_copy(pf_ctx* ctx, int64_t bytes, int reps, double* seconds) {
     if (!ctx || bytes < 16 || reps <= 0 || !seconds) return PF_EINVAL;
     HIPCK(hipSetDevice(ctx->device));
     const long long n16 = bytes / 16;
-    double2 *a = nullptr, *b = nullptr;
+    pf_d2 *a = nullptr, *b = nullptr;
     HIPCK(hipMalloc((void**)&a, 16 * (size_t)n16));
     if (hipMalloc((void**)&b, 16 * (size_t)n16) != hipSuccess) {
         hipFree(a);
         return fail(ctx, PF_EDEVICE, "pf_stream_copy: out of device memory");
     }
     hipMemsetAsync(a, 0, 16 * (size_t)n16, ctx->stream);
-    const int blocks = grid1(n16, 256, 1 << 16);
+    const int blocks = grid1((n16 + 3) / 4, 256, 1 << 16);
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);

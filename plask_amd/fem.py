@@ -30,12 +30,12 @@ EXPORTED = [
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
     "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
     "pf_capacity_set", "pf_dynamic_step", "pf_boundary_matrix_set", "pf_postprocess", "pf_interp_elements",
-    "pf_junction_state", "pf_junction_cond_set",
+    "pf_junction_state", "pf_junction_cond_set", "pf_set_multigrid_fusion",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
 KERNEL_NAMES = {0: "apply (k_apply3d/k_apply2d MODE_OP, fused PCG step)",
-                1: "residual (k_apply3d/k_apply2d MODE_RESG)",
+                1: "residual (k_apply3d/k_apply2d MODE_RESG, MODE_RESP)",
                 2: "line sweep (k_mg_smooth)", 3: "restriction (k_mg_restrict)",
                 4: "prolongation (k_mg_prolong_add)", 5: "line preconditioner (k_cg_part)",
                 6: "jacobi update (k_cg_update)", 7: "k_cg_r_to_f32", 8: "k_cg_z_from_f32"}
@@ -125,6 +125,7 @@ def load_library(path=LIB_PATH):
     L.pf_capacity_set.argtypes = [P, P, i64, C.c_double]
     L.pf_boundary_matrix_set.argtypes = [P, P, P, P, i64]
     L.pf_postprocess.argtypes = [P, I, P, P, P]
+    L.pf_set_multigrid_fusion.argtypes = [P, I]
     L.pf_interp_elements.argtypes = [P, I, P, i64, C.c_double, P]
     L.pf_dynamic_step.argtypes = [P, C.POINTER(SolveOpts), I, C.POINTER(SolveStats)]
     _lib = L
@@ -382,6 +383,10 @@ class FemContext:
         by the next multigrid solve."""
         self._ck(self.L.pf_set_multigrid(self.h, int(max_levels), int(coarse_sweeps), float(omega)),
                  "pf_set_multigrid")
+
+    def set_multigrid_fusion(self, on):
+        """3D: grid transfers inside the smoother / residual kernels (True) or separate."""
+        self._ck(self.L.pf_set_multigrid_fusion(self.h, 1 if on else 0), "pf_set_multigrid_fusion")
 
     def precond_apply(self, r, precision=PF_FP64, precond=PRECOND_LINE):
         """z = M^-1 r for one preconditioner (test hook; host buffers)."""

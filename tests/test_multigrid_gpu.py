@@ -43,12 +43,15 @@ def _levels(n0, n1):
     return lv
 
 
+@pytest.mark.parametrize("fuse", [False, True])
 @pytest.mark.parametrize("shape", [(9, 7, 11), (17, 9, 13), (33, 40, 30), (10, 6, 257), (6, 5, 3)])
-def test_vcycle_matches_oracle(shape):
-    """z = V r on the GPU (FP64 and FP32) vs the CPU statement of the same V-cycle."""
+def test_vcycle_matches_oracle(shape, fuse):
+    """z = V r on the GPU (FP64 and FP32) vs the CPU statement of the same V-cycle, with the
+    grid transfers as kernels of their own and fused into the sweeps / residuals."""
     g, axes, kh, kv, q, dn, dv = _rand_problem(shape, seed=21 + shape[0])
-    print("vcycle", shape, flush=True)
+    print("vcycle", shape, "fuse", fuse, flush=True)
     c = _ctx(axes, kh, kv, q, dn, dv)
+    c.set_multigrid_fusion(fuse)
     vc = MG.VCycle(axes, kh, kv, dn)
     r = np.random.default_rng(4).uniform(-1, 1, g.nnodes)
     r[dn] = 0.0   # PCG residuals vanish at Dirichlet nodes
@@ -76,6 +79,23 @@ def test_mg_solve_matches_band_cholesky(shape, precision):
     rel = np.abs(T - T_ref).max() / np.abs(T_ref).max()
     assert rel <= 1e-7, (precision, rel)
     c.close()
+
+
+@pytest.mark.parametrize("precision", [PF_FP64, PF_FP32_REFINE, PF_FP64_PREC32])
+def test_fused_transfers_give_the_same_solve(precision):
+    """The fused-transfer V-cycle is the same preconditioner: same iterations, same
+    solution to rounding."""
+    g, axes, kh, kv, q, dn, dv = _rand_problem((33, 40, 70), seed=12)
+    print("mg fused solve", precision, flush=True)
+    out = []
+    for fuse in (False, True):
+        c = _ctx(axes, kh, kv, q, dn, dv)
+        c.set_multigrid_fusion(fuse)
+        out.append(c.solve(x0=np.full(g.nnodes, 300.0), precision=precision, rtol=1e-10, precond=PRECOND_MG))
+        c.close()
+    (T0, s0), (T1, s1) = out
+    assert s0.iters == s1.iters
+    assert np.abs(T1 - T0).max() <= 1e-9 * np.abs(T0).max()
 
 
 def test_mg_iterations_on_layered_stack():

@@ -1,10 +1,11 @@
 """Device post-processing and providers (pf_postprocess, pf_interp_elements) on the GPU.
 
 * heat flux (saveHeatFluxes, solvers/thermal/static/therm2d.cpp:504-540,
-  therm3d.cpp:342-384): in a layered slab with a flux v through its top and a held bottom
-  the Q1 solution is piecewise linear, so every element's flux is exactly (0, -v) whatever
-  the layers' conductivities -- for device laws and for conductivities handed over
-  (thickness-dependent materials, evaluated with the leaf height as the reference does);
+  therm3d.cpp:342-384): in a layered slab with a flux through one face and the opposite
+  face held the Q1 solution is piecewise linear, so every element's flux equals the
+  applied one whatever the layers' conductivities -- for device laws and for
+  conductivities handed over (thickness-dependent materials, evaluated with the leaf
+  height as the reference does);
 * element fields at points: linear fields are reproduced exactly between midpoints and
   clamped to the outermost midpoint up to the mesh boundary (against SciPy's
   RegularGridInterpolator on clamped points), fill outside;
@@ -57,22 +58,32 @@ def _slab(cls, law=True):
 @pytest.mark.parametrize("law", [True, False])
 @pytest.mark.parametrize("cls", [Static2D, Static3D])
 def test_heat_flux_is_the_applied_flux(cls, law):
-    v = -2.0e5   # W/m^2 into the slab through the top
+    """2D: a flux v (W/m^2, outward positive) on the top, the bottom held: -k dT/dz = v in
+    every element.  3D: the reference puts a wall's flux on the element's local nodes 0-3
+    (therm3d.cpp:160-163), which is the geometric face only for a bottom wall, so there the
+    flux enters at the bottom and leaves through the held top: -k dT/dz = -v."""
+    v = -2.0e5   # heat into the slab
     s = _slab(cls, law)
     print("flux", cls.__name__, "law", law, flush=True)
-    s.heatflux_boundary.append(s.mesh.Top(), v)
+    if cls is Static3D:
+        s.temperature_boundary.clear()
+        s.temperature_boundary.append(s.mesh.Top(), 300.)
+        s.heatflux_boundary.append(s.mesh.Bottom(), v)
+        want = -v
+    else:
+        s.heatflux_boundary.append(s.mesh.Top(), v)
+        want = v
     s.compute(1)
     em = s.mesh.element_mesh()
     q = s.outHeatFlux(em)
     assert q.shape == (len(em), s.DIM)
-    # the flux leaves through the held bottom: -k dT/dz = v everywhere
-    assert np.abs(q[:, -1] - v).max() <= 1e-6 * abs(v)
+    assert np.abs(q[:, -1] - want).max() <= 1e-6 * abs(v)
     assert np.abs(q[:, :-1]).max() <= 1e-6 * abs(v)
     # the same at arbitrary points (device interpolation of the element field)
     rng = np.random.default_rng(1)
     pts = rng.uniform(0.0, 1.0, (200, s.DIM)) * np.array([a[-1] for a in s.mesh.axes])
     qp = s.outHeatFlux(pts)
-    assert np.abs(qp[:, -1] - v).max() <= 1e-6 * abs(v)
+    assert np.abs(qp[:, -1] - want).max() <= 1e-6 * abs(v)
 
 
 def test_heat_flux_matches_the_gradient_through_the_c_abi():
@@ -147,5 +158,6 @@ def test_shockley_fields_against_the_oracle():
     fh = RegularGridInterpolator(mids, heat.reshape(len(mids[1]), len(mids[0])).T, method="linear")
     assert np.abs(hp - fh(clipped)).max() <= 1e-9 * np.abs(heat).max()
     # the same fields through the C ABI's post-processing entry
-    assert np.abs(s._core.postprocess(PF_POST_CURRENT, None, None, True) - cur).max() <= 1e-9 * np.abs(cur).max()
+    got = s._core.postprocess(PF_POST_CURRENT, None, None, True).reshape(-1, 2)
+    assert np.abs(got - cur.reshape(-1, 2)).max() <= 1e-9 * np.abs(cur).max()
     assert np.abs(s._core.postprocess(PF_POST_JOULE, None, None, True) - heat).max() <= 1e-9 * np.abs(heat).max()
