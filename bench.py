@@ -19,8 +19,11 @@ timed on this host: the reference's NSPCG (CG + IC(0)) compiled from its sources
 build's serial IC(0)-PCG restatement, both on the reference's 14-diagonal storage.
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,7 +35,7 @@ sys.path.insert(0, ROOT)
 from plask_amd import workloads  # noqa: E402
 from plask_amd.fem import (KCAT_APPLY, KCAT_MG_COARSE, KCAT_MG_RESIDUAL, KCAT_MG_SMOOTH,  # noqa: E402
                            KCAT_MG_TRANSFER, KCAT_PRECOND, PF_FP32_REFINE, PF_FP32_RELIABLE, PF_FP64, PF_FP64_PREC32,
-                           PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG, FemContext)
+                           PRECOND_JACOBI, PRECOND_LINE, PRECOND_MG, KERNEL_NAMES, FemContext)
 
 
 HBM_PEAK_GBS = 8000.0
@@ -113,6 +116,77 @@ def per_kernel(kstats):
         k["share"] = k["seconds"] / max(total, 1e-30)
     coarse = sum(r["seconds"] for r in kstats if r["level"] > 0)
     return dict(kernels=out, total_s=total, coarse_share=coarse / max(total, 1e-30))
+
+
+def _kernel_class(name):
+    """Kernel class of a dispatch (the names pf_kernel_stats reports), or None for kernels
+    outside the PCG iterations (set-up, initial residual, norms, copies)."""
+    for key, k in (("k_mg_smooth", 2), ("k_mg_restrict", 3), ("k_mg_prolong_add", 4), ("k_cg_part", 5),
+                   ("k_cg_update", 6)):
+        if key in name:
+            return KERNEL_NAMES[k]
+    m = re.search(r"k_apply(?:3d|2d)<[^,]+,\s*(\d)", name)
+    if m:
+        return {0: KERNEL_NAMES[0], 2: KERNEL_NAMES[1], 3: KERNEL_NAMES[1]}.get(int(m.group(1)))
+    return None
+
+
+def _pmc_dispatches(folder, counter):
+    """(kernel name, grid size, value) of every dispatch in a rocprofv3 --pmc csv output
+    folder, in dispatch order."""
+    rows = []
+    for f in sorted(glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r.get("Dispatch_Id", len(rows))), r["Kernel_Name"], int(r["Grid_Size"]),
+                                 float(r["Counter_Value"])))
+    rows.sort()
+    return [r[1:] for r in rows]
+
+
+def traffic_from(fetch_dir, write_dir, size, copy_bytes=1 << 30):
+    """HBM bytes per launch of each kernel class from two rocprofv3 --pmc runs of
+    `bench.py --counters` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each counter
+    scaled by what it reads for the k_stream_copy launches of known size.  Dispatches
+    that exit at once after the stop (below 1 % of the largest value of the same kernel
+    and grid) are left out; a class's figure is the mean over its launches on every
+    multigrid level, as the roofline's achieved bytes are."""
+    out = {"workload": "thermal3d_%dx%dx%d" % size,
+           "unit": "HBM bytes per launch, mean over the launches of all multigrid levels",
+           "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE -- python bench.py --counters "
+                     "(one FP64 multigrid solve); python bench.py --traffic-from FETCH_DIR WRITE_DIR",
+           "calibration": {"kernel": "k_stream_copy, %d bytes read + %d written" % (copy_bytes, copy_bytes)},
+           "kernels": {}, "per_grid": {}}
+    per = {}
+    for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_dispatches(folder, counter)
+        copy = [v for n, g, v in rows if "k_stream_copy" in n]
+        if not copy:
+            raise SystemExit("traffic_from: no k_stream_copy dispatch in %s" % folder)
+        factor = copy_bytes / 1024.0 / float(np.median(copy))
+        out["calibration"][counter + "_KiB"] = float(np.median(copy))
+        out["calibration"][counter.split("_")[0].lower() + "_factor"] = round(factor, 4)
+        groups = {}
+        for n, g, v in rows:
+            c = _kernel_class(n)
+            if c is not None:
+                groups.setdefault((c, n, g), []).append(v)
+        for (c, n, g), vals in groups.items():
+            top = max(vals)
+            kept = [v for v in vals if v >= 0.01 * top] if top > 0 else vals
+            rec = per.setdefault((c, n, g), {"launches": len(kept)})
+            rec[counter] = 1024.0 * factor * float(np.mean(kept))
+    for (c, n, g), rec in sorted(per.items(), key=lambda kv: (kv[0][0], -kv[0][2])):
+        b = rec.get("FETCH_SIZE", 0.0) + rec.get("WRITE_SIZE", 0.0)
+        out["per_grid"].setdefault(c, []).append({"kernel": n, "grid": g, "launches": rec["launches"],
+                                                  "read_bytes": int(rec.get("FETCH_SIZE", 0.0)),
+                                                  "written_bytes": int(rec.get("WRITE_SIZE", 0.0)),
+                                                  "bytes": int(b)})
+    for c, lst in out["per_grid"].items():
+        n = sum(r["launches"] for r in lst)
+        out["kernels"][c] = int(sum(r["launches"] * r["bytes"] for r in lst) / max(n, 1))
+    return out
 
 
 def roofline_line(by_kernel, size, copy_gbs):
@@ -375,7 +449,14 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="short run for hardware-counter profiles: stream-copy calibration, one FP64 "
                          "multigrid solve")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc output folders of --counters runs (FETCH_SIZE, "
+                         "WRITE_SIZE) into HBM bytes per launch per kernel class (JSON on stdout)")
     args = ap.parse_args()
+    if args.traffic_from:
+        size = tuple(int(v) for v in args.size.split(","))
+        print(json.dumps(traffic_from(args.traffic_from[0], args.traffic_from[1], size), indent=1))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the solve path has no multi-GPU "
               "decomposition); use --gpus 1", file=sys.stderr)

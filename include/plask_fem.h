@@ -86,11 +86,12 @@ typedef struct pf_ctx pf_ctx;
                             zero potential, electr2d.cpp:176), i.e. ||b|| of the correction system */
 #define PF_NORM_B 1      /* ||B|| over the free rows */
 #define PF_NORM_B_FULL 2 /* ||B|| including the Dirichlet rows (B exactly as assembled by the reference) */
-#define PF_NORM_START 3  /* max(||B - A X_start||, 1e-4 ||B - A X_init||), X_start the start vector of this
-                            solve: the start residual reduced by rtol, the way a Picard loop that warm-starts
-                            each linear solve from the previous loop's solution matches a direct solve per
-                            loop (the 1e-4 floor keeps a converged loop from asking for more than FP64 CG
-                            attains, ~3e-14 ||B - A X_init|| on the 3D junction systems) */
+#define PF_NORM_START 3  /* max(||B - A X_start||, f ||B - A X_init||), f = max(1e-4, 1e-13 / rtol), X_start
+                            the start vector of this solve: the start residual reduced by rtol, the way a
+                            Picard loop that warm-starts each linear solve from the previous loop's solution
+                            matches a direct solve per loop (the floor keeps a converged loop from asking for
+                            more than FP64 CG attains, ~3e-14 ||B - A X_init|| on the 3D junction systems:
+                            the target never goes below 1e-13 ||B - A X_init||) */
 
 typedef struct {
     int precision;    /* PF_FP64 | PF_FP32_REFINE | PF_FP32_RELIABLE | PF_FP64_PREC32 */
@@ -362,10 +363,26 @@ int pf_element_means(pf_ctx* ctx, double* out);
 
 /* Temperature-dependent conductivity law per element, k = k0 (300 / T_avg)^alpha
  * (alpha = 0: constant); pf_coeffs_update evaluates it on the device from the current
- * solution, replacing Material::thermk(T) calls of setMatrix (therm3d.cpp:211-213). */
+ * solution, replacing Material::thermk(T) calls of setMatrix (therm3d.cpp:211-213).  An
+ * element whose k0_h is NaN keeps the conductivities it has (junction elements, whose
+ * conductivity the junction update owns: loadConductivities, electr2d.cpp:331-353). */
 int pf_coeffs_powerlaw(pf_ctx* ctx, int equation, const double* k0_h, const double* k0_v, const double* a_h,
                        const double* a_v, int64_t nelem);
 int pf_coeffs_update(pf_ctx* ctx);
+
+/* The same law evaluated at the element temperatures of another context's solution on the
+ * same grid and device, device to device: the electrical solver's conductivities at the
+ * thermal solver's temperatures (ElectricalFem3DSolver::loadConductivities reading
+ * inTemperature, electr3d.cpp:324-345, with inTemperature = thermal outTemperature,
+ * thermoelectric.py:69-84; the linear interpolation of a same-mesh temperature at element
+ * midpoints is the corner mean the law uses). */
+int pf_coeffs_update_from(pf_ctx* ctx, const pf_ctx* temperature_src);
+
+/* Element heat source of dst from another context's field on the same grid and device,
+ * device to device.  what = PF_POST_JOULE: the Joule heat density of src's last
+ * pf_currents (thermal.inHeat = electrical.outHeat, thermoelectric.py:69-84; the element
+ * field read at its own element midpoints is the value itself). */
+int pf_source_from(pf_ctx* dst, const pf_ctx* src, int what);
 
 /* Transient thermal solve (DynamicThermalFem2DSolver / DynamicThermalFem3DSolver,
  * solvers/thermal/dynamic/femT2d.cpp, femT3d.cpp: the theta method of setMatrix

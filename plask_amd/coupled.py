@@ -11,7 +11,9 @@ U[1.4, 2.2] V on the top nodes with x^2 + y^2 <= 10^2 um^2, 0 V and 300 K at the
 bottom.  The loop is ``outer`` iterations of (electrical.compute(6),
 thermal.compute(1)) as ThermoElectric.compute (thermoelectric.py:209-211), run once
 with FP64 PCG and once with FP32 PCG + FP64 refinement; the final temperatures are
-compared (and a third run with FP64 CG and the preconditioner in FP32).  The 3D solvers
+compared (and a third run with FP64 CG and the preconditioner in FP32).  The coupled fields
+stay on the device (ThermoElectric.link); a fourth run passes them through the providers'
+host arrays, for comparison.  The 3D solvers
 precondition with the multigrid V-cycle by default.
 """
 import argparse
@@ -60,25 +62,31 @@ def build(seed=5, shape=(96, 96, 192), nlayers=12, junction=6, width=100.0, cont
     return te, dict(seed=seed, shape=shape, layers=t, sigma=sig, k0=k0, alpha=alpha, js=js, beta=beta, V=V)
 
 
-def run(te, outer=20, tfreq=6, verbose=False):
-    """outer x (electrical.compute(tfreq), thermal.compute(1)); returns timings."""
+def run(te, outer=20, tfreq=6, verbose=False, link=True):
+    """outer x (electrical.compute(tfreq), thermal.compute(1)); returns timings.  link: the
+    coupled fields stay on the device (ThermoElectric.link), else they pass through the
+    providers' host arrays."""
     te.invalidate()
     te.initialize()
+    linked = te.link() if link else False
     t_el = t_th = 0.0
     hist = []
-    for _ in range(outer):
-        t0 = time.perf_counter()
-        verr = te.electrical.compute(tfreq)
-        t1 = time.perf_counter()
-        terr = te.thermal.compute(1)
-        t2 = time.perf_counter()
-        t_el += t1 - t0
-        t_th += t2 - t1
-        hist.append((float(verr), float(terr)))
-        if verbose:
-            print("outer %d: electrical err %.3g %% (%.2f s), thermal err %.3g K (%.2f s)" %
-                  (len(hist), verr, t1 - t0, terr, t2 - t1), file=sys.stderr, flush=True)
-    return dict(electrical_s=t_el, thermal_s=t_th, total_s=t_el + t_th, history=hist,
+    try:
+        for _ in range(outer):
+            t0 = time.perf_counter()
+            verr = te.electrical.compute(tfreq)
+            t1 = time.perf_counter()
+            terr = te.thermal.compute(1)
+            t2 = time.perf_counter()
+            t_el += t1 - t0
+            t_th += t2 - t1
+            hist.append((float(verr), float(terr)))
+            if verbose:
+                print("outer %d: electrical err %.3g %% (%.2f s), thermal err %.3g K (%.2f s)" %
+                      (len(hist), verr, t1 - t0, terr, t2 - t1), file=sys.stderr, flush=True)
+    finally:
+        te.unlink()
+    return dict(electrical_s=t_el, thermal_s=t_th, total_s=t_el + t_th, history=hist, linked=linked,
                 current_mA=te.get_total_current(), Tmax=float(te.thermal._temps().max()))
 
 
@@ -93,7 +101,8 @@ def main(argv=None):
     shape = tuple(int(v) for v in a.shape.split(","))
     out = {}
     T = {}
-    for name, prec in (("fp64", 0), ("fp32_refine", 1), ("fp64_cg_fp32_precond", 3)):
+    for name, prec, link in (("fp64", 0, True), ("fp32_refine", 1, True), ("fp64_cg_fp32_precond", 3, True),
+                             ("fp64_host_exchange", 0, False)):
         te, info = build(shape=shape)
         te.thermal.iterative.precision = prec
         te.electrical.iterative.precision = prec
@@ -101,7 +110,7 @@ def main(argv=None):
         t0 = time.perf_counter()
         te.initialize()
         t_init = time.perf_counter() - t0
-        r = run(te, a.outer, verbose=True)
+        r = run(te, a.outer, verbose=True, link=link)
         r["init_s"] = t_init
         T[name] = te.thermal._temps().copy()
         out[name] = r
@@ -110,13 +119,17 @@ def main(argv=None):
               file=sys.stderr, flush=True)
     d = np.abs(T["fp32_refine"] - T["fp64"]).max() / np.abs(T["fp64"]).max()
     d3 = np.abs(T["fp64_cg_fp32_precond"] - T["fp64"]).max() / np.abs(T["fp64"]).max()
+    dh = np.abs(T["fp64_host_exchange"] - T["fp64"]).max() / np.abs(T["fp64"]).max()
     summ = dict(config=dict(info, layers=None), outer=a.outer, thermal_rtol=a.rtol,
                 preconditioner="multigrid V-cycle (solver default)",
                 fp64={k: v for k, v in out["fp64"].items() if k != "history"},
                 fp32_refine={k: v for k, v in out["fp32_refine"].items() if k != "history"},
                 fp64_cg_fp32_precond={k: v for k, v in out["fp64_cg_fp32_precond"].items() if k != "history"},
+                fp64_host_exchange={k: v for k, v in out["fp64_host_exchange"].items() if k != "history"},
                 speedup=out["fp64"]["total_s"] / out["fp32_refine"]["total_s"],
+                speedup_device_exchange=out["fp64_host_exchange"]["total_s"] / out["fp64"]["total_s"],
                 max_rel_T_diff=float(d), max_rel_T_diff_fp32_precond=float(d3),
+                max_rel_T_diff_host_exchange=float(dh),
                 history_fp64=out["fp64"]["history"])
     print(json.dumps(summ))
     return summ

@@ -2041,6 +2041,7 @@ __global__ void k_powerlaw(ElemGrid g, long long E, const double* __restrict__ x
                            double* __restrict__ kh, double* __restrict__ kv, float* __restrict__ kh32,
                            float* __restrict__ kv32) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        if (isnan(k0h[e])) continue;  // the element keeps its conductivities
         int i, j, k;
         elem_coords(g, e, i, j, k);
         double s = 0.;

@@ -456,6 +456,7 @@ void ThermalCore::update_conductivity() {
 double ThermalCore::compute(int loops) {
     if (!initialized_) initialize();
     log(LOG_INFO, id + ": Running thermal calculations");
+    if (heat_src_) check(pf_source_from(ctx_, heat_src_->context(), PF_POST_JOULE), "heat from the linked solver");
     int loop = 0;
     double err = 0.;
     toterr = 0.;
@@ -585,11 +586,38 @@ void ElectricalCore::initialize() {
     if (!have_mesh_) throw std::runtime_error(id + ": no mesh");
     std::vector<double> v(N_, 0.);
     check(pf_solution_set(ctx_, v.data()), "initial potential");
-    std::vector<double> z(E_ * dim_, 0.);
     loopno = 0;
     jcond_.assign(jelems_.size(), start_cond_);
     jcond_h_.assign(jelems_.size(), start_cond_h_);
+    coeffs_live_ = jcond_on_device_ = false;
     initialized_ = true;
+}
+
+void ElectricalCore::link_temperature(FemCore* src, const std::vector<double>& s0h, const std::vector<double>& s0v,
+                                      const std::vector<double>& ah, const std::vector<double>& av) {
+    if (!src) {
+        temp_src_ = nullptr;
+        return;
+    }
+    if ((int64_t)s0h.size() != E_ || (int64_t)s0v.size() != E_ || (int64_t)ah.size() != E_ ||
+        (int64_t)av.size() != E_)
+        throw std::invalid_argument(id + ": conductivity law arrays need one value per element");
+    check(pf_coeffs_powerlaw(ctx_, PF_ELECTRICAL, s0h.data(), s0v.data(), ah.data(), av.data(), E_),
+          "conductivity law");
+    temp_src_ = src;
+}
+
+// the junction conductivities the device holds (saveConductivities, electr2d.cpp:355-361)
+void ElectricalCore::fetch_jcond() {
+    if (!jelems_.empty()) {
+        std::vector<double> ch(E_), cv(E_);
+        check(pf_get_coeffs(ctx_, ch.data(), cv.data()), "conductivity");
+        for (size_t q = 0; q < jelems_.size(); ++q) {
+            jcond_[q] = cv[jelems_[q]];
+            jcond_h_[q] = ch[jelems_[q]];
+        }
+    }
+    jcond_on_device_ = false;
 }
 
 double ElectricalCore::compute(int loops) {
@@ -597,12 +625,19 @@ double ElectricalCore::compute(int loops) {
     if ((int64_t)sh_.size() != E_) throw std::runtime_error(id + ": conductivities not set");
     log(LOG_INFO, id + ": Running electrical calculations");
     // loadConductivities (electr2d.cpp:331-353): materials, then the saved junction values
-    std::vector<double> kh = sh_, kv = sv_;
-    for (size_t q = 0; q < jelems_.size(); ++q) {
-        kh[jelems_[q]] = jcond_h_[q];
-        kv[jelems_[q]] = (std::isnan(jcond_[q]) || std::abs(jcond_[q]) < 1e-16) ? 1e-16 : jcond_[q];
+    if (temp_src_ && coeffs_live_) {
+        // on the device: the law at the linked temperatures; junction elements keep theirs
+        check(pf_coeffs_update_from(ctx_, temp_src_->context()), "conductivity at the linked temperatures");
+    } else {
+        if (jcond_on_device_) fetch_jcond();
+        std::vector<double> kh = sh_, kv = sv_;
+        for (size_t q = 0; q < jelems_.size(); ++q) {
+            kh[jelems_[q]] = jcond_h_[q];
+            kv[jelems_[q]] = (std::isnan(jcond_[q]) || std::abs(jcond_[q]) < 1e-16) ? 1e-16 : jcond_[q];
+        }
+        check(pf_coeffs_set(ctx_, PF_ELECTRICAL, kh.data(), kv.data(), E_), "conductivity");
+        coeffs_live_ = true;
     }
-    check(pf_coeffs_set(ctx_, PF_ELECTRICAL, kh.data(), kv.data(), E_), "conductivity");
     const bool noactive = jelems_.empty();
     const double minj = 100e-7;  // electr2d.cpp:391
     unsigned loop = 0;
@@ -634,15 +669,9 @@ double ElectricalCore::compute(int loops) {
         log(LOG_RESULT, fmt("%s: Loop %d(%d): max(j%s) = %g kA/cm2, error = %g%%", id.c_str(), loop, loopno,
                             noactive ? "" : "@junc", mcur, err));
     } while ((!iter.converged || err > maxerr) && (loops == 0 || loop < (unsigned)loops));
-    // saveConductivities (electr2d.cpp:355-361)
-    if (!jelems_.empty()) {
-        std::vector<double> ch(E_), cv(E_);
-        check(pf_get_coeffs(ctx_, ch.data(), cv.data()), "conductivity");
-        for (size_t q = 0; q < jelems_.size(); ++q) {
-            jcond_[q] = cv[jelems_[q]];
-            jcond_h_[q] = ch[jelems_[q]];
-        }
-    }
+    // saveConductivities (electr2d.cpp:355-361): linked, the junction values stay on the device
+    jcond_on_device_ = true;
+    if (!temp_src_) fetch_jcond();
     return toterr;
 }
 

@@ -60,6 +60,7 @@ class FemCore {
     std::vector<double> interp_elements(int what, const double* pts, int64_t n, double fill);
     void invalidate() { initialized_ = false; }
     bool initialized() const { return initialized_; }
+    pf_ctx* context() const { return ctx_; }
 
     IterParams iter;
     std::function<void(int, const std::string&)> logger;
@@ -108,6 +109,9 @@ class ThermalCore : public FemCore {
                         const std::vector<double>& ambient);
     double compute(int loops);
     void initialize();
+    // heat source taken on the device from another core's Joule heat (same grid and device)
+    // at the start of every compute, instead of the source of set_heat; nullptr unlinks
+    void link_heat(FemCore* src) { heat_src_ = src; }
 
     double inittemp = 300.;
     double maxerr = 0.05;  // K
@@ -132,6 +136,7 @@ class ThermalCore : public FemCore {
 
   protected:
     std::function<void(const double*, double*, double*, int64_t)> kcb_;
+    FemCore* heat_src_ = nullptr;
 };
 
 // DynamicThermalFem2DSolver / DynamicThermalFem3DSolver time loop
@@ -183,6 +188,14 @@ class ElectricalCore : public FemCore {
     // junction element it writes the in-plane and vertical conductivities; empty: Shockley's law
     using JunctionLaw = std::function<void(const double* U, const double* jy, double* cond_h, double* cond_v, int64_t n)>;
     void set_junction_law(JunctionLaw law) { law_ = std::move(law); }
+    // conductivities evaluated on the device at another core's temperatures (same grid and
+    // device) by the law s0 (300 / T_avg)^alpha per element (NaN s0_h: a junction element,
+    // whose conductivity the junction update owns); the first compute after initialize
+    // still takes the conductivities of set_conductivity.  src = nullptr unlinks.
+    void link_temperature(FemCore* src, const std::vector<double>& s0h, const std::vector<double>& s0v,
+                          const std::vector<double>& ah, const std::vector<double>& av);
+    // the device holds this core's conductivities (material and junction values)
+    bool device_coeffs() const { return coeffs_live_; }
     double compute(int loops);
     void initialize();
     std::vector<double> currents();  // E * dim, kA/cm^2
@@ -203,6 +216,10 @@ class ElectricalCore : public FemCore {
     double start_cond_h_ = 0.;             // and its c00
     JunctionLaw law_;
     bool have_flags_ = false;
+    FemCore* temp_src_ = nullptr;
+    bool coeffs_live_ = false;      // device kh / kv hold the material and junction conductivities
+    bool jcond_on_device_ = false;  // junction conductivities newer on the device than jcond_
+    void fetch_jcond();
 };
 
 }  // namespace plask_amd

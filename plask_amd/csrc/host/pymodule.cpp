@@ -109,6 +109,8 @@ PYBIND11_MODULE(_core, m) {
              })
         .def("initialize", &ThermalCore::initialize)
         .def("compute", &ThermalCore::compute, py::arg("loops") = 0)
+        // the linked core must outlive the link (the caller unlinks with None)
+        .def("link_heat", [](ThermalCore& s, FemCore* src) { s.link_heat(src); }, py::arg("src").none(true))
         .def_readwrite("inittemp", &ThermalCore::inittemp)
         .def_readwrite("maxerr", &ThermalCore::maxerr)
         .def_readwrite("loopno", &ThermalCore::loopno)
@@ -176,6 +178,11 @@ PYBIND11_MODULE(_core, m) {
              })
         .def("initialize", &ElectricalCore::initialize)
         .def("compute", &ElectricalCore::compute, py::arg("loops") = 0)
+        .def("link_temperature",
+             [](ElectricalCore& s, FemCore* src, const arr<double>& s0h, const arr<double>& s0v, const arr<double>& ah,
+                const arr<double>& av) { s.link_temperature(src, vec(s0h), vec(s0v), vec(ah), vec(av)); },
+             py::arg("src").none(true), py::arg("s0h"), py::arg("s0v"), py::arg("ah"), py::arg("av"))
+        .def_property_readonly("device_coeffs", &ElectricalCore::device_coeffs)
         .def("currents", [](ElectricalCore& s) { return to_np(s.currents()); })
         .def("heats", [](ElectricalCore& s) { return to_np(s.heats()); })
         .def("conductivity_h", [](ElectricalCore& s) { return to_np(s.conductivity_h()); })

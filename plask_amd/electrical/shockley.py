@@ -170,6 +170,58 @@ class _ShockleyBase(_sb.FemSolver):
         are grouped by (material, role class); materials with a closed-form law
         (cond_law: s0 (300/T)^alpha) are evaluated for a whole group at once."""
         E = len(self._mat)
+        self._build_cond_groups()
+        sh = np.empty(E)
+        sv = np.empty(E)
+        T = np.asarray(T, dtype=np.float64)
+        for kind, mm, ix in self._cond_groups:
+            if kind in ("act", "none"):
+                sh[ix] = sv[ix] = 0.0
+            elif kind == "p":
+                sh[ix] = sv[ix] = self.pcond
+            elif kind == "n":
+                sh[ix] = sv[ix] = self.ncond
+            else:
+                law = mm.cond_law()
+                if law is not None:
+                    t = 300.0 / T[ix]
+                    sh[ix] = law[0] * t ** law[2]
+                    sv[ix] = law[1] * t ** law[3]
+                else:
+                    for e in ix:
+                        v = mm.cond(float(T[e]))
+                        sh[e], sv[e] = (v, v) if np.isscalar(v) else (float(v[0]), float(v[1]))
+        return sh, sv
+
+    def _cond_law_arrays(self):
+        """The element conductivities as the device law s0 (300 / T)^alpha: (s0_h, s0_v,
+        alpha_h, alpha_v) per element, NaN s0_h at the junction elements (their conductivity
+        is the junction update's), when every material conductivity is such a law or
+        constant and beta and js do not depend on temperature; else None."""
+        if isinstance(self, _ActiveCondBase):   # the junction law takes the temperature
+            return None
+        if any(callable(v) for v in list(self._beta.values()) + list(self._js.values())):
+            return None
+        E = len(self._mat)
+        self._build_cond_groups()
+        s0h, s0v, ah, av = np.zeros(E), np.zeros(E), np.zeros(E), np.zeros(E)
+        for kind, mm, ix in self._cond_groups:
+            if kind == "p":
+                s0h[ix] = s0v[ix] = self.pcond
+            elif kind == "n":
+                s0h[ix] = s0v[ix] = self.ncond
+            elif kind == "mat":
+                law = mm.cond_law()
+                if law is None:
+                    return None
+                s0h[ix], s0v[ix], ah[ix], av[ix] = law
+        s0h[self._excl], s0v[self._excl] = _sb.MASKED_K
+        ah[self._excl] = av[self._excl] = 0.0
+        s0h[np.asarray(self._jelems, dtype=np.int64)] = np.nan
+        return s0h, s0v, ah, av
+
+    def _build_cond_groups(self):
+        """Elements grouped by (material, role class), once per initialization."""
         if getattr(self, "_cond_groups", None) is None:
             leaves = self._geometry._leaves
             nl = len(leaves)
@@ -196,27 +248,6 @@ class _ShockleyBase(_sb.FemSolver):
                 ix = np.concatenate([order[bounds[k]:bounds[k + 1]] for k in ks])
                 if len(ix):
                     self._cond_groups.append((key[0], mm, np.sort(ix)))
-        sh = np.empty(E)
-        sv = np.empty(E)
-        T = np.asarray(T, dtype=np.float64)
-        for kind, mm, ix in self._cond_groups:
-            if kind in ("act", "none"):
-                sh[ix] = sv[ix] = 0.0
-            elif kind == "p":
-                sh[ix] = sv[ix] = self.pcond
-            elif kind == "n":
-                sh[ix] = sv[ix] = self.ncond
-            else:
-                law = mm.cond_law()
-                if law is not None:
-                    t = 300.0 / T[ix]
-                    sh[ix] = law[0] * t ** law[2]
-                    sv[ix] = law[1] * t ** law[3]
-                else:
-                    for e in ix:
-                        v = mm.cond(float(T[e]))
-                        sh[e], sv[e] = (v, v) if np.isscalar(v) else (float(v[0]), float(v[1]))
-        return sh, sv
 
     def compute(self, loops=0):
         """Run electrical calculations; returns the maximum current-density change (%)."""
@@ -224,11 +255,14 @@ class _ShockleyBase(_sb.FemSolver):
         c = self._core
         c.maxerr = float(self.maxerr)
         c.stable = (str(self.convergence).lower() == "stable")
-        T = self.inTemperature(self._element_mesh())
-        sh, sv = self._load_conductivities(T)
-        sh[self._excl], sv[self._excl] = _sb.MASKED_K   # masked-out (empty) elements
-        c.set_conductivity(sh, sv)
-        self._junction_setup(np.asarray(T, dtype=np.float64))
+        if not (getattr(self, "_temp_linked", False) and c.device_coeffs):
+            T = self.inTemperature(self._element_mesh())
+            sh, sv = self._load_conductivities(T)
+            sh[self._excl], sv[self._excl] = _sb.MASKED_K   # masked-out (empty) elements
+            c.set_conductivity(sh, sv)
+            self._junction_setup(np.asarray(T, dtype=np.float64))
+        # else: linked to a thermal solver on the same mesh -- the core evaluates the
+        # conductivity law at its temperatures on the device (junction parameters constant)
         nodes, vals = self.voltage_boundary.resolve(self.mesh, self.geometry)
         if len(self._isolated):  # nodes of the masked-out region: held at 0 V, coupled to nothing
             nodes = np.concatenate([nodes, self._isolated])
@@ -242,15 +276,10 @@ class _ShockleyBase(_sb.FemSolver):
         """Per junction element the temperature of its column's element in the junction's
         middle row (electr2d.cpp:255)."""
         m = [len(a) - 1 for a in self._axes]
-        ti = np.empty(len(self._jelems), dtype=np.int64)
-        for q, (e, n) in enumerate(zip(self._jelems, self._jid)):
-            kb, kt, _ = self._junctions[n]
-            kmid = (kb + kt) // 2
-            if self.DIM == 2:
-                ti[q] = e % m[0] + m[0] * kmid
-            else:
-                ti[q] = e % (m[0] * m[1]) + m[0] * m[1] * kmid
-        return T[ti]
+        plane = m[0] if self.DIM == 2 else m[0] * m[1]
+        e = np.asarray(self._jelems, dtype=np.int64)
+        kmid = np.array([0 if j is None else (j[0] + j[1]) // 2 for j in self._junctions], dtype=np.int64)
+        return np.asarray(T)[e % plane + plane * kmid[np.asarray(self._jid, dtype=np.int64)]]
 
     def _junction_setup(self, T):
         """beta, js per junction element at its temperature (BetaSolver getBeta/getJs and the

@@ -30,7 +30,8 @@ EXPORTED = [
     "pf_set_multigrid", "pf_multigrid_levels", "pf_precond_apply_host", "pf_graphs_enabled", "pf_kernel_stats",
     "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
     "pf_capacity_set", "pf_dynamic_step", "pf_boundary_matrix_set", "pf_postprocess", "pf_interp_elements",
-    "pf_junction_state", "pf_junction_cond_set", "pf_set_multigrid_fusion",
+    "pf_junction_state", "pf_junction_cond_set", "pf_set_multigrid_fusion", "pf_coeffs_update_from",
+    "pf_source_from",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
@@ -126,6 +127,11 @@ def load_library(path=LIB_PATH):
     L.pf_boundary_matrix_set.argtypes = [P, P, P, P, i64]
     L.pf_postprocess.argtypes = [P, I, P, P, P]
     L.pf_set_multigrid_fusion.argtypes = [P, I]
+    L.pf_coeffs_update_from.argtypes = [P, P]
+    L.pf_currents.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pf_get_currents.argtypes = [P, P, P]
+    L.pf_get_coeffs.argtypes = [P, P, P]
+    L.pf_source_from.argtypes = [P, P, I]
     L.pf_interp_elements.argtypes = [P, I, P, i64, C.c_double, P]
     L.pf_dynamic_step.argtypes = [P, C.POINTER(SolveOpts), I, C.POINTER(SolveStats)]
     _lib = L
@@ -336,6 +342,22 @@ class FemContext:
         self._ck(self.L.pf_rhs_host(self.h, _p(b)), "pf_rhs_host")
         return b
 
+    def currents(self):
+        """Element current densities (E x dim, kA/cm^2) and Joule heat (E, W/m^3) of the
+        device solution taken as the potential (pf_currents, pf_get_currents)."""
+        dj2, j2 = C.c_double(), C.c_double()
+        self._ck(self.L.pf_currents(self.h, C.byref(dj2), C.byref(j2)), "pf_currents")
+        cur = np.zeros(self.nelems * self.dim)
+        heat = np.zeros(self.nelems)
+        self._ck(self.L.pf_get_currents(self.h, _p(cur), _p(heat)), "pf_get_currents")
+        return cur.reshape(-1, self.dim), heat
+
+    def coeffs(self):
+        """The element conductivities on the device (in-plane, vertical)."""
+        kh, kv = np.zeros(self.nelems), np.zeros(self.nelems)
+        self._ck(self.L.pf_get_coeffs(self.h, _p(kh), _p(kv)), "pf_get_coeffs")
+        return kh, kv
+
     def kernel_times(self, n=2):
         """(average seconds, timed launches) per kernel class KCAT_* of the last profiled solve."""
         s = np.zeros(n)
@@ -366,6 +388,15 @@ class FemContext:
     def coeffs_update(self):
         """Element conductivities from the law of set_powerlaw at the current element temperatures."""
         self._ck(self.L.pf_coeffs_update(self.h), "pf_coeffs_update")
+
+    def coeffs_update_from(self, other):
+        """The law of set_powerlaw at the element temperatures of another context's solution
+        (same grid and device), device to device."""
+        self._ck(self.L.pf_coeffs_update_from(self.h, other.h), "pf_coeffs_update_from")
+
+    def source_from(self, other, what=PF_POST_JOULE):
+        """Element heat source from another context's Joule heat (same grid and device)."""
+        self._ck(self.L.pf_source_from(self.h, other.h, int(what)), "pf_source_from")
 
     def set_launch(self, rows, planes):
         self._ck(self.L.pf_set_launch(self.h, int(rows), int(planes)), "pf_set_launch")

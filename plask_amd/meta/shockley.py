@@ -12,6 +12,8 @@ h5py, which is not part of this build: ``save`` is accepted and logged as skippe
 """
 import logging
 
+import numpy as np
+
 from ..electrical.shockley import Shockley2D, Shockley3D, ShockleyCyl
 from ..thermal.static import Static2D, Static3D, StaticCyl
 
@@ -64,17 +66,60 @@ class ThermoElectric:
         self.thermal.invalidate()
         self.electrical.invalidate()
 
+    def link(self):
+        """Exchange the coupled fields on the device from now on when the two solvers share
+        one mesh (the same axes) and the electrical conductivities are closed-form laws of
+        temperature (or constants) with temperature-independent junction parameters, and
+        the thermal mesh has no masked-out elements: the thermal core copies the Joule heat
+        into its source, the electrical core evaluates its conductivity law at the thermal
+        temperatures, both device to device, instead of the providers' host arrays.  The
+        values are the providers' (a same-mesh element field read at element midpoints, the
+        linear interpolation of nodal temperatures at element midpoints as the corner mean).
+        Returns whether the link was made; unlink() ends it."""
+        th, el = self.thermal, self.electrical
+        th.initialize()
+        el.initialize()
+        if th._core is None or el._core is None or len(th._axes) != len(el._axes):
+            return False
+        if not all(len(a) == len(b) and np.array_equal(a, b) for a, b in zip(th._axes, el._axes)):
+            return False
+        if th._excl.any():
+            return False
+        law = el._cond_law_arrays()
+        if law is None:
+            return False
+        el._core.link_temperature(th._core, *law)
+        th._core.link_heat(el._core)
+        el._temp_linked = th._heat_linked = True
+        return True
+
+    def unlink(self):
+        """Back to the providers' host exchange."""
+        th, el = self.thermal, self.electrical
+        if getattr(th, "_heat_linked", False):
+            th._core.link_heat(None)
+            th._heat_linked = False
+        if getattr(el, "_temp_linked", False):
+            z = np.zeros(0)
+            el._core.link_temperature(None, z, z, z, z)
+            el._temp_linked = False
+
     def compute(self, save=True, invalidate=True, group="ThermoElectric"):
         """Alternate electrical.compute(tfreq) and thermal.compute(1) until both
-        errors are below their maxerr (thermoelectric.py:205-211)."""
+        errors are below their maxerr (thermoelectric.py:205-211), with the coupled fields
+        exchanged on the device where link() allows."""
         if invalidate:
             self.invalidate()
         self.initialize()
-        verr = 2.0 * self.electrical.maxerr
-        terr = 2.0 * self.thermal.maxerr
-        while terr > self.thermal.maxerr or verr > self.electrical.maxerr:
-            verr = self.electrical.compute(self.tfreq)
-            terr = self.thermal.compute(1)
+        self.link()
+        try:
+            verr = 2.0 * self.electrical.maxerr
+            terr = 2.0 * self.thermal.maxerr
+            while terr > self.thermal.maxerr or verr > self.electrical.maxerr:
+                verr = self.electrical.compute(self.tfreq)
+                terr = self.thermal.compute(1)
+        finally:
+            self.unlink()
         log.info("%s: thermoelectric computations finished (thermal err %g K, electrical err %g %%)",
                  self.id, terr, verr)
         if save:

@@ -151,9 +151,12 @@ class _StaticBase(_sb.FemSolver):
         c = self._core
         c.inittemp = float(self.inittemp)
         c.maxerr = float(self.maxerr)
-        q = np.array(self.inHeat(self._element_mesh()), dtype=np.float64)
-        q[self._excl] = 0.0
-        c.set_heat(q)
+        if not getattr(self, "_heat_linked", False):
+            q = np.array(self.inHeat(self._element_mesh()), dtype=np.float64)
+            q[self._excl] = 0.0
+            c.set_heat(q)
+        # else: linked to an electrical solver on the same mesh -- the core copies its Joule
+        # heat into the source on the device
         nodes, vals = self.temperature_boundary.resolve(self.mesh, self.geometry)
         if len(self._isolated):  # nodes of the masked-out region hold the initial temperature
             nodes = np.concatenate([nodes, self._isolated])

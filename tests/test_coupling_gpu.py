@@ -1,0 +1,159 @@
+"""Device-resident coupling of the thermo-electrical loop (pf_source_from,
+pf_coeffs_update_from, ThermoElectric.link) on the GPU.
+
+* pf_source_from(PF_POST_JOULE): a thermal context whose source is copied on the device from
+  an electrical context's Joule heat solves exactly as one handed the same heat from the host
+  (thermal.inHeat = electrical.outHeat, thermoelectric.py:69-84);
+* pf_coeffs_update_from: a law s0 (300 / T_avg)^alpha evaluated at another context's
+  temperatures equals the law evaluated on the host at the element corner means, and
+  elements with NaN s0 keep their conductivities (junction elements);
+* ThermoElectric3D with the fields linked on the device reproduces the loop that passes them
+  through the providers' host arrays (thermoelectric.py:181-219): same loop errors, same
+  temperatures and potentials, for constant and temperature-dependent conductivities.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from plask_amd import geometry, material, mesh
+from plask_amd.fem import PF_ELECTRICAL, FemContext, FemError
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(seed, shape=(7, 6, 9)):
+    rng = np.random.default_rng(seed)
+    axes = [np.cumsum(np.r_[0.0, rng.uniform(0.2, 1.5, n - 1)]) for n in shape]
+    return rng, axes, O.Grid(axes)
+
+
+def test_source_from_joule_heat_equals_host_heat():
+    rng, axes, g = _grid(1)
+    E, N = g.nelems, g.nnodes
+    el = FemContext()
+    el.set_grid(axes)
+    el.set_coeffs(rng.uniform(10.0, 1e3, E), rng.uniform(10.0, 1e3, E), equation=PF_ELECTRICAL)
+    el.set_solution(rng.uniform(0.0, 1.5, N))
+    cur, heat = el.currents()
+    assert heat.min() > 0
+    plane = len(axes[0]) * len(axes[1])
+    dn = np.arange(plane, dtype=np.int64)
+    out = []
+    for linked in (False, True):
+        th = FemContext()
+        th.set_grid(axes)
+        th.set_coeffs(np.full(E, 20.0), np.full(E, 7.0))
+        th.set_dirichlet(dn, np.full(plane, 300.0))
+        if linked:
+            th.source_from(el)
+        else:
+            th.set_source(heat)
+        T, st = th.solve(x0=np.full(N, 300.0), rtol=1e-12)
+        out.append((T, th.rhs()))
+        th.close()
+    assert np.array_equal(out[0][1], out[1][1])     # the same load vector, bit for bit
+    assert np.array_equal(out[0][0], out[1][0])
+    # another grid is refused, and so is a source without currents
+    th = FemContext()
+    th.set_grid([axes[0][:-1], axes[1], axes[2]])
+    with pytest.raises(FemError):
+        th.source_from(el)
+    th.close()
+    fresh = FemContext()
+    fresh.set_grid(axes)
+    th = FemContext()
+    th.set_grid(axes)
+    with pytest.raises(FemError):
+        th.source_from(fresh)
+    for c in (th, fresh, el):
+        c.close()
+
+
+def test_coeffs_update_from_another_contexts_temperatures():
+    rng, axes, g = _grid(2)
+    E, N = g.nelems, g.nnodes
+    th = FemContext()
+    th.set_grid(axes)
+    T = rng.uniform(280.0, 420.0, N)
+    th.set_solution(T)
+    el = FemContext()
+    el.set_grid(axes)
+    k0 = (rng.uniform(1.0, 1e3, E), rng.uniform(1.0, 1e3, E))
+    a = (rng.uniform(-1.5, 1.5, E), rng.uniform(-1.5, 1.5, E))
+    keep = rng.random(E) < 0.2
+    k0h = np.where(keep, np.nan, k0[0])
+    held = (rng.uniform(1.0, 2.0, E), rng.uniform(3.0, 4.0, E))
+    el.set_coeffs(held[0], held[1], equation=PF_ELECTRICAL)
+    el.set_powerlaw(k0h, k0[1], a[0], a[1], equation=PF_ELECTRICAL)
+    el.coeffs_update_from(th)
+    kh, kv = el.coeffs()
+    # host law at the element corner means
+    n0, n1, n2 = (len(x) for x in axes)
+    Tn = T.reshape(n2, n1, n0)
+    tm = sum(Tn[k:n2 - 1 + k, j:n1 - 1 + j, i:n0 - 1 + i] for i in (0, 1) for j in (0, 1) for k in (0, 1)).ravel() / 8
+    wh = np.where(keep, held[0], k0[0] * (300.0 / tm) ** a[0])
+    wv = np.where(keep, held[1], k0[1] * (300.0 / tm) ** a[1])
+    assert np.allclose(kh, wh, rtol=1e-13, atol=0)
+    assert np.allclose(kv, wv, rtol=1e-13, atol=0)
+    assert np.array_equal(kh[keep], held[0][keep]) and np.array_equal(kv[keep], held[1][keep])
+    th.close()
+    el.close()
+
+
+@material.simple()
+class HotCond(material.Material):
+    """A conductivity falling with temperature, s = 2e3 (300/T)^1.7 S/m (a closed-form law)."""
+    def thermk(self, T, h=float("inf")):
+        return (40.0 * (300.0 / T) ** 1.1, 25.0 * (300.0 / T) ** 1.1)
+
+    def cond(self, T):
+        return (2e3 * (300.0 / T) ** 1.7, 2e3 * (300.0 / T) ** 1.7)
+
+    def cond_law(self):
+        return (2e3, 2e3, 1.7, 1.7)
+
+
+def _hot_device():
+    from plask_amd.meta.shockley import ThermoElectric3D
+    stack = geometry.Stack3D(xcenter=0.0, ycenter=0.0)
+    hot = HotCond()
+    stack.append(geometry.Cuboid(40.0, 40.0, 2.0, hot))
+    stack.append(geometry.Cuboid(40.0, 40.0, 0.02, "GaAs", role="active"))
+    stack.append(geometry.Cuboid(40.0, 40.0, 3.0, hot))
+    te = ThermoElectric3D("hot")
+    te.geometry = geometry.Cartesian3D(stack)
+    xy = np.linspace(-20.0, 20.0, 11)
+    te.mesh = mesh.Rectangular3D(xy, xy, np.array([0.0, 0.5, 1.0, 1.5, 2.0, 2.02, 2.75, 3.5, 4.25, 5.02]))
+    te.thermal.temperature_boundary.append(mesh.Rectangular3D.Bottom(), 300.0)
+    te.electrical.voltage_boundary.append(mesh.Rectangular3D.Top(), 1.2)
+    te.electrical.voltage_boundary.append(mesh.Rectangular3D.Bottom(), 0.0)
+    te.electrical.beta = 18.0
+    te.electrical.js = 30.0
+    return te
+
+
+@pytest.mark.parametrize("which", ["seeded", "hot"])
+def test_linked_loop_reproduces_the_host_exchange(which):
+    """seeded: constant conductivities, the linked loop is the host loop bit for bit up to
+    the order of the element-mean sums; hot: conductivities s0 (300/T)^1.7 at a few kelvin of
+    self-heating -- evaluated at 300 K or at stale temperatures they would be off by ~1e-2
+    and the potentials by far more than the tolerance."""
+    from plask_amd import coupled
+    print("linked loop", which, flush=True)
+    res = []
+    for link in (False, True):
+        te = coupled.build(shape=(20, 20, 40))[0] if which == "seeded" else _hot_device()
+        te.thermal.iterative.maxerr = 1e-7
+        r = coupled.run(te, outer=4, tfreq=3, link=link)
+        assert r["linked"] == link
+        res.append((r, te.thermal._temps().copy(), te.electrical._volt().copy(), te))
+    (r0, T0, V0, te0), (r1, T1, V1, te1) = res
+    assert not getattr(te1.thermal, "_heat_linked", False)   # unlinked after the run
+    tol_t, tol_v = (1e-11, 1e-10) if which == "seeded" else (1e-9, 1e-7)
+    print("T rise %.4g K, diffs T %.3e V %.3e" % (T0.max() - 300.0, np.abs(T1 - T0).max() / np.abs(T0).max(),
+                                                  np.abs(V1 - V0).max() / np.abs(V0).max()), flush=True)
+    assert np.abs(T1 - T0).max() <= tol_t * np.abs(T0).max()
+    assert np.abs(V1 - V0).max() <= tol_v * np.abs(V0).max()
+    h0, h1 = np.array(r0["history"]), np.array(r1["history"])
+    assert np.allclose(h1, h0, rtol=1e-5, atol=1e-9), (h0, h1)
+    assert T0.max() - 300.0 > (1e-3 if which == "hot" else 1e-6)
