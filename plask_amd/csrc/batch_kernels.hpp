@@ -108,88 +108,12 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
     return s;
 }
 
-// Node-centric 2D operator row: y = M A0 (M x) + (I-M) x (RES = false) or
-// y = M (0 - A0 x) (RES = true, electrical: no load vector).  Element matrix of
-// therm2d.cpp:225-228 / electr2d.cpp:296-304, x r_mid for cylindrical geometry.
-template <bool RES>
-__device__ inline double row2d(const BatchInst& I, const double* __restrict__ h0, const double* __restrict__ h1,
-                               const double* __restrict__ rm, const double* __restrict__ kh,
-                               const double* __restrict__ kv, const uint64_t* __restrict__ dmask,
-                               const double* __restrict__ x, int i, int k) {
-    const long long nb = I.node_off;
-    const long long node = nb + (long long)k * I.n0 + i;
-    const bool di = dbit(dmask, node);
-    const double xi = x[node - nb];
-    if (di) return RES ? 0. : xi;
-    double s = 0.;
-    for (int c = 0; c < 4; ++c) {
-        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
-        if (ei < 0 || ek < 0 || ei >= I.n0 - 1 || ek >= I.n1 - 1) continue;
-        const long long e = (long long)ek * (I.n0 - 1) + ei;
-        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
-        const double r = I.cyl ? rm[I.h0_off + ei] : 1.;
-        const double kx = r * kh[I.elem_off + e] * h / w, ky = r * kv[I.elem_off + e] * w / h;
-        // local position of node i in this element
-        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
-        for (int b = 0; b < 4; ++b) {
-            const int bx = b & 1, by = b >> 1;
-            const int ni = ei + bx, nk = ek + by;
-            const long long nn = nb + (long long)nk * I.n0 + ni;
-            if (!RES && dbit(dmask, nn)) continue;
-            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
-            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
-            s += (kx * Sx * My + ky * Mx * Sy) * x[nn - nb];
-        }
-    }
-    return RES ? -s : s;
-}
-
 // Error-free transformations (Knuth's TwoSum; TwoProduct with a fused multiply-add,
 // Ogita, Rump & Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 26, 2005).
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
     s = a + b;
     const double bv = s - a;
     e = (a - (s - bv)) + (b - bv);
-}
-
-// Residual row r = M (0 - A0 x) of row2d<true>, with the products and their sum carried in
-// double-double (Dot2): the residual of a nearly converged potential is then accurate to
-// about the rounding of the result, not to eps * |A0| |x|, so CG restarted from it can
-// reduce the true residual below the level its recursive residual drifts to.  mag
-// receives sum_j |a_ij x_j|, the row of |A0| |x| (no FP64 potential has a residual much
-// below eps times it).
-__device__ inline double row2d_res_comp(const BatchInst& I, const double* __restrict__ h0,
-                                        const double* __restrict__ h1, const double* __restrict__ rm,
-                                        const double* __restrict__ kh, const double* __restrict__ kv,
-                                        const uint64_t* __restrict__ dmask, const double* __restrict__ x, int i,
-                                        int k, double& mag) {
-    const long long nb = I.node_off;
-    const long long node = nb + (long long)k * I.n0 + i;
-    mag = 0.;
-    if (dbit(dmask, node)) return 0.;
-    double hi = 0., lo = 0.;
-    for (int c = 0; c < 4; ++c) {
-        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
-        if (ei < 0 || ek < 0 || ei >= I.n0 - 1 || ek >= I.n1 - 1) continue;
-        const long long e = (long long)ek * (I.n0 - 1) + ei;
-        const double w = h0[I.h0_off + ei], h = h1[I.h1_off + ek];
-        const double r = I.cyl ? rm[I.h0_off + ei] : 1.;
-        const double kx = r * kh[I.elem_off + e] * h / w, ky = r * kv[I.elem_off + e] * w / h;
-        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
-        for (int b = 0; b < 4; ++b) {
-            const int bx = b & 1, by = b >> 1;
-            const long long nn = nb + (long long)(ek + by) * I.n0 + (ei + bx);
-            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
-            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
-            const double a = kx * Sx * My + ky * Mx * Sy, xv = x[nn - nb];
-            const double pr = a * xv, pe = fma(a, xv, -pr);
-            double se;
-            two_sum(hi, pr, hi, se);
-            lo += pe + se;
-            mag += fabs(pr);
-        }
-    }
-    return -(hi + lo);
 }
 
 // One level of an instance as the workgroup sees it (level 0: the instance grid).
@@ -207,33 +131,151 @@ struct LView {
 
 __device__ __forceinline__ bool lbit(const LView& V, long long node) { return dbit(V.mask, V.mbase + node); }
 
-// Node-centric operator row on a level (same element matrices as row2d):
-// RES = false: (M A0 M x + (I - M) x)_node, RES = true: (M (0 - A0 x))_node.
-template <bool RES>
-__device__ inline double row2d_l(const LView& V, const double* __restrict__ x, int i, int k) {
-    const long long node = (long long)k * V.n0 + i;
-    const bool di = lbit(V, node);
-    const double xi = x[node];
-    if (di) return RES ? 0. : xi;
-    double s = 0.;
-    for (int c = 0; c < 4; ++c) {
-        const int ei = i - 1 + (c & 1), ek = k - 1 + (c >> 1);
-        if (ei < 0 || ek < 0 || ei >= V.n0 - 1 || ek >= V.n1 - 1) continue;
-        const long long e = (long long)ek * (V.n0 - 1) + ei;
-        const double w = V.h0[ei], h = V.h1[ek];
-        const double r = V.cyl ? V.rm[ei] : 1.;
-        const double kx = r * V.kh[e] * h / w, ky = r * V.kv[e] * w / h;
-        const int ax = (c & 1) ? 0 : 1, ay = (c >> 1) ? 0 : 1;
-        for (int b = 0; b < 4; ++b) {
-            const int bx = b & 1, by = b >> 1;
-            const long long nn = (long long)(ek + by) * V.n0 + ei + bx;
-            if (!RES && lbit(V, nn)) continue;
-            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
-            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
-            s += (kx * Sx * My + ky * Mx * Sy) * x[nn];
+// Operator rows of a level marched along axis 1.  A work item is one column i and a range
+// of rows (columns times row ranges cover at least the block); it carries the 3 x 3 node
+// window around its node and the element coefficients of the two element rows there in
+// registers, so a node costs three potential loads, four conductivity loads and three
+// mask bits instead of the ~30 loads of a node-centric row, and the loads of the next row
+// are issued before the current row is formed.  Element matrix of therm2d.cpp:225-228 /
+// electr2d.cpp:296-304 (x r_mid for cylindrical geometry), summed element by element.
+// KIND 0: y = M A0 M x + (I - M) x; returns the sum of x y over the rows formed;
+// KIND 1: y = b + M (0 - A0 x) (b null: 0); returns 0;
+// KIND 2: y = M (0 - A0 x) with the products and their sum carried in double-double (Dot2):
+//   the residual of a nearly converged potential is then accurate to about the rounding of
+//   the result, not to eps |A0| |x|, so CG restarted from it can reduce the true residual
+//   below the level its recursive residual drifts to; returns the sum of y^2, and bw the max
+//   over the rows of |y_i| / (|A0| |x|)_i (no FP64 potential has a residual much below eps
+//   times that row).
+template <int BS, int KIND>
+__device__ inline double march_l(const LView& V, const double* __restrict__ x, const double* __restrict__ b,
+                                 double* __restrict__ y, double& bw) {
+    const int n0 = V.n0, n1 = V.n1;
+    const int G = n0 >= BS ? 1 : BS / n0;  // row ranges per column
+    const int R = (n1 + G - 1) / G;
+    double acc = 0.;
+    bw = 0.;
+    for (int w = threadIdx.x; w < G * n0; w += BS) {
+        const int g = w / n0, i = w - g * n0;
+        const int k0 = g * R, k1 = min(n1, k0 + R);
+        if (k0 >= k1) continue;
+        const bool eL = i >= 1, eR = i < n0 - 1;
+        const int im = max(i - 1, 0), ip = min(i + 1, n0 - 1);
+        const int eiL = max(i - 1, 0), eiR = min(i, n0 - 2);
+        const double wL = V.h0[eiL], wR = V.h0[eiR];
+        const double rL = V.cyl ? V.rm[eiL] : 1., rR = V.cyl ? V.rm[eiR] : 1.;
+        // node row kk (clamped) into xr / mr
+        auto load_row = [&](int kk, double (&xr)[3], bool (&mr)[3]) {
+            const int kc = min(max(kk, 0), n1 - 1);
+            const long long base = (long long)kc * n0;
+            xr[0] = x[base + im];
+            xr[1] = x[base + i];
+            xr[2] = x[base + ip];
+            mr[0] = lbit(V, base + im);
+            mr[1] = lbit(V, base + i);
+            mr[2] = lbit(V, base + ip);
+        };
+        // element row ek (clamped): raw conductivities and the height
+        auto load_elems = [&](int ek, double (&kr)[4], double& hr) {
+            const int ec = min(max(ek, 0), n1 - 2);
+            const long long eb = (long long)ec * (n0 - 1);
+            kr[0] = V.kh[eb + eiL];
+            kr[1] = V.kv[eb + eiL];
+            kr[2] = V.kh[eb + eiR];
+            kr[3] = V.kv[eb + eiR];
+            hr = V.h1[ec];
+        };
+        double xw[3][3], xn[3], kn[4], hn, ke[2][4], he[2];
+        bool mw[3][3], mn[3];
+        load_row(k0 - 1, xw[0], mw[0]);
+        load_row(k0, xw[1], mw[1]);
+        load_elems(k0 - 1, ke[0], he[0]);
+        load_row(k0 + 1, xn, mn);
+        load_elems(k0, kn, hn);
+        for (int k = k0; k < k1; ++k) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                xw[2][d] = xn[d];
+                mw[2][d] = mn[d];
+            }
+#pragma unroll
+            for (int d = 0; d < 4; ++d) ke[1][d] = kn[d];
+            he[1] = hn;
+            load_row(k + 2, xn, mn);
+            load_elems(k + 1, kn, hn);
+            const long long node = (long long)k * n0 + i;
+            const bool di = mw[1][1];
+            double out;
+            if (KIND == 2) {
+                double hi = 0., lo = 0., mag = 0.;
+                if (!di) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const int cx = c & 1, cy = c >> 1;
+                        const bool ok = (cx ? eR : eL) && (cy ? (k < n1 - 1) : (k >= 1));
+                        if (!ok) continue;
+                        const double wv = cx ? wR : wL, rv = cx ? rR : rL, h = he[cy];
+                        const double kx = rv * ke[cy][2 * cx] * h / wv, ky = rv * ke[cy][2 * cx + 1] * wv / h;
+                        const int ax = cx ? 0 : 1, ay = cy ? 0 : 1;
+#pragma unroll
+                        for (int bb = 0; bb < 4; ++bb) {
+                            const int bx = bb & 1, by = bb >> 1;
+                            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
+                            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
+                            const double a = kx * Sx * My + ky * Mx * Sy, xv = xw[cy + by][cx + bx];
+                            const double pr = a * xv, pe = fma(a, xv, -pr);
+                            double se;
+                            two_sum(hi, pr, hi, se);
+                            lo += pe + se;
+                            mag += fabs(pr);
+                        }
+                    }
+                }
+                out = di ? 0. : -(hi + lo);
+                acc += out * out;
+                if (mag > 0.) bw = fmax(bw, fabs(out) / mag);
+            } else {
+                double s = 0.;
+                if (!di) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const int cx = c & 1, cy = c >> 1;
+                        const bool ok = (cx ? eR : eL) && (cy ? (k < n1 - 1) : (k >= 1));
+                        if (!ok) continue;
+                        const double wv = cx ? wR : wL, rv = cx ? rR : rL, h = he[cy];
+                        const double kx = rv * ke[cy][2 * cx] * h / wv, ky = rv * ke[cy][2 * cx + 1] * wv / h;
+                        const int ax = cx ? 0 : 1, ay = cy ? 0 : 1;
+#pragma unroll
+                        for (int bb = 0; bb < 4; ++bb) {
+                            const int bx = bb & 1, by = bb >> 1;
+                            if (KIND == 0 && mw[cy + by][cx + bx]) continue;
+                            const double Sx = (ax == bx) ? 1. : -1., Sy = (ay == by) ? 1. : -1.;
+                            const double Mx = (ax == bx) ? (1. / 3.) : (1. / 6.), My = (ay == by) ? (1. / 3.) : (1. / 6.);
+                            s += (kx * Sx * My + ky * Mx * Sy) * xw[cy + by][cx + bx];
+                        }
+                    }
+                }
+                if (KIND == 0) {
+                    out = di ? xw[1][1] : s;
+                    acc += xw[1][1] * out;
+                } else {
+                    out = (b ? b[node] : 0.) + (di ? 0. : -s);
+                }
+            }
+            y[node] = out;
+            // shift the window one row up
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                xw[0][d] = xw[1][d];
+                mw[0][d] = mw[1][d];
+                xw[1][d] = xw[2][d];
+                mw[1][d] = mw[2][d];
+            }
+#pragma unroll
+            for (int d = 0; d < 4; ++d) ke[0][d] = ke[1][d];
+            he[0] = he[1];
         }
     }
-    return RES ? -s : s;
+    return acc;
 }
 
 // Thomas factors of the tridiagonal part of the level's matrix along axis 1 (per column)
@@ -327,8 +369,8 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
 // r = b - A x on a level (zero at Dirichlet nodes, where b is zero)
 template <int BS>
 __device__ inline void residual_l(const LView& V, const double* b, const double* x, double* r) {
-    const long long N = (long long)V.n0 * V.n1;
-    for (long long n = threadIdx.x; n < N; n += BS) r[n] = b[n] + row2d_l<true>(V, x, (int)(n % V.n0), (int)(n / V.n0));
+    double bw;
+    march_l<BS, 1>(V, x, b, r, bw);
     __syncthreads();
 }
 
@@ -472,7 +514,8 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
         // computed by the launch that starts the instance
         double bb = RS.bb;
         if (S.loops == 0) {
-            for (long long n = tid; n < N; n += BS) r[n] = row2d<true>(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0);
+            double bw0;
+            march_l<BS, 1>(V0, x, nullptr, r, bw0);
             __syncthreads();
             bb = 0.;
             for (long long n = tid; n < N; n += BS) bb += r[n] * r[n];
@@ -513,14 +556,8 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
             // true residual (compensated) and the componentwise backward error of x
             double bwd = 0.;
             auto true_residual = [&]() {
-                double acc = 0., bw = 0.;
-                for (long long n = tid; n < N; n += BS) {
-                    double mg;
-                    const double v = row2d_res_comp(I, h0, h1, rm, kh, kv, dmask, x, n % n0, n / n0, mg);
-                    r[n] = v;
-                    acc += v * v;
-                    if (mg > 0.) bw = fmax(bw, fabs(v) / mg);
-                }
+                double bw = 0.;
+                const double acc = march_l<BS, 2>(V0, x, nullptr, r, bw);
                 bwd = wg_max(bw, red);
                 return wg_sum(acc, red);
             };
@@ -572,12 +609,8 @@ __global__ __launch_bounds__(BS) void k_batch_shockley2d(const BatchInst* __rest
                 for (long long n = tid; n < N; n += BS) p[n] = z[n];
                 __syncthreads();
                 for (;;) {
-                    double pq = 0.;
-                    for (long long n = tid; n < N; n += BS) {
-                        const double v = row2d<false>(I, h0, h1, rm, kh, kv, dmask, p, n % n0, n / n0);
-                        q[n] = v;
-                        pq += p[n] * v;
-                    }
+                    double bwq;
+                    double pq = march_l<BS, 0>(V0, p, nullptr, q, bwq);
                     pq = wg_sum(pq, red);
                     if (!(pq > 0.)) {
                         cst = -6;
