@@ -422,7 +422,22 @@ def config3(cpu_seconds):
     work = lambda i: float(nr[i] * nz[i]) * float(nr[i] + 1) ** 2
     c = sum(dt for _, dt in cpu) / sum(work(i) for i, _ in cpu)
     cpu_total = c * sum(loops[i] * work(i) for i in range(len(insts)))
+    # algorithmic bytes (FP64, each value read or written once per pass): per CG iteration
+    # the marched operator apply (p, two conductivities, q: 32 B/node), the x / r update
+    # (48), p = z + beta p and (r, z) (40) and the V-cycle, 232 B per node of every level
+    # (two line sweeps 56 + 64, two marched residuals 2 x 40, restriction 12, prolongation
+    # 20) over levels of 2 N nodes in all (axis 0 halved per level); per Picard loop the line
+    # factors and coarse conductivities of all levels, the junction update and the currents
+    # (~150 B/node)
+    nodes = (nr * nz).astype(np.float64)
+    bytes_alg = float(np.sum(nodes * (its * (32 + 48 + 40 + 232 * 2) + loops * 150)))
+    roofline = {"bound": "hbm", "kernel": "k_batch_shockley2d (whole batch)", "achieved": round(bytes_alg / t / 1e9, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bytes_alg / t / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes": bytes_alg,
+                "note": "one workgroup per instance: each instance's passes are chains of dependent loads on one CU; "
+                        "the time is the slowest instances' (largest, most CG iterations), not the total bytes"}
     return dict(workload="shockley_cyl_batch_256", instances=len(insts), nodes_total=int((nr * nz).sum()),
+                roofline=roofline,
                 gpu_s=t, block=blk, workgroups=nwg, autotune_s=t_tune, picard_loops_mean=float(loops.mean()),
                 picard_loops_max=int(loops.max()), cg_iters_total=int(its.sum()),
                 converged=int((status == 0).sum()), stopped_at_maxloops=int((status == 2).sum()),

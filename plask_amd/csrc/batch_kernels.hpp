@@ -316,17 +316,19 @@ __device__ inline void line_factors_l(const LView& V) {
 template <int BS>
 __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* tmp, const double* xin, double* xout,
                                     double om) {
-    // The loads of a chunk of U planes do not depend on the recurrence: they are all
-    // issued before its U steps, so the sweep waits for memory once per chunk, not once
-    // per plane.  A chunk reads all of its nodes before writing any (rhs may be tmp,
-    // xin may be xout).
-    constexpr int U = 16;
+    // The loads of a chunk of U planes do not depend on the recurrence: they are all issued
+    // before its U steps, and the next chunk's before the current chunk's steps (two
+    // register buffers), so the sweep waits for memory about once per two chunks.  A chunk
+    // is read before any of its nodes is written, and the chunk read ahead lies beyond the
+    // rows being written (rhs may be tmp, xin may be xout).  Reads past either end are
+    // clamped to the last row and unused.
+    constexpr int U = 8;
     const int tid = threadIdx.x;
     const int n1 = V.n1;
     for (int i = tid; i < V.n0; i += BS) {
         double d = 0.;
-        for (int k0 = 0; k0 < n1; k0 += U) {
-            double rv[U], av[U], wv[U];
+        double ra[U], aa[U], wa[U], rb[U], ab[U], wb[U];
+        auto ldf = [&](int k0, double (&rv)[U], double (&av)[U], double (&wv)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = min(k0 + u, n1 - 1);
@@ -335,6 +337,8 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
                 av[u] = V.la[node];
                 wv[u] = V.lw[node];
             }
+        };
+        auto stf = [&](int k0, const double (&rv)[U], const double (&av)[U], const double (&wv)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (k0 + u < n1) {
@@ -342,10 +346,17 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
                     tmp[(long long)(k0 + u) * V.n0 + i] = d;
                 }
             }
+        };
+        ldf(0, ra, aa, wa);
+        for (int k0 = 0; k0 < n1; k0 += 2 * U) {
+            ldf(k0 + U, rb, ab, wb);
+            stf(k0, ra, aa, wa);
+            ldf(k0 + 2 * U, ra, aa, wa);
+            stf(k0 + U, rb, ab, wb);
         }
         double zn = 0.;
-        for (int k0 = n1 - 1; k0 >= 0; k0 -= U) {
-            double tv[U], cv[U], xv[U];
+        double ta[U], ca[U], xa[U], tb[U], cb[U], xb[U];
+        auto ldb = [&](int k0, double (&tv)[U], double (&cv)[U], double (&xv)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = max(k0 - u, 0);
@@ -354,6 +365,8 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
                 cv[u] = V.lc[node];
                 xv[u] = xin ? xin[node] : 0.;
             }
+        };
+        auto stb = [&](int k0, const double (&tv)[U], const double (&cv)[U], const double (&xv)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (k0 - u >= 0) {
@@ -361,6 +374,13 @@ __device__ inline void line_sweep_l(const LView& V, const double* rhs, double* t
                     xout[(long long)(k0 - u) * V.n0 + i] = xv[u] + om * zn;
                 }
             }
+        };
+        ldb(n1 - 1, ta, ca, xa);
+        for (int k0 = n1 - 1; k0 >= 0; k0 -= 2 * U) {
+            ldb(k0 - U, tb, cb, xb);
+            stb(k0, ta, ca, xa);
+            ldb(k0 - 2 * U, ta, ca, xa);
+            stb(k0 - U, tb, cb, xb);
         }
     }
     __syncthreads();
