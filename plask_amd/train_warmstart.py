@@ -42,6 +42,70 @@ def load_weights(path):
         return z["params"].astype(np.float32), tuple(int(v) for v in z["shape"])
 
 
+def evaluate(weights, shape, held, precond=fem.PRECOND_MG, log=print):
+    """Held-out stacks (seeds 9000 + i) solved by PCG (multigrid by default) to 1e-8 of the
+    uniform start's residual: from the uniform inittemp start, from the network guess as it
+    is (unchecked), and from the guess checked by the residual norm; with, per stack, the
+    ratio of the guess's error energy to the uniform start's, estimated as (r, M r) with the
+    preconditioner M (the quantity CG's progress depends on; M approximates A^-1)."""
+    params, wshape = load_weights(weights)
+    if tuple(wshape) != tuple(shape):
+        raise ValueError("weights are for %s, not %s" % (wshape, shape))
+    ctx = fem.FemContext()
+    net = warmstart.WarmNet(shape, max_samples=1)
+    net.set_params(params)
+    ws = warmstart.WarmStart(net, shape)
+    rows = []
+    for j in range(held):
+        p = workloads.thermal_3d(seed=9000 + j, n=shape)
+        _setup(ctx, p)
+        if j == 0:
+            net.set_mask(warmstart.dirichlet_mask(p))
+        t0 = time.perf_counter()
+        st_u = ctx.solve_uniform(300.0, rtol=1e-8, precond=precond)
+        t_u = time.perf_counter() - t0
+        T = ctx.solution()
+        f = warmstart.features(p)
+        t0 = time.perf_counter()
+        ws.start_on_device(ctx, p, f)
+        t_g = time.perf_counter() - t0
+        xg = ctx.solution()
+        xg[p["dir_nodes"]] = p["dir_vals"]   # the solve scatters the Dirichlet values into its start
+        t0 = time.perf_counter()
+        st_f = ctx.solve_warm(rtol=1e-8, init_value=300.0, precond=precond)
+        t_f = time.perf_counter() - t0
+        ws.start_on_device(ctx, p, f)
+        st_c = ctx.solve_warm(rtol=1e-8, init_value=300.0, precond=precond, checked=True)
+        xu = np.full(ctx.nnodes, 300.0)
+        xu[p["dir_nodes"]] = p["dir_vals"]
+        B = ctx.rhs()
+        ru, rg = B - ctx.apply(xu), B - ctx.apply(xg)
+        eu = float(ru @ ctx.precond_apply(ru, fem.PF_FP64, precond))
+        eg = float(rg @ ctx.precond_apply(rg, fem.PF_FP64, precond))
+        rise = np.abs(T - 300.0).max()
+        rows.append(dict(seed=9000 + j, it_uniform=st_u.iters, it_net=st_f.iters, it_checked=st_c.iters,
+                         fallback=int(st_c.start), energy_ratio=eg / eu,
+                         resid_ratio=float(np.linalg.norm(rg) / np.linalg.norm(ru)),
+                         guess_err=float(np.abs(xg - T).max() / rise), ms_uniform=1e3 * t_u, ms_guess=1e3 * t_g,
+                         ms_net_solve=1e3 * t_f))
+        if (j + 1) % 50 == 0:
+            log("held-out %d/%d" % (j + 1, held))
+    ctx.close()
+    a = lambda k: np.array([r[k] for r in rows], dtype=np.float64)
+    summ = dict(shape=list(shape), held=held, precond=int(precond),
+                iters_uniform_mean=float(a("it_uniform").mean()), iters_net_mean=float(a("it_net").mean()),
+                iters_checked_mean=float(a("it_checked").mean()),
+                net_fewer_iters=int((a("it_net") < a("it_uniform")).sum()),
+                net_more_iters=int((a("it_net") > a("it_uniform")).sum()),
+                energy_ratio_median=float(np.median(a("energy_ratio"))),
+                energy_ratio_below_1=int((a("energy_ratio") < 1.0).sum()),
+                resid_ratio_median=float(np.median(a("resid_ratio"))),
+                guess_err_median=float(np.median(a("guess_err"))),
+                ms_uniform_mean=float(a("ms_uniform").mean()), ms_guess_mean=float(a("ms_guess").mean()),
+                ms_net_solve_mean=float(a("ms_net_solve").mean()), fallbacks=int(a("fallback").sum()))
+    return summ, rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--shape", default="64,64,128")
@@ -57,9 +121,18 @@ def main(argv=None):
                     help="weight of the vertical-conductance energy term (relative to the target's own); 0: off")
     ap.add_argument("--out", default="")
     ap.add_argument("--summary", default="")
+    ap.add_argument("--eval", default="", metavar="WEIGHTS",
+                    help="only evaluate these weights on the held-out stacks (multigrid PCG; see evaluate)")
     a = ap.parse_args(argv)
     shape = tuple(int(v) for v in a.shape.split(","))
     log = lambda *s: print(*s, file=sys.stderr, flush=True)
+    if a.eval:
+        summ, rows = evaluate(a.eval, shape, a.held, log=log)
+        if a.summary:
+            with open(a.summary, "w") as fh:
+                json.dump(dict(summary=summ, rows=rows), fh, indent=1)
+        print(json.dumps(summ))
+        return summ
     # the feature workers are forked before this process touches the GPU
     pool = mp.get_context("fork").Pool(a.workers)
     ctx = fem.FemContext()

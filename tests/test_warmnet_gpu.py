@@ -57,11 +57,16 @@ def test_forward_and_gradient_match_oracle():
     assert e <= 1e-2
 
 
-def test_trained_warm_start_reduces_iterations():
+def test_trained_warm_start_against_the_uniform_start():
     """Train on solver-generated stacks (32x32x64 nodes), then start PCG on
     held-out stacks from the network guess (checked start: the uniform inittemp
     start is taken when the guess has the larger residual): same tolerance and
-    solution, and iterations reported against the uniform start."""
+    solution, iterations reported against the uniform start.  What the network gains
+    on this problem class is nothing: on the 200 held-out 64x64x128 stacks its guess has
+    ~7x the error energy of the uniform start and an unchecked start costs 2.4 more
+    multigrid iterations, so the checked start falls back; asserted here are that a
+    fallback is exactly the uniform start's solve and that the checked start is never
+    worse than it."""
     from plask_amd import fem, warmstart, workloads
     shape = (32, 32, 64)
     ntrain, held = 48, list(range(9000, 9006))
@@ -97,9 +102,12 @@ def test_trained_warm_start_reduces_iterations():
         T1 = ctx.solution()
         assert st1.rel_resid <= 1e-8
         assert np.abs(T1 - T0).max() / np.abs(T0 - 300.0).max() <= 1e-5
+        if st1.start == 1:   # fell back: the uniform start itself, so the same CG run
+            assert st1.iters == st0.iters and np.array_equal(T1, T0)
         it0 += st0.iters
         it1 += st1.iters
         print("seed %d: uniform start %d its, network start %d its (fallback %d)" %
               (seed, st0.iters, st1.iters, st1.start), flush=True)
-    # the checked start never costs more than a couple of iterations over the uniform start
-    assert it1 <= it0 + 2 * len(held)
+    # the checked start is never worse than the uniform one
+    assert it1 <= it0
+
