@@ -282,6 +282,13 @@ int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omeg
  * either way.  No reference counterpart (tuning hook). */
 int pf_set_multigrid_fusion(pf_ctx* ctx, int on);
 
+/* 3D multigrid, levels below the solve grid: residual tiles ordered by in-plane row and
+ * line-sweep column groups in contiguous ranges per XCD (hardware block b runs on XCD
+ * b % 8), so that consecutive kernels of a level read what the previous one wrote from
+ * the same L2 (on = 1, the default), or the plain block order (0).  Same arithmetic either
+ * way.  No reference counterpart (tuning hook). */
+int pf_set_multigrid_xcd(pf_ctx* ctx, int on);
+
 /* Number of multigrid levels built (the solve grid included; 0 before the first
  * multigrid solve). */
 int pf_multigrid_levels(const pf_ctx* ctx);

@@ -206,15 +206,19 @@ __global__ __launch_bounds__(1024) void k_apply3d(Axes3<T> g, int BY, int KC, in
     // XCD-aware order: hardware block b runs on XCD b % 8; each XCD gets a contiguous
     // range of logical tiles (adjacent y tiles of a z chunk share halo rows in its L2).
     // Blocks past the last tile idle but still join the grid reduction.
+    // xcd 2: logical tiles y-tile major (an XCD covers a range of in-plane rows over all z
+    // chunks, the columns the multigrid sweeps of the same XCD handle).
     const int L = xcd ? (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
     const bool live = L < nb;
     const int tid = threadIdx.x;
     const int tx = tid % BX, ty = tid / BX;
     const bool active = live && ty < BY;
-    const int jb = (L % ytiles) * (BY - 1);
+    const int nzc = (g.n2 + KC - 1) / KC;
+    const int ytile = xcd == 2 ? L / nzc : L % ytiles, zchunk = xcd == 2 ? L % nzc : L / ytiles;
+    const int jb = ytile * (BY - 1);
     const int j = jb - 1 + ty;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
-    const int kb = live ? (L / ytiles) * KC : n2;
+    const int kb = live ? zchunk * KC : n2;
     const int ke = min(kb + KC, n2);
     const bool nvalid = active && j >= 0 && j < n1;
     const bool owned = nvalid && ty >= 1;
@@ -1414,7 +1418,8 @@ struct PartLB {
     static constexpr int value = (M * (int)sizeof(T) <= 64) ? 1024 : 512;
 };
 
-// Placement of one thread in a partitioned line block: column blockIdx.x * CB + cl,
+// Placement of one thread in a partitioned line block: column blk * CB + cl (blk: the
+// block's logical index),
 // planes [seg m, seg m + m) (lanes run over columns, so every load is a contiguous
 // row segment).  The last plane of every segment but the final one is a separator.
 struct PartPos {
@@ -1423,12 +1428,12 @@ struct PartPos {
     long long base;
 };
 
-__device__ inline PartPos part_pos(int ncol, int n, int P, int CB, long long plane, int m) {
+__device__ inline PartPos part_pos(int ncol, int n, int P, int CB, long long plane, int m, int blk) {
     PartPos q;
     q.t = threadIdx.x;
     q.cl = q.t % CB;
     q.seg = q.t / CB;
-    q.col = blockIdx.x * CB + q.cl;
+    q.col = blk * CB + q.cl;
     q.act = q.seg < P && q.col < ncol;
     const int k0 = q.seg * m;
     q.len = q.act ? min(m, n - k0) : 0;
@@ -1548,7 +1553,7 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_cg_part(
     T* __restrict__ z, PcgState* __restrict__ st, double* __restrict__ partials) {
     if (!START && st->done) return;
     extern __shared__ __align__(16) unsigned char part_smem[];
-    const PartPos pp = part_pos(ncol, n, P, CB, plane, M);
+    const PartPos pp = part_pos(ncol, n, P, CB, plane, M, (int)blockIdx.x);
     T rv[M], av[M], wv[M], cv[M], qv[M];
 #pragma unroll
     for (int u = 0; u < M; ++u) {
@@ -1636,10 +1641,16 @@ __global__ __launch_bounds__((PartLB<T, M>::value)) void k_mg_smooth(
     int ncol, int n, int P, int CB, long long plane, T* __restrict__ r, const T* __restrict__ q, const T* xin,
     T* xout, const T* __restrict__ rp, T omega, const T* __restrict__ la, const T* __restrict__ lw,
     const T* __restrict__ lcp, const T* __restrict__ schur, PcgState* __restrict__ st,
-    double* __restrict__ partials, MgXfer<T> xf) {
+    double* __restrict__ partials, MgXfer<T> xf, int xcd) {
     if (st->done) return;
     extern __shared__ __align__(16) unsigned char part_smem[];
-    const PartPos pp = part_pos(ncol, n, P, CB, plane, M);
+    // XCD-aware order (xcd != 0, grid a multiple of 8): hardware block b runs on XCD b % 8;
+    // each XCD takes a contiguous range of column groups, i.e. of the in-plane rows that
+    // the level's residual tiles on the same XCD cover, so each kernel of a level finds the
+    // vectors the previous one wrote in its own L2.  Blocks past the last group idle but
+    // still join the grid reduction.
+    const int blk = xcd ? (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const PartPos pp = part_pos(ncol, n, P, CB, plane, M, blk);
     T rv[M], av[M], wv[M], cv[M];
     const T alpha = UPD ? T(st->alpha) : T(0);
     // transfer tables of this thread's column (XF 1: its fine support; XF 2: its coarse cell)

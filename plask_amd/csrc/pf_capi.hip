@@ -133,6 +133,7 @@ struct pf_ctx {
     float *mg_r0_32 = nullptr;
     int mg_max_levels = 0;      // 0: coarsen until both in-plane axes have <= 3 nodes
     int mg_coarse_sweeps = 1;   // smoothing sweeps on the coarsest level
+    int mg_xcd = 1;             // multigrid levels >= 1: XCD-aligned residual tiles and sweep groups
     int mg_fuse = 0;            // 3D: grid transfers fused into the smoother / residual kernels (measured slower)
     double mg_omega = 0.65;     // damping of the line-Jacobi smoother
     // cached graphs of one chunk of iterations
@@ -1468,14 +1469,15 @@ static void mg_residual_launch(pf_ctx* ctx, int l, const MgView<T>& v, MgProl<T>
     while (KC > 1 && (long long)ytiles * ((n2 + KC - 1) / KC) < 1024) KC >>= 1;
     const int nb = ytiles * ((n2 + KC - 1) / KC), grid = (nb + 7) / 8 * 8;
     const size_t shm = tile_lds(BY, n0) / sizeof(double) * sizeof(T);
+    const int xo = ctx->mg_xcd ? 2 : ctx->xcd_order;
     if (pr.xc)
         pf_launch(ctx, (k_apply3d<T, MODE_RESP, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream,
-                           v.g, BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr,
+                           v.g, BY, KC, ytiles, nb, xo, (const T*)v.x, (const T*)nullptr,
                            (const T*)nullptr, (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv,
                            ctx->st, ctx->partials, pr);
     else
         pf_launch(ctx, (k_apply3d<T, MODE_RESG, false, false, 1>), dim3(grid), dim3(n0 * BY), shm, ctx->stream,
-                           v.g, BY, KC, ytiles, nb, ctx->xcd_order, (const T*)v.x, (const T*)nullptr,
+                           v.g, BY, KC, ytiles, nb, xo, (const T*)v.x, (const T*)nullptr,
                            (const T*)nullptr, (T*)nullptr, (T*)nullptr, v.r, (const T*)v.b, v.dmask, v.kh, v.kv,
                            ctx->st, ctx->partials, pr);
 }
@@ -1493,10 +1495,10 @@ static void mg_residual(pf_ctx* ctx, int l, const MgView<T>& v, MgProl<T> pr = M
 template <typename T, int M>
 static void mg_smooth_m(pf_ctx* ctx, dim3 grid, int threads, size_t lds, const PartShape& ps, int ncol, int n,
                         long long plane, bool upd, int fin, T* r, const T* q, const T* xin, T* xout, const T* rp,
-                        T omega, const MgView<T>& v, int xfk, const MgXfer<T>& xf) {
+                        T omega, const MgView<T>& v, int xfk, const MgXfer<T>& xf, int xcd) {
 #define PF_MGS(U, F, X)                                                                                          \
     pf_launch(ctx, (k_mg_smooth<T, M, U, F, X>), grid, dim3(threads), lds, ctx->stream, ncol, n, ps.P, ps.CB, \
-                       plane, r, q, xin, xout, rp, omega, v.la, v.lw, v.lcp, v.schur, ctx->st, ctx->partials, xf)
+                       plane, r, q, xin, xout, rp, omega, v.la, v.lw, v.lcp, v.schur, ctx->st, ctx->partials, xf, xcd)
     if (xfk == 1) {  // coarse levels: first sweep from zero, restriction fused
         PF_MGS(false, 0, 1);
     } else if (xfk == 2) {  // post-smoothing with the prolongation fused
@@ -1526,17 +1528,21 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
         ps.CB >>= 1;
         ps.threads = (ps.CB * ps.P + 63) / 64 * 64;
     }
-    const dim3 grid((ncol + ps.CB - 1) / ps.CB);
+    // coarse levels with the XCD-aligned order: the grid a multiple of 8 (idle blocks
+    // past the last column group)
+    const int xcd = (ctx->mg_xcd && v.level > 0) ? 1 : 0;
+    const int ngroups = (ncol + ps.CB - 1) / ps.CB;
+    const dim3 grid(xcd ? (ngroups + 7) / 8 * 8 : ngroups);
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
     const T om = omega > T(0) ? omega : T(ctx->mg_omega);
     const int hk = prof_kern<T>(ctx, PF_K_SMOOTH, v.level,
                                 bytes_smooth(v.nodes(), ncol, ps.P, (int)sizeof(T), upd, xin != nullptr, fin != 0) +
                                     xbytes);
     switch (ps.m) {
-        case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
-        case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
-        case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
-        case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf); break;
+        case 4: mg_smooth_m<T, 4>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf, xcd); break;
+        case 8: mg_smooth_m<T, 8>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf, xcd); break;
+        case 16: mg_smooth_m<T, 16>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf, xcd); break;
+        case 32: mg_smooth_m<T, 32>(ctx, grid, ps.threads, lds, ps, ncol, n, ncol, upd, fin, r, q, xin, xout, rp, om, v, xfk, xf, xcd); break;
     }
     prof_close(ctx, hk);
 }
@@ -2930,6 +2936,15 @@ int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omeg
     HIPCK(hipSetDevice(ctx->device));
     if (ctx->stream) HIPCK(hipStreamSynchronize(ctx->stream));
     mg_free(ctx);  // rebuilt by the next multigrid solve
+    drop_graphs(ctx);
+    return PF_OK;
+}
+
+int pf_set_multigrid_xcd(pf_ctx* ctx, int on) {
+    if (!ctx) return PF_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    if (ctx->stream) HIPCK(hipStreamSynchronize(ctx->stream));
+    ctx->mg_xcd = on ? 1 : 0;
     drop_graphs(ctx);
     return PF_OK;
 }

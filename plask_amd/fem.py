@@ -31,7 +31,7 @@ EXPORTED = [
     "pf_coeffs_powerlaw", "pf_coeffs_update", "pf_solution_prolong_dev", "pf_net_output_device",
     "pf_capacity_set", "pf_dynamic_step", "pf_boundary_matrix_set", "pf_postprocess", "pf_interp_elements",
     "pf_junction_state", "pf_junction_cond_set", "pf_set_multigrid_fusion", "pf_coeffs_update_from",
-    "pf_source_from",
+    "pf_source_from", "pf_set_multigrid_xcd",
 ]
 
 # kernel ids of pf_kernel_stats (PF_K_*) and the kernel functions they stand for
@@ -127,6 +127,7 @@ def load_library(path=LIB_PATH):
     L.pf_boundary_matrix_set.argtypes = [P, P, P, P, i64]
     L.pf_postprocess.argtypes = [P, I, P, P, P]
     L.pf_set_multigrid_fusion.argtypes = [P, I]
+    L.pf_set_multigrid_xcd.argtypes = [P, I]
     L.pf_coeffs_update_from.argtypes = [P, P]
     L.pf_currents.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.pf_get_currents.argtypes = [P, P, P]
@@ -414,6 +415,10 @@ class FemContext:
         by the next multigrid solve."""
         self._ck(self.L.pf_set_multigrid(self.h, int(max_levels), int(coarse_sweeps), float(omega)),
                  "pf_set_multigrid")
+
+    def set_multigrid_xcd(self, on):
+        """3D multigrid: coarse-level residual tiles and sweep groups in XCD-aligned order."""
+        self._ck(self.L.pf_set_multigrid_xcd(self.h, 1 if on else 0), "pf_set_multigrid_xcd")
 
     def set_multigrid_fusion(self, on):
         """3D: grid transfers inside the smoother / residual kernels (True) or separate."""
