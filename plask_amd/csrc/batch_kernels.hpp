@@ -395,32 +395,80 @@ __device__ inline void residual_l(const LView& V, const double* b, const double*
 }
 
 // b_c = P^T r_f (coarse Dirichlet nodes: 0) and x_f += P x_c (fine Dirichlet nodes skipped)
+// The transfers run over work items of one column and a range of rows (as march_l); the
+// column's tables are read once, and the rows go in groups of 4 whose loads are all issued
+// before any of their stores (independent rows: memory waits once per group, not per row).
 template <int BS>
 __device__ inline void restrict_l(const LView& F, const LView& Cv, const int* cx, const int* mx, const double* wx,
-                                  const double* rf, double* bc) {
-    const long long Nc = (long long)Cv.n0 * Cv.n1;
-    for (long long n = threadIdx.x; n < Nc; n += BS) {
-        const int I = (int)(n % Cv.n0), k = (int)(n / Cv.n0);
-        double s = 0.;
-        if (!lbit(Cv, n)) {
-            const int ilo = I > 0 ? cx[I - 1] + 1 : 0, ihi = I < Cv.n0 - 1 ? cx[I + 1] - 1 : cx[I];
-            for (int i = ilo; i <= ihi; ++i) s += tweight(mx[i], wx[i], I) * rf[(long long)k * F.n0 + i];
+                                  const double* __restrict__ rf, double* __restrict__ bc) {
+    const int n0 = Cv.n0, n1 = Cv.n1;
+    const int G = n0 >= BS ? 1 : BS / n0;
+    const int R = (n1 + G - 1) / G;
+    for (int w = threadIdx.x; w < G * n0; w += BS) {
+        const int g = w / n0, I = w - g * n0;
+        const int k0 = g * R, k1 = min(n1, k0 + R);
+        const int ilo = I > 0 ? cx[I - 1] + 1 : 0, ihi = I < n0 - 1 ? cx[I + 1] - 1 : cx[I];
+        // up to 3 fine columns (absent ones: weight 0 at a valid column)
+        int ii[3];
+        double wt[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int f = ilo + a;
+            const bool ok = f <= ihi;
+            ii[a] = ok ? f : ilo;
+            wt[a] = ok ? tweight(mx[ii[a]], wx[ii[a]], I) : 0.;
         }
-        bc[n] = s;
+        constexpr int U = 4;
+        for (int kb = k0; kb < k1; kb += U) {
+            double v[U][3];
+            bool m[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(kb + u, k1 - 1);
+                const double* row = rf + (long long)k * F.n0;
+                v[u][0] = row[ii[0]];
+                v[u][1] = row[ii[1]];
+                v[u][2] = row[ii[2]];
+                m[u] = lbit(Cv, (long long)k * n0 + I);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (kb + u < k1)
+                    bc[(long long)(kb + u) * n0 + I] = m[u] ? 0. : wt[0] * v[u][0] + wt[1] * v[u][1] + wt[2] * v[u][2];
+            }
+        }
     }
     __syncthreads();
 }
 
 template <int BS>
-__device__ inline void prolong_l(const LView& F, const LView& Cv, const int* mx, const double* wx, const double* xc,
-                                 double* xf) {
-    const long long Nf = (long long)F.n0 * F.n1;
-    for (long long n = threadIdx.x; n < Nf; n += BS) {
-        if (lbit(F, n)) continue;
-        const int i = (int)(n % F.n0), k = (int)(n / F.n0);
+__device__ inline void prolong_l(const LView& F, const LView& Cv, const int* mx, const double* wx,
+                                 const double* __restrict__ xc, double* __restrict__ xf) {
+    const int n0 = F.n0, n1 = F.n1;
+    const int G = n0 >= BS ? 1 : BS / n0;
+    const int R = (n1 + G - 1) / G;
+    for (int w = threadIdx.x; w < G * n0; w += BS) {
+        const int g = w / n0, i = w - g * n0;
+        const int k0 = g * R, k1 = min(n1, k0 + R);
         const int I = mx[i], I1 = min(I + 1, Cv.n0 - 1);
         const double a = wx[i];
-        xf[n] += (1. - a) * xc[(long long)k * Cv.n0 + I] + a * xc[(long long)k * Cv.n0 + I1];
+        constexpr int U = 4;
+        for (int kb = k0; kb < k1; kb += U) {
+            double xo[U], c0[U], c1[U];
+            bool m[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(kb + u, k1 - 1);
+                xo[u] = xf[(long long)k * n0 + i];
+                c0[u] = xc[(long long)k * Cv.n0 + I];
+                c1[u] = xc[(long long)k * Cv.n0 + I1];
+                m[u] = lbit(F, (long long)k * n0 + i);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (kb + u < k1 && !m[u]) xf[(long long)(kb + u) * n0 + i] = xo[u] + ((1. - a) * c0[u] + a * c1[u]);
+            }
+        }
     }
     __syncthreads();
 }
