@@ -647,11 +647,13 @@ int pf_grid_set(pf_ctx* ctx, int dim, int geom, const double* ax0, int64_t n0, c
     ctx->have_source = false;
     ctx->have_solution = false;
     // launch shapes: tile blocks of BY node rows (BY - 1 owned) x whole x rows, KC planes;
-    // measured best at 128 x 128 x 256: 4 rows, 8 planes (FP64) / 16 planes (FP32)
+    // measured best at 128 x 128 x 256: 4 rows, 16 planes in both precisions (FP64: 8.23 ms
+    // per multigrid solve against 8.32 with 8 planes, 8.47 with 32, 8.32 with 6 rows; a
+    // prefetch depth of 2 planes is slower, 8.52)
     if (dim == 3) {
         const int by = (int)std::max<int64_t>(2, std::min<int64_t>(16, 512 / n0));
         ctx->BY[0] = ctx->BY[1] = by;
-        ctx->KC[0] = (int)std::max<int64_t>(1, std::min<int64_t>(8, n2));
+        ctx->KC[0] = (int)std::max<int64_t>(1, std::min<int64_t>(16, n2));
         ctx->KC[1] = (int)std::max<int64_t>(1, std::min<int64_t>(16, n2));
         // long x rows: the tile kernel's LDS would pass 64 KiB, use row blocks
         const int kind = tile_lds(by, (int)n0) > 65536 ? 1 : 0;
