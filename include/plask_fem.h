@@ -285,9 +285,10 @@ int pf_set_multigrid_fusion(pf_ctx* ctx, int on);
 /* 3D multigrid, levels below the solve grid: residual tiles ordered by in-plane row and
  * line-sweep column groups in contiguous ranges per XCD (hardware block b runs on XCD
  * b % 8), so that consecutive kernels of a level read what the previous one wrote from
- * the same L2 (on = 1, the default), or the plain block order (0).  Same arithmetic either
- * way.  No reference counterpart (tuning hook). */
-int pf_set_multigrid_xcd(pf_ctx* ctx, int on);
+ * the same L2 (mode 1, the default); the same on the solve grid's residuals and sweeps
+ * too (2, measured slower: the solve grid's vectors do not fit the L2s); or the plain
+ * block order (0).  Same arithmetic in every mode.  No reference counterpart (tuning hook). */
+int pf_set_multigrid_xcd(pf_ctx* ctx, int mode);
 
 /* Number of multigrid levels built (the solve grid included; 0 before the first
  * multigrid solve). */

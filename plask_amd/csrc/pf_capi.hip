@@ -896,13 +896,16 @@ static void launch_apply(pf_ctx* ctx, const T* xin, const T* zin, const T* pin, 
         const int zt = (int)((ctx->n[2] + KC - 1) / KC);
         const int nb = ytiles * zt, grid = (nb + 7) / 8 * 8;
         const size_t shm = tile_lds(BY, BX) / sizeof(double) * sizeof(T);
+        // the multigrid residuals on the solve grid in the XCD-aligned order as well
+        // (pf_set_multigrid_xcd(2))
+        const int xo = (ctx->mg_xcd == 2 && (MODE == MODE_RESG || MODE == MODE_RESP) && ctx->xcd_order) ? 2 : ctx->xcd_order;
         if (ctx->pfd == 2)
             pf_launch(ctx, (k_apply3d<T, MODE, FUSEP, DOT, 2>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
-                               g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
+                               g3, BY, KC, ytiles, nb, xo, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
                                ctx->partials, pr);
         else
             pf_launch(ctx, (k_apply3d<T, MODE, FUSEP, DOT, 1>), dim3(grid), dim3(BX * BY), shm, ctx->stream,
-                               g3, BY, KC, ytiles, nb, ctx->xcd_order, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
+                               g3, BY, KC, ytiles, nb, xo, xin, zin, pin, pout, xsol, y, F, dmask, kh, kv, ctx->st,
                                ctx->partials, pr);
     } else {
         const int BX = (int)ctx->n[0];
@@ -1532,7 +1535,7 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
     }
     // coarse levels with the XCD-aligned order: the grid a multiple of 8 (idle blocks
     // past the last column group)
-    const int xcd = (ctx->mg_xcd && v.level > 0) ? 1 : 0;
+    const int xcd = (ctx->mg_xcd == 2 || (ctx->mg_xcd && v.level > 0)) ? 1 : 0;
     const int ngroups = (ncol + ps.CB - 1) / ps.CB;
     const dim3 grid(xcd ? (ngroups + 7) / 8 * 8 : ngroups);
     const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
@@ -2942,11 +2945,11 @@ int pf_set_multigrid(pf_ctx* ctx, int max_levels, int coarse_sweeps, double omeg
     return PF_OK;
 }
 
-int pf_set_multigrid_xcd(pf_ctx* ctx, int on) {
+int pf_set_multigrid_xcd(pf_ctx* ctx, int mode) {
     if (!ctx) return PF_EINVAL;
     HIPCK(hipSetDevice(ctx->device));
     if (ctx->stream) HIPCK(hipStreamSynchronize(ctx->stream));
-    ctx->mg_xcd = on ? 1 : 0;
+    ctx->mg_xcd = mode < 0 ? 0 : (mode > 2 ? 2 : mode);
     drop_graphs(ctx);
     return PF_OK;
 }

@@ -416,9 +416,10 @@ class FemContext:
         self._ck(self.L.pf_set_multigrid(self.h, int(max_levels), int(coarse_sweeps), float(omega)),
                  "pf_set_multigrid")
 
-    def set_multigrid_xcd(self, on):
-        """3D multigrid: coarse-level residual tiles and sweep groups in XCD-aligned order."""
-        self._ck(self.L.pf_set_multigrid_xcd(self.h, 1 if on else 0), "pf_set_multigrid_xcd")
+    def set_multigrid_xcd(self, mode):
+        """3D multigrid: residual tiles and sweep groups in XCD-aligned order on the coarse
+        levels (1, the default), on every level (2), or nowhere (0)."""
+        self._ck(self.L.pf_set_multigrid_xcd(self.h, int(mode)), "pf_set_multigrid_xcd")
 
     def set_multigrid_fusion(self, on):
         """3D: grid transfers inside the smoother / residual kernels (True) or separate."""
