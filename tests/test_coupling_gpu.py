@@ -157,3 +157,39 @@ def test_linked_loop_reproduces_the_host_exchange(which):
     h0, h1 = np.array(r0["history"]), np.array(r1["history"])
     assert np.allclose(h1, h0, rtol=1e-5, atol=1e-9), (h0, h1)
     assert T0.max() - 300.0 > (1e-3 if which == "hot" else 1e-6)
+
+
+@pytest.mark.parametrize("cyl", [False, True])
+def test_linked_loop_2d(cyl):
+    """ThermoElectric2D / ThermoElectricCyl: the linked loop against the host exchange on a
+    2D stack with the temperature-dependent conductivity (same checks as the 3D case)."""
+    from plask_amd import coupled
+    from plask_amd.meta.shockley import ThermoElectric2D, ThermoElectricCyl
+    print("linked loop 2d cyl", cyl, flush=True)
+    res = []
+    for link in (False, True):
+        hot = HotCond()
+        stack = geometry.Stack2D()
+        stack.append(geometry.Rectangle(30.0, 2.0, hot))
+        stack.append(geometry.Rectangle(30.0, 0.02, "GaAs", role="active"))
+        stack.append(geometry.Rectangle(30.0, 3.0, hot))
+        te = (ThermoElectricCyl if cyl else ThermoElectric2D)("hot2d")
+        te.geometry = geometry.Cylindrical(stack) if cyl else geometry.Cartesian2D(stack)
+        te.mesh = mesh.Rectangular2D(np.linspace(0.0, 30.0, 31),
+                                     np.array([0.0, 0.5, 1.0, 1.5, 2.0, 2.02, 2.75, 3.5, 4.25, 5.02]))
+        te.thermal.temperature_boundary.append(mesh.Rectangular2D.Bottom(), 300.0)
+        te.electrical.voltage_boundary.append(mesh.Rectangular2D.Top(), 1.2)
+        te.electrical.voltage_boundary.append(mesh.Rectangular2D.Bottom(), 0.0)
+        te.electrical.beta = 18.0
+        te.electrical.js = 30.0
+        te.thermal.iterative.maxerr = 1e-7
+        r = coupled.run(te, outer=4, tfreq=3, link=link)
+        assert r["linked"] == link
+        res.append((r, te.thermal._temps().copy(), te.electrical._volt().copy()))
+    (r0, T0, V0), (r1, T1, V1) = res
+    print("T rise %.4g K, diffs T %.3e V %.3e" % (T0.max() - 300.0, np.abs(T1 - T0).max() / np.abs(T0).max(),
+                                                  np.abs(V1 - V0).max() / np.abs(V0).max()), flush=True)
+    assert T0.max() - 300.0 > 1e-3
+    assert np.abs(T1 - T0).max() <= 1e-9 * np.abs(T0).max()
+    assert np.abs(V1 - V0).max() <= 1e-7 * np.abs(V0).max()
+    assert np.allclose(np.array(r1["history"]), np.array(r0["history"]), rtol=1e-5, atol=1e-9)
