@@ -1443,26 +1443,44 @@ __device__ inline PartPos part_pos(int ncol, int n, int P, int CB, long long pla
     return q;
 }
 
+// Width of the shuffle scan that solves a column's separator system in part_solve (the
+// power of two >= P - 1), or 0 where one thread walks each column instead: more than 64
+// separators, or fewer threads in the block than CB * W.
+__host__ __device__ inline int part_scan_width(int P, int CB, int S) {
+    int W = 1;
+    while (W < P - 1) W <<= 1;
+    return (W <= 64 && CB * W <= S) ? W : 0;
+}
+
+// LDS values of part_solve for blocks of S threads
+__host__ __device__ inline int part_lds_values(int P, int CB, int S) {
+    return 6 * (S + (part_scan_width(P, CB, S) ? P : 0));
+}
+
 // z = T^-1 r for the thread's segment (values in registers): interior solve
-// y = T_j^-1 r; the separator right-hand sides g = r_s - T_sI y go to LDS; one thread
-// per column solves S z_s = g; each interior is solved again with the separator
+// y = T_j^-1 r; the separator right-hand sides g = r_s - T_sI y go to LDS; S z_s = g is
+// solved per column (a shuffle scan, or one thread); each interior is solved again with the separator
 // values moved to its right-hand side.  Every thread of the block calls it (two
-// barriers); lds holds 6 * blockDim.x values.
+// barriers); lds holds part_lds_values() values: six arrays of P rows of CB values
+// (segment-major), with one pad per row where the scan runs, so that the lanes of a
+// column's scan, which step through the rows, fall in different banks.
 template <typename T, int M>
 __device__ inline void part_solve(const PartPos& q, int P, int CB, int ncol, const T* __restrict__ schur, T* lds,
                                   const T (&rv)[M], const T (&av)[M], const T (&wv)[M], const T (&cv)[M], T (&zv)[M]) {
-    const int S = blockDim.x;
+    const int S = blockDim.x, W = part_scan_width(P, CB, S);
+    const int CS = W ? CB + 1 : CB, SP = part_lds_values(P, CB, S) / 6;
     T* s_yf = lds;
-    T* s_g = s_yf + S;
-    T* s_eh = s_g + S;
-    T* s_sa = s_eh + S;
-    T* s_sw = s_sa + S;
-    T* s_sc = s_sw + S;
+    T* s_g = s_yf + SP;
+    T* s_eh = s_g + SP;
+    T* s_sa = s_eh + SP;
+    T* s_sw = s_sa + SP;
+    T* s_sc = s_sw + SP;
     const int t = q.t, cl = q.cl, seg = q.seg;
+    const int pt = seg * CS + cl;  // the thread's slot (threads past CB * P have none)
     if (q.sep) {
-        s_sa[t] = schur[(size_t)seg * ncol + q.col];
-        s_sw[t] = schur[(size_t)(P - 1 + seg) * ncol + q.col];
-        s_sc[t] = schur[(size_t)(2 * (P - 1) + seg) * ncol + q.col];
+        s_sa[pt] = schur[(size_t)seg * ncol + q.col];
+        s_sw[pt] = schur[(size_t)(P - 1 + seg) * ncol + q.col];
+        s_sc[pt] = schur[(size_t)(2 * (P - 1) + seg) * ncol + q.col];
     }
     // interior solve with the separators at zero
     T d[M];
@@ -1493,31 +1511,68 @@ __device__ inline void part_solve(const PartPos& q, int P, int CB, int ncol, con
             elo = av[u];
             ehi = cv[u];
         }
-    s_yf[t] = q.act ? d[0] : T(0);
-    s_g[t] = q.sep ? rs - elo * yl : T(0);
-    s_eh[t] = q.sep ? ehi : T(0);
+    if (seg < P) {
+        s_yf[pt] = q.act ? d[0] : T(0);
+        s_g[pt] = q.sep ? rs - elo * yl : T(0);
+        s_eh[pt] = q.sep ? ehi : T(0);
+    }
     __syncthreads();
-    if (seg == 0 && q.col < ncol) {
+    // The separator system S z_s = g of each column, S = L U from its Thomas factors
+    // (L: pivots 1/w, sub-diagonal a; U: unit, super-diagonal c').  Up to 64 separators:
+    // the lanes of W-wide groups (W the power of two >= P - 1; one group per column)
+    // take one separator each and run both bidiagonal solves as affine first-order
+    // recurrences x_j = a_j x_(j-1) + b_j, with a log2(W)-step scan over shuffles
+    // ((a, b) after (a', b') composes to (a a', a b' + b)) instead of a walk of P - 1
+    // dependent steps.  Longer lines walk each column with one thread.
+    const int ns = P - 1;
+    if (W) {
+        const int j = t & (W - 1), c = t / W;
+        const bool on = j < ns && c < CB && q.col - cl + c < ncol;
+        const int o = j * CS + c;
+        T a = T(0), b = T(0);
+        if (on) {
+            const T w = s_sw[o];
+            a = -s_sa[o] * w;
+            b = (s_g[o] - s_eh[o] * s_yf[o + CS]) * w;
+        }
+        for (int d = 1; d < W; d <<= 1) {  // L y = g
+            const T ap = __shfl_up(a, d, W), bp = __shfl_up(b, d, W);
+            if (j >= d) {
+                b = fma(a, bp, b);
+                a *= ap;
+            }
+        }
+        a = on ? -s_sc[o] : T(0);
+        if (!on) b = T(0);
+        for (int d = 1; d < W; d <<= 1) {  // U z = y
+            const T an = __shfl_down(a, d, W), bn = __shfl_down(b, d, W);
+            if (j + d < W) {
+                b = fma(a, bn, b);
+                a *= an;
+            }
+        }
+        if (on) s_g[o] = b;
+    } else if (seg == 0 && q.col < ncol) {
         T dq = T(0);
         for (int j = 0; j < P - 1; ++j) {
-            const int o = j * CB + cl;
-            const T g = s_g[o] - s_eh[o] * s_yf[o + CB];
+            const int o = j * CS + cl;
+            const T g = s_g[o] - s_eh[o] * s_yf[o + CS];
             const T dd = (g - s_sa[o] * dq) * s_sw[o];
             s_g[o] = dd;
             dq = dd;
         }
         T zq = T(0);
         for (int j = P - 2; j >= 0; --j) {
-            const int o = j * CB + cl;
+            const int o = j * CS + cl;
             const T zz = s_g[o] - s_sc[o] * zq;
             s_g[o] = zz;
             zq = zz;
         }
     }
     __syncthreads();
-    const T lo = (q.act && seg > 0) ? s_g[t - CB] : T(0);
-    const T elp = (q.act && seg > 0) ? s_eh[t - CB] : T(0);
-    const T hi = q.sep ? s_g[t] : T(0);
+    const T lo = (q.act && seg > 0) ? s_g[pt - CS] : T(0);
+    const T elp = (q.act && seg > 0) ? s_eh[pt - CS] : T(0);
+    const T hi = q.sep ? s_g[pt] : T(0);
     // interior solve with the separator values on the right-hand side
     dp = T(0);
 #pragma unroll

@@ -1098,7 +1098,7 @@ template <typename T, bool START>
 static void launch_part(pf_ctx* ctx, const PartShape& ps, int ncol, int n, long long plane, T* r, const T* q,
                         const T* la, const T* lw, const T* lcp, const T* schur, T* z) {
     const dim3 grid((ncol + ps.CB - 1) / ps.CB);
-    const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
+    const size_t lds = sizeof(T) * (size_t)part_lds_values(ps.P, ps.CB, ps.threads);
 #define PF_PART(MM)                                                                                            \
     pf_launch(ctx, (k_cg_part<T, MM, START>), grid, dim3(ps.threads), lds, ctx->stream, ncol, n, ps.P, ps.CB, \
                        plane, r, q, la, lw, lcp, schur, z, ctx->st, ctx->partials)
@@ -1538,7 +1538,7 @@ static void mg_smooth(pf_ctx* ctx, const MgView<T>& v, bool upd, int fin, T* r, 
     const int xcd = (ctx->mg_xcd == 2 || (ctx->mg_xcd && v.level > 0)) ? 1 : 0;
     const int ngroups = (ncol + ps.CB - 1) / ps.CB;
     const dim3 grid(xcd ? (ngroups + 7) / 8 * 8 : ngroups);
-    const size_t lds = 6 * sizeof(T) * (size_t)ps.threads;
+    const size_t lds = sizeof(T) * (size_t)part_lds_values(ps.P, ps.CB, ps.threads);
     const T om = omega > T(0) ? omega : T(ctx->mg_omega);
     const int hk = prof_kern<T>(ctx, PF_K_SMOOTH, v.level,
                                 bytes_smooth(v.nodes(), ncol, ps.P, (int)sizeof(T), upd, xin != nullptr, fin != 0) +
