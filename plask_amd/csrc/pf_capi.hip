@@ -89,6 +89,7 @@ struct pf_ctx {
     double dvals_sq = 0.;
     // line-factor set-up scratch: per-node line coefficients, per-segment sums
     double *sb = nullptr, *sc = nullptr, *sff = nullptr, *sfl = nullptr;
+    size_t nseg_cap = 0;  // entries of sff / sfl
     // interpolation workspace (axes, points, results)
     double *iax = nullptr, *ipts = nullptr, *iout = nullptr, *imid = nullptr;
     size_t ipts_cap = 0, iout_cap = 0;
@@ -426,6 +427,7 @@ static int line_config(pf_ctx* ctx) {
     if ((rc = dalloc(ctx, &ctx->sb, nseg ? (size_t)ctx->N : 0)) || (rc = dalloc(ctx, &ctx->sc, nseg ? (size_t)ctx->N : 0)) ||
         (rc = dalloc(ctx, &ctx->sff, nseg)) || (rc = dalloc(ctx, &ctx->sfl, nseg)))
         return rc;
+    ctx->nseg_cap = nseg;
     ctx->dirty = true;  // the factors depend on the partition
     drop_graphs(ctx);
     return PF_OK;
@@ -1198,9 +1200,25 @@ static int mg_refresh(pf_ctx* ctx) {
         hipLaunchKernelGGL(k_mg_coarsen_mask, dim3(grid1((L.N + 63) / 64, 256, 4096)), dim3(256), 0, ctx->stream, tr,
                            F ? F->dmask : ctx->dmask, L.dmask);
         const int ncol = L.n0 * L.n1;
-        // same segment lengths as the solve grid (longer segments on the small levels, with
-        // fewer separators in the serial Schur solve, measured slower: fewer waves per CU)
-        const PartShape s64 = part_shape<double>(ctx->pm64, n2, ncol), s32 = part_shape<float>(ctx->pm32, n2, ncol);
+        // Segment lengths: with the default lengths (8 planes FP64, 16 FP32 on the solve
+        // grid) the coarse levels, which are latency-bound, take shorter segments, i.e.
+        // shorter dependent chains per thread, as long as the separators (P - 1) stay few
+        // enough for the scan of the separator system (at most 64): 8 planes, and 4 on
+        // levels of at most 512 columns (measured per level on the 4.2M-DOF stack; longer
+        // segments were slower on every coarse level).  A requested length applies to
+        // every level.
+        int m64 = ctx->pm64, m32 = ctx->pm32;
+        if (ctx->line_m == 0) {
+            int mc = 0;
+            if ((n2 + 7) / 8 <= 65) mc = 8;
+            if (ncol <= 512 && (n2 + 3) / 4 <= 65) mc = 4;
+            if (mc && (size_t)((n2 + mc - 1) / mc) * ncol <= ctx->nseg_cap && part_shape<double>(mc, n2, ncol).m &&
+                part_shape<float>(mc, n2, ncol).m) {
+                m64 = std::min(m64, mc);
+                m32 = std::min(m32, mc);
+            }
+        }
+        const PartShape s64 = part_shape<double>(m64, n2, ncol), s32 = part_shape<float>(m32, n2, ncol);
         if (!s64.m || !s32.m) return fail(ctx, PF_EINVAL, "multigrid: line partition does not fit a coarse level");
         L.pm64 = s64.m;
         L.pm32 = s32.m;
