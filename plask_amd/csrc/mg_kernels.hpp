@@ -54,6 +54,22 @@ __device__ __forceinline__ void restrict_axis(int I, int nc, const int* __restri
     }
 }
 
+// (i, j, k) of node idx of an n0 x n1 x planes grid: 32-bit divisions below 2^31 nodes
+// (the 64-bit ones are long software sequences on the critical path of these short
+// kernels)
+__device__ __forceinline__ void split_node(long long idx, long long N, int n0, int n1, int& i, int& j, long long& k) {
+    if (N <= 0x7fffffffLL) {
+        const unsigned u = (unsigned)idx, row = u / (unsigned)n0, kk = row / (unsigned)n1;
+        i = (int)(u - row * (unsigned)n0);
+        j = (int)(row - kk * (unsigned)n1);
+        k = kk;
+    } else {
+        i = (int)(idx % n0);
+        j = (int)((idx / n0) % n1);
+        k = idx / ((long long)n0 * n1);
+    }
+}
+
 // b_c = R r_f (transpose of the interpolation), zero at coarse Dirichlet nodes; a
 // coarse node's support has at most 3 fine nodes per in-plane axis (coarsening by 2)
 template <typename T>
@@ -66,9 +82,9 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __r
     const long long Nc = planec * tr.n2;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nc;
          idx += (long long)gridDim.x * blockDim.x) {
-        const int I = (int)(idx % tr.n0c);
-        const int J = (int)((idx / tr.n0c) % tr.n1c);
-        const long long k = idx / planec;
+        int I, J;
+        long long k;
+        split_node(idx, Nc, tr.n0c, tr.n1c, I, J, k);
         int ii[3], jj[3];
         T wi[3], wj[3];
         restrict_axis(I, tr.n0c, tr.cx, tr.mx, wx, ii, wi);
@@ -98,9 +114,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(MgTransfer tr, const T* 
         // the Dirichlet test selects the value instead of skipping the node, so the
         // mask, table and vector loads do not wait on a branch
         const bool fixed = dbit(dmask_f, idx);
-        const int i = (int)(idx % tr.n0f);
-        const int j = (int)((idx / tr.n0f) % tr.n1f);
-        const long long k = idx / planef;
+        int i, j;
+        long long k;
+        split_node(idx, Nf, tr.n0f, tr.n1f, i, j, k);
         const int I = tr.mx[i], J = tr.my[j];
         const int I1 = min(I + 1, tr.n0c - 1), J1 = min(J + 1, tr.n1c - 1);
         const T a = wx[i], b = wy[j];

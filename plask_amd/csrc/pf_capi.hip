@@ -1651,7 +1651,7 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
                 h = open(cat(l, PF_KCAT_MG_TRANSFER));
                 const int hk = prof_kern<T>(ctx, PF_K_RESTRICT, l,
                                             sizeof(T) * (double)(v.nodes() + c.nodes()) + mask_bytes(c.nodes()));
-                pf_launch(ctx, (k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 4096)), dim3(256), 0,
+                pf_launch(ctx, (k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 1 << 20)), dim3(256), 0,
                                    ctx->stream, tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
                 prof_close(ctx, hk);
                 prof_close(ctx, h);
@@ -1678,7 +1678,7 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
         const long long nf = (long long)v.ncol * v.planes;
         int h = open(cat(l, PF_KCAT_MG_TRANSFER));
         const int hk = prof_kern<T>(ctx, PF_K_PROLONG, l, sizeof(T) * (double)(2 * nf + c.nodes()) + mask_bytes(nf));
-        pf_launch(ctx, (k_mg_prolong_add<T>), dim3(grid1(nf, 256, 4096)), dim3(256), 0, ctx->stream, tr, c.wx,
+        pf_launch(ctx, (k_mg_prolong_add<T>), dim3(grid1(nf, 256, 1 << 20)), dim3(256), 0, ctx->stream, tr, c.wx,
                            c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
         prof_close(ctx, hk);
         prof_close(ctx, h);
