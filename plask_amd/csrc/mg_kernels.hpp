@@ -28,6 +28,8 @@ struct MgTransfer {
     const int* cy;  // [n1c]
     const int* mx;  // [n0f]
     const int* my;  // [n1f]
+    const int* rx;  // [3 n0c] restriction rows: fine indices per coarse node
+    const int* ry;  // [3 n1c]
 };
 
 // weight of fine node i (left coarse index m, position w) in coarse node I
@@ -73,8 +75,8 @@ __device__ __forceinline__ void split_node(long long idx, long long N, int n0, i
 // b_c = R r_f (transpose of the interpolation), zero at coarse Dirichlet nodes; a
 // coarse node's support has at most 3 fine nodes per in-plane axis (coarsening by 2)
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __restrict__ wx,
-                                                     const T* __restrict__ wy, const T* __restrict__ rf,
+__global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __restrict__ rwx,
+                                                     const T* __restrict__ rwy, const T* __restrict__ rf,
                                                      T* __restrict__ bc, const uint64_t* __restrict__ dmask_c,
                                                      const PcgState* __restrict__ st) {
     if (st->done) return;
@@ -85,10 +87,16 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgTransfer tr, const T* __r
         int I, J;
         long long k;
         split_node(idx, Nc, tr.n0c, tr.n1c, I, J, k);
+        // the coarse node's restriction rows (the same as restrict_axis gives, tabulated)
         int ii[3], jj[3];
         T wi[3], wj[3];
-        restrict_axis(I, tr.n0c, tr.cx, tr.mx, wx, ii, wi);
-        restrict_axis(J, tr.n1c, tr.cy, tr.my, wy, jj, wj);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ii[a] = tr.rx[3 * I + a];
+            wi[a] = rwx[3 * I + a];
+            jj[a] = tr.ry[3 * J + a];
+            wj[a] = rwy[3 * J + a];
+        }
         const T* src = rf + k * planef;
         T s = T(0);
 #pragma unroll

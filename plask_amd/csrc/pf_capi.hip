@@ -51,6 +51,10 @@ struct MgLevel {
     int *cx = nullptr, *cy = nullptr, *mx = nullptr, *my = nullptr;
     double *wx = nullptr, *wy = nullptr;
     float *wx32 = nullptr, *wy32 = nullptr;
+    // restriction rows per coarse node of each axis: 3 fine indices and weights
+    int *rx = nullptr, *ry = nullptr;
+    double *rwx = nullptr, *rwy = nullptr;
+    float *rwx32 = nullptr, *rwy32 = nullptr;
     // partitioned line factors (same partition lengths as the solve grid)
     double *la = nullptr, *lw = nullptr, *lcp = nullptr, *schur = nullptr;
     float *la32 = nullptr, *lw32 = nullptr, *lcp32 = nullptr, *schur32 = nullptr;
@@ -355,7 +359,8 @@ static void mg_free(pf_ctx* ctx) {
                          (void**)&L.schur, (void**)&L.la32,  (void**)&L.lw32,    (void**)&L.lcp32, (void**)&L.schur32,
                          (void**)&L.b,     (void**)&L.x,     (void**)&L.r,       (void**)&L.b32,   (void**)&L.x32,
                          (void**)&L.r32,   (void**)&L.rmid,  (void**)&L.rmid32, (void**)&L.dsh,
-                         (void**)&L.dsh32};
+                         (void**)&L.dsh32, (void**)&L.rx,    (void**)&L.ry,     (void**)&L.rwx,
+                         (void**)&L.rwy,   (void**)&L.rwx32, (void**)&L.rwy32};
         for (void** q : ptrs) dfree(ctx, q);
     }
     ctx->mg.clear();
@@ -1167,6 +1172,8 @@ static MgTransfer mg_transfer(pf_ctx* ctx, size_t l) {
     tr.cx = L.cx;
     tr.cy = L.cy;
     tr.mx = L.mx;
+    tr.rx = L.rx;
+    tr.ry = L.ry;
     tr.my = L.my;
     return tr;
 }
@@ -1280,6 +1287,35 @@ static void mg_coarsen_axis(const std::vector<double>& f, bool on, std::vector<i
     for (int i : ci) c.push_back(f[i]);
 }
 
+// Restriction rows of one axis, per coarse node I: its (up to) 3 fine nodes (those strictly
+// between the neighbouring coarse nodes; absent ones repeat the first with weight 0) and
+// their interpolation weights in I, as restrict_axis computes them on the device, the FP32
+// weights in FP32 arithmetic like the kernel's.  Read by k_mg_restrict, so that its index
+// and weight loads do not wait on the coarse-index loads.
+static void mg_restrict_rows(const std::vector<int>& ci, const std::vector<int>& m, const std::vector<double>& w,
+                             std::vector<int>& ri, std::vector<double>& rw, std::vector<float>& rw32) {
+    const int nc = (int)ci.size();
+    ri.assign(3 * (size_t)nc, 0);
+    rw.assign(3 * (size_t)nc, 0.);
+    rw32.assign(3 * (size_t)nc, 0.f);
+    for (int I = 0; I < nc; ++I) {
+        const int lo = I > 0 ? ci[I - 1] + 1 : 0;
+        const int hi = I < nc - 1 ? ci[I + 1] - 1 : ci[I];
+        for (int a = 0; a < 3; ++a) {
+            const int i = lo + a, ix = i <= hi ? i : lo;
+            ri[3 * I + a] = ix;
+            if (i > hi) continue;
+            if (m[ix] == I) {
+                rw[3 * I + a] = 1. - w[ix];
+                rw32[3 * I + a] = 1.f - (float)w[ix];
+            } else if (m[ix] + 1 == I) {
+                rw[3 * I + a] = w[ix];
+                rw32[3 * I + a] = (float)w[ix];
+            }
+        }
+    }
+}
+
 // levels of the current grid: an in-plane axis (3D: 0 and 1, 2D: 0) is coarsened while
 // it has more than 3 nodes; at most mg_max_levels levels in all (0: no limit)
 static int mg_build(pf_ctx* ctx) {
@@ -1319,6 +1355,16 @@ static int mg_build(pf_ctx* ctx) {
             (rc = upload(ctx, &L.hy, hy)) || (rc = upload(ctx, &L.hx32, std::vector<float>(hx.begin(), hx.end()))) ||
             (rc = upload(ctx, &L.hy32, std::vector<float>(hy.begin(), hy.end()))))
             return rc;
+        {
+            std::vector<int> rx, ry;
+            std::vector<double> rwx, rwy;
+            std::vector<float> rwx32, rwy32;
+            mg_restrict_rows(cix, mx, wx, rx, rwx, rwx32);
+            mg_restrict_rows(ciy, my, wy, ry, rwy, rwy32);
+            if ((rc = upload(ctx, &L.rx, rx)) || (rc = upload(ctx, &L.ry, ry)) || (rc = upload(ctx, &L.rwx, rwx)) ||
+                (rc = upload(ctx, &L.rwy, rwy)) || (rc = upload(ctx, &L.rwx32, rwx32)) || (rc = upload(ctx, &L.rwy32, rwy32)))
+                return rc;
+        }
         if ((rc = dalloc(ctx, &L.kh, L.E)) || (rc = dalloc(ctx, &L.kv, L.E)) || (rc = dalloc(ctx, &L.kh32, L.E)) ||
             (rc = dalloc(ctx, &L.kv32, L.E)) || (rc = dalloc(ctx, &L.dmask, (L.N + 63) / 64)) ||
             (rc = dalloc(ctx, &L.la, L.N)) || (rc = dalloc(ctx, &L.lw, L.N)) || (rc = dalloc(ctx, &L.lcp, L.N)) ||
@@ -1342,7 +1388,7 @@ struct MgView {
     Axes3<T> g;   // 3D levels
     Axes2<T> g2;  // 2D levels
     int ncol, planes;
-    const T *kh, *kv, *la, *lw, *lcp, *schur, *wx, *wy;
+    const T *kh, *kv, *la, *lw, *lcp, *schur, *wx, *wy, *rwx, *rwy;
     const uint64_t* dmask;
     T *b, *x, *r;
     int pm;
@@ -1379,6 +1425,8 @@ template <> struct MgPick<double> {
     static const double* schur(const MgLevel& L) { return L.schur; }
     static const double* wx(const MgLevel& L) { return L.wx; }
     static const double* wy(const MgLevel& L) { return L.wy; }
+    static const double* rwx(const MgLevel& L) { return L.rwx; }
+    static const double* rwy(const MgLevel& L) { return L.rwy; }
     static double* b(const MgLevel& L) { return L.b; }
     static double* x(const MgLevel& L) { return L.x; }
     static double* r(const MgLevel& L) { return L.r; }
@@ -1408,6 +1456,8 @@ template <> struct MgPick<float> {
     static const float* schur(const MgLevel& L) { return L.schur32; }
     static const float* wx(const MgLevel& L) { return L.wx32; }
     static const float* wy(const MgLevel& L) { return L.wy32; }
+    static const float* rwx(const MgLevel& L) { return L.rwx32; }
+    static const float* rwy(const MgLevel& L) { return L.rwy32; }
     static float* b(const MgLevel& L) { return L.b32; }
     static float* x(const MgLevel& L) { return L.x32; }
     static float* r(const MgLevel& L) { return L.r32; }
@@ -1431,7 +1481,7 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
         v.lw = S::lw(ctx);
         v.lcp = S::lcp(ctx);
         v.schur = S::schur(ctx);
-        v.wx = v.wy = nullptr;
+        v.wx = v.wy = v.rwx = v.rwy = nullptr;
         v.dmask = ctx->dmask;
         v.b = r0;
         v.x = z0;
@@ -1452,6 +1502,8 @@ static MgView<T> mg_view(pf_ctx* ctx, int l, T* r0, T* z0) {
     v.schur = S::schur(L);
     v.wx = S::wx(L);
     v.wy = S::wy(L);
+    v.rwx = S::rwx(L);
+    v.rwy = S::rwy(L);
     v.dmask = L.dmask;
     v.b = S::b(L);
     v.x = S::x(L);
@@ -1666,7 +1718,7 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
                 const int hk = prof_kern<T>(ctx, PF_K_RESTRICT, l,
                                             sizeof(T) * (double)(v.nodes() + c.nodes()) + mask_bytes(c.nodes()));
                 pf_launch(ctx, (k_mg_restrict<T>), dim3(grid1(ctx->mg[l].N, 256, 1 << 20)), dim3(256), 0,
-                                   ctx->stream, tr, c.wx, c.wy, (const T*)v.r, c.b, c.dmask, ctx->st);
+                                   ctx->stream, tr, c.rwx, c.rwy, (const T*)v.r, c.b, c.dmask, ctx->st);
                 prof_close(ctx, hk);
                 prof_close(ctx, h);
             }
