@@ -258,9 +258,11 @@ int pf_set_launch(pf_ctx* ctx, int rows, int planes);
 /* Line preconditioner schedule (the preconditioner itself, the exact tridiagonal
  * solve along whole lines of the layer axis, does not change): seg = 4, 8, 16 or
  * 32 solves each line as segments of seg planes, one thread each, coupled through
- * the Schur complement of the segment separators (FP64 uses at most 16);
- * seg = 0 restores the default (8 FP64, 16 FP32); seg < 0 selects the sequential
- * one-thread-per-line kernel.  No counterpart in the reference (a tuning hook). */
+ * the Schur complement of the segment separators (FP64 uses at most 16), on every
+ * multigrid level; seg = 0 restores the default (8 FP64, 16 FP32 on the solve grid;
+ * the multigrid coarse levels take 8, and 4 on levels of at most 512 columns, where
+ * the separators stay at most 64); seg < 0 selects the sequential one-thread-per-line
+ * kernel.  No counterpart in the reference (a tuning hook). */
 int pf_set_line_segment(pf_ctx* ctx, int seg);
 
 /* Multigrid preconditioner (precond = 3): levels coarsen the in-plane axes (3D: axes 0
