@@ -136,6 +136,56 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(MgTransfer tr, const T* 
     }
 }
 
+// The same for an even number of fine nodes along axis 0, two nodes (i, i + 1), i even, per
+// thread: their interval indices differ by 0 or 1, so the pair needs three coarse values
+// per coarse row instead of four per node, and the pair's table, mask and vector accesses
+// are single (8- or 16-byte) loads.  The interpolation formula per node is that of
+// k_mg_prolong_add (results agree to rounding: the compiler may contract it into FMAs
+// differently).
+template <typename T>
+struct alignas(2 * sizeof(T)) PairOf {
+    T a, b;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong_add2(MgTransfer tr, const T* __restrict__ wx,
+                                                         const T* __restrict__ wy, const T* __restrict__ xc,
+                                                         T* __restrict__ xf, const uint64_t* __restrict__ dmask_f,
+                                                         const PcgState* __restrict__ st) {
+    if (st->done) return;
+    const long long planec = (long long)tr.n0c * tr.n1c;
+    const int h0 = tr.n0f / 2;  // pairs per row (n0f even)
+    const long long Np = (long long)h0 * tr.n1f * tr.n2;
+    for (long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x; pid < Np;
+         pid += (long long)gridDim.x * blockDim.x) {
+        int ip, j;
+        long long k;
+        split_node(pid, Np, h0, tr.n1f, ip, j, k);
+        const int i = 2 * ip;
+        const long long idx = 2 * pid;  // the pair's first fine node (rows hold whole pairs)
+        const uint64_t w = dmask_f[idx >> 6];  // idx even: both nodes in one mask word
+        const bool fixed0 = (w >> (idx & 63)) & 1ull, fixed1 = (w >> ((idx + 1) & 63)) & 1ull;
+        const int2 Ip = *reinterpret_cast<const int2*>(tr.mx + i);
+        const PairOf<T> ap = *reinterpret_cast<const PairOf<T>*>(wx + i);
+        const PairOf<T> xo = *reinterpret_cast<const PairOf<T>*>(xf + idx);
+        const int J = tr.my[j], J1 = min(J + 1, tr.n1c - 1);
+        const T b = wy[j];
+        const int I0 = Ip.x, I0n = min(I0 + 1, tr.n0c - 1), I1n = min(Ip.y + 1, tr.n0c - 1);
+        const T* r0 = xc + k * planec + (long long)J * tr.n0c;
+        const T* r1 = xc + k * planec + (long long)J1 * tr.n0c;
+        const T u0 = r0[I0], u1 = r0[I0n], u2 = r0[I1n];
+        const T v0 = r1[I0], v1 = r1[I0n], v2 = r1[I1n];
+        const bool same = Ip.y == I0;  // else Ip.y == I0 + 1
+        const T lo0 = (T(1) - ap.a) * u0 + ap.a * u1, hi0 = (T(1) - ap.a) * v0 + ap.a * v1;
+        const T lo1 = (T(1) - ap.b) * (same ? u0 : u1) + ap.b * (same ? u1 : u2);
+        const T hi1 = (T(1) - ap.b) * (same ? v0 : v1) + ap.b * (same ? v1 : v2);
+        PairOf<T> out;
+        out.a = fixed0 ? xo.a : xo.a + ((T(1) - b) * lo0 + b * hi0);
+        out.b = fixed1 ? xo.b : xo.b + ((T(1) - b) * lo1 + b * hi1);
+        *reinterpret_cast<PairOf<T>*>(xf + idx) = out;
+    }
+}
+
 // Coarse element conductivities: coarse element (I, J, k) covers the fine elements
 // [cx[I], cx[I+1]) x [cy[J], cy[J+1]) of element layer k; area-weighted means.
 __global__ __launch_bounds__(256) void k_mg_coarsen_k(MgTransfer tr, const double* __restrict__ hxf,

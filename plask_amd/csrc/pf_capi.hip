@@ -1744,8 +1744,12 @@ static void mg_vcycle(pf_ctx* ctx, T* r, const T* q, T* z, bool upd, int fin) {
         const long long nf = (long long)v.ncol * v.planes;
         int h = open(cat(l, PF_KCAT_MG_TRANSFER));
         const int hk = prof_kern<T>(ctx, PF_K_PROLONG, l, sizeof(T) * (double)(2 * nf + c.nodes()) + mask_bytes(nf));
-        pf_launch(ctx, (k_mg_prolong_add<T>), dim3(grid1(nf, 256, 1 << 20)), dim3(256), 0, ctx->stream, tr, c.wx,
-                           c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
+        if (tr.n0f % 2 == 0)  // two fine nodes of a row per thread
+            pf_launch(ctx, (k_mg_prolong_add2<T>), dim3(grid1(nf / 2, 256, 1 << 20)), dim3(256), 0, ctx->stream, tr,
+                      c.wx, c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
+        else
+            pf_launch(ctx, (k_mg_prolong_add<T>), dim3(grid1(nf, 256, 1 << 20)), dim3(256), 0, ctx->stream, tr, c.wx,
+                      c.wy, (const T*)c.x, v.x, v.dmask, ctx->st);
         prof_close(ctx, hk);
         prof_close(ctx, h);
         h = open(cat(l, PF_KCAT_MG_RESIDUAL));
