@@ -1218,8 +1218,10 @@ __global__ void k_line_setup2d(Axes2<double> g, const double* __restrict__ kh, c
 //   z_I = T_II^-1 (r_I - T_Is z_s),   S z_s = r_s - T_sI T_II^-1 r_I,
 //   S = T_ss - T_sI T_II^-1 T_Is   (tridiagonal, SPD, size P - 1).
 // T_II is block diagonal (one tridiagonal block per interior), so each segment
-// is solved by its own thread with its planes held in registers; one thread per
-// column then solves the small S system.  Stored per node:
+// is solved by its own thread with its planes held in registers; the small S system
+// of each column is then solved from its Thomas factors (by a shuffle scan over the
+// separators, or one thread's walk for lines of more than 64 separators; part_solve).
+// Stored per node:
 //   interior plane: la = coupling to the previous interior plane (0 on the
 //     first), lw = LDL^T pivot inverse, lcp = c' (0 on the last);
 //   separator: la = coupling to the plane below, lcp = coupling to the plane
